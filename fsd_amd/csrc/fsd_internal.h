@@ -1,0 +1,252 @@
+// Internal declarations of the MI355X FSD library (libfsd.so).
+//
+// Data layout in HBM (all device arrays owned by the context unless noted):
+//   particles   pos float4[N], generalised vectors in the reference layouts;
+//   neighbours  fixed-stride list nlist[i*maxn + k] with counts nneigh[i];
+//   lubrication per-pair records (neighbour, unit vector, five "12" resistance
+//               functions) + one symmetric 6x6 self block per particle,
+//               built once per step and reused by every R_FU apply;
+//   PSE grids   11 real fields [11][M] (M = Nx*Ny*Nz, z fastest) and their
+//               half spectra [11][Nx*Ny*(Nz/2+1)] (rocFFT R2C/C2R);
+//   IC(0)       block-sparse lower factor on the RCM-permuted particle graph:
+//               per permuted particle a dense 6x6 diagonal block and one 6x6
+//               block per lower neighbour (36 floats, row-major).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <rocfft/rocfft.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "fsd.h"
+
+#define FSD_MAXP 14        // pruned neighbours incl. self (kissing number 12 + self + 1)
+#define FSD_PSTRIDE 16     // stride of the pruned list
+
+struct BoxF {
+    float Lx, Ly, Lz, xy, xz, yz;
+    float iLx, iLy, iLz;
+};
+
+// HOOMD BoxDim::minImage with rint rounding, evaluated without contraction so
+// that cutoff decisions round exactly like the oracle.
+__host__ __device__ inline void box_min_image(const BoxF &b, float &dx, float &dy, float &dz)
+{
+#pragma clang fp contract(off)
+    float img = rintf(dz * b.iLz);
+    float t = b.Lz * b.xz; t = t * img; dx = dx - t;
+    t = b.Lz * b.yz; t = t * img; dy = dy - t;
+    t = b.Lz * img; dz = dz - t;
+    img = rintf(dy * b.iLy);
+    t = b.Ly * b.xy; t = t * img; dx = dx - t;
+    t = b.Ly * img; dy = dy - t;
+    img = rintf(dx * b.iLx);
+    t = b.Lx * img; dx = dx - t;
+}
+
+__host__ __device__ inline float dist2_nc(float dx, float dy, float dz)
+{
+#pragma clang fp contract(off)
+    float s = dx * dx;
+    float t = dy * dy;
+    s = s + t;
+    t = dz * dz;
+    return s + t;
+}
+
+// HOOMD BoxDim::makeFraction
+__host__ __device__ inline void box_make_fraction(const BoxF &b, float px, float py, float pz,
+                                                  float &fx, float &fy, float &fz)
+{
+#pragma clang fp contract(off)
+    float dx = px + 0.5f * b.Lx, dy = py + 0.5f * b.Ly, dz = pz + 0.5f * b.Lz;
+    float t = b.yz * b.xy; t = b.xz - t; t = t * pz;
+    float u = b.xy * py; t = t + u; dx = dx - t;
+    u = b.yz * pz; dy = dy - u;
+    fx = dx / b.Lx; fy = dy / b.Ly; fz = dz / b.Lz;
+}
+
+// HOOMD BoxDim::wrap (one image shift per axis, x then y then z)
+__host__ __device__ inline void box_wrap(const BoxF &b, float &x, float &y, float &z, int &ix, int &iy, int &iz)
+{
+    const float hx = 0.5f * b.Lx, hy = 0.5f * b.Ly, hz = 0.5f * b.Lz;
+    float tilt_x = (b.xz - b.xy * b.yz) * z + b.xy * y;
+    if (x >= hx + tilt_x) { x -= b.Lx; ix++; }
+    else if (x < -hx + tilt_x) { x += b.Lx; ix--; }
+    float tilt_y = b.yz * z;
+    if (y >= hy + tilt_y) { y -= b.Ly; x -= b.Ly * b.xy; iy++; }
+    else if (y < -hy + tilt_y) { y += b.Ly; x += b.Ly * b.xy; iy--; }
+    if (z >= hz) { z -= b.Lz; y -= b.Lz * b.yz; x -= b.Lz * b.xz; iz++; }
+    else if (z < -hz) { z += b.Lz; y += b.Lz * b.yz; x += b.Lz * b.xz; iz--; }
+}
+
+struct LubTable {
+    const float *dist;   // 1000
+    const float *vals;   // 22000
+    float table_min, table_dr;
+};
+
+// Interpolate `ncol` resistance functions starting at column c0
+// (Lubrication.cu:172-231); contact_row < 0 disables the r <= 2.001 clamp.
+__device__ inline void lub_lookup(const LubTable &t, float dist, int c0, int ncol, int contact_row, float *out)
+{
+    if ((double)dist <= 2.001 && contact_row >= 0) {
+        for (int c = 0; c < ncol; ++c) out[c] = t.vals[22 * contact_row + c0 + c];
+        return;
+    }
+    float arg = (float)(((double)dist - 2.0) / (double)t.table_min);
+    int ind = (int)(log10f(arg) / t.table_dr);
+    ind = ind < 0 ? 0 : ind;
+    if (ind >= 999) {  // r in [~3.99968, 4): the reference reads past the distance array; row 999 is zero
+        for (int c = 0; c < ncol; ++c) out[c] = t.vals[22 * 999 + c0 + c];
+        return;
+    }
+    float dl = t.dist[ind], du = t.dist[ind + 1];
+    float fac = (dist - dl) / (du - dl);
+    for (int c = 0; c < ncol; ++c) {
+        float lo = t.vals[22 * ind + c0 + c], hi = t.vals[22 * (ind + 1) + c0 + c];
+        out[c] = lo + (hi - lo) * fac;
+    }
+}
+
+// ------------------------------------------------------------------ context
+struct ProfSlot {
+    const char *name;
+    int64_t launches;
+    double ms;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct fsd_ctx {
+    fsd_params p;
+    fsd_pse_info pse;
+    std::string err;
+    hipStream_t stream = nullptr;
+    BoxF box;
+    int N = 0;
+
+    // tables
+    float *d_ewald = nullptr; int ewald_nR = 0;
+    std::vector<float> h_ewald;
+    float *d_res_dist = nullptr, *d_res_vals = nullptr; bool have_table = false;
+    float table_min = 0.0001f, table_dr = 0.004305f;
+
+    // neighbour list
+    int maxn = 0;
+    int *d_nneigh = nullptr, *d_nlist = nullptr;
+    int ncell[3] = {0, 0, 0}; int ncells = 0; bool use_cells = false;
+    int *d_cell_of = nullptr, *d_cell_count = nullptr, *d_cell_start = nullptr, *d_cell_list = nullptr;
+    int *d_flags = nullptr;     // [0] nlist overflow, [1] lub overflow, [2] ichol fail, [3] solve spin timeout
+    int *h_flags = nullptr;     // pinned mirror
+    bool nlist_valid = false;
+    int64_t nlist_total = 0;
+
+    // lubrication pair records
+    int maxlub = 0;
+    int *d_lub_n = nullptr, *d_lub_j = nullptr;
+    float4 *d_lub_a = nullptr;  // (rx, ry, rz, XA12-YA12)
+    float4 *d_lub_b = nullptr;  // (YA12, YB12, XC12-YC12, YC12)
+    float *d_lub_self = nullptr; // 36 per particle (row-major 6x6)
+    bool lub_valid = false;
+
+    // PSE
+    int M = 0, Mh = 0;
+    float *d_grid = nullptr;      // 11*M real
+    float2 *d_spec = nullptr;     // 11*Mh complex
+    float4 *d_gridk = nullptr;    // Mh
+    rocfft_plan plan_fwd = nullptr, plan_inv = nullptr;
+    rocfft_execution_info info_fwd = nullptr, info_inv = nullptr;
+    void *d_fftwork = nullptr; size_t fftwork_size = 0;
+    int tiles[3] = {0, 0, 0}; int ntiles = 0; int tile = 8;
+    int *d_porig = nullptr;       // 3 ints per particle: stencil origin (unwrapped)
+    int *d_tile_count = nullptr, *d_tile_start = nullptr, *d_tile_list = nullptr, *d_tile_of = nullptr;
+    float *d_wave = nullptr;      // 11 per particle: wave-space u (3) and gradient (8)
+    float4 *d_rec = nullptr;      // 3 per particle: force + couplet records of the current apply
+    bool spread_valid = false;
+    const float *spread_pos = nullptr;
+
+    // preconditioner (blocks on the permuted particle graph)
+    int *d_pn = nullptr, *d_plist = nullptr, *d_pless = nullptr;
+    int *d_perm = nullptr, *d_iperm = nullptr;
+    std::vector<int> h_pn, h_plist, h_perm;
+    int *d_lrow = nullptr;        // N+1: start of each permuted particle's lower blocks
+    int *d_lcol = nullptr;        // lower block columns (permuted ids), ascending
+    int *d_urow = nullptr, *d_ucol = nullptr, *d_upos = nullptr;  // upper adjacency for L^T solves
+    float *d_Ablk = nullptr;      // 36 per lower block (A values)
+    float *d_Adiag = nullptr;     // 36 per particle (A diagonal block, full)
+    float *d_Lblk = nullptr, *d_Ldiag = nullptr;
+    int *d_isolated = nullptr;    // per permuted particle
+    int64_t nlow = 0, nlow_cap = 0;
+    int64_t nnz_ref = 0;          // reference CSR nonzeros
+    float ichol_relaxer = 1.0f;
+    int ichol_attempts = 0;
+    bool precond_valid = false;
+    // level schedules of the forward (lower) and backward (upper) particle DAGs
+    int nlev_f = 0, nlev_b = 0;
+    std::vector<int> h_lev_f_ptr, h_lev_b_ptr;
+    int *d_lev_f = nullptr, *d_lev_b = nullptr;
+    float *d_tmp6a = nullptr, *d_tmp6b = nullptr;
+    float *d_g12 = nullptr;       // G b_F and G b_U of the saddle preconditioner
+
+    // saddle / GMRES
+    float *d_V = nullptr, *d_w = nullptr, *d_t = nullptr, *d_t2 = nullptr;
+    double *d_part = nullptr; int part_blocks = 0; double *h_part = nullptr;
+    float *d_rhs = nullptr, *d_sol = nullptr;
+    float *d_force = nullptr;
+    float *d_mask1 = nullptr, *d_mask2 = nullptr;
+    float *d_ori_zero = nullptr;
+
+    // profiling
+    bool prof = false;
+    std::vector<ProfSlot> slots;
+};
+
+// kernel-launch helper with error capture
+#define FSD_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);               \
+            return FSD_E_HIP;                                                           \
+        }                                                                               \
+    } while (0)
+
+static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+// profiling scopes
+void prof_begin(fsd_ctx *ctx, int slot, hipEvent_t *ev);
+void prof_end(fsd_ctx *ctx, int slot, hipEvent_t ev);
+enum { PROF_SPREAD = 0, PROF_GREEN, PROF_CONTRACT, PROF_FFT, PROF_REAL, PROF_RFU, PROF_TRSV, PROF_ICHOL,
+       PROF_GMRES_VEC, PROF_NLIST, PROF_PRECOND_BUILD, PROF_COUNT };
+
+// kernel groups (launchers live with their kernels)
+int k_build_nlist(fsd_ctx *ctx, const float4 *pos);
+int k_build_lub_pairs(fsd_ctx *ctx, const float4 *pos);
+int k_rfu_apply(fsd_ctx *ctx, const float *U, float *F, const float *xF, float sign_out);
+int k_rfe(fsd_ctx *ctx, const float4 *pos, float shear_rate, const float *ori, float *F);
+int k_rsu(fsd_ctx *ctx, const float4 *pos, const float *U, float *S);
+int k_rse(fsd_ctx *ctx, const float4 *pos, float shear_rate, const float *ori, float *S);
+int k_setup_spread(fsd_ctx *ctx, const float4 *pos);
+int k_gridk(fsd_ctx *ctx);
+int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int parts, const float *addU);
+int k_prune(fsd_ctx *ctx, const float4 *pos);
+int k_build_blocks(fsd_ctx *ctx, const float4 *pos);
+int k_ichol(fsd_ctx *ctx, float lambda, int *failed);
+int k_precond_apply2(fsd_ctx *ctx, const float *b0, const float *b1, float *y0, float *y1, int n_rhs);
+int k_export_csr(fsd_ctx *ctx, const float4 *pos, int32_t *h_rowptr, int32_t *h_col, float *h_val,
+                 int32_t *h_prow, int32_t *h_pcol, float *h_L, int64_t cap);
+int k_force_law(fsd_ctx *ctx, const float4 *pos, float T_ext, float *F);
+int k_strain_rhs(fsd_ctx *ctx, float *rhs, float shear_rate, const float *ori, const float *force);
+int k_euler_shear(fsd_ctx *ctx, float4 *pos, float *ori, float4 *pos_gb, int *image, const float *vel,
+                  const float *mask1, const float *noise, float dt, float shear_rate, float B1);
+// vector helpers
+int k_axpby(fsd_ctx *ctx, int64_t n, float a, const float *x, float b, const float *y, float *z);
+int k_copy(fsd_ctx *ctx, int64_t n, const float *x, float *y);
+int k_fill(fsd_ctx *ctx, int64_t n, float v, float *y);
+int k_multidot(fsd_ctx *ctx, int64_t n, int k, const float *V, int64_t ldv, const float *w, double *out);
+int k_multi_axpy(fsd_ctx *ctx, int64_t n, int k, const float *V, int64_t ldv, const double *h, float *w);
+int k_nrm2(fsd_ctx *ctx, int64_t n, const float *x, double *out);
+int k_saddle_precond(fsd_ctx *ctx, const float *b, float *x);
+int k_saddle_multiply(fsd_ctx *ctx, const float4 *pos, const float *x, float *y);
+
+// host RCM (George & Liu GENRCM); phead[N+1], plist packed with self loops
+void host_rcm(int N, const int *phead, const int *plist, int *perm);

@@ -1,0 +1,529 @@
+// PSE far-field grand mobility on MI355X.
+//
+//   wave space:  particle records -> tile-gather spreading (no global atomics)
+//                -> rocFFT R2C (11 batched fields) -> Green's-function scaling on
+//                the half spectrum -> rocFFT C2R -> per-particle contraction;
+//   real space:  table-interpolated FTS pair sum, 16 lanes per particle, with an
+//                epilogue that adds the wave-space result, converts the velocity
+//                gradient to (W, E) and writes the generalised velocity.
+// Restates Mobility.cu:131-1542 and Helper_Mobility.cu:88-410.
+#include "fsd_internal.h"
+
+#define FSD_TILE 8
+
+// ------------------------------------------------------------- k vectors
+// Mobility_SetGridk_kernel (Helper_Mobility.cu:351-410) on the R2C half
+// spectrum: index (i, j, k) with k in [0, Nz/2].
+__global__ void gridk_kernel(float4 *gk, int Nx, int Ny, int Nz, BoxF b, float xi, float eta, float twopi_k)
+{
+    const int Nzh = Nz / 2 + 1;
+    const int64_t Mh = (int64_t)Nx * Ny * Nzh;
+    int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= Mh) return;
+    int i = (int)(tid / ((int64_t)Ny * Nzh));
+    int j = (int)((tid / Nzh) % Ny);
+    int k = (int)(tid % Nzh);
+    double kx = (i < (Nx + 1) / 2) ? i : i - Nx;
+    double ky = (((j < (Ny + 1) / 2) ? j : j - Ny) - (double)b.xy * kx * b.Ly / b.Lx) / b.Ly;
+    double kz = ((k < (Nz + 1) / 2) ? k : k - Nz) / (double)b.Lz;
+    kx = kx / b.Lx;
+    kx *= twopi_k; ky *= twopi_k; kz *= twopi_k;
+    double k2 = kx * kx + ky * ky + kz * kz;
+    double xisq = (double)xi * xi;
+    double w = 0.0;
+    if (!(i == 0 && j == 0 && k == 0))
+        w = 6.0 * 3.1415926536 * (1.0 + k2 / 4.0 / xisq) * exp(-(1.0 - eta) * k2 / 4.0 / xisq) / k2
+            / (double)((int64_t)Nx * Ny * Nz);
+    gk[tid] = make_float4((float)kx, (float)ky, (float)kz, (float)w);
+}
+
+int k_gridk(fsd_ctx *ctx)
+{
+    const double twopi = 2.0 * (ctx->p.parity ? 3.1416926536 : 3.14159265358979323846);
+    hipLaunchKernelGGL(gridk_kernel, dim3(nblk(ctx->Mh, 256)), dim3(256), 0, ctx->stream, ctx->d_gridk, ctx->pse.Nx,
+                       ctx->pse.Ny, ctx->pse.Nz, ctx->box, ctx->p.xi, ctx->pse.eta, (float)twopi);
+    FSD_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------- spreading set-up (per step)
+// stencil origin exactly as Mobility.cu:203-225 (float fraction, truncation)
+__global__ void stencil_origin_kernel(const float4 *pos, int N, BoxF b, int Nx, int Ny, int Nz, int P, int tile,
+                                      int tx_n, int ty_n, int tz_n, int *orig, int *tile_of, int *tile_count)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float4 p = pos[i];
+    float fx, fy, fz;
+    box_make_fraction(b, p.x, p.y, p.z, fx, fy, fz);
+    fx *= (float)Nx; fy *= (float)Ny; fz *= (float)Nz;
+    int x = (int)fx, y = (int)fy, z = (int)fz;
+    const int Pd2 = P / 2;
+    int ox = x - Pd2 + 1 - (P % 2) * ((fx - (float)x) < 0.5f);
+    int oy = y - Pd2 + 1 - (P % 2) * ((fy - (float)y) < 0.5f);
+    int oz = z - Pd2 + 1 - (P % 2) * ((fz - (float)z) < 0.5f);
+    orig[3 * i] = ox; orig[3 * i + 1] = oy; orig[3 * i + 2] = oz;
+    int wx = ((ox % Nx) + Nx) % Nx, wy = ((oy % Ny) + Ny) % Ny, wz = ((oz % Nz) + Nz) % Nz;
+    int t = ((wx / tile) * ty_n + (wy / tile)) * tz_n + (wz / tile);
+    tile_of[i] = t;
+    atomicAdd(&tile_count[t], 1);
+}
+
+__global__ void tile_scatter_kernel(const int *tile_of, int N, const int *start, int *fill, int *list)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int t = tile_of[i];
+    int s = atomicAdd(&fill[t], 1);
+    list[start[t] + s] = i;
+}
+
+__global__ void tile_sort_kernel(const int *start, int ntiles, int *list)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ntiles) return;
+    int s = start[c], e = start[c + 1];
+    for (int a = s + 1; a < e; ++a) {
+        int key = list[a], q = a - 1;
+        while (q >= s && list[q] > key) { list[q + 1] = list[q]; --q; }
+        list[q + 1] = key;
+    }
+}
+
+int scan_excl(fsd_ctx *ctx, const int *in, int *out, int n);
+
+int k_setup_spread(fsd_ctx *ctx, const float4 *pos)
+{
+    const int N = ctx->N;
+    FSD_HIP(hipMemsetAsync(ctx->d_tile_count, 0, sizeof(int) * ctx->ntiles, ctx->stream));
+    hipLaunchKernelGGL(stencil_origin_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, pos, N, ctx->box,
+                       ctx->pse.Nx, ctx->pse.Ny, ctx->pse.Nz, ctx->pse.P, ctx->tile, ctx->tiles[0], ctx->tiles[1],
+                       ctx->tiles[2], ctx->d_porig, ctx->d_tile_of, ctx->d_tile_count);
+    if (scan_excl(ctx, ctx->d_tile_count, ctx->d_tile_start, ctx->ntiles)) return FSD_E_HIP;
+    FSD_HIP(hipMemsetAsync(ctx->d_tile_count, 0, sizeof(int) * ctx->ntiles, ctx->stream));
+    hipLaunchKernelGGL(tile_scatter_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, ctx->d_tile_of, N,
+                       ctx->d_tile_start, ctx->d_tile_count, ctx->d_tile_list);
+    hipLaunchKernelGGL(tile_sort_kernel, dim3(nblk(ctx->ntiles, 64)), dim3(64), 0, ctx->stream, ctx->d_tile_start,
+                       ctx->ntiles, ctx->d_tile_list);
+    FSD_HIP(hipGetLastError());
+    ctx->spread_valid = true;
+    ctx->spread_pos = (const float *)pos;
+    return 0;
+}
+
+// ------------------------------------------------- force + couplet records
+// per particle: F (3) and the 8 stored couplet components (Mobility_TS2C_kernel,
+// Helper_Mobility.cu:294-336) packed as 3 float4
+__global__ void fc_records_kernel(const float *gF, int N, float4 *rec)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float *f = gF + 6 * (int64_t)i;
+    const float *s = gF + 6 * (int64_t)N + 5 * (int64_t)i;
+    float Lx = -f[3], Ly = -f[4], Lz = -f[5];
+    float Sxx = s[0], Sxy = s[1], Sxz = s[2], Syz = s[3], Syy = s[4];
+    float C0 = Sxx, C1 = Sxy + 0.5f * Lz, C2 = Sxz - 0.5f * Ly, C3 = Syz + 0.5f * Lx;
+    float C4 = Syy, C5 = Sxy - 0.5f * Lz, C6 = Sxz + 0.5f * Ly, C7 = Syz - 0.5f * Lx;
+    rec[3 * (int64_t)i] = make_float4(f[0], f[1], f[2], C0);
+    rec[3 * (int64_t)i + 1] = make_float4(C1, C2, C3, C4);
+    rec[3 * (int64_t)i + 2] = make_float4(C5, C6, C7, 0.f);
+}
+
+// ------------------------------------------------------------ spreading
+struct SpreadGeom {
+    int Nx, Ny, Nz, P, tile, tnx, tny, tnz;
+    float hx, hy, hz, Lx2, Ly2, Lz2, xy;
+    float prefac, expfac;
+};
+
+// one workgroup per tile of TILE^3 grid points (one thread per point); the
+// particles whose stencils reach the tile are streamed through LDS in chunks
+// and every point accumulates its 11 fields in registers.
+__global__ void __launch_bounds__(512) spread_kernel(SpreadGeom g, const float4 *pos, const int *orig,
+                                                     const float4 *rec, const int *tile_start, const int *tile_list,
+                                                     float *grid, int64_t M)
+{
+    constexpr int CH = 256;
+    __shared__ int s_tiles[27];
+    __shared__ int s_ntiles;
+    __shared__ int s_idx[CH];
+    __shared__ int s_o[CH][3];
+    __shared__ float s_p[CH][3];
+    __shared__ float4 s_r[CH][3];
+
+    const int tile_id = blockIdx.x;
+    const int tz_i = tile_id % g.tnz, ty_i = (tile_id / g.tnz) % g.tny, tx_i = tile_id / (g.tnz * g.tny);
+    const int x0 = tx_i * g.tile, y0 = ty_i * g.tile, z0 = tz_i * g.tile;
+    const int lt = threadIdx.x;
+    const int lx = lt / (g.tile * g.tile), ly = (lt / g.tile) % g.tile, lz = lt % g.tile;
+    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    const bool inside = (gx < g.Nx) && (gy < g.Ny) && (gz < g.Nz);
+
+    if (lt == 0) {
+        // origin tiles covering [t0 - P + 1, t0 + tile - 1] per axis (periodic)
+        int lists[3][3], cnt[3];
+        const int n_[3] = {g.Nx, g.Ny, g.Nz}, s0[3] = {x0, y0, z0};
+        for (int d = 0; d < 3; ++d) {
+            cnt[d] = 0;
+            int last = -1;
+            for (int o = s0[d] - g.P + 1; o <= s0[d] + g.tile - 1; ++o) {
+                int w = ((o % n_[d]) + n_[d]) % n_[d];
+                int t = w / g.tile;
+                bool seen = false;
+                for (int q = 0; q < cnt[d]; ++q) seen |= (lists[d][q] == t);
+                if (!seen && t != last && cnt[d] < 3) lists[d][cnt[d]++] = t;
+                last = t;
+            }
+        }
+        int n = 0;
+        for (int a = 0; a < cnt[0]; ++a)
+            for (int c = 0; c < cnt[1]; ++c)
+                for (int e = 0; e < cnt[2]; ++e) s_tiles[n++] = (lists[0][a] * g.tny + lists[1][c]) * g.tnz + lists[2][e];
+        s_ntiles = n;
+    }
+    __syncthreads();
+
+    float acc[11];
+#pragma unroll
+    for (int c = 0; c < 11; ++c) acc[c] = 0.f;
+
+    const int ntl = s_ntiles;
+    for (int tt = 0; tt < ntl; ++tt) {
+        const int t = s_tiles[tt];
+        const int beg = tile_start[t], end = tile_start[t + 1];
+        for (int base = beg; base < end; base += CH) {
+            const int nch = min(CH, end - base);
+            __syncthreads();
+            if (lt < nch) {
+                int i = tile_list[base + lt];
+                s_idx[lt] = i;
+                s_o[lt][0] = orig[3 * i]; s_o[lt][1] = orig[3 * i + 1]; s_o[lt][2] = orig[3 * i + 2];
+                float4 p = pos[i];
+                s_p[lt][0] = p.x; s_p[lt][1] = p.y; s_p[lt][2] = p.z;
+                s_r[lt][0] = rec[3 * (int64_t)i]; s_r[lt][1] = rec[3 * (int64_t)i + 1]; s_r[lt][2] = rec[3 * (int64_t)i + 2];
+            }
+            __syncthreads();
+            if (inside) {
+                for (int q = 0; q < nch; ++q) {
+                    int tx = gx - s_o[q][0]; tx += (tx < 0) ? g.Nx : 0; tx -= (tx >= g.Nx) ? g.Nx : 0;
+                    int ty = gy - s_o[q][1]; ty += (ty < 0) ? g.Ny : 0; ty -= (ty >= g.Ny) ? g.Ny : 0;
+                    int tz = gz - s_o[q][2]; tz += (tz < 0) ? g.Nz : 0; tz -= (tz >= g.Nz) ? g.Nz : 0;
+                    if (tx < g.P && ty < g.P && tz < g.P) {
+                        // unwrapped grid node of this stencil point, relative to the particle
+                        const float uy = g.hy * (float)(s_o[q][1] + ty) - g.Ly2;
+                        const float dx = g.hx * (float)(s_o[q][0] + tx) - g.Lx2 + g.xy * uy - s_p[q][0];
+                        const float dy = uy - s_p[q][1];
+                        const float dz = g.hz * (float)(s_o[q][2] + tz) - g.Lz2 - s_p[q][2];
+                        const float w = g.prefac * __expf(-g.expfac * (dx * dx + dy * dy + dz * dz));
+                        const float4 r0 = s_r[q][0], r1 = s_r[q][1], r2 = s_r[q][2];
+                        acc[0] += w * r0.x; acc[1] += w * r0.y; acc[2] += w * r0.z;
+                        acc[3] += w * r0.w; acc[4] += w * r1.x; acc[5] += w * r1.y; acc[6] += w * r1.z;
+                        acc[7] += w * r1.w; acc[8] += w * r2.x; acc[9] += w * r2.y; acc[10] += w * r2.z;
+                    }
+                }
+            }
+        }
+    }
+    if (inside) {
+        const int64_t id = ((int64_t)gx * g.Ny + gy) * g.Nz + gz;
+#pragma unroll
+        for (int c = 0; c < 11; ++c) grid[c * M + id] = acc[c];
+    }
+}
+
+// ------------------------------------------------------------ Green kernel
+// Mobility_WaveSpace_Green_kernel (Mobility.cu:315-479) on the half spectrum.
+// Field order: Fx Fy Fz | C_xx C_xy C_xz C_yz C_yy C_yx C_zx C_zy.
+__global__ void green_kernel(float2 *spec, const float4 *gk, int64_t Mh)
+{
+    int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= Mh) return;
+    float2 v[11];
+#pragma unroll
+    for (int c = 0; c < 11; ++c) v[c] = spec[c * Mh + id];
+    const float4 tk = gk[id];
+    float2 o[11];
+#pragma unroll
+    for (int c = 0; c < 11; ++c) o[c] = make_float2(0.f, 0.f);
+    if (id != 0) {
+        const float k[3] = {tk.x, tk.y, tk.z};
+        const float ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        const float kk = sqrtf(ksq);
+        // couplet matrix rows (x, y, z)
+        const float2 Cm[3][3] = {{v[3], v[4], v[5]}, {v[8], v[7], v[6]}, {v[9], v[10], make_float2(-(v[3].x + v[7].x), -(v[3].y + v[7].y))}};
+        float2 kdF;
+        kdF.x = (k[0] * v[0].x + k[1] * v[1].x + k[2] * v[2].x) / ksq;
+        kdF.y = (k[0] * v[0].y + k[1] * v[1].y + k[2] * v[2].y) / ksq;
+        float2 Cdk[3];
+        for (int a = 0; a < 3; ++a) {
+            Cdk[a].x = Cm[a][0].x * k[0] + Cm[a][1].x * k[1] + Cm[a][2].x * k[2];
+            Cdk[a].y = Cm[a][0].y * k[0] + Cm[a][1].y * k[1] + Cm[a][2].y * k[2];
+        }
+        float2 kCk;
+        kCk.x = (Cdk[0].x * k[0] + Cdk[1].x * k[1] + Cdk[2].x * k[2]) / ksq;
+        kCk.y = (Cdk[0].y * k[0] + Cdk[1].y * k[1] + Cdk[2].y * k[2]) / ksq;
+        const float sk = sinf(kk), ck = cosf(kk);
+        const float j0 = sk / kk;
+        const float jt = 3.0f * (sk - kk * ck) / (ksq * kk);
+        const float Buf = tk.w * j0 * j0;
+        const float Buc = -tk.w * j0 * jt;        // UC (imaginary coupling)
+        const float Bdf = tk.w * j0 * jt;         // DF (imaginary coupling)
+        const float q = (sk - kk * ck) / (ksq * kk);
+        const float Bdc = tk.w * 9.0f * q * q;    // DC
+        for (int a = 0; a < 3; ++a) {
+            float2 ff = v[a];
+            float2 u;
+            u.x = (ff.x - k[a] * kdF.x) * Buf;
+            u.y = (ff.y - k[a] * kdF.y) * Buf;
+            // + i (Cdk_a - k_a kCk) Buc
+            u.x += -(Cdk[a].y - k[a] * kCk.y) * Buc;
+            u.y += (Cdk[a].x - k[a] * kCk.x) * Buc;
+            o[a] = u;
+        }
+        // gradient slots in field order: (x,x) (x,y) (x,z) (y,z) (y,y) (y,x) (z,x) (z,y)
+        const int sa[8] = {0, 0, 0, 1, 1, 1, 2, 2}, sc[8] = {0, 1, 2, 2, 1, 0, 0, 1};
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int a = sa[s], c = sc[s];
+            const float kk_ac = k[a] * k[c];
+            float2 z;
+            z.x = v[a].x * k[c] - kk_ac * kdF.x;
+            z.y = v[a].y * k[c] - kk_ac * kdF.y;
+            float2 d;
+            d.x = -z.y * Bdf + (Cdk[a].x * k[c] - kk_ac * kCk.x) * Bdc;
+            d.y = z.x * Bdf + (Cdk[a].y * k[c] - kk_ac * kCk.y) * Bdc;
+            o[3 + s] = d;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 11; ++c) spec[c * Mh + id] = o[c];
+}
+
+// ------------------------------------------------------------ contraction
+// Mobility_WaveSpace_ContractU/D (Mobility.cu:503-841): one wavefront per
+// particle, lanes over the (y, z) stencil pairs, loop over x.  Output per
+// particle: u (3) then the gradient in the reference's transposed slots.
+__global__ void __launch_bounds__(256) contract_kernel(SpreadGeom g, const float4 *pos, const int *orig,
+                                                       const float *grid, int64_t M, int N, float quadw_prefac,
+                                                       float *wave)
+{
+    const int wave_id = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave_id >= N) return;
+    const int i = wave_id;
+    const int ox = orig[3 * i], oy = orig[3 * i + 1], oz = orig[3 * i + 2];
+    const float4 p = pos[i];
+    float acc[11];
+#pragma unroll
+    for (int c = 0; c < 11; ++c) acc[c] = 0.f;
+    const int P = g.P;
+    for (int yz = lane; yz < P * P; yz += 64) {
+        const int ty = yz / P, tz = yz % P;
+        int y = oy + ty; const float uy = g.hy * (float)y - g.Ly2;
+        int z = oz + tz; const float dz = g.hz * (float)z - g.Lz2 - p.z;
+        const float dy = uy - p.y;
+        y = ((y % g.Ny) + g.Ny) % g.Ny;
+        z = ((z % g.Nz) + g.Nz) % g.Nz;
+        for (int tx = 0; tx < P; ++tx) {
+            int x = ox + tx;
+            const float dx = g.hx * (float)x - g.Lx2 + g.xy * uy - p.x;
+            x = ((x % g.Nx) + g.Nx) % g.Nx;
+            const float w = quadw_prefac * __expf(-g.expfac * (dx * dx + dy * dy + dz * dz));
+            const int64_t id = ((int64_t)x * g.Ny + y) * g.Nz + z;
+#pragma unroll
+            for (int c = 0; c < 11; ++c) acc[c] += w * grid[c * M + id];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 11; ++c)
+        for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
+    if (lane == 0) {
+        float *w = wave + 11 * (int64_t)i;
+        w[0] = acc[0]; w[1] = acc[1]; w[2] = acc[2];
+        // fields XX XY XZ YZ YY YX ZX ZY -> slots {XX, YX, ZX, ZY, YY, XY, XZ, YZ}
+        w[3] = acc[3]; w[4] = acc[8]; w[5] = acc[9]; w[6] = acc[10];
+        w[7] = acc[7]; w[8] = acc[4]; w[9] = acc[5]; w[10] = acc[6];
+    }
+}
+
+// ------------------------------------------------------------ real space
+struct RealParams {
+    BoxF b;
+    float rcut, dr, self_x, self_y;
+    int ewald_n;
+};
+
+// Mobility_RealSpace_kernel (Mobility.cu:1046-1226) + LinearCombination/Add4
+// + Mobility_D2WE_kernel + Saddle_MakeGeneralizedU_kernel (+ the saddle's
+// x_U term), G = 16 lanes per particle.
+template <int G>
+__global__ void __launch_bounds__(256) realspace_kernel(RealParams rp, const float4 *pos, const float4 *rec,
+                                                        const float4 *tab, const int *nneigh, const int *nlist,
+                                                        int maxn, int N, const float *wave, const float *addU,
+                                                        int use_real, float *gU)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = gid / G, lane = gid % G;
+    const bool active = i < N;
+    float u[3] = {0.f, 0.f, 0.f};
+    float D[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (active && use_real) {
+        const float4 pi = pos[i];
+        const float mind2 = rp.dr * rp.dr, maxd2 = rp.rcut * rp.rcut;
+        const int nn = nneigh[i];
+        for (int k = lane; k < nn; k += G) {
+            const int j = nlist[(int64_t)i * maxn + k];
+            const float4 pj = pos[j];
+            float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            box_min_image(rp.b, dx, dy, dz);
+            const float d2 = dist2_nc(dx, dy, dz);
+            if (!(d2 < maxd2 && d2 >= mind2)) continue;
+            const float dist = sqrtf(d2);
+            float xr = (float)rp.ewald_n * (dist - rp.dr);
+            xr = xr / (rp.rcut - rp.dr);
+            int r_ind = (int)floorf(xr);
+            r_ind = r_ind < 0 ? 0 : (r_ind > rp.ewald_n - 1 ? rp.ewald_n - 1 : r_ind);
+            const float fac = dist / rp.dr - (float)r_ind - 1.0f;
+            const float4 m1 = tab[2 * r_ind], p1 = tab[2 * r_ind + 2];
+            const float4 m2 = tab[2 * r_ind + 1], p2 = tab[2 * r_ind + 3];
+            const float f1 = m1.x + (p1.x - m1.x) * fac;
+            const float f2 = m1.y + (p1.y - m1.y) * fac;
+            const float g1 = -(m1.z + (p1.z - m1.z) * fac);
+            const float g2 = -(m1.w + (p1.w - m1.w) * fac);
+            const float h1 = m2.x + (p2.x - m2.x) * fac;
+            const float h2 = m2.y + (p2.y - m2.y) * fac;
+            const float h3 = m2.z + (p2.z - m2.z) * fac;
+            const float R[3] = {dx / dist, dy / dist, dz / dist};
+            const float4 r0 = rec[3 * (int64_t)j], r1 = rec[3 * (int64_t)j + 1], r2 = rec[3 * (int64_t)j + 2];
+            const float Fj[3] = {r0.x, r0.y, r0.z};
+            const float C[8] = {r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z};
+            const float rdotf = R[0] * Fj[0] + R[1] * Fj[1] + R[2] * Fj[2];
+            const float Cm[3][3] = {{C[0], C[1], C[2]}, {C[5], C[4], C[3]}, {C[6], C[7], -(C[0] + C[4])}};
+            float Cdr[3], rdC[3];
+            for (int a = 0; a < 3; ++a) {
+                Cdr[a] = Cm[a][0] * R[0] + Cm[a][1] * R[1] + Cm[a][2] * R[2];
+                rdC[a] = R[0] * Cm[0][a] + R[1] * Cm[1][a] + R[2] * Cm[2][a];
+            }
+            const float rCr = R[0] * Cdr[0] + R[1] * Cdr[1] + R[2] * Cdr[2];
+            for (int a = 0; a < 3; ++a) {
+                u[a] += f1 * Fj[a] + (f2 - f1) * rdotf * R[a];
+                u[a] += g1 * (Cdr[a] - rCr * R[a]) + g2 * (rdC[a] - 4.f * rCr * R[a]);
+            }
+            const int A[8] = {0, 0, 0, 1, 1, 1, 2, 2};
+            const int B[8] = {0, 1, 2, 2, 1, 0, 0, 1};
+            const int Cs[8] = {0, 5, 6, 7, 4, 1, 2, 3};
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int a = A[c], bb = B[c];
+                const float del = (a == bb) ? 1.f : 0.f;
+                D[c] += -g1 * (R[a] * Fj[bb] - rdotf * R[a] * R[bb]) - g2 * (del * rdotf + Fj[a] * R[bb] - 4.f * rdotf * R[a] * R[bb]);
+                D[c] += h1 * (C[c] - 4.f * C[Cs[c]]) + h2 * (R[a] * Cdr[bb] - rCr * R[a] * R[bb])
+                        + h3 * (del * rCr + Cdr[a] * R[bb] + R[a] * rdC[bb] + rdC[a] * R[bb] - 6.f * rCr * R[a] * R[bb] - C[Cs[c]]);
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        for (int o = G / 2; o > 0; o >>= 1) u[a] += __shfl_xor(u[a], o, G);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        for (int o = G / 2; o > 0; o >>= 1) D[c] += __shfl_xor(D[c], o, G);
+    if (!active || lane != 0) return;
+    const float4 r0 = rec[3 * (int64_t)i], r1 = rec[3 * (int64_t)i + 1], r2 = rec[3 * (int64_t)i + 2];
+    if (use_real) {
+        const float C[8] = {r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z};
+        u[0] += rp.self_x * r0.x; u[1] += rp.self_x * r0.y; u[2] += rp.self_x * r0.z;
+        D[0] += rp.self_y * (C[0] - 4.f * C[0]); D[1] += rp.self_y * (C[1] - 4.f * C[5]);
+        D[2] += rp.self_y * (C[2] - 4.f * C[6]); D[3] += rp.self_y * (C[3] - 4.f * C[7]);
+        D[4] += rp.self_y * (C[4] - 4.f * C[4]); D[5] += rp.self_y * (C[5] - 4.f * C[1]);
+        D[6] += rp.self_y * (C[6] - 4.f * C[2]); D[7] += rp.self_y * (C[7] - 4.f * C[3]);
+    }
+    if (wave) {
+        const float *w = wave + 11 * (int64_t)i;
+        u[0] += w[0]; u[1] += w[1]; u[2] += w[2];
+        for (int c = 0; c < 8; ++c) D[c] += w[3 + c];
+    }
+    // D2WE (Helper_Mobility.cu:235-283)
+    const float W0 = 0.5f * (D[3] - D[7]), W1 = 0.5f * (D[6] - D[2]), W2 = 0.5f * (D[1] - D[5]);
+    const float e0 = D[0], e1 = 0.5f * (D[1] + D[5]), e2 = 0.5f * (D[2] + D[6]), e3 = 0.5f * (D[3] + D[7]), e4 = D[4];
+    float *o6 = gU + 6 * (int64_t)i;
+    float add[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (addU)
+        for (int a = 0; a < 6; ++a) add[a] = addU[6 * (int64_t)i + a];
+    o6[0] = u[0] + add[0]; o6[1] = u[1] + add[1]; o6[2] = u[2] + add[2];
+    o6[3] = W0 + add[3]; o6[4] = W1 + add[4]; o6[5] = W2 + add[5];
+    float *o5 = gU + 6 * (int64_t)N + 5 * (int64_t)i;
+    o5[0] = 2.f * e0 + e4; o5[1] = 2.f * e1; o5[2] = 2.f * e2; o5[3] = 2.f * e3; o5[4] = 2.f * e4 + e0;
+}
+
+// ------------------------------------------------------------ driver
+// gU = M gF (+ addU on the first 6N rows); parts: 1 real, 2 wave, 3 both
+int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int parts, const float *addU)
+{
+    const int N = ctx->N;
+    const fsd_pse_info &ps = ctx->pse;
+    float4 *rec = ctx->d_rec;
+    hipLaunchKernelGGL(fc_records_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, gF, N, rec);
+    FSD_HIP(hipGetLastError());
+    SpreadGeom g;
+    g.Nx = ps.Nx; g.Ny = ps.Ny; g.Nz = ps.Nz; g.P = ps.P; g.tile = ctx->tile;
+    g.tnx = ctx->tiles[0]; g.tny = ctx->tiles[1]; g.tnz = ctx->tiles[2];
+    g.hx = ps.gridh_x; g.hy = ps.gridh_y; g.hz = ps.gridh_z;
+    g.Lx2 = ctx->box.Lx / 2.0f; g.Ly2 = ctx->box.Ly / 2.0f; g.Lz2 = ctx->box.Lz / 2.0f; g.xy = ctx->box.xy;
+    g.prefac = ps.prefac; g.expfac = ps.expfac;
+    const float *wave = nullptr;
+    if (parts & 2) {
+        if (!ctx->spread_valid || ctx->spread_pos != (const float *)pos) {
+            int rc = k_setup_spread(ctx, pos);
+            if (rc) return rc;
+        }
+        hipEvent_t ev;
+        prof_begin(ctx, PROF_SPREAD, &ev);
+        hipLaunchKernelGGL(spread_kernel, dim3(ctx->ntiles), dim3(ctx->tile * ctx->tile * ctx->tile), 0, ctx->stream,
+                           g, pos, ctx->d_porig, rec, ctx->d_tile_start, ctx->d_tile_list, ctx->d_grid, (int64_t)ctx->M);
+        FSD_HIP(hipGetLastError());
+        prof_end(ctx, PROF_SPREAD, ev);
+        prof_begin(ctx, PROF_FFT, &ev);
+        {
+            void *in[1] = {ctx->d_grid};
+            void *out[1] = {ctx->d_spec};
+            if (rocfft_execute(ctx->plan_fwd, in, out, ctx->info_fwd) != rocfft_status_success) {
+                ctx->err = "rocfft forward failed";
+                return FSD_E_HIP;
+            }
+        }
+        prof_end(ctx, PROF_FFT, ev);
+        prof_begin(ctx, PROF_GREEN, &ev);
+        hipLaunchKernelGGL(green_kernel, dim3(nblk(ctx->Mh, 256)), dim3(256), 0, ctx->stream, ctx->d_spec,
+                           ctx->d_gridk, (int64_t)ctx->Mh);
+        FSD_HIP(hipGetLastError());
+        prof_end(ctx, PROF_GREEN, ev);
+        prof_begin(ctx, PROF_FFT, &ev);
+        {
+            void *in[1] = {ctx->d_spec};
+            void *out[1] = {ctx->d_grid};
+            if (rocfft_execute(ctx->plan_inv, in, out, ctx->info_inv) != rocfft_status_success) {
+                ctx->err = "rocfft inverse failed";
+                return FSD_E_HIP;
+            }
+        }
+        prof_end(ctx, PROF_FFT, ev);
+        prof_begin(ctx, PROF_CONTRACT, &ev);
+        hipLaunchKernelGGL(contract_kernel, dim3(nblk((int64_t)N * 64, 256)), dim3(256), 0, ctx->stream, g, pos,
+                           ctx->d_porig, ctx->d_grid, (int64_t)ctx->M, N, ps.quadW * ps.prefac, ctx->d_wave);
+        FSD_HIP(hipGetLastError());
+        prof_end(ctx, PROF_CONTRACT, ev);
+        wave = ctx->d_wave;
+    }
+    RealParams rp;
+    rp.b = ctx->box; rp.rcut = ps.rcut; rp.dr = 0.001f; rp.self_x = ps.self_x; rp.self_y = ps.self_y;
+    rp.ewald_n = ps.ewald_n;
+    hipEvent_t ev;
+    prof_begin(ctx, PROF_REAL, &ev);
+    hipLaunchKernelGGL(realspace_kernel<16>, dim3(nblk((int64_t)N * 16, 256)), dim3(256), 0, ctx->stream, rp, pos,
+                       rec, reinterpret_cast<const float4 *>(ctx->d_ewald), ctx->d_nneigh, ctx->d_nlist, ctx->maxn, N,
+                       wave, addU, (parts & 1) ? 1 : 0, gU);
+    FSD_HIP(hipGetLastError());
+    prof_end(ctx, PROF_REAL, ev);
+    return 0;
+}

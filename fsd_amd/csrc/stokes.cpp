@@ -1,0 +1,987 @@
+// Host side of the MI355X FSD step: context set-up (Stokes::setParams,
+// setResistanceTable, setSparseMath, AllocateWorkSpaces), the preconditioner
+// build around the host RCM, the GMRES driver (FP32 Arnoldi on the device,
+// FP64 Hessenberg/Givens on the host) and the step sequencer
+// (Stokes::integrateStepOne -> Stokes_StepOne), exported through fsd.h.
+#include "fsd_internal.h"
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+extern "C" {
+int fsd_ewald_table_size(float ewald_cut, float ewald_dr);
+void fsd_ewald_table_fill(float xi, int nR, float *tab);
+void fsd_ewald_self(float xi, float self_out[2]);
+}
+
+// ------------------------------------------------------------ profiling
+static const char *kProfNames[PROF_COUNT] = {"spread", "green", "contract", "fft", "realspace", "rfu",
+                                             "trsv", "ichol", "gmres_vec", "nlist", "precond_build"};
+
+void prof_begin(fsd_ctx *ctx, int slot, hipEvent_t *ev)
+{
+    *ev = nullptr;
+    if (!ctx->prof) return;
+    hipEventCreate(ev);
+    hipEventRecord(*ev, ctx->stream);
+}
+
+void prof_end(fsd_ctx *ctx, int slot, hipEvent_t ev)
+{
+    if (!ctx->prof || !ev) return;
+    hipEvent_t e2;
+    hipEventCreate(&e2);
+    hipEventRecord(e2, ctx->stream);
+    ctx->slots[slot].pending.push_back({ev, e2});
+}
+
+static void prof_collect(fsd_ctx *ctx)
+{
+    hipStreamSynchronize(ctx->stream);
+    for (auto &s : ctx->slots) {
+        for (auto &pr : s.pending) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, pr.first, pr.second);
+            s.ms += ms;
+            s.launches += 1;
+            hipEventDestroy(pr.first);
+            hipEventDestroy(pr.second);
+        }
+        s.pending.clear();
+    }
+}
+
+// ------------------------------------------------------------ parameters
+// Stokes::setParams (Stokes.cc:131-261) and the spreading constants of
+// gpu_stokes_Mwave_wrap (Mobility.cu:930-935).
+static int pse_params(fsd_ctx *ctx)
+{
+    const fsd_params &p = ctx->p;
+    fsd_pse_info &o = ctx->pse;
+    memset(&o, 0, sizeof(o));
+    const float xi = p.xi, error = p.error;
+    o.rcut = sqrtf(-logf(error)) / xi;
+    const int kmax = (int)(2.0 * sqrtf(-logf(error)) * xi) + 1;
+    if (o.rcut > p.Lx / 2.0 || o.rcut > p.Ly / 2.0 || o.rcut > p.Lz / 2.0) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "real space Ewald cutoff %g exceeds half the box; increase xi", o.rcut);
+        ctx->err = buf;
+        return FSD_E_PARAM;
+    }
+    const float L[3] = {p.Lx, p.Ly, p.Lz};
+    int n[3];
+    std::vector<int> good;
+    for (int p2 = 1; p2 <= 512; p2 *= 2)
+        for (int p3 = 1; p2 * p3 <= 512; p3 *= 3)
+            for (int p5 = 1; p2 * p3 * p5 <= 512; p5 *= 5)
+                if (p2 * p3 * p5 >= 8) good.push_back(p2 * p3 * p5);
+    std::sort(good.begin(), good.end());
+    for (int d = 0; d < 3; ++d) {
+        int m = (int)(kmax * L[d] / (2.0 * 3.1415926536) * 2.0) + 1;
+        auto it = std::lower_bound(good.begin(), good.end(), m);
+        if (it == good.end()) { ctx->err = "grid larger than 512 points per axis"; return FSD_E_PARAM; }
+        n[d] = *it;
+    }
+    o.Nx = n[0]; o.Ny = n[1]; o.Nz = n[2];
+    const float gamma = p.max_strain, gamma2 = gamma * gamma;
+    const float lambda = (float)(1.0 + gamma2 / 2.0 + gamma * sqrtf((float)(1.0 + gamma2 / 4.0)));
+    o.gridh_x = p.Lx / (float)o.Nx; o.gridh_y = p.Ly / (float)o.Ny; o.gridh_z = p.Lz / (float)o.Nz;
+    float gm = 1.0f;
+    while (erfcf(gm / sqrtf((float)(2.0 * lambda))) > error) gm = (float)(gm + 0.01);
+    o.gaussm = gm;
+    o.P = (int)(gm * gm / 3.1415926536) + 1;
+    const float w = (float)(o.P * o.gridh_x / 2.0);
+    const float xisq = xi * xi;
+    o.eta = (float)((2.0 * w / gm) * (2.0 * w / gm) * xisq);
+    if (o.P > std::min(o.Nx, std::min(o.Ny, o.Nz))) { ctx->err = "Gaussian support exceeds the grid"; return FSD_E_PARAM; }
+    o.ewald_n = (int)(o.rcut / 0.001f - 1.0f);
+    float self[2];
+    fsd_ewald_self(xi, self);
+    o.self_x = self[0]; o.self_y = self[1];
+    o.quadW = o.gridh_x * o.gridh_y * o.gridh_z;
+    o.prefac = (float)((2.0 * xisq / 3.1415926536 / o.eta) * sqrtf((float)(2.0 * xisq / 3.1415926536 / o.eta)));
+    o.expfac = (float)(2.0 * xisq / o.eta);
+    return 0;
+}
+
+static void set_box(fsd_ctx *ctx, float Lx, float Ly, float Lz, float xy, float xz, float yz)
+{
+    BoxF &b = ctx->box;
+    b.Lx = Lx; b.Ly = Ly; b.Lz = Lz; b.xy = xy; b.xz = xz; b.yz = yz;
+    b.iLx = 1.0f / Lx; b.iLy = 1.0f / Ly; b.iLz = 1.0f / Lz;
+}
+
+// cells along each axis for the neighbour list (nearest plane distances)
+static void cell_dims(fsd_ctx *ctx)
+{
+    const BoxF &b = ctx->box;
+    const double a1[3] = {b.Lx, 0, 0}, a2[3] = {b.xy * b.Ly, b.Ly, 0}, a3[3] = {b.xz * b.Lz, b.yz * b.Lz, b.Lz};
+    auto cross = [](const double *u, const double *v, double *o) {
+        o[0] = u[1] * v[2] - u[2] * v[1]; o[1] = u[2] * v[0] - u[0] * v[2]; o[2] = u[0] * v[1] - u[1] * v[0];
+    };
+    double c23[3], c31[3], c12[3];
+    cross(a2, a3, c23); cross(a3, a1, c31); cross(a1, a2, c12);
+    const double V = (double)b.Lx * b.Ly * b.Lz;
+    auto nrm = [](const double *u) { return std::sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]); };
+    const double d[3] = {V / nrm(c23), V / nrm(c31), V / nrm(c12)};
+    const double rl = ctx->pse.rcut + ctx->p.r_buff;
+    ctx->use_cells = true;
+    for (int a = 0; a < 3; ++a) {
+        ctx->ncell[a] = (int)std::floor(d[a] / rl);
+        if (ctx->ncell[a] < 3) ctx->use_cells = false;
+    }
+    ctx->ncells = ctx->use_cells ? ctx->ncell[0] * ctx->ncell[1] * ctx->ncell[2] : 0;
+}
+
+#define CK(call)                                                                                    \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) {                                                                     \
+            ctx->err = std::string(#call) + ": " + hipGetErrorString(e_);                           \
+            return FSD_E_HIP;                                                               \
+        }                                                                                           \
+    } while (0)
+
+#define RC(call)                                                                                    \
+    do {                                                                                            \
+        int r_ = (call);                                                                            \
+        if (r_ < 0) return r_;                                                                      \
+    } while (0)
+
+static int alloc_all(fsd_ctx *ctx)
+{
+    const int N = ctx->N;
+    const int64_t n17 = 17 * (int64_t)N;
+    // neighbour list capacity from the density (list radius rcut + r_buff)
+    const double rl = ctx->pse.rcut + ctx->p.r_buff;
+    const double rho = N / ((double)ctx->box.Lx * ctx->box.Ly * ctx->box.Lz);
+    int est = (int)(4.18879 * rl * rl * rl * rho * 1.6) + 32;
+    est = std::min(est, N);
+    ctx->maxn = ((est + 31) / 32) * 32;
+    CK(hipMalloc(&ctx->d_nneigh, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_nlist, sizeof(int) * (size_t)N * ctx->maxn));
+    if (ctx->use_cells) {
+        CK(hipMalloc(&ctx->d_cell_of, sizeof(int) * N));
+        CK(hipMalloc(&ctx->d_cell_count, sizeof(int) * ctx->ncells));
+        CK(hipMalloc(&ctx->d_cell_start, sizeof(int) * (ctx->ncells + 1)));
+        CK(hipMalloc(&ctx->d_cell_list, sizeof(int) * N));
+    }
+    CK(hipMalloc(&ctx->d_flags, sizeof(int) * 8));
+    CK(hipHostMalloc((void **)&ctx->h_flags, sizeof(int) * 8, hipHostMallocDefault));
+    // lubrication records
+    ctx->maxlub = 48;
+    CK(hipMalloc(&ctx->d_lub_n, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_lub_j, sizeof(int) * (size_t)N * ctx->maxlub));
+    CK(hipMalloc(&ctx->d_lub_a, sizeof(float4) * (size_t)N * ctx->maxlub));
+    CK(hipMalloc(&ctx->d_lub_b, sizeof(float4) * (size_t)N * ctx->maxlub));
+    CK(hipMalloc(&ctx->d_lub_self, sizeof(float) * 36 * (size_t)N));
+    // grids and FFT
+    const fsd_pse_info &ps = ctx->pse;
+    ctx->M = ps.Nx * ps.Ny * ps.Nz;
+    ctx->Mh = ps.Nx * ps.Ny * (ps.Nz / 2 + 1);
+    CK(hipMalloc(&ctx->d_grid, sizeof(float) * 11 * (size_t)ctx->M));
+    CK(hipMalloc(&ctx->d_spec, sizeof(float2) * 11 * (size_t)ctx->Mh));
+    CK(hipMalloc(&ctx->d_gridk, sizeof(float4) * (size_t)ctx->Mh));
+    ctx->tile = 8;
+    ctx->tiles[0] = (ps.Nx + 7) / 8; ctx->tiles[1] = (ps.Ny + 7) / 8; ctx->tiles[2] = (ps.Nz + 7) / 8;
+    ctx->ntiles = ctx->tiles[0] * ctx->tiles[1] * ctx->tiles[2];
+    CK(hipMalloc(&ctx->d_porig, sizeof(int) * 3 * N));
+    CK(hipMalloc(&ctx->d_tile_of, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_tile_list, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_tile_count, sizeof(int) * ctx->ntiles));
+    CK(hipMalloc(&ctx->d_tile_start, sizeof(int) * (ctx->ntiles + 1)));
+    CK(hipMalloc(&ctx->d_wave, sizeof(float) * 11 * (size_t)N));
+    CK(hipMalloc(&ctx->d_rec, sizeof(float4) * 3 * (size_t)N));
+    {
+        size_t lengths[3] = {(size_t)ps.Nz, (size_t)ps.Ny, (size_t)ps.Nx};
+        if (rocfft_plan_create(&ctx->plan_fwd, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                               rocfft_precision_single, 3, lengths, 11, nullptr) != rocfft_status_success ||
+            rocfft_plan_create(&ctx->plan_inv, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                               rocfft_precision_single, 3, lengths, 11, nullptr) != rocfft_status_success) {
+            ctx->err = "rocfft_plan_create failed";
+            return FSD_E_HIP;
+        }
+        size_t w1 = 0, w2 = 0;
+        rocfft_plan_get_work_buffer_size(ctx->plan_fwd, &w1);
+        rocfft_plan_get_work_buffer_size(ctx->plan_inv, &w2);
+        ctx->fftwork_size = std::max(w1, w2);
+        if (ctx->fftwork_size) CK(hipMalloc(&ctx->d_fftwork, ctx->fftwork_size));
+        rocfft_execution_info_create(&ctx->info_fwd);
+        rocfft_execution_info_create(&ctx->info_inv);
+        rocfft_execution_info_set_stream(ctx->info_fwd, ctx->stream);
+        rocfft_execution_info_set_stream(ctx->info_inv, ctx->stream);
+        if (ctx->fftwork_size) {
+            rocfft_execution_info_set_work_buffer(ctx->info_fwd, ctx->d_fftwork, ctx->fftwork_size);
+            rocfft_execution_info_set_work_buffer(ctx->info_inv, ctx->d_fftwork, ctx->fftwork_size);
+        }
+    }
+    // Ewald table
+    ctx->ewald_nR = ps.ewald_n + 1;
+    ctx->h_ewald.resize(8 * (size_t)ctx->ewald_nR);
+    fsd_ewald_table_fill(ctx->p.xi, ctx->ewald_nR, ctx->h_ewald.data());
+    CK(hipMalloc(&ctx->d_ewald, sizeof(float) * ctx->h_ewald.size()));
+    CK(hipMemcpy(ctx->d_ewald, ctx->h_ewald.data(), sizeof(float) * ctx->h_ewald.size(), hipMemcpyHostToDevice));
+    CK(hipMalloc(&ctx->d_res_dist, sizeof(float) * 1000));
+    CK(hipMalloc(&ctx->d_res_vals, sizeof(float) * 22000));
+    // preconditioner
+    CK(hipMalloc(&ctx->d_pn, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_plist, sizeof(int) * FSD_PSTRIDE * (size_t)N));
+    CK(hipMalloc(&ctx->d_pless, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_perm, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_iperm, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_lrow, sizeof(int) * (N + 1)));
+    CK(hipMalloc(&ctx->d_urow, sizeof(int) * (N + 1)));
+    ctx->nlow_cap = 4 * (int64_t)N + 64;
+    CK(hipMalloc(&ctx->d_lcol, sizeof(int) * ctx->nlow_cap));
+    CK(hipMalloc(&ctx->d_ucol, sizeof(int) * ctx->nlow_cap));
+    CK(hipMalloc(&ctx->d_upos, sizeof(int) * ctx->nlow_cap));
+    CK(hipMalloc(&ctx->d_Ablk, sizeof(float) * 36 * ctx->nlow_cap));
+    CK(hipMalloc(&ctx->d_Lblk, sizeof(float) * 36 * ctx->nlow_cap));
+    CK(hipMalloc(&ctx->d_Adiag, sizeof(float) * 36 * (size_t)N));
+    CK(hipMalloc(&ctx->d_Ldiag, sizeof(float) * 36 * (size_t)N));
+    CK(hipMalloc(&ctx->d_lev_f, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_lev_b, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_tmp6a, sizeof(float) * 12 * (size_t)N));
+    CK(hipMalloc(&ctx->d_tmp6b, sizeof(float) * 12 * (size_t)N));
+    CK(hipMalloc(&ctx->d_g12, sizeof(float) * 12 * (size_t)N));
+    // GMRES
+    const int R = ctx->p.gmres_restart;
+    CK(hipMalloc(&ctx->d_V, sizeof(float) * n17 * (R + 1)));
+    CK(hipMalloc(&ctx->d_w, sizeof(float) * n17));
+    CK(hipMalloc(&ctx->d_t, sizeof(float) * n17));
+    CK(hipMalloc(&ctx->d_t2, sizeof(float) * n17));
+    ctx->part_blocks = (int)((n17 + 4095) / 4096);
+    CK(hipMalloc(&ctx->d_part, sizeof(double) * ctx->part_blocks * 64 + sizeof(double) * 256));
+    CK(hipHostMalloc((void **)&ctx->h_part, sizeof(double) * 256, hipHostMallocDefault));
+    CK(hipMalloc(&ctx->d_rhs, sizeof(float) * n17));
+    CK(hipMalloc(&ctx->d_sol, sizeof(float) * n17));
+    CK(hipMemset(ctx->d_sol, 0, sizeof(float) * n17));
+    CK(hipMalloc(&ctx->d_force, sizeof(float) * 6 * (size_t)N));
+    // squirmer masks (Stokes.cc:1318-1346, index == tag)
+    std::vector<float> m1(N), m2(N);
+    for (int i = 0; i < N; ++i) {
+        const bool first = (uint32_t)i < ctx->p.N_mix;
+        m1[i] = first ? 1.0f : ctx->p.coef_B1_mask;
+        m2[i] = first ? 1.0f : ctx->p.coef_B2_mask;
+    }
+    CK(hipMalloc(&ctx->d_mask1, sizeof(float) * N));
+    CK(hipMalloc(&ctx->d_mask2, sizeof(float) * N));
+    CK(hipMemcpy(ctx->d_mask1, m1.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    CK(hipMemcpy(ctx->d_mask2, m2.data(), sizeof(float) * N, hipMemcpyHostToDevice));
+    return 0;
+}
+
+static void free_all(fsd_ctx *ctx)
+{
+    void *ptrs[] = {ctx->d_nneigh, ctx->d_nlist, ctx->d_cell_of, ctx->d_cell_count, ctx->d_cell_start,
+                    ctx->d_cell_list, ctx->d_flags, ctx->d_lub_n, ctx->d_lub_j, ctx->d_lub_a, ctx->d_lub_b,
+                    ctx->d_lub_self, ctx->d_grid, ctx->d_spec, ctx->d_gridk, ctx->d_fftwork, ctx->d_porig,
+                    ctx->d_tile_of, ctx->d_tile_list, ctx->d_tile_count, ctx->d_tile_start, ctx->d_wave, ctx->d_rec,
+                    ctx->d_ewald, ctx->d_res_dist, ctx->d_res_vals, ctx->d_pn, ctx->d_plist, ctx->d_pless,
+                    ctx->d_perm, ctx->d_iperm, ctx->d_lrow, ctx->d_urow, ctx->d_lcol, ctx->d_ucol, ctx->d_upos,
+                    ctx->d_Ablk, ctx->d_Lblk, ctx->d_Adiag, ctx->d_Ldiag, ctx->d_lev_f, ctx->d_lev_b, ctx->d_tmp6a,
+                    ctx->d_tmp6b, ctx->d_g12, ctx->d_V, ctx->d_w, ctx->d_t, ctx->d_t2, ctx->d_part, ctx->d_rhs, ctx->d_sol,
+                    ctx->d_force, ctx->d_mask1, ctx->d_mask2};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (ctx->h_flags) hipHostFree(ctx->h_flags);
+    if (ctx->h_part) hipHostFree(ctx->h_part);
+    if (ctx->plan_fwd) rocfft_plan_destroy(ctx->plan_fwd);
+    if (ctx->plan_inv) rocfft_plan_destroy(ctx->plan_inv);
+    if (ctx->info_fwd) rocfft_execution_info_destroy(ctx->info_fwd);
+    if (ctx->info_inv) rocfft_execution_info_destroy(ctx->info_inv);
+    for (auto &s : ctx->slots)
+        for (auto &pr : s.pending) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
+}
+
+// ------------------------------------------------------------ preconditioner
+static int grow_lower(fsd_ctx *ctx, int64_t need)
+{
+    if (need <= ctx->nlow_cap) return 0;
+    const int64_t cap = need + need / 4 + 64;
+    hipFree(ctx->d_lcol); hipFree(ctx->d_ucol); hipFree(ctx->d_upos); hipFree(ctx->d_Ablk); hipFree(ctx->d_Lblk);
+    CK(hipMalloc(&ctx->d_lcol, sizeof(int) * cap));
+    CK(hipMalloc(&ctx->d_ucol, sizeof(int) * cap));
+    CK(hipMalloc(&ctx->d_upos, sizeof(int) * cap));
+    CK(hipMalloc(&ctx->d_Ablk, sizeof(float) * 36 * cap));
+    CK(hipMalloc(&ctx->d_Lblk, sizeof(float) * 36 * cap));
+    ctx->nlow_cap = cap;
+    return 0;
+}
+
+// Precondition_Wrap (Precondition.cu:1326-1497)
+static int build_precond(fsd_ctx *ctx, const float4 *pos)
+{
+    const int N = ctx->N;
+    if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    if (!ctx->have_table) { ctx->err = "resistance table not set"; return FSD_E_STATE; }
+    hipEvent_t ev;
+    prof_begin(ctx, PROF_PRECOND_BUILD, &ev);
+    RC(k_prune(ctx, pos));
+    ctx->h_pn.resize(N);
+    std::vector<int> pl_stride((size_t)N * FSD_PSTRIDE);
+    CK(hipMemcpyAsync(ctx->h_pn.data(), ctx->d_pn, sizeof(int) * N, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipMemcpyAsync(pl_stride.data(), ctx->d_plist, sizeof(int) * pl_stride.size(), hipMemcpyDeviceToHost,
+                      ctx->stream));
+    CK(hipMemcpyAsync(ctx->h_flags + 2, ctx->d_flags + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    if (ctx->h_flags[2]) { ctx->err = "more than 15 neighbours inside the preconditioner cutoff"; return FSD_E_CAPACITY; }
+    // packed adjacency (self loops included), as Precondition_Reorder passes to genrcm
+    std::vector<int> phead(N + 1);
+    phead[0] = 0;
+    for (int i = 0; i < N; ++i) phead[i + 1] = phead[i] + ctx->h_pn[i];
+    ctx->h_plist.resize(phead[N]);
+    for (int i = 0; i < N; ++i)
+        for (int s = 0; s < ctx->h_pn[i]; ++s) ctx->h_plist[phead[i] + s] = pl_stride[(size_t)i * FSD_PSTRIDE + s];
+    ctx->h_perm.resize(N);
+    host_rcm(N, phead.data(), ctx->h_plist.data(), ctx->h_perm.data());
+    std::vector<int> iperm(N);
+    for (int P = 0; P < N; ++P) iperm[ctx->h_perm[P]] = P;
+    // lower and upper block adjacency on the permuted graph
+    std::vector<int> lrow(N + 1), lcol, urow(N + 1, 0), ucol, upos;
+    lcol.reserve(2 * (size_t)N);
+    std::vector<int> tmp;
+    int64_t nnz_ref = 0;
+    for (int P = 0; P < N; ++P) {
+        const int i = ctx->h_perm[P];
+        lrow[P] = (int)lcol.size();
+        tmp.clear();
+        for (int s = phead[i]; s < phead[i + 1]; ++s) {
+            const int Q = iperm[ctx->h_plist[s]];
+            if (Q < P) tmp.push_back(Q);
+        }
+        std::sort(tmp.begin(), tmp.end());
+        lcol.insert(lcol.end(), tmp.begin(), tmp.end());
+        for (int Q : tmp) urow[Q + 1]++;
+        nnz_ref += (ctx->h_pn[i] > 1) ? 36 * (int64_t)ctx->h_pn[i] : 6;
+    }
+    lrow[N] = (int)lcol.size();
+    for (int P = 0; P < N; ++P) urow[P + 1] += urow[P];
+    ucol.resize(lcol.size());
+    upos.resize(lcol.size());
+    {
+        std::vector<int> fill(urow.begin(), urow.end() - 1);
+        for (int P = 0; P < N; ++P)
+            for (int e = lrow[P]; e < lrow[P + 1]; ++e) {
+                const int Q = lcol[e];          // block (P, Q) with Q < P -> upper entry of row Q
+                ucol[fill[Q]] = P;
+                upos[fill[Q]] = e;
+                fill[Q]++;
+            }
+    }
+    ctx->nlow = (int64_t)lcol.size();
+    ctx->nnz_ref = nnz_ref;
+    // level schedules: forward level = 1 + max level of the lower neighbours,
+    // backward level = 1 + max level of the upper neighbours (counted from N-1)
+    {
+        std::vector<int> lf(N, 0), lb(N, 0);
+        int mf = 0, mb = 0;
+        for (int P = 0; P < N; ++P) {
+            int l = 0;
+            for (int e = lrow[P]; e < lrow[P + 1]; ++e) l = std::max(l, lf[lcol[e]] + 1);
+            lf[P] = l;
+            mf = std::max(mf, l);
+        }
+        for (int P = N - 1; P >= 0; --P) {
+            int l = 0;
+            for (int e = urow[P]; e < urow[P + 1]; ++e) l = std::max(l, lb[ucol[e]] + 1);
+            lb[P] = l;
+            mb = std::max(mb, l);
+        }
+        auto bucket = [N](const std::vector<int> &lev, int nl, std::vector<int> &ptr, std::vector<int> &items) {
+            ptr.assign(nl + 1, 0);
+            for (int P = 0; P < N; ++P) ptr[lev[P] + 1]++;
+            for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+            items.resize(N);
+            std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+            for (int P = 0; P < N; ++P) items[fill[lev[P]]++] = P;
+        };
+        std::vector<int> itf, itb;
+        ctx->nlev_f = mf + 1;
+        ctx->nlev_b = mb + 1;
+        bucket(lf, ctx->nlev_f, ctx->h_lev_f_ptr, itf);
+        bucket(lb, ctx->nlev_b, ctx->h_lev_b_ptr, itb);
+        CK(hipMemcpyAsync(ctx->d_lev_f, itf.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+        CK(hipMemcpyAsync(ctx->d_lev_b, itb.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+    }
+    RC(grow_lower(ctx, ctx->nlow + 1));
+    CK(hipMemcpyAsync(ctx->d_perm, ctx->h_perm.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(ctx->d_iperm, iperm.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(ctx->d_lrow, lrow.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
+    CK(hipMemcpyAsync(ctx->d_urow, urow.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
+    if (ctx->nlow) {
+        CK(hipMemcpyAsync(ctx->d_lcol, lcol.data(), sizeof(int) * ctx->nlow, hipMemcpyHostToDevice, ctx->stream));
+        CK(hipMemcpyAsync(ctx->d_ucol, ucol.data(), sizeof(int) * ctx->nlow, hipMemcpyHostToDevice, ctx->stream));
+        CK(hipMemcpyAsync(ctx->d_upos, upos.data(), sizeof(int) * ctx->nlow, hipMemcpyHostToDevice, ctx->stream));
+    }
+    RC(k_build_blocks(ctx, pos));
+    // IC(0) with the relaxer doubling on a non-positive pivot (Precondition.cu:1274-1288, 1455-1489)
+    ctx->ichol_attempts = 0;
+    float lambda = ctx->ichol_relaxer;
+    for (;;) {
+        int failed = 0;
+        RC(k_ichol(ctx, lambda, &failed));
+        ctx->ichol_attempts++;
+        if (!failed) break;
+        lambda *= 2.0f;
+        if (lambda > 1.0e7f) { ctx->err = "IC(0) did not succeed for any relaxer"; return FSD_E_ICHOL; }
+    }
+    ctx->ichol_relaxer = lambda;
+    prof_end(ctx, PROF_PRECOND_BUILD, ev);
+    ctx->precond_valid = true;
+    return 0;
+}
+
+// ------------------------------------------------------------ GMRES
+// Solvers_Saddle with CUSP's gmres semantics (Solvers.cu:55-126): left
+// preconditioned restarted GMRES, converged when the preconditioned residual
+// estimate |s_{i+1}| <= tol ||b||.  Arnoldi vectors in FP32 on the device,
+// classical Gram-Schmidt with one reorthogonalisation and FP64 dot products;
+// Hessenberg, Givens rotations and the small triangular solve in FP64 on the host.
+static int gmres(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
+                 fsd_solve_stats *st)
+{
+    const int64_t n = 17 * (int64_t)ctx->N;
+    float *V = ctx->d_V, *w = ctx->d_w, *t = ctx->d_t;
+    double *d_h = ctx->d_part + (int64_t)ctx->part_blocks * 64;   // 256 doubles of scratch after the partials
+    std::vector<double> H((size_t)(R + 1) * R, 0.0), cs(R), sn(R), g(R + 1), yv(R + 1);
+    auto hostsync = [&](int count) -> int {
+        CK(hipMemcpyAsync(ctx->h_part, d_h, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    };
+    RC(k_nrm2(ctx, n, b, d_h));
+    RC(hostsync(1));
+    const double bn = std::sqrt(ctx->h_part[0]);
+    const double target = (double)tol * bn;
+    int iters = 0, restarts = 0;
+    double resid = 0.0;
+    for (;;) {
+        RC(k_saddle_multiply(ctx, pos, x, t));
+        RC(k_axpby(ctx, n, 1.0f, t, -1.0f, b, t));
+        RC(k_saddle_precond(ctx, t, w));
+        RC(k_nrm2(ctx, n, w, d_h));
+        RC(hostsync(1));
+        const double beta = std::sqrt(ctx->h_part[0]);
+        resid = beta;
+        if (resid <= target || iters >= maxit || beta == 0.0) break;
+        RC(k_axpby(ctx, n, (float)(-1.0 / beta), w, 0.0f, nullptr, V));
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int i = -1;
+        do {
+            ++i;
+            ++iters;
+            float *vi = V + (int64_t)i * n, *vn = V + (int64_t)(i + 1) * n;
+            RC(k_saddle_multiply(ctx, pos, vi, t));
+            RC(k_saddle_precond(ctx, t, w));
+            hipEvent_t ev;
+            prof_begin(ctx, PROF_GMRES_VEC, &ev);
+            // CGS2: two passes of h = V^T w, w -= V h
+            RC(k_multidot(ctx, n, i + 1, V, n, w, d_h));
+            RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h, w));
+            RC(k_multidot(ctx, n, i + 1, V, n, w, d_h + 64));
+            RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h + 64, w));
+            RC(k_nrm2(ctx, n, w, d_h + 128));
+            prof_end(ctx, PROF_GMRES_VEC, ev);
+            CK(hipMemcpyAsync(ctx->h_part, d_h, sizeof(double) * 129, hipMemcpyDeviceToHost, ctx->stream));
+            CK(hipStreamSynchronize(ctx->stream));
+            const double hn = std::sqrt(ctx->h_part[128]);
+            for (int k = 0; k <= i; ++k) H[(size_t)k * R + i] = ctx->h_part[k] + ctx->h_part[64 + k];
+            H[(size_t)(i + 1) * R + i] = hn;
+            RC(k_axpby(ctx, n, hn > 0.0 ? (float)(1.0 / hn) : 0.0f, w, 0.0f, nullptr, vn));
+            for (int k = 0; k < i; ++k) {
+                const double a = H[(size_t)k * R + i], c = H[(size_t)(k + 1) * R + i];
+                H[(size_t)k * R + i] = cs[k] * a + sn[k] * c;
+                H[(size_t)(k + 1) * R + i] = -sn[k] * a + cs[k] * c;
+            }
+            const double a = H[(size_t)i * R + i], c = H[(size_t)(i + 1) * R + i];
+            if (c == 0.0) { cs[i] = 1.0; sn[i] = 0.0; }
+            else if (std::fabs(c) > std::fabs(a)) { double tt = a / c; sn[i] = 1.0 / std::sqrt(1.0 + tt * tt); cs[i] = tt * sn[i]; }
+            else { double tt = c / a; cs[i] = 1.0 / std::sqrt(1.0 + tt * tt); sn[i] = tt * cs[i]; }
+            H[(size_t)i * R + i] = cs[i] * a + sn[i] * c;
+            H[(size_t)(i + 1) * R + i] = 0.0;
+            g[i + 1] = -sn[i] * g[i];
+            g[i] = cs[i] * g[i];
+            resid = std::fabs(g[i + 1]);
+        } while (!(resid <= target) && iters < maxit && i < R - 1);
+        // y = H^-1 g ; x += V y
+        for (int j = i; j >= 0; --j) {
+            double s = g[j];
+            for (int k = j + 1; k <= i; ++k) s -= H[(size_t)j * R + k] * yv[k];
+            yv[j] = s / H[(size_t)j * R + j];
+        }
+        for (int j = 0; j <= i; ++j) ctx->h_part[j] = -yv[j];
+        CK(hipMemcpyAsync(d_h, ctx->h_part, sizeof(double) * (i + 1), hipMemcpyHostToDevice, ctx->stream));
+        RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h, x));
+        CK(hipStreamSynchronize(ctx->stream));
+        ++restarts;
+        if (resid <= target || iters >= maxit) break;
+    }
+    if (st) {
+        st->iterations = iters;
+        st->restarts = restarts;
+        st->residual = (float)(resid / (bn > 0 ? bn : 1.0));
+    }
+    return (resid <= target || bn == 0.0) ? 0 : FSD_W_NOT_CONVERGED;
+}
+
+static int true_residual(fsd_ctx *ctx, const float4 *pos, const float *b, const float *x, float *out)
+{
+    const int64_t n = 17 * (int64_t)ctx->N;
+    double *d_h = ctx->d_part + (int64_t)ctx->part_blocks * 64;
+    RC(k_saddle_multiply(ctx, pos, x, ctx->d_t));
+    RC(k_axpby(ctx, n, 1.0f, b, -1.0f, ctx->d_t, ctx->d_t));
+    RC(k_nrm2(ctx, n, ctx->d_t, d_h));
+    RC(k_nrm2(ctx, n, b, d_h + 1));
+    CK(hipMemcpyAsync(ctx->h_part, d_h, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    *out = (float)(std::sqrt(ctx->h_part[0]) / std::max(std::sqrt(ctx->h_part[1]), 1e-300));
+    return 0;
+}
+
+// ------------------------------------------------------------ C ABI
+extern "C" {
+
+int fsd_create(const fsd_params *p, fsd_ctx **out)
+{
+    *out = nullptr;
+    fsd_ctx *ctx = new fsd_ctx();
+    ctx->p = *p;
+    fsd_params &q = ctx->p;
+    if (q.gmres_tol <= 0.f) q.gmres_tol = q.error;
+    if (q.gmres_restart <= 0) q.gmres_restart = 50;
+    if (q.gmres_restart > 63) q.gmres_restart = 63;
+    if (q.gmres_maxiter <= 0) q.gmres_maxiter = 1000;
+    if (q.rlub <= 0.f) q.rlub = 4.0f;
+    if (q.rp <= 0.f) q.rp = 2.1f;
+    if (q.r_buff < 0.f) q.r_buff = 0.4f;
+    if (q.N <= 0 || q.xi <= 0.f || q.error <= 0.f || q.error >= 1.f || q.Lx <= 0.f || q.Ly <= 0.f || q.Lz <= 0.f) {
+        ctx->err = "invalid fsd_params";
+        *out = ctx;
+        return FSD_E_PARAM;
+    }
+    ctx->N = q.N;
+    set_box(ctx, q.Lx, q.Ly, q.Lz, q.xy, q.xz, q.yz);
+    int rc = pse_params(ctx);
+    if (rc) { *out = ctx; return rc; }
+    cell_dims(ctx);
+    static bool fft_ready = false;
+    if (!fft_ready) { rocfft_setup(); fft_ready = true; }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        ctx->err = "hipStreamCreate failed";
+        *out = ctx;
+        return FSD_E_HIP;
+    }
+    ctx->slots.resize(PROF_COUNT);
+    for (int s = 0; s < PROF_COUNT; ++s) { ctx->slots[s].name = kProfNames[s]; ctx->slots[s].launches = 0; ctx->slots[s].ms = 0; }
+    rc = alloc_all(ctx);
+    *out = ctx;
+    if (rc) return rc;
+    return k_gridk(ctx) ? FSD_E_HIP : 0;
+}
+
+void fsd_destroy(fsd_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *fsd_last_error(const fsd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int fsd_get_pse_info(const fsd_ctx *ctx, fsd_pse_info *out)
+{
+    *out = ctx->pse;
+    return 0;
+}
+
+int fsd_get_ewald_table(const fsd_ctx *ctx, float *h_tab, int32_t capacity)
+{
+    if ((size_t)capacity < ctx->h_ewald.size()) return FSD_E_CAPACITY;
+    memcpy(h_tab, ctx->h_ewald.data(), sizeof(float) * ctx->h_ewald.size());
+    return 0;
+}
+
+int fsd_set_resistance_table(fsd_ctx *ctx, const float *h_dist, const float *h_vals)
+{
+    CK(hipMemcpy(ctx->d_res_dist, h_dist, sizeof(float) * 1000, hipMemcpyHostToDevice));
+    CK(hipMemcpy(ctx->d_res_vals, h_vals, sizeof(float) * 22000, hipMemcpyHostToDevice));
+    ctx->have_table = true;
+    ctx->lub_valid = false;
+    ctx->precond_valid = false;
+    return 0;
+}
+
+int fsd_set_friction(fsd_ctx *ctx, const char *type, float h0, const float *h_alpha6)
+{
+    (void)h0; (void)h_alpha6;
+    std::string t = type ? type : "NONE";
+    for (auto &c : t) c = (char)toupper(c);
+    if (t == "NONE") return 0;
+    ctx->err = "friction type " + t + " is not supported";
+    return FSD_E_PARAM;
+}
+
+int fsd_set_temperature(fsd_ctx *ctx, float T)
+{
+    ctx->p.T = T;
+    return 0;
+}
+
+int fsd_set_box(fsd_ctx *ctx, float Lx, float Ly, float Lz, float xy, float xz, float yz)
+{
+    if (Lx != ctx->box.Lx || Ly != ctx->box.Ly || Lz != ctx->box.Lz) {
+        ctx->err = "box lengths are fixed at creation; only the tilt may change";
+        return FSD_E_PARAM;
+    }
+    set_box(ctx, Lx, Ly, Lz, xy, xz, yz);
+    ctx->nlist_valid = false;
+    ctx->spread_valid = false;
+    return k_gridk(ctx) ? FSD_E_HIP : 0;
+}
+
+int fsd_build_nlist(fsd_ctx *ctx, const float *d_pos4)
+{
+    int rc = k_build_nlist(ctx, reinterpret_cast<const float4 *>(d_pos4));
+    if (rc) return rc;
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_get_nlist(fsd_ctx *ctx, int32_t *h_nneigh, int32_t *h_nbrs, int64_t capacity, int64_t *total)
+{
+    if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    const int N = ctx->N;
+    std::vector<int> nn(N), lst((size_t)N * ctx->maxn);
+    CK(hipMemcpy(nn.data(), ctx->d_nneigh, sizeof(int) * N, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(lst.data(), ctx->d_nlist, sizeof(int) * lst.size(), hipMemcpyDeviceToHost));
+    int64_t tot = 0;
+    for (int i = 0; i < N; ++i) {
+        std::vector<int> v(lst.begin() + (size_t)i * ctx->maxn, lst.begin() + (size_t)i * ctx->maxn + nn[i]);
+        std::sort(v.begin(), v.end());
+        if (h_nneigh) h_nneigh[i] = nn[i];
+        for (int j : v) {
+            if (h_nbrs && tot < capacity) h_nbrs[tot] = j;
+            ++tot;
+        }
+    }
+    *total = tot;
+    return tot > capacity && h_nbrs ? FSD_E_CAPACITY : 0;
+}
+
+static int ensure_lub(fsd_ctx *ctx, const float4 *pos)
+{
+    if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    if (!ctx->lub_valid) RC(k_build_lub_pairs(ctx, pos));
+    return 0;
+}
+
+int fsd_rfu(fsd_ctx *ctx, const float *d_pos4, const float *d_U6, float *d_F6)
+{
+    const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
+    ctx->lub_valid = false;
+    RC(ensure_lub(ctx, pos));
+    RC(k_rfu_apply(ctx, d_U6, d_F6, nullptr, 1.0f));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_rfe(fsd_ctx *ctx, const float *d_pos4, float shear_rate, const float *d_ori3, float *d_F6)
+{
+    if (!ctx->nlist_valid || !ctx->have_table) { ctx->err = "neighbour list or table missing"; return FSD_E_STATE; }
+    RC(k_rfe(ctx, reinterpret_cast<const float4 *>(d_pos4), shear_rate, d_ori3, d_F6));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_rsu(fsd_ctx *ctx, const float *d_pos4, const float *d_U6, float *d_S5)
+{
+    if (!ctx->nlist_valid || !ctx->have_table) { ctx->err = "neighbour list or table missing"; return FSD_E_STATE; }
+    RC(k_rsu(ctx, reinterpret_cast<const float4 *>(d_pos4), d_U6, d_S5));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_rse(fsd_ctx *ctx, const float *d_pos4, float shear_rate, const float *d_ori3, float *d_S5)
+{
+    if (!ctx->nlist_valid || !ctx->have_table) { ctx->err = "neighbour list or table missing"; return FSD_E_STATE; }
+    RC(k_rse(ctx, reinterpret_cast<const float4 *>(d_pos4), shear_rate, d_ori3, d_S5));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_mobility(fsd_ctx *ctx, const float *d_pos4, const float *d_gF11, float *d_gU11, int32_t parts)
+{
+    if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    ctx->spread_valid = false;
+    RC(k_mobility(ctx, reinterpret_cast<const float4 *>(d_pos4), d_gF11, d_gU11, parts, nullptr));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_build_precond(fsd_ctx *ctx, const float *d_pos4, fsd_solve_stats *stats)
+{
+    RC(build_precond(ctx, reinterpret_cast<const float4 *>(d_pos4)));
+    if (stats) {
+        stats->ichol_relaxer = ctx->ichol_relaxer;
+        stats->ichol_attempts = ctx->ichol_attempts;
+        stats->nnz = ctx->nnz_ref;
+    }
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_precond_apply(fsd_ctx *ctx, const float *d_x6, float *d_y6)
+{
+    if (!ctx->precond_valid) { ctx->err = "preconditioner not built"; return FSD_E_STATE; }
+    RC(k_precond_apply2(ctx, d_x6, d_x6, d_y6, d_y6, 1));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// Reference-format export (pattern of Precondition_RFU_kernel + coo2csr and of
+// cusolverSpXcsrpermHost; values of the factor on the permuted pattern).
+int fsd_get_precond(fsd_ctx *ctx, int32_t *h_pn, int32_t *h_plist, int64_t plist_cap, int32_t *h_perm,
+                    int32_t *h_rowptr, int32_t *h_col, float *h_val, int32_t *h_prow, int32_t *h_pcol, float *h_L,
+                    int64_t nnz_cap, int64_t *nnz_out)
+{
+    if (!ctx->precond_valid) { ctx->err = "preconditioner not built"; return FSD_E_STATE; }
+    const int N = ctx->N;
+    const std::vector<int> &pn = ctx->h_pn, &perm = ctx->h_perm;
+    std::vector<int> phead(N + 1, 0);
+    for (int i = 0; i < N; ++i) phead[i + 1] = phead[i] + pn[i];
+    if (h_pn) memcpy(h_pn, pn.data(), sizeof(int) * N);
+    if (h_plist) {
+        if ((int64_t)ctx->h_plist.size() > plist_cap) return FSD_E_CAPACITY;
+        memcpy(h_plist, ctx->h_plist.data(), sizeof(int) * ctx->h_plist.size());
+    }
+    if (h_perm) memcpy(h_perm, perm.data(), sizeof(int) * N);
+    if (nnz_out) *nnz_out = ctx->nnz_ref;
+    if (!h_rowptr && !h_col && !h_val && !h_prow && !h_pcol && !h_L) return 0;
+    if (ctx->nnz_ref > nnz_cap) return FSD_E_CAPACITY;
+    std::vector<int> iperm(N);
+    for (int P = 0; P < N; ++P) iperm[perm[P]] = P;
+    std::vector<int> lrow(N + 1), lcol(ctx->nlow);
+    std::vector<float> Ab(36 * (size_t)ctx->nlow), Ad(36 * (size_t)N), Lb(36 * (size_t)ctx->nlow), Ld(36 * (size_t)N);
+    CK(hipMemcpy(lrow.data(), ctx->d_lrow, sizeof(int) * (N + 1), hipMemcpyDeviceToHost));
+    if (ctx->nlow) {
+        CK(hipMemcpy(lcol.data(), ctx->d_lcol, sizeof(int) * ctx->nlow, hipMemcpyDeviceToHost));
+        CK(hipMemcpy(Ab.data(), ctx->d_Ablk, sizeof(float) * Ab.size(), hipMemcpyDeviceToHost));
+        CK(hipMemcpy(Lb.data(), ctx->d_Lblk, sizeof(float) * Lb.size(), hipMemcpyDeviceToHost));
+    }
+    CK(hipMemcpy(Ad.data(), ctx->d_Adiag, sizeof(float) * Ad.size(), hipMemcpyDeviceToHost));
+    CK(hipMemcpy(Ld.data(), ctx->d_Ldiag, sizeof(float) * Ld.size(), hipMemcpyDeviceToHost));
+    // value of block (row particle i, column particle j), 6x6 element (a, c), original ids
+    auto blockval = [&](const std::vector<float> &lowv, const std::vector<float> &diagv, int Pi, int Pj, int a,
+                        int c) -> float {
+        if (Pi == Pj) return diagv[(size_t)Pi * 36 + a * 6 + c];
+        const int R0 = std::max(Pi, Pj), C0 = std::min(Pi, Pj);
+        const int *b = lcol.data() + lrow[R0], *e = lcol.data() + lrow[R0 + 1];
+        const int pos = (int)(std::lower_bound(b, e, C0) - lcol.data());
+        const float *blk = lowv.data() + (size_t)pos * 36;
+        return (Pi > Pj) ? blk[a * 6 + c] : blk[c * 6 + a];
+    };
+    // unpermuted CSR in the reference's COO order (rows ascending, neighbours ascending)
+    if (h_rowptr || h_col || h_val) {
+        int64_t k = 0;
+        for (int i = 0; i < N; ++i) {
+            const bool iso = pn[i] <= 1;
+            for (int a = 0; a < 6; ++a) {
+                const int row = 6 * i + a;
+                if (h_rowptr) h_rowptr[row] = (int)k;
+                if (iso) {
+                    if (h_col) h_col[k] = row;
+                    if (h_val) h_val[k] = 0.0f;
+                    ++k;
+                    continue;
+                }
+                for (int s = phead[i]; s < phead[i + 1]; ++s) {
+                    const int j = ctx->h_plist[s];
+                    for (int c = 0; c < 6; ++c) {
+                        if (h_col) h_col[k] = 6 * j + c;
+                        if (h_val) h_val[k] = blockval(Ab, Ad, iperm[i], iperm[j], a, c);
+                        ++k;
+                    }
+                }
+            }
+        }
+        if (h_rowptr) h_rowptr[6 * N] = (int)k;
+    }
+    // permuted CSR (B = A(p,p), columns ascending) and the factor on that pattern
+    if (h_prow || h_pcol || h_L) {
+        int64_t k = 0;
+        std::vector<int> cols;
+        for (int P = 0; P < N; ++P) {
+            const int i = perm[P];
+            const bool iso = pn[i] <= 1;
+            cols.clear();
+            for (int s = phead[i]; s < phead[i + 1]; ++s) cols.push_back(iperm[ctx->h_plist[s]]);
+            std::sort(cols.begin(), cols.end());
+            for (int a = 0; a < 6; ++a) {
+                const int row = 6 * P + a;
+                if (h_prow) h_prow[row] = (int)k;
+                if (iso) {
+                    if (h_pcol) h_pcol[k] = row;
+                    if (h_L) h_L[k] = Ld[(size_t)P * 36 + a * 6 + a];
+                    ++k;
+                    continue;
+                }
+                for (int Q : cols)
+                    for (int c = 0; c < 6; ++c) {
+                        const int col = 6 * Q + c;
+                        if (h_pcol) h_pcol[k] = col;
+                        if (h_L) h_L[k] = (col > row) ? 0.0f : blockval(Lb, Ld, P, Q, a, c);
+                        ++k;
+                    }
+            }
+        }
+        if (h_prow) h_prow[6 * N] = (int)k;
+    }
+    return 0;
+}
+
+int fsd_saddle_multiply(fsd_ctx *ctx, const float *d_pos4, const float *d_x17, float *d_y17)
+{
+    const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
+    RC(ensure_lub(ctx, pos));
+    ctx->spread_valid = false;
+    RC(k_saddle_multiply(ctx, pos, d_x17, d_y17));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_saddle_precond(fsd_ctx *ctx, const float *d_b17, float *d_x17)
+{
+    if (!ctx->precond_valid) { ctx->err = "preconditioner not built"; return FSD_E_STATE; }
+    const int64_t n = 17 * (int64_t)ctx->N;
+    RC(k_copy(ctx, n, d_b17, ctx->d_t2));
+    RC(k_saddle_precond(ctx, ctx->d_t2, d_x17));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_saddle_solve(fsd_ctx *ctx, const float *d_pos4, const float *d_rhs17, float *d_sol17,
+                     int32_t want_true_residual, fsd_solve_stats *stats)
+{
+    const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
+    if (!ctx->precond_valid) { ctx->err = "preconditioner not built"; return FSD_E_STATE; }
+    RC(ensure_lub(ctx, pos));
+    ctx->spread_valid = false;
+    fsd_solve_stats st;
+    memset(&st, 0, sizeof st);
+    st.true_residual = -1.f;
+    int rc = gmres(ctx, pos, d_rhs17, d_sol17, ctx->p.gmres_tol, ctx->p.gmres_maxiter, ctx->p.gmres_restart, &st);
+    if (rc < 0) return rc;
+    if (want_true_residual) RC(true_residual(ctx, pos, d_rhs17, d_sol17, &st.true_residual));
+    st.ichol_relaxer = ctx->ichol_relaxer;
+    st.ichol_attempts = ctx->ichol_attempts;
+    st.nnz = ctx->nnz_ref;
+    if (stats) *stats = st;
+    CK(hipStreamSynchronize(ctx->stream));
+    return rc;
+}
+
+int fsd_force_law(fsd_ctx *ctx, const float *d_pos4, float T_ext, float *d_F6)
+{
+    if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    RC(k_force_law(ctx, reinterpret_cast<const float4 *>(d_pos4), T_ext, d_F6));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_euler_shear(fsd_ctx *ctx, float *d_pos4, float *d_ori3, float *d_pos_gb4, int32_t *d_image3,
+                    const float *d_vel6, const float *d_mask1, const float *d_noise3, float dt, float shear_rate)
+{
+    RC(k_euler_shear(ctx, reinterpret_cast<float4 *>(d_pos4), d_ori3, reinterpret_cast<float4 *>(d_pos_gb4),
+                     d_image3, d_vel6, d_mask1 ? d_mask1 : ctx->d_mask1, d_noise3, dt, shear_rate, ctx->p.sqm_B1));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// Stokes::integrateStepOne -> Stokes_StepOne (Stokes.cc:1197-1683, Stokes.cu:119-273)
+int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, float *d_ori3, float *d_pos_gb4,
+             int32_t *d_image3, const float *d_force6, const float *d_noise_ang3, float *d_vel11,
+             fsd_solve_stats *stats)
+{
+    const int N = ctx->N;
+    const int64_t n17 = 17 * (int64_t)N;
+    float4 *pos = reinterpret_cast<float4 *>(d_pos4);
+    if (!ctx->have_table) { ctx->err = "resistance table not set"; return FSD_E_STATE; }
+    if (ctx->p.T > 0.f) { ctx->err = "Brownian motion (T > 0) is not available in this build"; return FSD_E_PARAM; }
+    // neighbour list (HOOMD NeighborList::compute) and sheared k vectors
+    RC(k_build_nlist(ctx, pos));
+    RC(k_gridk(ctx));
+    // preconditioner (Precondition_Wrap)
+    RC(build_precond(ctx, pos));
+    // applied force: injected, or the reference force law with the external torque
+    if (d_force6) {
+        RC(k_copy(ctx, 6 * (int64_t)N, d_force6, ctx->d_force));
+    } else {
+        const float T_ext = (float)(ctx->p.T_ext * std::cos(ctx->p.omega_ext * (double)timestep * ctx->p.dt));
+        RC(k_force_law(ctx, pos, T_ext, ctx->d_force));
+    }
+    // right-hand side (Integrator_ComputeVelocity, Integrator.cu:744-860)
+    RC(k_fill(ctx, 11 * (int64_t)N, 0.f, d_vel11));
+    RC(k_fill(ctx, n17, 0.f, ctx->d_rhs));
+    RC(k_strain_rhs(ctx, ctx->d_rhs, shear_rate, d_ori3, ctx->d_force));
+    RC(k_rfe(ctx, pos, shear_rate, d_ori3, ctx->d_rhs + 11 * (int64_t)N));
+    RC(k_build_lub_pairs(ctx, pos));
+    RC(k_setup_spread(ctx, pos));
+    // saddle point solve, warm-started from the previous step's solution
+    fsd_solve_stats st;
+    memset(&st, 0, sizeof st);
+    st.true_residual = -1.f;
+    int rc = gmres(ctx, pos, ctx->d_rhs, ctx->d_sol, ctx->p.gmres_tol, ctx->p.gmres_maxiter, ctx->p.gmres_restart, &st);
+    if (rc < 0) return rc;
+    RC(k_axpby(ctx, 6 * (int64_t)N, 1.0f, ctx->d_sol + 11 * (int64_t)N, 1.0f, d_vel11, d_vel11));
+    if (ctx->p.period > 0 && (int)(timestep + 1) % ctx->p.period == 0) {
+        float *S = d_vel11 + 6 * (int64_t)N;
+        RC(k_axpby(ctx, 5 * (int64_t)N, 1.0f, ctx->d_sol + 6 * (int64_t)N, 1.0f, S, S));
+        RC(k_rsu(ctx, pos, d_vel11, S));
+        RC(k_rse(ctx, pos, shear_rate, d_ori3, S));
+    }
+    // explicit Euler with shear (Integrator_ExplicitEuler_Shear_kernel)
+    RC(k_euler_shear(ctx, pos, d_ori3, reinterpret_cast<float4 *>(d_pos_gb4), d_image3, d_vel11, ctx->d_mask1,
+                     d_noise_ang3, ctx->p.dt, shear_rate, ctx->p.sqm_B1));
+    CK(hipStreamSynchronize(ctx->stream));
+    st.ichol_relaxer = ctx->ichol_relaxer;
+    st.ichol_attempts = ctx->ichol_attempts;
+    st.nnz = ctx->nnz_ref;
+    // relaxer bookkeeping (Stokes.cc:1666)
+    ctx->ichol_relaxer = ((timestep % 5 == 0) || (ctx->ichol_relaxer > 1024.0f)) ? 1.0f : ctx->ichol_relaxer;
+    if (stats) *stats = st;
+    return rc;
+}
+
+int fsd_profile_enable(fsd_ctx *ctx, int32_t enable)
+{
+    ctx->prof = enable != 0;
+    return 0;
+}
+
+int fsd_profile_get(fsd_ctx *ctx, const char *name, int64_t *launches, double *total_ms)
+{
+    prof_collect(ctx);
+    for (auto &s : ctx->slots)
+        if (strcmp(s.name, name) == 0) {
+            *launches = s.launches;
+            *total_ms = s.ms;
+            return 0;
+        }
+    return FSD_E_PARAM;
+}
+
+int fsd_profile_reset(fsd_ctx *ctx)
+{
+    prof_collect(ctx);
+    for (auto &s : ctx->slots) { s.launches = 0; s.ms = 0.0; }
+    return 0;
+}
+
+}  // extern "C"

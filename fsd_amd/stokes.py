@@ -1,0 +1,298 @@
+"""Python mirror of the reference's C++ ``Stokes`` integration method
+(Stokes.h:43-101) on top of the C ABI (include/fsd.h).
+
+Device buffers are torch CUDA tensors (torch is used only for device memory);
+every call synchronises torch's stream before handing pointers to the
+library, and the library drains its own stream before returning.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .tables import load_resistance_table
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("fsd_amd needs a GPU: torch.cuda.is_available() is False")
+    return torch
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Stokes:
+    """One integrator instance: owns an fsd_ctx (device workspaces, tables,
+    warm start, IC(0) relaxer) for N particles in a fixed periodic box.
+
+    Method names follow the reference class: setParams / setResistanceTable /
+    setFriction / setSparseMath / AllocateWorkSpaces / integrateStepOne /
+    integrateStepTwo / setT / setShear.
+    """
+
+    def __init__(self, N, box, T=0.0, seed=0, xi=0.5, error=1e-3, fileprefix="stresslet", period=0,
+                 ndsr=1e-1, kappa=1 / 0.05, k_n=1e4, beta_AF=0.0, epsq=1e-5, sqm_B1=0.0, sqm_B2=0.0,
+                 N_mix=1, coef_B1_mask=1.0, coef_B2_mask=1.0, rot_diff=0.0, T_ext=0.0, omega_ext=0.0,
+                 dt=1e-3, max_strain=0.5, gmres_tol=None, gmres_restart=50, gmres_maxiter=1000,
+                 parity=True):
+        self.lib = _lib.load()
+        self.torch = _torch()
+        L = box if not np.isscalar(box) else (box, box, box, 0.0, 0.0, 0.0)
+        L = tuple(L) + (0.0,) * (6 - len(L))
+        p = _lib.FsdParams()
+        p.N = int(N)
+        p.Lx, p.Ly, p.Lz, p.xy, p.xz, p.yz = [float(v) for v in L]
+        p.T = float(T); p.seed = int(seed) & 0xFFFFFFFF
+        p.xi = float(xi); p.error = float(error); p.max_strain = float(max_strain); p.dt = float(dt)
+        p.period = int(period)
+        p.ndsr, p.kappa, p.k_n, p.beta_AF, p.epsq = float(ndsr), float(kappa), float(k_n), float(beta_AF), float(epsq)
+        p.sqm_B1, p.sqm_B2 = float(sqm_B1), float(sqm_B2)
+        p.N_mix = int(N_mix)
+        p.coef_B1_mask, p.coef_B2_mask = float(coef_B1_mask), float(coef_B2_mask)
+        p.rot_diff, p.T_ext, p.omega_ext = float(rot_diff), float(T_ext), float(omega_ext)
+        p.gmres_tol = float(gmres_tol) if gmres_tol is not None else float(error)
+        p.gmres_restart = int(gmres_restart); p.gmres_maxiter = int(gmres_maxiter)
+        p.rlub = 4.0; p.rp = 2.1; p.r_buff = 0.4
+        p.parity = 1 if parity else 0
+        self.params = p
+        self.N = int(N)
+        self.fileprefix = fileprefix
+        self.shear_function = None
+        self.max_strain = max_strain
+        self.ctx = C.c_void_p()
+        rc = self.lib.fsd_create(C.byref(p), C.byref(self.ctx))
+        self._check(rc)
+        self.setResistanceTable()
+
+    # ---- error handling
+    def _check(self, rc):
+        if rc < 0:
+            msg = self.lib.fsd_last_error(self.ctx).decode() if self.ctx else "create failed"
+            raise _lib.FsdError(rc, msg)
+        return rc
+
+    def __del__(self):
+        try:
+            if self.ctx:
+                self.lib.fsd_destroy(self.ctx)
+                self.ctx = C.c_void_p()
+        except Exception:
+            pass
+
+    def _sync(self):
+        self.torch.cuda.synchronize()
+
+    # ---- reference set-up methods
+    def setParams(self):
+        """parameters are derived at construction (Stokes.cc:118-696)"""
+        return self.pse_info()
+
+    def setResistanceTable(self):
+        dist, vals = load_resistance_table()
+        self._check(self.lib.fsd_set_resistance_table(self.ctx, dist.ctypes.data_as(C.c_void_p),
+                                                      vals.ctypes.data_as(C.c_void_p)))
+
+    def setFriction(self, friction_type="NONE", h0=0.0, alpha=(0.0,) * 6):
+        a = np.asarray(alpha, dtype=np.float32)
+        self._check(self.lib.fsd_set_friction(self.ctx, friction_type.upper().encode(), float(h0),
+                                              a.ctypes.data_as(C.c_void_p)))
+
+    def setSparseMath(self):
+        """sparse workspaces are allocated at construction (Stokes_SparseMath.cc:15-163)"""
+
+    def AllocateWorkSpaces(self):
+        """workspaces are allocated at construction (Stokes.cc:937-1063)"""
+
+    def setT(self, T):
+        self._check(self.lib.fsd_set_temperature(self.ctx, float(T)))
+
+    def setShear(self, shear_func, max_strain=0.5):
+        self.shear_function = shear_func
+        self.max_strain = max_strain
+
+    def set_box_tilt(self, xy=0.0, xz=0.0, yz=0.0):
+        p = self.params
+        self._check(self.lib.fsd_set_box(self.ctx, p.Lx, p.Ly, p.Lz, float(xy), float(xz), float(yz)))
+
+    def pse_info(self):
+        info = _lib.FsdPseInfo()
+        self.lib.fsd_get_pse_info(self.ctx, C.byref(info))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    # ---- the step
+    def integrateStepOne(self, timestep, state, force6=None, noise_ang=None, shear_rate=None):
+        """advance `state` (a ParticleState of device tensors) by one step.
+        Returns the solve statistics; velocities are left in state.velocity."""
+        if shear_rate is None:
+            shear_rate = self.shear_function.get_shear_rate(timestep) if self.shear_function else 0.0
+        st = _lib.FsdSolveStats()
+        self._sync()
+        rc = self.lib.fsd_step(self.ctx, int(timestep), float(shear_rate), _ptr(state.pos), _ptr(state.ori),
+                               _ptr(state.pos_gb), _ptr(state.image), _ptr(force6), _ptr(noise_ang),
+                               _ptr(state.velocity), C.byref(st))
+        self._check(rc)
+        d = st.as_dict()
+        d["converged"] = rc == 0
+        return d
+
+    def integrateStepTwo(self, timestep):
+        """no second half step (Stokes.cc:1688-1690)"""
+
+    # ---- operator level (tests / benchmarks)
+    def build_nlist(self, pos):
+        self._sync()
+        self._check(self.lib.fsd_build_nlist(self.ctx, _ptr(pos)))
+
+    def get_nlist(self):
+        N = self.N
+        nn = np.zeros(N, dtype=np.int32)
+        tot = C.c_int64(0)
+        self._check(self.lib.fsd_get_nlist(self.ctx, nn.ctypes.data_as(C.c_void_p), None, 0, C.byref(tot)))
+        nbrs = np.zeros(max(tot.value, 1), dtype=np.int32)
+        self._check(self.lib.fsd_get_nlist(self.ctx, nn.ctypes.data_as(C.c_void_p),
+                                           nbrs.ctypes.data_as(C.c_void_p), tot.value, C.byref(tot)))
+        return nn, nbrs[:tot.value]
+
+    def _out(self, n):
+        return self.torch.zeros(n, dtype=self.torch.float32, device="cuda")
+
+    def rfu(self, pos, U):
+        F = self._out(6 * self.N)
+        self._sync()
+        self._check(self.lib.fsd_rfu(self.ctx, _ptr(pos), _ptr(U), _ptr(F)))
+        return F
+
+    def rfe(self, pos, shear_rate, ori=None, F=None):
+        F = self._out(6 * self.N) if F is None else F
+        self._sync()
+        self._check(self.lib.fsd_rfe(self.ctx, _ptr(pos), float(shear_rate), _ptr(ori), _ptr(F)))
+        return F
+
+    def rsu(self, pos, U, S=None):
+        S = self._out(5 * self.N) if S is None else S
+        self._sync()
+        self._check(self.lib.fsd_rsu(self.ctx, _ptr(pos), _ptr(U), _ptr(S)))
+        return S
+
+    def rse(self, pos, shear_rate, ori=None, S=None):
+        S = self._out(5 * self.N) if S is None else S
+        self._sync()
+        self._check(self.lib.fsd_rse(self.ctx, _ptr(pos), float(shear_rate), _ptr(ori), _ptr(S)))
+        return S
+
+    def mobility(self, pos, gF, parts=3):
+        gU = self._out(11 * self.N)
+        self._sync()
+        self._check(self.lib.fsd_mobility(self.ctx, _ptr(pos), _ptr(gF), _ptr(gU), int(parts)))
+        return gU
+
+    def build_precond(self, pos):
+        st = _lib.FsdSolveStats()
+        self._sync()
+        self._check(self.lib.fsd_build_precond(self.ctx, _ptr(pos), C.byref(st)))
+        return st.as_dict()
+
+    def get_precond(self, with_values=True):
+        N = self.N
+        nnz = C.c_int64(0)
+        pn = np.zeros(N, dtype=np.int32)
+        perm = np.zeros(N, dtype=np.int32)
+        self._check(self.lib.fsd_get_precond(self.ctx, pn.ctypes.data_as(C.c_void_p), None, 0,
+                                             perm.ctypes.data_as(C.c_void_p), None, None, None, None, None, None,
+                                             0, C.byref(nnz)))
+        plist = np.zeros(int(pn.sum()), dtype=np.int32)
+        nz = nnz.value
+        rowptr = np.zeros(6 * N + 1, dtype=np.int32)
+        col = np.zeros(nz, dtype=np.int32)
+        val = np.zeros(nz, dtype=np.float32)
+        prow = np.zeros(6 * N + 1, dtype=np.int32)
+        pcol = np.zeros(nz, dtype=np.int32)
+        Lv = np.zeros(nz, dtype=np.float32)
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(self.lib.fsd_get_precond(self.ctx, vp(pn), vp(plist), plist.size, vp(perm), vp(rowptr),
+                                             vp(col), vp(val), vp(prow), vp(pcol), vp(Lv), nz, C.byref(nnz)))
+        return dict(pn=pn, plist=plist, perm=perm, rowptr=rowptr, col=col, val=val, prow=prow, pcol=pcol, L=Lv)
+
+    def precond_apply(self, x6):
+        y = self._out(6 * self.N)
+        self._sync()
+        self._check(self.lib.fsd_precond_apply(self.ctx, _ptr(x6), _ptr(y)))
+        return y
+
+    def saddle_multiply(self, pos, x):
+        y = self._out(17 * self.N)
+        self._sync()
+        self._check(self.lib.fsd_saddle_multiply(self.ctx, _ptr(pos), _ptr(x), _ptr(y)))
+        return y
+
+    def saddle_precond(self, b):
+        x = self._out(17 * self.N)
+        self._sync()
+        self._check(self.lib.fsd_saddle_precond(self.ctx, _ptr(b), _ptr(x)))
+        return x
+
+    def saddle_solve(self, pos, rhs, x0=None, true_residual=True):
+        x = self._out(17 * self.N) if x0 is None else x0.clone()
+        st = _lib.FsdSolveStats()
+        self._sync()
+        rc = self.lib.fsd_saddle_solve(self.ctx, _ptr(pos), _ptr(rhs), _ptr(x), 1 if true_residual else 0,
+                                       C.byref(st))
+        self._check(rc)
+        d = st.as_dict()
+        d["converged"] = rc == 0
+        return x, d
+
+    def force_law(self, pos, T_ext=0.0):
+        F = self._out(6 * self.N)
+        self._sync()
+        self._check(self.lib.fsd_force_law(self.ctx, _ptr(pos), float(T_ext), _ptr(F)))
+        return F
+
+    def get_ewald_table(self):
+        info = self.pse_info()
+        t = np.zeros(8 * (info["ewald_n"] + 1), dtype=np.float32)
+        self._check(self.lib.fsd_get_ewald_table(self.ctx, t.ctypes.data_as(C.c_void_p), t.size))
+        return t
+
+    # ---- profiling
+    def profile(self, enable=True):
+        self.lib.fsd_profile_enable(self.ctx, 1 if enable else 0)
+
+    def profile_get(self, name):
+        n = C.c_int64(0)
+        ms = C.c_double(0)
+        self._check(self.lib.fsd_profile_get(self.ctx, name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def profile_reset(self):
+        self.lib.fsd_profile_reset(self.ctx)
+
+
+class ParticleState:
+    """Device-resident particle arrays in the layouts of the reference's
+    HOOMD arrays: positions (float4), orientation (stored by the reference in
+    the accelerations array), unwrapped positions (in the velocities array),
+    images, and the 11N Velocity output."""
+
+    def __init__(self, pos, ori=None, image=None, device="cuda"):
+        torch = _torch()
+        pos = np.asarray(pos, dtype=np.float32)
+        N = pos.shape[0]
+        p4 = np.zeros((N, 4), dtype=np.float32)
+        p4[:, :3] = pos[:, :3]
+        self.N = N
+        self.pos = torch.from_numpy(p4.reshape(-1)).to(device)
+        self.pos_gb = self.pos.clone()
+        if ori is None:
+            ori = np.zeros((N, 3), dtype=np.float32)
+            ori[:, 0] = 1.0
+        self.ori = torch.from_numpy(np.ascontiguousarray(ori, dtype=np.float32).reshape(-1)).to(device)
+        img = np.zeros((N, 3), dtype=np.int32) if image is None else np.asarray(image, dtype=np.int32)
+        self.image = torch.from_numpy(img.reshape(-1)).to(device)
+        self.velocity = torch.zeros(11 * N, dtype=torch.float32, device=device)
+
+    def positions(self):
+        return self.pos.view(-1, 4)[:, :3].cpu().numpy()

@@ -1,0 +1,186 @@
+/*
+ * fsd.h - C ABI of the MI355X Fast Stokesian Dynamics velocity/integration step.
+ *
+ * This is the boundary the host side (the C++ `Stokes` integrator and the
+ * Python `integrate.PSEv3` class) calls into.  It replaces the reference's
+ * host->device seam `Stokes_StepOne` (Stokes.cuh:81-108) and the setup calls
+ * of `class Stokes` (Stokes.h:43-101); the operator-level entry points below
+ * expose the individual functions of that step so they can be tested and
+ * timed on their own.
+ *
+ * Conventions
+ *   - every array argument named d_* is a device pointer (HIP device memory)
+ *     owned by the caller; h_* arguments are host pointers;
+ *   - particle arrays use the reference layouts:
+ *       pos, pos_gb     float4[N]  (x, y, z, w)             (HOOMD positions)
+ *       ori             float[3N]  unit director             (Stokes.cc:1234)
+ *       image           int[3N]                              (HOOMD images)
+ *       force/torque    float[6N]  [Fx Fy Fz Tx Ty Tz]      (AppliedForce)
+ *       velocity        float[11N] [U,W (6N) | E or S (5N)] (Velocity)
+ *       saddle vectors  float[17N] [F,T (6N) | S (5N) | U,W (6N)];
+ *   - return value 0 = success; negative = error (fsd_last_error has text);
+ *     FSD_W_NOT_CONVERGED (> 0) = a solve stopped at its iteration limit;
+ *   - each call is enqueued on the context's stream and returns after that
+ *     stream has drained, so device buffers may be reused on return.
+ */
+#ifndef FSD_H
+#define FSD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FSD_OK 0
+#define FSD_E_PARAM (-1)          /* invalid parameter (setParams' exit() cases) */
+#define FSD_E_HIP (-2)            /* HIP / rocFFT runtime failure (gpuErrchk) */
+#define FSD_E_CAPACITY (-3)       /* a neighbour or sparse capacity was exceeded */
+#define FSD_E_ICHOL (-4)          /* IC(0) failed after the relaxer limit */
+#define FSD_E_STATE (-5)          /* call order violated (e.g. no nlist yet) */
+#define FSD_W_NOT_CONVERGED 1     /* GMRES hit its iteration limit (Solvers.cu:121-125) */
+
+typedef struct fsd_ctx fsd_ctx;
+
+/* Constructor arguments of Stokes (Stokes.cc:47-83, integrate.py:40-55) plus
+ * the box and the knobs the reference hard-codes (Stokes.cc:1505-1506,
+ * Solvers.cu:105-117, integrate.py:86). */
+typedef struct fsd_params {
+    int32_t N;                       /* particles in the group */
+    float Lx, Ly, Lz, xy, xz, yz;    /* periodic box (HOOMD BoxDim) */
+    float T;                         /* kT of the bath */
+    uint32_t seed;
+    float xi;                        /* Ewald splitting parameter */
+    float error;                     /* PSE error tolerance */
+    float max_strain;                /* box strain bound for the spreading support */
+    float dt;
+    int32_t period;                  /* stresslet output period (0 = never) */
+    float ndsr, kappa, k_n, beta_AF, epsq;  /* interparticle force law */
+    float sqm_B1, sqm_B2;            /* squirmer modes */
+    uint32_t N_mix;
+    float coef_B1_mask, coef_B2_mask, rot_diff, T_ext, omega_ext;
+    /* not settable in the reference (defaults in brackets) */
+    float gmres_tol;                 /* <= 0: = error [error] */
+    int32_t gmres_restart;           /* <= 0: 50 */
+    int32_t gmres_maxiter;           /* <= 0: 1000 */
+    float rlub;                      /* <= 0: 4.0 */
+    float rp;                        /* <= 0: 2.1 */
+    float r_buff;                    /* < 0: 0.4 */
+    int32_t parity;                  /* 1: reproduce the reference's k-vector constant (Helper_Mobility.cu:389-391) */
+} fsd_params;
+
+/* Derived PSE parameters (Stokes.cc:131-261). */
+typedef struct fsd_pse_info {
+    int32_t Nx, Ny, Nz, P;
+    float gaussm, eta, rcut, gridh_x, gridh_y, gridh_z;
+    int32_t ewald_n;
+    float self_x, self_y;
+    float prefac, expfac, quadW;
+} fsd_pse_info;
+
+typedef struct fsd_solve_stats {
+    int32_t iterations;              /* GMRES iterations (Arnoldi steps) */
+    int32_t restarts;
+    float residual;                  /* final preconditioned residual estimate / ||b|| */
+    float true_residual;             /* ||b - A x|| / ||b||, if requested (< 0 otherwise) */
+    float ichol_relaxer;             /* lambda actually used by IC(0) */
+    int32_t ichol_attempts;
+    int64_t nnz;                     /* preconditioner nonzeros (reference CSR count) */
+    int32_t nlist_total;             /* entries in the neighbour list */
+} fsd_solve_stats;
+
+/* ---- lifecycle: Stokes::Stokes + setParams + setSparseMath + AllocateWorkSpaces
+ *      (Stokes.cc:47-83, 118-696, 937-1063; Stokes_SparseMath.cc:15-163) ---- */
+int fsd_create(const fsd_params *p, fsd_ctx **out);
+void fsd_destroy(fsd_ctx *ctx);
+const char *fsd_last_error(const fsd_ctx *ctx);
+int fsd_get_pse_info(const fsd_ctx *ctx, fsd_pse_info *out);
+
+/* Stokes::setResistanceTable (Stokes_ResistanceTable.cc:34-23057):
+ * h_dist[1000], h_vals[22000] in the reference's order. */
+int fsd_set_resistance_table(fsd_ctx *ctx, const float *h_dist, const float *h_vals);
+/* Stokes::setFriction (Stokes.cc:1132-1191); "NONE" (the default) is accepted,
+ * other types return FSD_E_PARAM. */
+int fsd_set_friction(fsd_ctx *ctx, const char *type, float h0, const float *h_alpha6);
+/* Stokes::setT / set_params (Stokes.h:64-67, integrate.py:139-149) */
+int fsd_set_temperature(fsd_ctx *ctx, float T);
+/* box tilt for a deforming box (HOOMD box update between steps) */
+int fsd_set_box(fsd_ctx *ctx, float Lx, float Ly, float Lz, float xy, float xz, float yz);
+/* real-space Ewald table (Stokes.cc:344-678): h_tab[8*(ewald_n+1)] */
+int fsd_get_ewald_table(const fsd_ctx *ctx, float *h_tab, int32_t capacity);
+
+/* ---- the step: Stokes::integrateStepOne -> Stokes_StepOne
+ *      (Stokes.cc:1197-1683, Stokes.cu:119-273) ----
+ * d_force6: NULL -> the reference's interparticle force law
+ *           (Stokes_SetForce_manually_kernel, Helper_Stokes.cu:65-172);
+ *           otherwise the applied force/torque to use instead.
+ * d_noise_ang3: rotational noise (NULL -> none)
+ * d_vel11: output Velocity array (disturbance U,W then stresslet on output steps).
+ * Positions, orientations, unwrapped positions and images are advanced in place. */
+int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate,
+             float *d_pos4, float *d_ori3, float *d_pos_gb4, int32_t *d_image3,
+             const float *d_force6, const float *d_noise_ang3, float *d_vel11,
+             fsd_solve_stats *stats);
+
+/* ---- operator-level entry points ---- */
+/* HOOMD NeighborList::compute (Stokes.cc:1216): full list, r < rcut + r_buff. */
+int fsd_build_nlist(fsd_ctx *ctx, const float *d_pos4);
+/* copy the current list to the host: h_nneigh[N]; h_nbrs gets each particle's
+ * neighbours ascending, packed; returns the total in *total. */
+int fsd_get_nlist(fsd_ctx *ctx, int32_t *h_nneigh, int32_t *h_nbrs, int64_t capacity, int64_t *total);
+
+/* Lubrication_RFU_kernel (Lubrication.cu:69-309): d_F6 = R_FU d_U6 */
+int fsd_rfu(fsd_ctx *ctx, const float *d_pos4, const float *d_U6, float *d_F6);
+/* Lubrication_RFE_kernel (Lubrication.cu:350-608): d_F6 += R_FE : E */
+int fsd_rfe(fsd_ctx *ctx, const float *d_pos4, float shear_rate, const float *d_ori3, float *d_F6);
+/* Lubrication_RSU_kernel (Lubrication.cu:650-911): d_S5 -= R_SU d_U6 */
+int fsd_rsu(fsd_ctx *ctx, const float *d_pos4, const float *d_U6, float *d_S5);
+/* Lubrication_RSE_kernel (Lubrication.cu:953-1211): d_S5 += R_SE : E */
+int fsd_rse(fsd_ctx *ctx, const float *d_pos4, float shear_rate, const float *d_ori3, float *d_S5);
+
+/* Mobility_GeneralizedMobility (Mobility.cu:1488-1542): d_gU11 = M d_gF11;
+ * parts: 1 real space only, 2 wave space only, 3 both. */
+int fsd_mobility(fsd_ctx *ctx, const float *d_pos4, const float *d_gF11, float *d_gU11, int32_t parts);
+
+/* Precondition_Wrap (Precondition.cu:1326-1497): prune, build R_FU^nf, RCM,
+ * IC(0) with relaxer doubling.  Uses the current neighbour list. */
+int fsd_build_precond(fsd_ctx *ctx, const float *d_pos4, fsd_solve_stats *stats);
+/* Export of the preconditioner in the reference's formats:
+ *   h_pn[N] pruned counts (self included), h_plist packed ascending (total = sum h_pn),
+ *   h_perm[N] RCM permutation (new -> old),
+ *   h_rowptr[6N+1], h_col[nnz], h_val[nnz]: R_FU^nf CSR before reordering (coo2csr of
+ *     Precondition_RFU_kernel's COO), h_prow/h_pcol: pattern after csrperm,
+ *   h_L[nnz]: IC(0) factor on the permuted pattern (upper part zero).
+ * Any pointer may be NULL; capacities are in elements. */
+int fsd_get_precond(fsd_ctx *ctx, int32_t *h_pn, int32_t *h_plist, int64_t plist_cap, int32_t *h_perm,
+                    int32_t *h_rowptr, int32_t *h_col, float *h_val, int32_t *h_prow, int32_t *h_pcol,
+                    float *h_L, int64_t nnz_cap, int64_t *nnz_out);
+/* Precondition_Saddle_RFUmultiply (Precondition.cu:1802-1890): d_y6 = P^T L^-T L^-1 P d_x6 */
+int fsd_precond_apply(fsd_ctx *ctx, const float *d_x6, float *d_y6);
+
+/* Saddle_Multiply (Saddle.cu:80-151) */
+int fsd_saddle_multiply(fsd_ctx *ctx, const float *d_pos4, const float *d_x17, float *d_y17);
+/* Saddle_Preconditioner (Saddle.cu:171-264) */
+int fsd_saddle_precond(fsd_ctx *ctx, const float *d_b17, float *d_x17);
+/* Solvers_Saddle (Solvers.cu:55-126): CUSP-semantics left-preconditioned GMRES;
+ * d_sol17 holds the initial guess on entry. want_true_residual adds one mat-vec. */
+int fsd_saddle_solve(fsd_ctx *ctx, const float *d_pos4, const float *d_rhs17, float *d_sol17,
+                     int32_t want_true_residual, fsd_solve_stats *stats);
+
+/* Stokes_SetForce_manually_kernel (Helper_Stokes.cu:65-172) */
+int fsd_force_law(fsd_ctx *ctx, const float *d_pos4, float T_ext, float *d_F6);
+/* Integrator_ExplicitEuler_Shear_kernel (Integrator.cu:126-228) */
+int fsd_euler_shear(fsd_ctx *ctx, float *d_pos4, float *d_ori3, float *d_pos_gb4, int32_t *d_image3,
+                    const float *d_vel6, const float *d_mask1, const float *d_noise3,
+                    float dt, float shear_rate);
+
+/* ---- timing of the hot kernels (HIP events on the context's stream) ---- */
+/* enable: record an event pair around every launch of the named kernel classes */
+int fsd_profile_enable(fsd_ctx *ctx, int32_t enable);
+/* name: "spread", "green", "contract", "fft", "realspace", "rfu", "trsv", "ichol",
+ * "gmres_vec", "nlist", "precond_build"; returns launches and total milliseconds */
+int fsd_profile_get(fsd_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
+int fsd_profile_reset(fsd_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSD_H */

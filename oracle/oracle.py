@@ -1,0 +1,327 @@
+"""Python front end of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the benchmark's
+CPU-baseline leg as the checker; the product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+_P = C.c_void_p
+
+
+class Box(C.Structure):
+    _fields_ = [("Lx", C.c_float), ("Ly", C.c_float), ("Lz", C.c_float),
+                ("xy", C.c_float), ("xz", C.c_float), ("yz", C.c_float)]
+
+
+class Pse(C.Structure):
+    _fields_ = [("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int), ("P", C.c_int),
+                ("gaussm", C.c_float), ("eta", C.c_float), ("xi", C.c_float), ("error", C.c_float),
+                ("rcut", C.c_float), ("gridh", C.c_float * 3), ("ewald_n", C.c_int), ("ewald_dr", C.c_float),
+                ("self", C.c_float * 2), ("prefac", C.c_float), ("expfac", C.c_float), ("quadW", C.c_float)]
+
+
+class Lub(C.Structure):
+    _fields_ = [("dist", _P), ("vals", _P), ("table_min", C.c_float), ("table_dr", C.c_float),
+                ("rlub", C.c_float), ("rp", C.c_float)]
+
+
+class Csr(C.Structure):
+    _fields_ = [("n", C.c_int), ("nnz", C.c_int), ("rowptr", C.POINTER(C.c_int)),
+                ("col", C.POINTER(C.c_int)), ("val", C.POINTER(C.c_double))]
+
+
+class System(C.Structure):
+    _fields_ = [("N", C.c_int), ("pos4", _P), ("box", Box), ("pse", C.POINTER(Pse)), ("ewald_tab", _P),
+                ("gridk4", _P), ("nneigh", _P), ("head", _P), ("nbrs", _P), ("lub", C.POINTER(Lub)),
+                ("L", C.POINTER(Csr)), ("perm", _P)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        lib = C.CDLL(LIB)
+        lib.orc_pse_params.argtypes = [C.POINTER(Box), C.c_float, C.c_float, C.c_float, C.POINTER(Pse)]
+        lib.orc_gridk.argtypes = [C.POINTER(Box), C.POINTER(Pse), C.c_int, _P]
+        lib.orc_nlist.argtypes = [C.c_int, _P, C.POINTER(Box), C.c_float, _P, _P, _P, C.c_int]
+        for fn in ("orc_rfu", "orc_rsu"):
+            getattr(lib, fn).argtypes = [C.c_int, _P, C.POINTER(Box), _P, _P, _P, C.POINTER(Lub), _P, _P]
+        for fn in ("orc_rfe", "orc_rse"):
+            getattr(lib, fn).argtypes = [C.c_int, _P, C.POINTER(Box), _P, _P, _P, C.POINTER(Lub), C.c_double,
+                                         C.c_double, _P, _P, _P]
+        lib.orc_mobility.argtypes = [C.c_int, _P, C.POINTER(Box), C.POINTER(Pse), _P, _P, _P, _P, _P, _P, _P,
+                                     C.c_int]
+        lib.orc_prune.argtypes = [C.c_int, _P, C.POINTER(Box), _P, _P, _P, C.c_float, _P, _P, _P, _P, C.c_int]
+        lib.orc_build_rfu_csr.argtypes = [C.c_int, _P, C.POINTER(Box), _P, _P, _P, _P, C.POINTER(Lub),
+                                          C.POINTER(Csr)]
+        lib.orc_rcm.argtypes = [C.c_int, _P, _P, _P]
+        lib.orc_permute.argtypes = [C.POINTER(Csr), _P, C.c_int, C.POINTER(Csr)]
+        lib.orc_ichol.argtypes = [C.POINTER(Csr), C.c_double]
+        lib.orc_precond_apply.argtypes = [C.POINTER(Csr), _P, C.c_int, _P, _P]
+        lib.orc_csr_free.argtypes = [C.POINTER(Csr)]
+        lib.orc_saddle_multiply.argtypes = [C.POINTER(System), _P, _P]
+        lib.orc_saddle_precond.argtypes = [C.POINTER(System), _P, _P]
+        lib.orc_gmres.argtypes = [C.POINTER(System), _P, _P, C.c_int, C.c_int, C.c_double, C.c_int,
+                                  C.POINTER(C.c_double)]
+        lib.orc_force_law.argtypes = [C.c_int, _P, C.POINTER(Box), _P, _P, _P, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_double, _P]
+        lib.orc_euler_shear.argtypes = [C.c_int, _P, _P, _P, _P, C.POINTER(Box), _P, C.c_double, _P, _P,
+                                        C.c_double, C.c_double]
+        lib.fsd_ewald_table_fill.argtypes = [C.c_float, C.c_int, _P]
+        lib.fsd_ewald_functions.argtypes = [C.c_double, C.c_double, _P]
+        lib.fsd_ewald_self.argtypes = [C.c_float, _P]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def make_box(L, xy=0.0, xz=0.0, yz=0.0):
+    if np.isscalar(L):
+        L = (L, L, L)
+    return Box(float(L[0]), float(L[1]), float(L[2]), float(xy), float(xz), float(yz))
+
+
+class Oracle:
+    """Holds one configuration (positions, box, tables, neighbour list) and
+    exposes the oracle operators on it with numpy arrays."""
+
+    def __init__(self, pos, L, xi=0.5, error=1e-3, max_strain=0.5, xy=0.0, res_table=None, parity=True,
+                 r_buff=0.4, rlub=4.0, rp=2.1):
+        self.lib = load()
+        pos = np.asarray(pos, dtype=np.float32)
+        if pos.shape[1] == 3:
+            p4 = np.zeros((pos.shape[0], 4), dtype=np.float32)
+            p4[:, :3] = pos
+            pos = p4
+        self.pos4 = np.ascontiguousarray(pos)
+        self.N = self.pos4.shape[0]
+        self.box = make_box(L, xy)
+        self.pse = Pse()
+        rc = self.lib.orc_pse_params(C.byref(self.box), xi, error, max_strain, C.byref(self.pse))
+        if rc != 0:
+            raise ValueError("orc_pse_params failed: %d" % rc)
+        self.nR = self.pse.ewald_n + 1
+        self.ewald = np.zeros(8 * self.nR, dtype=np.float32)
+        self.lib.fsd_ewald_table_fill(xi, self.nR, _p(self.ewald))
+        M = self.pse.Nx * self.pse.Ny * self.pse.Nz
+        self.gridk = np.zeros(4 * M, dtype=np.float64)
+        self.lib.orc_gridk(C.byref(self.box), C.byref(self.pse), 1 if parity else 0, _p(self.gridk))
+        self.rlist = self.pse.rcut + r_buff
+        self.build_nlist()
+        if res_table is None:
+            from fsd_amd.tables import load_resistance_table
+            res_table = load_resistance_table()
+        self.res_dist = np.ascontiguousarray(res_table[0], dtype=np.float32)
+        self.res_vals = np.ascontiguousarray(res_table[1], dtype=np.float32)
+        self.lub = Lub(_p(self.res_dist), _p(self.res_vals), 0.0001, 0.004305, rlub, rp)
+        self.L = None
+        self.perm = None
+
+    # ---- neighbour list
+    def build_nlist(self):
+        N = self.N
+        self.nneigh = np.zeros(N, dtype=np.int32)
+        self.head = np.zeros(N, dtype=np.int32)
+        cap = max(N * 64, 1024)
+        while True:
+            self.nbrs = np.zeros(cap, dtype=np.int32)
+            tot = self.lib.orc_nlist(N, _p(self.pos4), C.byref(self.box), self.rlist, _p(self.nneigh),
+                                     _p(self.head), _p(self.nbrs), cap)
+            if tot <= cap:
+                break
+            cap = tot
+        self.nbrs = self.nbrs[:tot].copy()
+        return self.nneigh, self.nbrs
+
+    def _nl(self):
+        return _p(self.nneigh), _p(self.head), _p(self.nbrs)
+
+    # ---- lubrication
+    def rfu(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        F = np.zeros(6 * self.N)
+        self.lib.orc_rfu(self.N, _p(self.pos4), C.byref(self.box), *self._nl(), C.byref(self.lub), _p(U), _p(F))
+        return F
+
+    def rsu(self, U, S=None):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        S = np.zeros(5 * self.N) if S is None else np.array(S, dtype=np.float64)
+        self.lib.orc_rsu(self.N, _p(self.pos4), C.byref(self.box), *self._nl(), C.byref(self.lub), _p(U), _p(S))
+        return S
+
+    def rfe(self, shear_rate, F=None, B2=0.0, mask2=None, ori=None):
+        F = np.zeros(6 * self.N) if F is None else np.array(F, dtype=np.float64)
+        m = None if mask2 is None else np.ascontiguousarray(mask2, dtype=np.float32)
+        o = None if ori is None else np.ascontiguousarray(ori, dtype=np.float32)
+        self.lib.orc_rfe(self.N, _p(self.pos4), C.byref(self.box), *self._nl(), C.byref(self.lub), shear_rate, B2,
+                         None if m is None else _p(m), None if o is None else _p(o), _p(F))
+        return F
+
+    def rse(self, shear_rate, S=None, B2=0.0, mask2=None, ori=None):
+        S = np.zeros(5 * self.N) if S is None else np.array(S, dtype=np.float64)
+        m = None if mask2 is None else np.ascontiguousarray(mask2, dtype=np.float32)
+        o = None if ori is None else np.ascontiguousarray(ori, dtype=np.float32)
+        self.lib.orc_rse(self.N, _p(self.pos4), C.byref(self.box), *self._nl(), C.byref(self.lub), shear_rate, B2,
+                         None if m is None else _p(m), None if o is None else _p(o), _p(S))
+        return S
+
+    # ---- mobility
+    def mobility(self, gF, parts=3):
+        gF = np.ascontiguousarray(gF, dtype=np.float64)
+        gU = np.zeros(11 * self.N)
+        self.lib.orc_mobility(self.N, _p(self.pos4), C.byref(self.box), C.byref(self.pse), _p(self.ewald),
+                              _p(self.gridk), *self._nl(), _p(gF), _p(gU), parts)
+        return gU
+
+    # ---- preconditioner
+    def prune(self):
+        N = self.N
+        pn = np.zeros(N, dtype=np.int32)
+        phead = np.zeros(N + 1, dtype=np.int32)
+        pless = np.zeros(N, dtype=np.int32)
+        cap = 16 * N
+        plist = np.zeros(cap, dtype=np.int32)
+        tot = self.lib.orc_prune(N, _p(self.pos4), C.byref(self.box), *self._nl(), self.lub.rp, _p(pn), _p(phead),
+                                 _p(plist), _p(pless), cap)
+        self.pn, self.phead, self.plist, self.pless = pn, phead, plist[:tot].copy(), pless
+        return pn, phead, self.plist, pless
+
+    def build_precond(self, lam=1.0):
+        """prune, R_FU^nf CSR, RCM, permute, IC(0) with relaxer doubling"""
+        self.prune()
+        A = Csr()
+        self.lib.orc_build_rfu_csr(self.N, _p(self.pos4), C.byref(self.box), _p(self.pn), _p(self.phead),
+                                   _p(self.plist), _p(self.pless), C.byref(self.lub), C.byref(A))
+        self.A = A
+        self.A_rowptr = np.ctypeslib.as_array(A.rowptr, shape=(A.n + 1,)).copy()
+        self.A_col = np.ctypeslib.as_array(A.col, shape=(A.nnz,)).copy()
+        self.A_val = np.ctypeslib.as_array(A.val, shape=(A.nnz,)).copy()
+        self.perm = np.zeros(self.N, dtype=np.int32)
+        self.lib.orc_rcm(self.N, _p(self.phead), _p(self.plist), _p(self.perm))
+        attempts = 0
+        while True:
+            B = Csr()
+            self.lib.orc_permute(C.byref(A), _p(self.perm), self.N, C.byref(B))
+            if attempts == 0:
+                self.B_rowptr = np.ctypeslib.as_array(B.rowptr, shape=(B.n + 1,)).copy()
+                self.B_col = np.ctypeslib.as_array(B.col, shape=(B.nnz,)).copy()
+                self.B_val = np.ctypeslib.as_array(B.val, shape=(B.nnz,)).copy()
+            fail = self.lib.orc_ichol(C.byref(B), lam)
+            attempts += 1
+            if fail == 0:
+                break
+            self.lib.orc_csr_free(C.byref(B))
+            lam *= 2.0
+        self.L = B
+        self.L_val = np.ctypeslib.as_array(B.val, shape=(B.nnz,)).copy()
+        self.lam = lam
+        self.attempts = attempts
+        return self
+
+    def precond_apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(6 * self.N)
+        self.lib.orc_precond_apply(C.byref(self.L), _p(self.perm), self.N, _p(x), _p(y))
+        return y
+
+    # ---- saddle point
+    def _system(self):
+        s = System()
+        s.N = self.N
+        s.pos4 = _p(self.pos4)
+        s.box = self.box
+        s.pse = C.pointer(self.pse)
+        s.ewald_tab = _p(self.ewald)
+        s.gridk4 = _p(self.gridk)
+        s.nneigh, s.head, s.nbrs = self._nl()
+        s.lub = C.pointer(self.lub)
+        s.L = C.pointer(self.L) if self.L is not None else None
+        s.perm = _p(self.perm) if self.perm is not None else None
+        self._sys = s
+        return s
+
+    def saddle_multiply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(17 * self.N)
+        self.lib.orc_saddle_multiply(C.byref(self._system()), _p(x), _p(y))
+        return y
+
+    def saddle_precond(self, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(17 * self.N)
+        self.lib.orc_saddle_precond(C.byref(self._system()), _p(b), _p(x))
+        return x
+
+    def gmres(self, b, x0=None, restart=50, maxit=1000, tol=1e-10, use_precond=True):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(17 * self.N) if x0 is None else np.array(x0, dtype=np.float64)
+        r = C.c_double(0)
+        it = self.lib.orc_gmres(C.byref(self._system()), _p(b), _p(x), restart, maxit, tol,
+                                1 if use_precond else 0, C.byref(r))
+        return x, it, r.value
+
+    # ---- force / integration
+    def force_law(self, ndsr=0.1, k_n=1e4, kappa=20.0, beta=0.0, epsq=1e-5, T_ext=0.0):
+        F = np.zeros(6 * self.N)
+        self.lib.orc_force_law(self.N, _p(self.pos4), C.byref(self.box), *self._nl(), ndsr, k_n, kappa, beta, epsq,
+                               T_ext, _p(F))
+        return F
+
+    def rhs(self, force6, shear_rate):
+        """saddle right-hand side at T = 0 (Integrator.cu:831-844)"""
+        N = self.N
+        b = np.zeros(17 * N)
+        b[6 * N + 1:11 * N:5] += shear_rate
+        b[11 * N:] -= np.asarray(force6, dtype=np.float64)
+        b[11 * N:] = self.rfe(shear_rate, F=b[11 * N:])
+        return b
+
+    def __del__(self):
+        try:
+            if self.L is not None:
+                self.lib.orc_csr_free(C.byref(self.L))
+            if getattr(self, "A", None) is not None:
+                self.lib.orc_csr_free(C.byref(self.A))
+        except Exception:
+            pass
+
+
+def pse_params(L, xi=0.5, error=1e-3, max_strain=0.5):
+    lib = load()
+    box = make_box(L)
+    p = Pse()
+    rc = lib.orc_pse_params(C.byref(box), xi, error, max_strain, C.byref(p))
+    if rc != 0:
+        raise ValueError("orc_pse_params failed: %d" % rc)
+    return p
+
+
+def ewald_functions(r, xi):
+    lib = load()
+    out = np.zeros(7)
+    lib.fsd_ewald_functions(float(r), float(xi), _p(out))
+    return out
+
+
+def ewald_table(xi, nR):
+    lib = load()
+    t = np.zeros(8 * nR, dtype=np.float32)
+    lib.fsd_ewald_table_fill(xi, nR, _p(t))
+    return t

@@ -1,0 +1,184 @@
+"""GPU parity of every operator of the FSD step against the FP64 oracle.
+
+Integer and index results (neighbour sets, pruned lists, RCM permutation,
+sparsity patterns) must match bit-exactly; floating-point results within the
+tolerance written in each test (FP32 kernels vs an FP64 restatement of the
+same discretisation; the north-star bar is 1e-5 relative).
+"""
+import numpy as np
+import pytest
+
+from tests._cases import packing, rel_err, rel_err2, forces, random_vec, sorted_lists
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "C1": (343, 0.20, 11),
+    "C2": (4096, 0.30, 12),
+}
+
+
+def _setup(case, **kw):
+    import torch
+    from fsd_amd.stokes import Stokes
+    from oracle.oracle import Oracle
+    N, phi, seed = CASES[case]
+    pos, L = packing(N, phi, seed)
+    print("case", case, "N", N, "phi", phi, "seed", seed, flush=True)
+    st = Stokes(N, L, **kw)
+    p4 = np.zeros((N, 4), dtype=np.float32)
+    p4[:, :3] = pos
+    dpos = torch.from_numpy(p4.reshape(-1)).cuda()
+    st.build_nlist(dpos)
+    orc = Oracle(pos, L)
+    return st, orc, dpos, N
+
+
+def _t(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
+
+
+@pytest.mark.parametrize("case", ["C1", "C2"])
+def test_nlist_bitexact(case):
+    st, orc, dpos, N = _setup(case)
+    nn, nb = st.get_nlist()
+    got = sorted_lists(nn, nb)
+    ref = sorted_lists(orc.nneigh, orc.nbrs)
+    assert got == ref
+
+
+@pytest.mark.parametrize("case", ["C1", "C2"])
+def test_rfu_parity(case):
+    st, orc, dpos, N = _setup(case)
+    U = random_vec(6 * N, 1)
+    F = st.rfu(dpos, _t(U)).cpu().numpy()
+    Fo = orc.rfu(U.astype(np.float32))
+    e = rel_err(F, Fo)
+    print("RFU rel err", e)
+    assert e < 1e-5
+
+
+@pytest.mark.parametrize("case", ["C1", "C2"])
+def test_rfe_rsu_rse_parity(case):
+    st, orc, dpos, N = _setup(case)
+    F = st.rfe(dpos, 1.0).cpu().numpy()
+    assert rel_err(F, orc.rfe(1.0)) < 1e-5
+    U = random_vec(6 * N, 2).astype(np.float32)
+    S = st.rsu(dpos, _t(U)).cpu().numpy()
+    assert rel_err(S, orc.rsu(U)) < 1e-5
+    S = st.rse(dpos, 1.0).cpu().numpy()
+    assert rel_err(S, orc.rse(1.0)) < 1e-5
+
+
+@pytest.mark.parametrize("case", ["C1", "C2"])
+@pytest.mark.parametrize("parts", [1, 2, 3])
+def test_mobility_parity(case, parts):
+    st, orc, dpos, N = _setup(case)
+    gF = random_vec(11 * N, 3).astype(np.float32)
+    gU = st.mobility(dpos, _t(gF), parts).cpu().numpy()
+    gUo = orc.mobility(gF, parts)
+    e = rel_err(gU, gUo)
+    print("mobility parts", parts, "rel err", e, "l2", rel_err2(gU, gUo))
+    assert e < 1e-5
+
+
+@pytest.mark.parametrize("case", ["C1", "C2"])
+def test_precond_pattern_bitexact(case):
+    st, orc, dpos, N = _setup(case)
+    info = st.build_precond(dpos)
+    g = st.get_precond()
+    orc.build_precond()
+    assert np.array_equal(g["pn"], orc.pn)
+    assert np.array_equal(g["plist"], orc.plist)
+    assert np.array_equal(g["perm"], orc.perm)
+    assert np.array_equal(g["rowptr"], orc.A_rowptr)
+    assert np.array_equal(g["col"], orc.A_col)
+    assert np.array_equal(g["prow"], orc.B_rowptr)
+    assert np.array_equal(g["pcol"], orc.B_col)
+    assert info["nnz"] == len(orc.A_col)
+    assert info["ichol_relaxer"] == orc.lam
+    ev = rel_err(g["val"], orc.A_val)
+    eL = rel_err(g["L"], orc.L_val)
+    print("A values rel err", ev, "L values rel err", eL, "lambda", orc.lam)
+    assert ev < 1e-5
+    assert eL < 1e-4
+
+
+@pytest.mark.parametrize("case", ["C1", "C2"])
+def test_precond_apply_parity(case):
+    st, orc, dpos, N = _setup(case)
+    st.build_precond(dpos)
+    orc.build_precond()
+    x = random_vec(6 * N, 4).astype(np.float32)
+    y = st.precond_apply(_t(x)).cpu().numpy()
+    yo = orc.precond_apply(x)
+    e = rel_err(y, yo)
+    print("G v rel err", e)
+    assert e < 1e-4
+
+
+@pytest.mark.parametrize("case", ["C1", "C2"])
+def test_saddle_operators_parity(case):
+    st, orc, dpos, N = _setup(case)
+    st.build_precond(dpos)
+    orc.build_precond()
+    x = random_vec(17 * N, 5).astype(np.float32)
+    y = st.saddle_multiply(dpos, _t(x)).cpu().numpy()
+    yo = orc.saddle_multiply(x)
+    e1 = rel_err(y, yo)
+    z = st.saddle_precond(_t(x)).cpu().numpy()
+    zo = orc.saddle_precond(x)
+    e2 = rel_err(z, zo)
+    print("saddle multiply rel err", e1, "saddle precond rel err", e2)
+    assert e1 < 1e-5
+    assert e2 < 1e-4
+
+
+def test_solve_C1_velocities():
+    """one C1 velocity solve (GMRES tol 1e-6) against the oracle solved to 1e-11"""
+    st, orc, dpos, N = _setup("C1", gmres_tol=1e-6)
+    st.build_precond(dpos)
+    orc.build_precond()
+    F = forces(N, 21)
+    b = orc.rhs(F, 0.0)
+    x, info = st.saddle_solve(dpos, _t(b))
+    xo, it, res = orc.gmres(b, tol=1e-11)
+    U, Uo = x.cpu().numpy()[11 * N:], xo[11 * N:]
+    e = rel_err2(U, Uo)
+    print("GMRES iters", info["iterations"], "true residual", info["true_residual"], "velocity rel err", e,
+          "oracle iters", it)
+    assert info["converged"]
+    assert info["true_residual"] < 1e-5
+    assert e < 1e-5
+
+
+def test_step_C1():
+    """fsd_step with injected forces: velocities equal the oracle's solve and
+    positions move by (U + shear) dt"""
+    import torch
+    from fsd_amd.stokes import ParticleState
+    st, orc, dpos, N = _setup("C1", gmres_tol=1e-6, dt=1e-3)
+    pos, L = packing(*CASES["C1"])
+    state = ParticleState(pos)
+    F = forces(N, 22)
+    info = st.integrateStepOne(0, state, force6=_t(F), shear_rate=0.0)
+    vel = state.velocity.cpu().numpy()
+    orc.build_precond()
+    b = orc.rhs(F, 0.0)
+    xo, it, res = orc.gmres(b, tol=1e-11)
+    e = rel_err2(vel[:6 * N], xo[11 * N:])
+    print("step iterations", info["iterations"], "velocity rel err", e)
+    assert e < 1e-5
+    newpos = state.positions()
+    disp = newpos - pos
+    disp -= L * np.rint(disp / L)
+    expect = vel[:6 * N].reshape(N, 6)[:, :3] * 1e-3
+    assert np.abs(disp - expect).max() < 1e-5
+
+
+def test_force_law_parity():
+    st, orc, dpos, N = _setup("C1")
+    F = st.force_law(dpos, T_ext=0.3).cpu().numpy()
+    Fo = orc.force_law(T_ext=0.3)
+    assert rel_err(F, Fo) < 1e-5
