@@ -17,6 +17,10 @@ FSD_E_ICHOL = -4
 FSD_E_STATE = -5
 FSD_W_NOT_CONVERGED = 1
 
+# kernel classes timed by fsd_profile_* (bit k of the enable mask)
+PROF_NAMES = ["spread", "green", "contract", "fft", "realspace", "rfu", "trsv", "ichol", "gmres_vec", "nlist",
+              "precond_build"]
+
 
 class FsdParams(C.Structure):
     _fields_ = [
@@ -43,6 +47,7 @@ class FsdParams(C.Structure):
         ("rp", C.c_float),
         ("r_buff", C.c_float),
         ("parity", C.c_int32),
+        ("gmres_true_tol", C.c_float),
     ]
 
 
@@ -66,7 +71,10 @@ class FsdSolveStats(C.Structure):
         ("ichol_relaxer", C.c_float),
         ("ichol_attempts", C.c_int32),
         ("nnz", C.c_int64),
-        ("nlist_total", C.c_int32),
+        ("nlist_total", C.c_int64),
+        ("lub_pairs", C.c_int64),
+        ("lower_blocks", C.c_int64),
+        ("levels", C.c_int32),
     ]
 
     def as_dict(self):

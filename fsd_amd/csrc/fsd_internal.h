@@ -147,6 +147,7 @@ struct fsd_ctx {
     float4 *d_lub_b = nullptr;  // (YA12, YB12, XC12-YC12, YC12)
     float *d_lub_self = nullptr; // 36 per particle (row-major 6x6)
     bool lub_valid = false;
+    int64_t lub_total = 0;
 
     // PSE
     int M = 0, Mh = 0;
@@ -196,7 +197,7 @@ struct fsd_ctx {
     float *d_ori_zero = nullptr;
 
     // profiling
-    bool prof = false;
+    unsigned prof = 0;            // bit mask of timed kernel classes
     std::vector<ProfSlot> slots;
 };
 

@@ -42,6 +42,22 @@ __global__ void __launch_bounds__(1024) scan_excl_kernel(const int *in, int *out
     if (tid == 0) out[n] = carry;
 }
 
+// total of n ints into *out (single workgroup)
+__global__ void __launch_bounds__(1024) sum_int_kernel(const int *in, int n, int *out)
+{
+    __shared__ int part[16];
+    int s = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) s += in[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < 16; ++w) t += part[w];
+        *out = t;
+    }
+}
+
 int scan_excl(fsd_ctx *ctx, const int *in, int *out, int n)
 {
     hipLaunchKernelGGL(scan_excl_kernel, dim3(1), dim3(1024), 0, ctx->stream, in, out, n);
@@ -153,7 +169,7 @@ int k_build_nlist(fsd_ctx *ctx, const float4 *pos)
     hipEvent_t ev;
     prof_begin(ctx, PROF_NLIST, &ev);
     for (int attempt = 0; attempt < 2; ++attempt) {
-        FSD_HIP(hipMemsetAsync(ctx->d_flags, 0, 4 * sizeof(int), ctx->stream));
+        FSD_HIP(hipMemsetAsync(ctx->d_flags, 0, 8 * sizeof(int), ctx->stream));
         if (ctx->use_cells) {
             FSD_HIP(hipMemsetAsync(ctx->d_cell_count, 0, sizeof(int) * ctx->ncells, ctx->stream));
             hipLaunchKernelGGL(cell_assign_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, pos, N, ctx->box,
@@ -172,9 +188,11 @@ int k_build_nlist(fsd_ctx *ctx, const float4 *pos)
                                rl2, ctx->maxn, ctx->d_nneigh, ctx->d_nlist, ctx->d_flags);
         }
         FSD_HIP(hipGetLastError());
-        FSD_HIP(hipMemcpyAsync(ctx->h_flags, ctx->d_flags, 4 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        hipLaunchKernelGGL(sum_int_kernel, dim3(1), dim3(1024), 0, ctx->stream, ctx->d_nneigh, N, ctx->d_flags + 4);
+        FSD_HIP(hipMemcpyAsync(ctx->h_flags, ctx->d_flags, 8 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         FSD_HIP(hipStreamSynchronize(ctx->stream));
         int need = ctx->h_flags[0];
+        ctx->nlist_total = ctx->h_flags[4];
         if (need == 0) break;
         if (attempt == 1) { ctx->err = "neighbour list overflow after regrow"; return FSD_E_CAPACITY; }
         // grow and rebuild
@@ -255,8 +273,10 @@ int k_build_lub_pairs(fsd_ctx *ctx, const float4 *pos)
                            ctx->d_nneigh, ctx->d_nlist, ctx->maxn, t, rl2, ctx->maxlub, ctx->d_lub_n, ctx->d_lub_j,
                            ctx->d_lub_a, ctx->d_lub_b, ctx->d_lub_self, ctx->d_flags + 1);
         FSD_HIP(hipGetLastError());
-        FSD_HIP(hipMemcpyAsync(ctx->h_flags + 1, ctx->d_flags + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        hipLaunchKernelGGL(sum_int_kernel, dim3(1), dim3(1024), 0, ctx->stream, ctx->d_lub_n, N, ctx->d_flags + 5);
+        FSD_HIP(hipMemcpyAsync(ctx->h_flags + 1, ctx->d_flags + 1, 5 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         FSD_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->lub_total = ctx->h_flags[5];
         int need = ctx->h_flags[1];
         if (need == 0) break;
         if (attempt == 1) { ctx->err = "lubrication pair capacity exceeded"; return FSD_E_CAPACITY; }

@@ -23,14 +23,14 @@ static const char *kProfNames[PROF_COUNT] = {"spread", "green", "contract", "fft
 void prof_begin(fsd_ctx *ctx, int slot, hipEvent_t *ev)
 {
     *ev = nullptr;
-    if (!ctx->prof) return;
+    if (!(ctx->prof & (1u << slot))) return;
     hipEventCreate(ev);
     hipEventRecord(*ev, ctx->stream);
 }
 
 void prof_end(fsd_ctx *ctx, int slot, hipEvent_t ev)
 {
-    if (!ctx->prof || !ev) return;
+    if (!(ctx->prof & (1u << slot)) || !ev) return;
     hipEvent_t e2;
     hipEventCreate(&e2);
     hipEventRecord(e2, ctx->stream);
@@ -441,8 +441,36 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
 // estimate |s_{i+1}| <= tol ||b||.  Arnoldi vectors in FP32 on the device,
 // classical Gram-Schmidt with one reorthogonalisation and FP64 dot products;
 // Hessenberg, Givens rotations and the small triangular solve in FP64 on the host.
+static int true_residual(fsd_ctx *ctx, const float4 *pos, const float *b, const float *x, float *out);
+
+static int gmres_cusp(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
+                      fsd_solve_stats *st);
+
+// CUSP-semantics GMRES; with gmres_true_tol > 0 the solve is repeated (warm
+// started, tolerance lowered by the observed ratio) until the true relative
+// residual ||b - A x|| / ||b|| also meets gmres_true_tol.
 static int gmres(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
                  fsd_solve_stats *st)
+{
+    const float ttol = ctx->p.gmres_true_tol;
+    int rc = gmres_cusp(ctx, pos, b, x, tol, maxit, R, st);
+    if (rc < 0 || ttol <= 0.f) return rc;
+    int total = st ? st->iterations : 0, restarts = st ? st->restarts : 0;
+    float tres = 0.f;
+    for (int round = 0; round < 6; ++round) {
+        RC(true_residual(ctx, pos, b, x, &tres));
+        if (tres <= ttol || total >= maxit) break;
+        tol = tol * std::max(0.05f, 0.5f * ttol / tres);
+        rc = gmres_cusp(ctx, pos, b, x, tol, maxit - total, R, st);
+        if (rc < 0) return rc;
+        if (st) { total += st->iterations; restarts += st->restarts; }
+    }
+    if (st) { st->iterations = total; st->restarts = restarts; st->true_residual = tres; }
+    return (tres <= ttol) ? 0 : FSD_W_NOT_CONVERGED;
+}
+
+static int gmres_cusp(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
+                      fsd_solve_stats *st)
 {
     const int64_t n = 17 * (int64_t)ctx->N;
     float *V = ctx->d_V, *w = ctx->d_w, *t = ctx->d_t;
@@ -882,6 +910,10 @@ int fsd_saddle_solve(fsd_ctx *ctx, const float *d_pos4, const float *d_rhs17, fl
     st.ichol_relaxer = ctx->ichol_relaxer;
     st.ichol_attempts = ctx->ichol_attempts;
     st.nnz = ctx->nnz_ref;
+    st.nlist_total = ctx->nlist_total;
+    st.lub_pairs = ctx->lub_total;
+    st.lower_blocks = ctx->nlow;
+    st.levels = ctx->nlev_f;
     if (stats) *stats = st;
     CK(hipStreamSynchronize(ctx->stream));
     return rc;
@@ -953,6 +985,10 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     st.ichol_relaxer = ctx->ichol_relaxer;
     st.ichol_attempts = ctx->ichol_attempts;
     st.nnz = ctx->nnz_ref;
+    st.nlist_total = ctx->nlist_total;
+    st.lub_pairs = ctx->lub_total;
+    st.lower_blocks = ctx->nlow;
+    st.levels = ctx->nlev_f;
     // relaxer bookkeeping (Stokes.cc:1666)
     ctx->ichol_relaxer = ((timestep % 5 == 0) || (ctx->ichol_relaxer > 1024.0f)) ? 1.0f : ctx->ichol_relaxer;
     if (stats) *stats = st;
@@ -961,7 +997,7 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
 
 int fsd_profile_enable(fsd_ctx *ctx, int32_t enable)
 {
-    ctx->prof = enable != 0;
+    ctx->prof = (unsigned)enable;
     return 0;
 }
 

@@ -65,6 +65,7 @@ typedef struct fsd_params {
     float rp;                        /* <= 0: 2.1 */
     float r_buff;                    /* < 0: 0.4 */
     int32_t parity;                  /* 1: reproduce the reference's k-vector constant (Helper_Mobility.cu:389-391) */
+    float gmres_true_tol;            /* > 0: also require ||b - A x|| / ||b|| <= this (extra mat-vecs) */
 } fsd_params;
 
 /* Derived PSE parameters (Stokes.cc:131-261). */
@@ -84,7 +85,10 @@ typedef struct fsd_solve_stats {
     float ichol_relaxer;             /* lambda actually used by IC(0) */
     int32_t ichol_attempts;
     int64_t nnz;                     /* preconditioner nonzeros (reference CSR count) */
-    int32_t nlist_total;             /* entries in the neighbour list */
+    int64_t nlist_total;             /* entries in the neighbour list */
+    int64_t lub_pairs;               /* lubrication pair records (r < rlub) */
+    int64_t lower_blocks;            /* 6x6 blocks of the IC(0) factor below the diagonal */
+    int32_t levels;                  /* level-schedule depth of the triangular solves */
 } fsd_solve_stats;
 
 /* ---- lifecycle: Stokes::Stokes + setParams + setSparseMath + AllocateWorkSpaces
@@ -173,7 +177,8 @@ int fsd_euler_shear(fsd_ctx *ctx, float *d_pos4, float *d_ori3, float *d_pos_gb4
                     float dt, float shear_rate);
 
 /* ---- timing of the hot kernels (HIP events on the context's stream) ---- */
-/* enable: record an event pair around every launch of the named kernel classes */
+/* enable: bit mask of kernel classes to time (bit k = k-th name below, -1 = all, 0 = off);
+ * an event pair is recorded around every launch of a selected class */
 int fsd_profile_enable(fsd_ctx *ctx, int32_t enable);
 /* name: "spread", "green", "contract", "fft", "realspace", "rfu", "trsv", "ichol",
  * "gmres_vec", "nlist", "precond_build"; returns launches and total milliseconds */
