@@ -44,14 +44,15 @@ def algorithmic_bytes(cls, N, info, pse, st):
         return 4 * st["nlist_total"] + (16 + 48 + 44 + 24 + 44) * N
     if cls == "rfu":         # per pair: index + 2 float4; per particle: self block, U, x_F, out
         return 36 * st["lub_pairs"] + (144 + 24 + 24 + 24) * N
-    if cls == "trsv":        # one level launch ~ 1/levels of a sweep; a sweep reads all L blocks and 2 RHS
-        sweep = 144 * (st["lower_blocks"] + N) + 2 * (24 + 24 + 24) * N + 4 * N
-        return sweep / max(st["levels"], 1)
-    if cls == "ichol":
-        sweep = 144 * 2 * (st["lower_blocks"] + N) + 144 * (st["lower_blocks"] + N)
-        return sweep / max(st["levels"], 1)
-    if cls == "gmres_vec":   # CGS2 multidot + axpy on 17N vectors, ~26 basis vectors on average
-        return 4 * 17 * N * 26 * 4
+    if cls == "trsv":        # one application G b for 2 right-hand sides: both sweeps read every packed
+        # block (144 B) and its dependency index (4 B); per particle two inverse diagonal blocks (288 B), three
+        # index words (12 B), b in (48 B), y out and back in (96 B), x out (48 B), result out (48 B)
+        return (144 + 4) * 2 * st["lower_blocks"] + (288 + 12 + 48 + 96 + 48 + 48) * N
+    if cls == "ichol":       # one factorisation: A blocks in, L blocks out (lower and diagonal)
+        return 144 * 2 * (st["lower_blocks"] + N)
+    if cls == "gmres_vec":   # CGS2 on 17N-float vectors with k basis vectors: (4k + 7) vector passes,
+        k = 25.5             # k averaged over a restart cycle of 50
+        return 4 * 17 * N * (4 * k + 7)
     return None
 
 

@@ -18,9 +18,12 @@
 #include <string>
 #include <vector>
 #include "fsd.h"
+#include "sweep_plan.h"
 
 #define FSD_MAXP 14        // pruned neighbours incl. self (kissing number 12 + self + 1)
 #define FSD_PSTRIDE 16     // stride of the pruned list
+#define SW_THREADS 1024    // threads of the single-workgroup triangular sweeps
+#define SW_TAB_MAX 16384   // ints of sweep schedule held in LDS (64 KiB)
 
 struct BoxF {
     float Lx, Ly, Lz, xy, xz, yz;
@@ -187,6 +190,21 @@ struct fsd_ctx {
     int *d_lev_f = nullptr, *d_lev_b = nullptr;
     float *d_tmp6a = nullptr, *d_tmp6b = nullptr;
     float *d_g12 = nullptr;       // G b_F and G b_U of the saddle preconditioner
+    // single-workgroup sweeps: schedule table (levels, jagged-diagonal records) and packed operands
+    struct SweepOffsets { int nlev, levrow, levjd, rec; } sw_f{}, sw_b{};
+    SweepPlan plan_f, plan_b;
+    std::vector<int> h_sw_tab;
+    bool sw_ok = false;
+    int sw_ntab = 0;
+    int *d_sw_tab = nullptr;
+    int *d_sw_fcol = nullptr, *d_sw_bcol = nullptr, *d_sw_fsrc = nullptr, *d_sw_bsrc = nullptr;  // nlow_cap
+    float *d_sw_fblk = nullptr, *d_sw_bblk = nullptr;                                            // 36 nlow_cap
+    float *d_sw_fdinv = nullptr, *d_sw_bdinv = nullptr;                                          // 36 N
+    int *d_sw_forder = nullptr, *d_sw_border = nullptr;                                          // N
+    int *d_sw_fin = nullptr;      // forward position -> particle (input gather)
+    int *d_sw_fyb = nullptr;      // forward position -> backward position (copy of y)
+    int *d_sw_bout = nullptr;     // backward position -> particle (output scatter)
+    float *d_sw_work = nullptr;   // 4 x 12N: gathered input, y (forward order), y (backward order), x
 
     // saddle / GMRES
     float *d_V = nullptr, *d_w = nullptr, *d_t = nullptr, *d_t2 = nullptr;
@@ -232,6 +250,7 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
 int k_prune(fsd_ctx *ctx, const float4 *pos);
 int k_build_blocks(fsd_ctx *ctx, const float4 *pos);
 int k_ichol(fsd_ctx *ctx, float lambda, int *failed);
+int k_sweep_pack(fsd_ctx *ctx);
 int k_precond_apply2(fsd_ctx *ctx, const float *b0, const float *b1, float *y0, float *y1, int n_rhs);
 int k_export_csr(fsd_ctx *ctx, const float4 *pos, int32_t *h_rowptr, int32_t *h_col, float *h_val,
                  int32_t *h_prow, int32_t *h_pcol, float *h_L, int64_t cap);

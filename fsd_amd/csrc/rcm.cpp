@@ -1,10 +1,13 @@
 // Reverse Cuthill-McKee ordering of the pruned particle graph, following
 // George & Liu's SPARSPAK GENRCM (the routine the reference calls at
 // Precondition.cu:1009; rcm.cpp:870-1120, 2211-2262, 2340-2477, 3152-3574).
-// The traversal order and tie-breaking (stable insertion sort by degree,
-// degrees counted with the self loop the pruned list carries, pseudo-
-// peripheral root from the last BFS level) are kept so the permutation is the
-// reference's.  Indices are 0-based here.
+// The traversal order and tie-breaking are kept so the permutation is the
+// reference's: degrees counted with the self loop the pruned list carries,
+// pseudo-peripheral root from the last BFS level, and the children of a node
+// ordered as rcm.cpp:3340-3362 orders them -- its insertion sort never
+// compares the first new child (loop `while ( fnbr < l )`), so that child
+// stays first and only the remaining children are sorted (stably) by
+// increasing degree.  Indices are 0-based here.
 #include <vector>
 #include <algorithm>
 
@@ -110,7 +113,8 @@ void host_rcm(int N, const int *phead, const int *plist, int *perm)
                 const int w = g.adj[e];
                 if (mask[w]) { mask[w] = 0; out[n_out++] = w; }
             }
-            std::stable_sort(out + first, out + n_out, [&](int a, int b) { return deg[a] < deg[b]; });
+            if (n_out - first > 2)
+                std::stable_sort(out + first + 1, out + n_out, [&](int a, int b) { return deg[a] < deg[b]; });
         }
         std::reverse(out, out + n_out);
         placed += n_out;

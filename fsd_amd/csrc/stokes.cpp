@@ -4,6 +4,7 @@
 // FP64 Hessenberg/Givens on the host) and the step sequencer
 // (Stokes::integrateStepOne -> Stokes_StepOne), exported through fsd.h.
 #include "fsd_internal.h"
+#include "sweep_plan.h"
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -243,6 +244,18 @@ static int alloc_all(fsd_ctx *ctx)
     CK(hipMalloc(&ctx->d_Ldiag, sizeof(float) * 36 * (size_t)N));
     CK(hipMalloc(&ctx->d_lev_f, sizeof(int) * N));
     CK(hipMalloc(&ctx->d_lev_b, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_sw_tab, sizeof(int) * SW_TAB_MAX));
+    CK(hipMalloc(&ctx->d_sw_fcol, sizeof(int) * ctx->nlow_cap));
+    CK(hipMalloc(&ctx->d_sw_bcol, sizeof(int) * ctx->nlow_cap));
+    CK(hipMalloc(&ctx->d_sw_fsrc, sizeof(int) * ctx->nlow_cap));
+    CK(hipMalloc(&ctx->d_sw_bsrc, sizeof(int) * ctx->nlow_cap));
+    CK(hipMalloc(&ctx->d_sw_fblk, sizeof(float) * 36 * ctx->nlow_cap));
+    CK(hipMalloc(&ctx->d_sw_bblk, sizeof(float) * 36 * ctx->nlow_cap));
+    CK(hipMalloc(&ctx->d_sw_fdinv, sizeof(float) * 36 * (size_t)N));
+    CK(hipMalloc(&ctx->d_sw_bdinv, sizeof(float) * 36 * (size_t)N));
+    for (int **pp : {&ctx->d_sw_forder, &ctx->d_sw_border, &ctx->d_sw_fin, &ctx->d_sw_fyb, &ctx->d_sw_bout})
+        CK(hipMalloc(pp, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_sw_work, sizeof(float) * 48 * (size_t)N));
     CK(hipMalloc(&ctx->d_tmp6a, sizeof(float) * 12 * (size_t)N));
     CK(hipMalloc(&ctx->d_tmp6b, sizeof(float) * 12 * (size_t)N));
     CK(hipMalloc(&ctx->d_g12, sizeof(float) * 12 * (size_t)N));
@@ -283,7 +296,9 @@ static void free_all(fsd_ctx *ctx)
                     ctx->d_perm, ctx->d_iperm, ctx->d_lrow, ctx->d_urow, ctx->d_lcol, ctx->d_ucol, ctx->d_upos,
                     ctx->d_Ablk, ctx->d_Lblk, ctx->d_Adiag, ctx->d_Ldiag, ctx->d_lev_f, ctx->d_lev_b, ctx->d_tmp6a,
                     ctx->d_tmp6b, ctx->d_g12, ctx->d_V, ctx->d_w, ctx->d_t, ctx->d_t2, ctx->d_part, ctx->d_rhs, ctx->d_sol,
-                    ctx->d_force, ctx->d_mask1, ctx->d_mask2};
+                    ctx->d_force, ctx->d_mask1, ctx->d_mask2, ctx->d_sw_tab, ctx->d_sw_fcol, ctx->d_sw_bcol,
+                    ctx->d_sw_fsrc, ctx->d_sw_bsrc, ctx->d_sw_fblk, ctx->d_sw_bblk, ctx->d_sw_fdinv, ctx->d_sw_bdinv,
+                    ctx->d_sw_forder, ctx->d_sw_border, ctx->d_sw_fin, ctx->d_sw_fyb, ctx->d_sw_bout, ctx->d_sw_work};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->h_flags) hipHostFree(ctx->h_flags);
@@ -301,12 +316,16 @@ static int grow_lower(fsd_ctx *ctx, int64_t need)
 {
     if (need <= ctx->nlow_cap) return 0;
     const int64_t cap = need + need / 4 + 64;
-    hipFree(ctx->d_lcol); hipFree(ctx->d_ucol); hipFree(ctx->d_upos); hipFree(ctx->d_Ablk); hipFree(ctx->d_Lblk);
-    CK(hipMalloc(&ctx->d_lcol, sizeof(int) * cap));
-    CK(hipMalloc(&ctx->d_ucol, sizeof(int) * cap));
-    CK(hipMalloc(&ctx->d_upos, sizeof(int) * cap));
-    CK(hipMalloc(&ctx->d_Ablk, sizeof(float) * 36 * cap));
-    CK(hipMalloc(&ctx->d_Lblk, sizeof(float) * 36 * cap));
+    CK(hipStreamSynchronize(ctx->stream));
+    for (int *q : {ctx->d_lcol, ctx->d_ucol, ctx->d_upos, ctx->d_sw_fcol, ctx->d_sw_bcol, ctx->d_sw_fsrc,
+                   ctx->d_sw_bsrc})
+        hipFree(q);
+    for (float *q : {ctx->d_Ablk, ctx->d_Lblk, ctx->d_sw_fblk, ctx->d_sw_bblk}) hipFree(q);
+    for (int **pp : {&ctx->d_lcol, &ctx->d_ucol, &ctx->d_upos, &ctx->d_sw_fcol, &ctx->d_sw_bcol, &ctx->d_sw_fsrc,
+                     &ctx->d_sw_bsrc})
+        CK(hipMalloc(pp, sizeof(int) * cap));
+    for (float **pp : {&ctx->d_Ablk, &ctx->d_Lblk, &ctx->d_sw_fblk, &ctx->d_sw_bblk})
+        CK(hipMalloc(pp, sizeof(float) * 36 * cap));
     ctx->nlow_cap = cap;
     return 0;
 }
@@ -406,8 +425,54 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
         CK(hipMemcpyAsync(ctx->d_lev_f, itf.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
         CK(hipMemcpyAsync(ctx->d_lev_b, itb.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
+        // schedules of the single-workgroup sweeps
+        build_sweep_plan(N, lf.data(), ctx->nlev_f, lrow.data(), lcol.data(), nullptr, ctx->plan_f);
+        build_sweep_plan(N, lb.data(), ctx->nlev_b, urow.data(), ucol.data(), upos.data(), ctx->plan_b);
     }
     RC(grow_lower(ctx, ctx->nlow + 1));
+    {
+        const SweepPlan &f = ctx->plan_f, &b = ctx->plan_b;
+        std::vector<int> &tab = ctx->h_sw_tab;
+        tab.clear();
+        auto put = [&tab](const SweepPlan &pl, fsd_ctx::SweepOffsets &o) {
+            o.nlev = pl.nlev;
+            o.levrow = (int)tab.size();
+            tab.insert(tab.end(), pl.levrow.begin(), pl.levrow.end());
+            o.levjd = (int)tab.size();
+            tab.insert(tab.end(), pl.levjd.begin(), pl.levjd.end());
+            o.rec = (int)tab.size();
+            tab.insert(tab.end(), pl.rec.begin(), pl.rec.end());
+        };
+        put(f, ctx->sw_f);
+        put(b, ctx->sw_b);
+        ctx->sw_ntab = (int)tab.size();
+        // LDS holds the schedule; the kernel addresses its operands with 32-bit byte offsets
+        ctx->sw_ok = ctx->sw_ntab <= SW_TAB_MAX && ctx->nlow < (int64_t)29000000 && N < 80000000;
+        if (ctx->sw_ok) {
+            std::vector<int> fin(N), fyb(N), bout(N);
+            for (int k = 0; k < N; ++k) {
+                fin[k] = ctx->h_perm[f.order[k]];
+                fyb[k] = b.pos[f.order[k]];
+                bout[k] = ctx->h_perm[b.order[k]];
+            }
+            auto up = [ctx](int *d, const int *h, size_t n) {
+                return hipMemcpyAsync(d, h, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream);
+            };
+            CK(up(ctx->d_sw_tab, tab.data(), tab.size()));
+            CK(up(ctx->d_sw_forder, f.order.data(), N));
+            CK(up(ctx->d_sw_border, b.order.data(), N));
+            CK(up(ctx->d_sw_fin, fin.data(), N));
+            CK(up(ctx->d_sw_fyb, fyb.data(), N));
+            CK(up(ctx->d_sw_bout, bout.data(), N));
+            if (ctx->nlow) {
+                CK(up(ctx->d_sw_fcol, f.ecol.data(), ctx->nlow));
+                CK(up(ctx->d_sw_fsrc, f.esrc.data(), ctx->nlow));
+                CK(up(ctx->d_sw_bcol, b.ecol.data(), ctx->nlow));
+                CK(up(ctx->d_sw_bsrc, b.esrc.data(), ctx->nlow));
+            }
+            CK(hipStreamSynchronize(ctx->stream));   // host vectors above go out of scope
+        }
+    }
     CK(hipMemcpyAsync(ctx->d_perm, ctx->h_perm.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
     CK(hipMemcpyAsync(ctx->d_iperm, iperm.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
     CK(hipMemcpyAsync(ctx->d_lrow, lrow.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
@@ -430,6 +495,7 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
         if (lambda > 1.0e7f) { ctx->err = "IC(0) did not succeed for any relaxer"; return FSD_E_ICHOL; }
     }
     ctx->ichol_relaxer = lambda;
+    if (ctx->sw_ok) RC(k_sweep_pack(ctx));
     prof_end(ctx, PROF_PRECOND_BUILD, ev);
     ctx->precond_valid = true;
     return 0;

@@ -860,10 +860,12 @@ static int rcm_component(int root, const int *xadj, const int *adj, int *mask, i
                 if (mask[nb]) { mask[nb] = 0; perm[lnbr++] = nb; }
             }
             if (lnbr - fnbr <= 1) continue;
-            /* stable insertion sort of perm[fnbr..lnbr) by increasing degree */
+            /* insertion sort by increasing degree as rcm.cpp:3340-3362 runs it: its
+             * loop `while ( fnbr < l )` never compares the first new neighbour, so
+             * perm[fnbr] stays first and only perm[fnbr+1..lnbr) is sorted (stable) */
             for (int k = fnbr + 1; k < lnbr; ++k) {
                 int nb = perm[k], l = k - 1;
-                while (l >= fnbr && deg[perm[l]] > deg[nb]) { perm[l + 1] = perm[l]; --l; }
+                while (l > fnbr && deg[perm[l]] > deg[nb]) { perm[l + 1] = perm[l]; --l; }
                 perm[l + 1] = nb;
             }
         }

@@ -136,8 +136,11 @@ def test_saddle_operators_parity(case):
 
 
 def test_solve_C1_velocities():
-    """one C1 velocity solve (GMRES tol 1e-6) against the oracle solved to 1e-11"""
-    st, orc, dpos, N = _setup("C1", gmres_tol=1e-6)
+    """one C1 velocity solve against the oracle solved to 1e-11: GMRES with the
+    reference's stopping rule at tol 1e-6, continued until the true relative
+    residual is below 1e-5 (the rule bounds the preconditioned residual only;
+    FP32 vectors put the floor of the true residual near 4e-6)"""
+    st, orc, dpos, N = _setup("C1", gmres_tol=1e-6, gmres_true_tol=1e-5)
     st.build_precond(dpos)
     orc.build_precond()
     F = forces(N, 21)

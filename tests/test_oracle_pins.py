@@ -1,0 +1,216 @@
+"""CPU pins of the FP64 oracle: closed forms, invariants, the reference's own
+resistance table and RCM, and golden vectors.  Each test is chosen so that a
+plausible mistake in the oracle (dropped term, wrong sign or index, transposed
+operand) fails it."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from _cases import packing
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def _oracle(*a, **kw):
+    from oracle.oracle import Oracle
+    return Oracle(*a, **kw)
+
+
+def _grand(o, N, cols, parts=3):
+    """columns `cols` of the 11N x 11N grand mobility"""
+    out = np.zeros((11 * N, len(cols)))
+    for k, c in enumerate(cols):
+        g = np.zeros(11 * N)
+        g[c] = 1.0
+        out[:, k] = o.mobility(g, parts)
+    return out
+
+
+# ------------------------------------------------------------ resistance table
+def test_resistance_table_anchors():
+    """Stokes_ResistanceTable.cc: 1000 rows, h = r - 2 log-spaced from 1e-4 with
+    step 0.004305 in log10; the anchors the lookup's clamps rely on"""
+    from fsd_amd.tables import load_resistance_table, CONTACT_ROW
+    dist, vals = load_resistance_table()
+    v = vals.reshape(1000, 22)
+    assert dist.shape == (1000,) and vals.shape == (22000,)
+    assert dist[0] == np.float32(2.0001)
+    assert v[0, 0] == np.float32(2500.839791) and v[0, 1] == np.float32(-2500.837257)
+    assert CONTACT_ROW == 232 and dist[232] == np.float32(2.000997)
+    assert dist[999] == np.float32(4.0) and np.all(v[999] == 0.0)
+    # log10 spacing whose rounded step 0.004305 is the lookup's divisor
+    h = dist[:999].astype(np.float64) - 2.0
+    assert np.all(np.diff(h) > 0)
+    assert abs(np.log10(h[998] / h[0]) / 998 - 0.004305) < 1e-6
+    # pair and self squeeze terms diverge together: XA11 + XA12 stays O(1) at contact
+    assert abs(v[0, 0] + v[0, 1]) < 0.01 and v[0, 0] > 2000
+
+
+def test_rfu_two_sphere_table_values():
+    """R_FU of two spheres along x at a table node returns the table's XA/YA/XC/YC
+    entries (Lubrication.cu:Lubrication_RFU_kernel); along y the roles swap"""
+    from fsd_amd.tables import load_resistance_table
+    dist, vals = load_resistance_table()
+    v = vals.reshape(1000, 22)
+    row = 500
+    r = float(dist[row])
+    L = 30.0
+    for axis in (0, 1):
+        pos = np.zeros((2, 3))
+        pos[1, axis] = r
+        o = _oracle(pos, L)
+        for comp, (c11, c12) in {axis: (0, 1), (axis + 1) % 3: (2, 3), 3 + axis: (6, 7),
+                                 3 + (axis + 1) % 3: (8, 9)}.items():
+            U = np.zeros(12)
+            U[comp] = 1.0
+            F = o.rfu(U)
+            assert F[comp] == pytest.approx(v[row, c11], rel=2e-3)
+            assert F[6 + comp] == pytest.approx(v[row, c12], rel=2e-3)
+
+
+def test_rfu_symmetric():
+    """R_FU is symmetric: x . R y == y . R x on a C1 packing"""
+    pos, L = packing(343, 0.2, 11)
+    o = _oracle(pos, L)
+    rng = np.random.default_rng(0)
+    x, y = rng.standard_normal((2, 6 * 343))
+    a, b = x @ o.rfu(y), y @ o.rfu(x)
+    assert abs(a - b) <= 1e-10 * abs(a)
+
+
+# ------------------------------------------------------------ mobility (PSE)
+@pytest.mark.parametrize("L", [20.0, 30.0])
+def test_hasimoto_self_mobility(L):
+    """one sphere in a periodic cube (the pi typo off): translational self-mobility
+    1 - 2.837297/L + 4.18879/L^3 (Hasimoto 1959; units 6 pi eta a = 1), rotational
+    0.75 (1 - (4 pi / 3)/L^3) (the periodic rotlet's -T/(6 eta L^3))"""
+    o = _oracle(np.zeros((1, 3)), L, error=1e-6, parity=False)
+    M = _grand(o, 1, [0, 3])
+    has = 1.0 - 2.837297 / L + 4.18879 / L ** 3
+    assert M[0, 0] == pytest.approx(has, rel=1e-6)
+    assert M[1, 0] == pytest.approx(0.0, abs=1e-9) and M[2, 0] == pytest.approx(0.0, abs=1e-9)
+    assert M[3, 1] == pytest.approx(0.75 * (1.0 - 4.18879 / L ** 3), rel=2e-6)
+
+
+def test_rpy_pair_limit():
+    """pair blocks in a large box: free-space RPY (overlapping form for r < 2)
+    plus the leading periodic shift -2.837297/L; the next correction is O(r^2/L^3)"""
+    L = 80.0
+    pos = np.array([[0.0, 0.0, 0.0], [1.8, 2.4, 0.0], [-0.9, 0.0, 1.2]])   # r = 3.0 and r = 1.5 from particle 0
+    o = _oracle(pos, L, error=1e-6, parity=False)
+    M = _grand(o, 3, [0, 1, 2])
+    for j in (1, 2):
+        rv = pos[j] - pos[0]
+        r = np.linalg.norm(rv)
+        e = rv / r
+        if r >= 2:
+            R = (3 / (4 * r) + 1 / (2 * r ** 3)) * np.eye(3) + (3 / (4 * r) - 3 / (2 * r ** 3)) * np.outer(e, e)
+        else:
+            R = (1 - 9 * r / 32) * np.eye(3) + (3 * r / 32) * np.outer(e, e)
+        blk = M[6 * j:6 * j + 3, :]
+        assert np.abs(blk - (R - 2.837297 / L * np.eye(3))).max() < 1e-4, (r, blk, R)
+
+
+def test_ewald_split_independence():
+    """the grand mobility does not depend on the Ewald splitting xi for pairs
+    inside every xi's real-space cutoff (pairs beyond a cutoff carry the
+    truncation error the reference's rcut rule allows)"""
+    pos = np.array([[0.0, 0.0, 0.0], [1.5, 2.0, 0.0], [0.0, -2.2, 1.0]])
+    L = 20.0
+    Ms = [_grand(_oracle(pos, L, xi=xi, error=1e-6, parity=False), 3, range(33)) for xi in (0.5, 0.6, 0.8)]
+    assert np.abs(Ms[1] - Ms[0]).max() < 5e-6
+    assert np.abs(Ms[2] - Ms[0]).max() < 5e-6
+
+
+def test_mobility_symmetric():
+    """the grand mobility is symmetric on a random dilute configuration"""
+    pos, L = packing(60, 0.05, 5)
+    o = _oracle(pos, L, error=1e-6, parity=False)
+    rng = np.random.default_rng(1)
+    x, y = rng.standard_normal((2, 11 * 60))
+    a, b = x @ o.mobility(y), y @ o.mobility(x)
+    assert abs(a - b) <= 1e-7 * abs(a)
+
+
+def test_ewald_table_matches_functions():
+    """the float table nodes r_k = (k+1) dr equal the closed-form functions
+    (fsd_ewald_functions) rounded to float (Stokes.cc: Ewald real-space table)"""
+    from oracle import oracle as om
+    xi = 0.5
+    nR = 2000
+    tab = om.ewald_table(xi, nR).reshape(nR, 8)
+    for k in (0, 499, 1999):
+        r = (k + 1) * 0.001
+        f = om.ewald_functions(r, xi)
+        assert tab[k, 0] == pytest.approx(f[0], rel=1e-6, abs=1e-7)
+        assert tab[k, 1] == pytest.approx(f[1], rel=1e-6, abs=1e-7)
+
+
+# ------------------------------------------------------------ preconditioner
+def test_ichol_reproduces_matrix_on_pattern():
+    """IC(0): (L L^T)_ij = B_ij + lambda * shift_ii on the stored lower pattern
+    (csric02 on the RCM-permuted R_FU^nf with the diagonal relaxer)"""
+    pos, L = packing(343, 0.2, 11)
+    o = _oracle(pos, L)
+    o.build_precond()
+    n = 6 * 343
+    rp, col, bval, lval = o.B_rowptr, o.B_col, o.B_val, o.L_val
+    Lm = np.zeros((n, n))
+    Bm = np.zeros((n, n))
+    for i in range(n):
+        for k in range(rp[i], rp[i + 1]):
+            j = col[k]
+            Bm[i, j] = bval[k]
+            if j <= i:
+                Lm[i, j] = lval[k]
+    shift = np.tile([1.0, 1.0, 1.0, 1.33333333, 1.33333333, 1.33333333], 343) * o.lam
+    LL = Lm @ Lm.T
+    mask = np.tril(Bm != 0)
+    target = Bm + np.diag(shift)
+    assert np.abs((LL - target)[mask]).max() < 1e-9 * np.abs(target).max()
+
+
+def _ref_rcm():
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_rcm.so")
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    lib.ref_genrcm.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    return lib
+
+
+def _oracle_rcm(phead, plist):
+    from oracle import oracle as om
+    N = len(phead) - 1
+    perm = np.zeros(N, dtype=np.int32)
+    om.load().orc_rcm(N, phead.ctypes.data_as(C.c_void_p), plist.ctypes.data_as(C.c_void_p),
+                      perm.ctypes.data_as(C.c_void_p))
+    return perm
+
+
+def test_rcm_golden_vectors():
+    """oracle RCM equals the reference GENRCM's permutations stored in
+    tests/golden/rcm_golden.npz (written by tests/golden/make_rcm_golden.py)"""
+    z = np.load(os.path.join(GOLDEN, "rcm_golden.npz"))
+    for name in z["names"]:
+        phead, plist, perm = z[name + "_phead"], z[name + "_plist"], z[name + "_perm"]
+        got = _oracle_rcm(phead.astype(np.int32), plist.astype(np.int32))
+        assert np.array_equal(got, perm), name
+
+
+def test_rcm_against_reference_build():
+    """oracle RCM equals the reference GENRCM (oracle/_ref) on pruned graphs of
+    packings, including isolated particles and several components"""
+    lib = _ref_rcm()
+    if lib is None:
+        pytest.skip("oracle/_ref/libref_rcm.so not built (reference tree not mounted)")
+    for N, phi, seed in [(343, 0.2, 11), (500, 0.05, 3), (1000, 0.35, 7)]:
+        pos, L = packing(N, phi, seed)
+        o = _oracle(pos, L)
+        pn, phead, plist, _ = o.prune()
+        ref = np.zeros(N, dtype=np.int32)
+        lib.ref_genrcm(N, int(phead[N]), phead.ctypes.data, plist.ctypes.data, ref.ctypes.data)
+        assert np.array_equal(_oracle_rcm(phead, plist), ref), (N, phi)
