@@ -75,6 +75,8 @@ class FsdSolveStats(C.Structure):
         ("lub_pairs", C.c_int64),
         ("lower_blocks", C.c_int64),
         ("levels", C.c_int32),
+        ("lanczos_ff", C.c_int32),
+        ("lanczos_nf", C.c_int32),
     ]
 
     def as_dict(self):
@@ -112,6 +114,10 @@ SIGNATURES = [
     ("fsd_saddle_solve", _I, [_P, _P, _P, _P, _I, C.POINTER(FsdSolveStats)]),
     ("fsd_force_law", _I, [_P, _P, _F, _P]),
     ("fsd_euler_shear", _I, [_P, _P, _P, _P, _P, _P, _P, _P, _F, _F]),
+    ("fsd_set_noise", _I, [_P, _P, _P, _P, _P]),
+    ("fsd_brownian_farfield", _I, [_P, _P, C.c_uint32, _P]),
+    ("fsd_brownian_nearfield", _I, [_P, _P, C.c_uint32, _P]),
+    ("fsd_rfd_drift", _I, [_P, _P, C.c_uint32, _P]),
     ("fsd_profile_enable", _I, [_P, _I]),
     ("fsd_profile_get", _I, [_P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     ("fsd_profile_reset", _I, [_P]),

@@ -24,6 +24,7 @@
 #define FSD_PSTRIDE 16     // stride of the pruned list
 #define SW_THREADS 1024    // threads of the single-workgroup triangular sweeps
 #define SW_TAB_MAX 16384   // ints of sweep schedule held in LDS (64 KiB)
+#define BRO_MMAX 100       // Lanczos basis bound of the Brownian samplers (m_max, Brownian_FarField.cu:613)
 
 struct BoxF {
     float Lx, Ly, Lz, xy, xz, yz;
@@ -162,7 +163,12 @@ struct fsd_ctx {
     void *d_fftwork = nullptr; size_t fftwork_size = 0;
     int tiles[3] = {0, 0, 0}; int ntiles = 0; int tile = 8;
     int *d_porig = nullptr;       // 3 ints per particle: stencil origin (unwrapped)
-    int *d_tile_count = nullptr, *d_tile_start = nullptr, *d_tile_list = nullptr, *d_tile_of = nullptr;
+    int *d_ocell = nullptr;       // N: grid cell of each particle's (wrapped) stencil origin
+    int *d_olist = nullptr;       // N: particles sorted by origin cell, ascending id inside a cell
+    int *d_ocount = nullptr;      // M: particles per origin cell (scratch)
+    int *d_ostart = nullptr;      // M+1: first position of each origin cell in d_olist
+    int *d_scan_tmp = nullptr;    // chunk totals of the multi-workgroup scan
+    int contract_lds = 0;         // LDS bytes of the tile contraction (0: per-particle contraction)
     float *d_wave = nullptr;      // 11 per particle: wave-space u (3) and gradient (8)
     float4 *d_rec = nullptr;      // 3 per particle: force + couplet records of the current apply
     bool spread_valid = false;
@@ -208,11 +214,24 @@ struct fsd_ctx {
 
     // saddle / GMRES
     float *d_V = nullptr, *d_w = nullptr, *d_t = nullptr, *d_t2 = nullptr;
+    float *d_Z = nullptr;         // M^-1 V of the right-preconditioned solve (allocated on first use)
     double *d_part = nullptr; int part_blocks = 0; double *h_part = nullptr;
     float *d_rhs = nullptr, *d_sol = nullptr;
     float *d_force = nullptr;
     float *d_mask1 = nullptr, *d_mask2 = nullptr;
     float *d_ori_zero = nullptr;
+
+    // Brownian motion (allocated on the first step with T > 0)
+    bool bro_alloc = false;
+    int m_lanczos_ff = 2, m_lanczos_nf = 2;   // Krylov sizes carried between steps (Stokes.cc:122-123)
+    float *d_lanV = nullptr;      // (BRO_MMAX + 1) x 11N Lanczos basis
+    float *d_lanw = nullptr, *d_lany0 = nullptr;           // 11N each
+    double *d_lan_h = nullptr, *h_lan = nullptr;           // 256 doubles (device, pinned host)
+    float *d_noise = nullptr;     // generated noise: 11N far real | 6M far grid | 6N near | 6N RFD
+    const float *inj_ff_real = nullptr, *inj_ff_grid = nullptr, *inj_nf = nullptr, *inj_rfd = nullptr;
+    float4 *d_posp = nullptr;     // shifted positions of the random finite difference
+    float *d_rfd_rhs = nullptr, *d_rfd_sol = nullptr;     // 17N each
+    int last_lanczos_ff = 0, last_lanczos_nf = 0;
 
     // profiling
     unsigned prof = 0;            // bit mask of timed kernel classes
@@ -245,12 +264,19 @@ int k_rfe(fsd_ctx *ctx, const float4 *pos, float shear_rate, const float *ori, f
 int k_rsu(fsd_ctx *ctx, const float4 *pos, const float *U, float *S);
 int k_rse(fsd_ctx *ctx, const float4 *pos, float shear_rate, const float *ori, float *S);
 int k_setup_spread(fsd_ctx *ctx, const float4 *pos);
+int k_contract_lds(fsd_ctx *ctx);
+int scan_excl_chunks(int n);
 int k_gridk(fsd_ctx *ctx);
 int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int parts, const float *addU);
 int k_prune(fsd_ctx *ctx, const float4 *pos);
 int k_build_blocks(fsd_ctx *ctx, const float4 *pos);
 int k_ichol(fsd_ctx *ctx, float lambda, int *failed);
 int k_sweep_pack(fsd_ctx *ctx);
+int k_normals(fsd_ctx *ctx, int64_t n, uint32_t step, uint32_t stream, float sd, float *out);
+int k_ff_noise(fsd_ctx *ctx, const float *noise, float fac);
+int k_wave_to_general(fsd_ctx *ctx, float *out);
+int k_shift_wrap(fsd_ctx *ctx, const float4 *pos, const float *psi6, float scale, float4 *out);
+int k_wave_contract(fsd_ctx *ctx, const float4 *pos);
 int k_precond_apply2(fsd_ctx *ctx, const float *b0, const float *b1, float *y0, float *y1, int n_rhs);
 int k_export_csr(fsd_ctx *ctx, const float4 *pos, int32_t *h_rowptr, int32_t *h_col, float *h_val,
                  int32_t *h_prow, int32_t *h_pcol, float *h_L, int64_t cap);

@@ -47,9 +47,12 @@ int k_gridk(fsd_ctx *ctx)
 }
 
 // ------------------------------------------------- spreading set-up (per step)
-// stencil origin exactly as Mobility.cu:203-225 (float fraction, truncation)
-__global__ void stencil_origin_kernel(const float4 *pos, int N, BoxF b, int Nx, int Ny, int Nz, int P, int tile,
-                                      int tx_n, int ty_n, int tz_n, int *orig, int *tile_of, int *tile_count)
+// Stencil origin exactly as Mobility.cu:203-225 (float fraction, truncation);
+// particles are then counting-sorted by the grid cell of their (wrapped)
+// origin, ascending particle id inside a cell, so spreading and contraction
+// can find the particles of any block of origin cells as contiguous runs.
+__global__ void stencil_origin_kernel(const float4 *pos, int N, BoxF b, int Nx, int Ny, int Nz, int P, int *orig,
+                                      int *ocell, int *ocount)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -64,25 +67,24 @@ __global__ void stencil_origin_kernel(const float4 *pos, int N, BoxF b, int Nx, 
     int oz = z - Pd2 + 1 - (P % 2) * ((fz - (float)z) < 0.5f);
     orig[3 * i] = ox; orig[3 * i + 1] = oy; orig[3 * i + 2] = oz;
     int wx = ((ox % Nx) + Nx) % Nx, wy = ((oy % Ny) + Ny) % Ny, wz = ((oz % Nz) + Nz) % Nz;
-    int t = ((wx / tile) * ty_n + (wy / tile)) * tz_n + (wz / tile);
-    tile_of[i] = t;
-    atomicAdd(&tile_count[t], 1);
+    const int c = (wx * Ny + wy) * Nz + wz;
+    ocell[i] = c;
+    atomicAdd(&ocount[c], 1);
 }
 
-__global__ void tile_scatter_kernel(const int *tile_of, int N, const int *start, int *fill, int *list)
+__global__ void ocell_scatter_kernel(const int *ocell, int N, const int *start, int *fill, int *list)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    int t = tile_of[i];
-    int s = atomicAdd(&fill[t], 1);
-    list[start[t] + s] = i;
+    const int c = ocell[i];
+    list[start[c] + atomicAdd(&fill[c], 1)] = i;
 }
 
-__global__ void tile_sort_kernel(const int *start, int ntiles, int *list)
+__global__ void ocell_sort_kernel(const int *start, int ncell, int *list)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ntiles) return;
-    int s = start[c], e = start[c + 1];
+    if (c >= ncell) return;
+    const int s = start[c], e = start[c + 1];
     for (int a = s + 1; a < e; ++a) {
         int key = list[a], q = a - 1;
         while (q >= s && list[q] > key) { list[q + 1] = list[q]; --q; }
@@ -90,21 +92,20 @@ __global__ void tile_sort_kernel(const int *start, int ntiles, int *list)
     }
 }
 
-int scan_excl(fsd_ctx *ctx, const int *in, int *out, int n);
+int scan_excl_large(fsd_ctx *ctx, const int *in, int *out, int n, int *tmp);
 
 int k_setup_spread(fsd_ctx *ctx, const float4 *pos)
 {
-    const int N = ctx->N;
-    FSD_HIP(hipMemsetAsync(ctx->d_tile_count, 0, sizeof(int) * ctx->ntiles, ctx->stream));
+    const int N = ctx->N, M = ctx->M;
+    FSD_HIP(hipMemsetAsync(ctx->d_ocount, 0, sizeof(int) * M, ctx->stream));
     hipLaunchKernelGGL(stencil_origin_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, pos, N, ctx->box,
-                       ctx->pse.Nx, ctx->pse.Ny, ctx->pse.Nz, ctx->pse.P, ctx->tile, ctx->tiles[0], ctx->tiles[1],
-                       ctx->tiles[2], ctx->d_porig, ctx->d_tile_of, ctx->d_tile_count);
-    if (scan_excl(ctx, ctx->d_tile_count, ctx->d_tile_start, ctx->ntiles)) return FSD_E_HIP;
-    FSD_HIP(hipMemsetAsync(ctx->d_tile_count, 0, sizeof(int) * ctx->ntiles, ctx->stream));
-    hipLaunchKernelGGL(tile_scatter_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, ctx->d_tile_of, N,
-                       ctx->d_tile_start, ctx->d_tile_count, ctx->d_tile_list);
-    hipLaunchKernelGGL(tile_sort_kernel, dim3(nblk(ctx->ntiles, 64)), dim3(64), 0, ctx->stream, ctx->d_tile_start,
-                       ctx->ntiles, ctx->d_tile_list);
+                       ctx->pse.Nx, ctx->pse.Ny, ctx->pse.Nz, ctx->pse.P, ctx->d_porig, ctx->d_ocell, ctx->d_ocount);
+    if (scan_excl_large(ctx, ctx->d_ocount, ctx->d_ostart, M, ctx->d_scan_tmp)) return FSD_E_HIP;
+    FSD_HIP(hipMemsetAsync(ctx->d_ocount, 0, sizeof(int) * M, ctx->stream));
+    hipLaunchKernelGGL(ocell_scatter_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, ctx->d_ocell, N,
+                       ctx->d_ostart, ctx->d_ocount, ctx->d_olist);
+    hipLaunchKernelGGL(ocell_sort_kernel, dim3(nblk(M, 256)), dim3(256), 0, ctx->stream, ctx->d_ostart, M,
+                       ctx->d_olist);
     FSD_HIP(hipGetLastError());
     ctx->spread_valid = true;
     ctx->spread_pos = (const float *)pos;
@@ -136,20 +137,42 @@ struct SpreadGeom {
     float prefac, expfac;
 };
 
-// one workgroup per tile of TILE^3 grid points (one thread per point); the
-// particles whose stencils reach the tile are streamed through LDS in chunks
-// and every point accumulates its 11 fields in registers.
+// The origin cells whose stencils reach a tile [t0, t0 + tile) on one axis are
+// t0 - P + 1 .. t0 + tile - 1 (wrapped): `halo_axis` lists them once each
+// (fewer when the grid is shorter than the list) and returns the count.
+__device__ int halo_axis(int t0, int tile, int P, int n, int *list)
+{
+    int cnt = 0;
+    for (int o = t0 - P + 1; o <= t0 + tile - 1; ++o) {
+        const int w = ((o % n) + n) % n;
+        bool seen = false;
+        for (int q = 0; q < cnt; ++q) seen |= (list[q] == w);
+        if (!seen) list[cnt++] = w;
+    }
+    return cnt;
+}
+
+// One workgroup per tile of TILE^3 grid points, one thread per point.  The
+// particles whose origin lies in the tile's halo of origin cells (the exact
+// set whose stencils can reach the tile) are found as runs of the
+// origin-cell-sorted list and streamed through LDS in chunks; every point
+// accumulates its 11 fields in registers and is written once (no atomics).
+#define SP_CH 256
+#define SP_MAXAX 32
 __global__ void __launch_bounds__(512) spread_kernel(SpreadGeom g, const float4 *pos, const int *orig,
-                                                     const float4 *rec, const int *tile_start, const int *tile_list,
+                                                     const float4 *rec, const int *ostart, const int *olist,
                                                      float *grid, int64_t M)
 {
-    constexpr int CH = 256;
-    __shared__ int s_tiles[27];
-    __shared__ int s_ntiles;
-    __shared__ int s_idx[CH];
-    __shared__ int s_o[CH][3];
-    __shared__ float s_p[CH][3];
-    __shared__ float4 s_r[CH][3];
+    __shared__ int s_ax[3][SP_MAXAX];
+    __shared__ int s_nax[3];
+    __shared__ int s_zrun[SP_MAXAX][2];   // z runs: [first cell, one past last cell)
+    __shared__ int s_nzrun;
+    __shared__ int s_seg_beg[SP_MAXAX * SP_MAXAX * 2];
+    __shared__ int s_seg_off[SP_MAXAX * SP_MAXAX * 2 + 1];
+    __shared__ int s_nseg;
+    __shared__ int s_o[SP_CH][3];
+    __shared__ float s_p[SP_CH][3];
+    __shared__ float4 s_r[SP_CH][3];
 
     const int tile_id = blockIdx.x;
     const int tz_i = tile_id % g.tnz, ty_i = (tile_id / g.tnz) % g.tny, tx_i = tile_id / (g.tnz * g.tny);
@@ -159,67 +182,78 @@ __global__ void __launch_bounds__(512) spread_kernel(SpreadGeom g, const float4 
     const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
     const bool inside = (gx < g.Nx) && (gy < g.Ny) && (gz < g.Nz);
 
-    if (lt == 0) {
-        // origin tiles covering [t0 - P + 1, t0 + tile - 1] per axis (periodic)
-        int lists[3][3], cnt[3];
-        const int n_[3] = {g.Nx, g.Ny, g.Nz}, s0[3] = {x0, y0, z0};
-        for (int d = 0; d < 3; ++d) {
-            cnt[d] = 0;
-            int last = -1;
-            for (int o = s0[d] - g.P + 1; o <= s0[d] + g.tile - 1; ++o) {
-                int w = ((o % n_[d]) + n_[d]) % n_[d];
-                int t = w / g.tile;
-                bool seen = false;
-                for (int q = 0; q < cnt[d]; ++q) seen |= (lists[d][q] == t);
-                if (!seen && t != last && cnt[d] < 3) lists[d][cnt[d]++] = t;
-                last = t;
-            }
-        }
-        int n = 0;
-        for (int a = 0; a < cnt[0]; ++a)
-            for (int c = 0; c < cnt[1]; ++c)
-                for (int e = 0; e < cnt[2]; ++e) s_tiles[n++] = (lists[0][a] * g.tny + lists[1][c]) * g.tnz + lists[2][e];
-        s_ntiles = n;
+    if (lt < 3) {
+        const int t0 = (lt == 0) ? x0 : (lt == 1) ? y0 : z0;
+        const int n = (lt == 0) ? g.Nx : (lt == 1) ? g.Ny : g.Nz;
+        s_nax[lt] = halo_axis(t0, g.tile, g.P, n, s_ax[lt]);
     }
     __syncthreads();
+    if (lt == 0) {
+        // group the z cells into runs of consecutive indices
+        int nr = 0;
+        for (int q = 0; q < s_nax[2]; ++q) {
+            const int z = s_ax[2][q];
+            if (nr > 0 && s_zrun[nr - 1][1] == z) s_zrun[nr - 1][1] = z + 1;
+            else { s_zrun[nr][0] = z; s_zrun[nr][1] = z + 1; ++nr; }
+        }
+        s_nzrun = nr;
+        s_nseg = s_nax[0] * s_nax[1] * nr;
+    }
+    __syncthreads();
+    const int nseg = s_nseg, nzr = s_nzrun, ny = s_nax[1];
+    for (int sg = lt; sg < nseg; sg += blockDim.x) {
+        const int zr = sg % nzr, xy = sg / nzr;
+        const int x = s_ax[0][xy / ny], y = s_ax[1][xy % ny];
+        const int cbase = (x * g.Ny + y) * g.Nz;
+        const int b0 = ostart[cbase + s_zrun[zr][0]], b1 = ostart[cbase + s_zrun[zr][1]];
+        s_seg_beg[sg] = b0;
+        s_seg_off[sg + 1] = b1 - b0;
+    }
+    __syncthreads();
+    if (lt == 0) {
+        s_seg_off[0] = 0;
+        for (int sg = 0; sg < nseg; ++sg) s_seg_off[sg + 1] += s_seg_off[sg];
+    }
+    __syncthreads();
+    const int total = s_seg_off[nseg];
 
     float acc[11];
 #pragma unroll
     for (int c = 0; c < 11; ++c) acc[c] = 0.f;
 
-    const int ntl = s_ntiles;
-    for (int tt = 0; tt < ntl; ++tt) {
-        const int t = s_tiles[tt];
-        const int beg = tile_start[t], end = tile_start[t + 1];
-        for (int base = beg; base < end; base += CH) {
-            const int nch = min(CH, end - base);
-            __syncthreads();
-            if (lt < nch) {
-                int i = tile_list[base + lt];
-                s_idx[lt] = i;
-                s_o[lt][0] = orig[3 * i]; s_o[lt][1] = orig[3 * i + 1]; s_o[lt][2] = orig[3 * i + 2];
-                float4 p = pos[i];
-                s_p[lt][0] = p.x; s_p[lt][1] = p.y; s_p[lt][2] = p.z;
-                s_r[lt][0] = rec[3 * (int64_t)i]; s_r[lt][1] = rec[3 * (int64_t)i + 1]; s_r[lt][2] = rec[3 * (int64_t)i + 2];
+    for (int base = 0; base < total; base += SP_CH) {
+        const int nch = min(SP_CH, total - base);
+        __syncthreads();
+        if (lt < nch) {
+            const int q = base + lt;
+            int lo = 0, hi = nseg - 1;          // segment sg with off[sg] <= q < off[sg + 1]
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_seg_off[mid] <= q) lo = mid; else hi = mid - 1;
             }
-            __syncthreads();
-            if (inside) {
-                for (int q = 0; q < nch; ++q) {
-                    int tx = gx - s_o[q][0]; tx += (tx < 0) ? g.Nx : 0; tx -= (tx >= g.Nx) ? g.Nx : 0;
-                    int ty = gy - s_o[q][1]; ty += (ty < 0) ? g.Ny : 0; ty -= (ty >= g.Ny) ? g.Ny : 0;
-                    int tz = gz - s_o[q][2]; tz += (tz < 0) ? g.Nz : 0; tz -= (tz >= g.Nz) ? g.Nz : 0;
-                    if (tx < g.P && ty < g.P && tz < g.P) {
-                        // unwrapped grid node of this stencil point, relative to the particle
-                        const float uy = g.hy * (float)(s_o[q][1] + ty) - g.Ly2;
-                        const float dx = g.hx * (float)(s_o[q][0] + tx) - g.Lx2 + g.xy * uy - s_p[q][0];
-                        const float dy = uy - s_p[q][1];
-                        const float dz = g.hz * (float)(s_o[q][2] + tz) - g.Lz2 - s_p[q][2];
-                        const float w = g.prefac * __expf(-g.expfac * (dx * dx + dy * dy + dz * dz));
-                        const float4 r0 = s_r[q][0], r1 = s_r[q][1], r2 = s_r[q][2];
-                        acc[0] += w * r0.x; acc[1] += w * r0.y; acc[2] += w * r0.z;
-                        acc[3] += w * r0.w; acc[4] += w * r1.x; acc[5] += w * r1.y; acc[6] += w * r1.z;
-                        acc[7] += w * r1.w; acc[8] += w * r2.x; acc[9] += w * r2.y; acc[10] += w * r2.z;
-                    }
+            const int i = olist[s_seg_beg[lo] + (q - s_seg_off[lo])];
+            s_o[lt][0] = orig[3 * i]; s_o[lt][1] = orig[3 * i + 1]; s_o[lt][2] = orig[3 * i + 2];
+            const float4 p = pos[i];
+            s_p[lt][0] = p.x; s_p[lt][1] = p.y; s_p[lt][2] = p.z;
+            s_r[lt][0] = rec[3 * (int64_t)i]; s_r[lt][1] = rec[3 * (int64_t)i + 1]; s_r[lt][2] = rec[3 * (int64_t)i + 2];
+        }
+        __syncthreads();
+        if (inside) {
+            for (int q = 0; q < nch; ++q) {
+                int tx = gx - s_o[q][0]; tx += (tx < 0) ? g.Nx : 0; tx -= (tx >= g.Nx) ? g.Nx : 0;
+                int ty = gy - s_o[q][1]; ty += (ty < 0) ? g.Ny : 0; ty -= (ty >= g.Ny) ? g.Ny : 0;
+                int tz = gz - s_o[q][2]; tz += (tz < 0) ? g.Nz : 0; tz -= (tz >= g.Nz) ? g.Nz : 0;
+                if (tx < g.P && ty < g.P && tz < g.P) {
+                    // unwrapped grid node of this stencil point, relative to the particle
+                    const float uy = g.hy * (float)(s_o[q][1] + ty) - g.Ly2;
+                    const float dx = g.hx * (float)(s_o[q][0] + tx) - g.Lx2 + g.xy * uy - s_p[q][0];
+                    const float dy = uy - s_p[q][1];
+                    const float dz = g.hz * (float)(s_o[q][2] + tz) - g.Lz2 - s_p[q][2];
+                    const float w = g.prefac * __expf(-g.expfac * (dx * dx + dy * dy + dz * dz));
+                    const float4 r0 = s_r[q][0], r1 = s_r[q][1], r2 = s_r[q][2];
+                    acc[0] += w * r0.x; acc[1] += w * r0.y; acc[2] += w * r0.z;
+                    acc[3] += w * r0.w; acc[4] += w * r1.x; acc[5] += w * r1.y; acc[6] += w * r1.z;
+                    acc[7] += w * r1.w; acc[8] += w * r2.x; acc[9] += w * r2.y; acc[10] += w * r2.z;
                 }
             }
         }
@@ -346,6 +380,113 @@ __global__ void __launch_bounds__(256) contract_kernel(SpreadGeom g, const float
     }
 }
 
+// Tile form of the contraction: one workgroup per tile of TILE^3 origin
+// cells stages the grid block [t0, t0 + TILE + P - 1)^3 of all 11 fields in
+// LDS (coalesced loads, wrapped indices), then contracts the particles whose
+// origin lies in the tile, one wavefront per particle, from LDS.  Requires
+// 11 (TILE + P - 1)^3 floats of LDS (TILE 8, P 6: 96.7 KB).
+#define CT_S 13
+__global__ void __launch_bounds__(512) contract_tile_kernel(SpreadGeom g, const float4 *pos, const int *orig,
+                                                            const int *ostart, const int *olist, const float *grid,
+                                                            int64_t M, float quadw_prefac, float *wave)
+{
+    extern __shared__ float s_g[];           // [11][S][S][S]
+    __shared__ int s_run_beg[64], s_run_off[65];
+    const int S = g.tile + g.P - 1;
+    const int S3 = S * S * S;
+    const int tile_id = blockIdx.x;
+    const int tz_i = tile_id % g.tnz, ty_i = (tile_id / g.tnz) % g.tny, tx_i = tile_id / (g.tnz * g.tny);
+    const int x0 = tx_i * g.tile, y0 = ty_i * g.tile, z0 = tz_i * g.tile;
+    const int lt = threadIdx.x;
+    // origin cells of the tile: columns (x, y), z in [z0, z1)
+    const int xe = min(x0 + g.tile, g.Nx), ye = min(y0 + g.tile, g.Ny), z1 = min(z0 + g.tile, g.Nz);
+    const int ncx = xe - x0, ncy = ye - y0;
+    if (lt < ncx * ncy) {
+        const int x = x0 + lt / ncy, y = y0 + lt % ncy;
+        const int cb = (x * g.Ny + y) * g.Nz;
+        const int b0 = ostart[cb + z0], b1 = ostart[cb + z1];
+        s_run_beg[lt] = b0;
+        s_run_off[lt + 1] = b1 - b0;
+    }
+    // grid block, wrapped
+    for (int idx = lt; idx < 11 * S3; idx += blockDim.x) {
+        const int c = idx / S3, r = idx - c * S3;
+        const int ax = r / (S * S), ay = (r / S) % S, az = r % S;
+        int x = x0 + ax, y = y0 + ay, z = z0 + az;
+        x -= (x >= g.Nx) ? g.Nx : 0; x -= (x >= g.Nx) ? g.Nx : 0;
+        y -= (y >= g.Ny) ? g.Ny : 0; y -= (y >= g.Ny) ? g.Ny : 0;
+        z -= (z >= g.Nz) ? g.Nz : 0; z -= (z >= g.Nz) ? g.Nz : 0;
+        s_g[idx] = grid[c * M + ((int64_t)x * g.Ny + y) * g.Nz + z];
+    }
+    __syncthreads();
+    if (lt == 0) {
+        s_run_off[0] = 0;
+        for (int q = 0; q < ncx * ncy; ++q) s_run_off[q + 1] += s_run_off[q];
+    }
+    __syncthreads();
+    const int nruns = ncx * ncy;
+    const int total = s_run_off[nruns];
+    const int wv = lt >> 6, lane = lt & 63, nw = blockDim.x >> 6;
+    const int P = g.P;
+    for (int q = wv; q < total; q += nw) {
+        int lo = 0, hi = nruns - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_run_off[mid] <= q) lo = mid; else hi = mid - 1;
+        }
+        const int i = olist[s_run_beg[lo] + (q - s_run_off[lo])];
+        const int ox = orig[3 * i], oy = orig[3 * i + 1], oz = orig[3 * i + 2];
+        const float4 p = pos[i];
+        // local block coordinates of the stencil's first node (wrapped origin - tile start)
+        const int bx = (((ox % g.Nx) + g.Nx) % g.Nx) - x0;
+        const int by = (((oy % g.Ny) + g.Ny) % g.Ny) - y0;
+        const int bz = (((oz % g.Nz) + g.Nz) % g.Nz) - z0;
+        float acc[11];
+#pragma unroll
+        for (int c = 0; c < 11; ++c) acc[c] = 0.f;
+        if (lane < P * P) {
+            const int ty = lane / P, tz = lane - P * ty;
+            const float uy = g.hy * (float)(oy + ty) - g.Ly2;
+            const float dz = g.hz * (float)(oz + tz) - g.Lz2 - p.z;
+            const float dy = uy - p.y;
+            for (int tx = 0; tx < P; ++tx) {
+                const float dx = g.hx * (float)(ox + tx) - g.Lx2 + g.xy * uy - p.x;
+                const float w = quadw_prefac * __expf(-g.expfac * (dx * dx + dy * dy + dz * dz));
+                const int r = ((bx + tx) * S + (by + ty)) * S + (bz + tz);
+#pragma unroll
+                for (int c = 0; c < 11; ++c) acc[c] += w * s_g[c * S3 + r];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 11; ++c)
+            for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
+        if (lane == 0) {
+            float *w = wave + 11 * (int64_t)i;
+            w[0] = acc[0]; w[1] = acc[1]; w[2] = acc[2];
+            // fields XX XY XZ YZ YY YX ZX ZY -> slots {XX, YX, ZX, ZY, YY, XY, XZ, YZ}
+            w[3] = acc[3]; w[4] = acc[8]; w[5] = acc[9]; w[6] = acc[10];
+            w[7] = acc[7]; w[8] = acc[4]; w[9] = acc[5]; w[10] = acc[6];
+        }
+    }
+}
+
+// LDS bytes of the tile contraction, 0 when it does not fit (then the
+// per-particle contraction runs); sets the launch attribute once
+int k_contract_lds(fsd_ctx *ctx)
+{
+    const int S = ctx->tile + ctx->pse.P - 1;
+    const size_t bytes = sizeof(float) * 11 * (size_t)S * S * S;
+    int dev = 0, maxlds = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return 0;
+    if (ctx->tile * ctx->tile > 64 || bytes + 1024 > (size_t)maxlds) return 0;   // 64 origin columns per tile
+    if (hipFuncSetAttribute((const void *)contract_tile_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes) != hipSuccess)
+        return 0;
+    return (int)bytes;
+}
+
 // ------------------------------------------------------------ real space
 struct RealParams {
     BoxF b;
@@ -456,6 +597,46 @@ __global__ void __launch_bounds__(256) realspace_kernel(RealParams rp, const flo
     o5[0] = 2.f * e0 + e4; o5[1] = 2.f * e1; o5[2] = 2.f * e2; o5[3] = 2.f * e3; o5[4] = 2.f * e4 + e0;
 }
 
+// inverse transform of the 11 half spectra in d_spec and contraction of the
+// grids at the particles into d_wave (u and the velocity gradient)
+int k_wave_contract(fsd_ctx *ctx, const float4 *pos)
+{
+    const int N = ctx->N;
+    const fsd_pse_info &ps = ctx->pse;
+    if (!ctx->spread_valid || ctx->spread_pos != (const float *)pos) {
+        const int rc = k_setup_spread(ctx, pos);
+        if (rc) return rc;
+    }
+    SpreadGeom g;
+    g.Nx = ps.Nx; g.Ny = ps.Ny; g.Nz = ps.Nz; g.P = ps.P; g.tile = ctx->tile;
+    g.tnx = ctx->tiles[0]; g.tny = ctx->tiles[1]; g.tnz = ctx->tiles[2];
+    g.hx = ps.gridh_x; g.hy = ps.gridh_y; g.hz = ps.gridh_z;
+    g.Lx2 = ctx->box.Lx / 2.0f; g.Ly2 = ctx->box.Ly / 2.0f; g.Lz2 = ctx->box.Lz / 2.0f; g.xy = ctx->box.xy;
+    g.prefac = ps.prefac; g.expfac = ps.expfac;
+    hipEvent_t ev;
+    prof_begin(ctx, PROF_FFT, &ev);
+    {
+        void *in[1] = {ctx->d_spec};
+        void *out[1] = {ctx->d_grid};
+        if (rocfft_execute(ctx->plan_inv, in, out, ctx->info_inv) != rocfft_status_success) {
+            ctx->err = "rocfft inverse failed";
+            return FSD_E_HIP;
+        }
+    }
+    prof_end(ctx, PROF_FFT, ev);
+    prof_begin(ctx, PROF_CONTRACT, &ev);
+    if (ctx->contract_lds)
+        hipLaunchKernelGGL(contract_tile_kernel, dim3(ctx->ntiles), dim3(512), ctx->contract_lds, ctx->stream, g, pos,
+                           ctx->d_porig, ctx->d_ostart, ctx->d_olist, ctx->d_grid, (int64_t)ctx->M,
+                           ps.quadW * ps.prefac, ctx->d_wave);
+    else
+        hipLaunchKernelGGL(contract_kernel, dim3(nblk((int64_t)N * 64, 256)), dim3(256), 0, ctx->stream, g, pos,
+                           ctx->d_porig, ctx->d_grid, (int64_t)ctx->M, N, ps.quadW * ps.prefac, ctx->d_wave);
+    FSD_HIP(hipGetLastError());
+    prof_end(ctx, PROF_CONTRACT, ev);
+    return 0;
+}
+
 // ------------------------------------------------------------ driver
 // gU = M gF (+ addU on the first 6N rows); parts: 1 real, 2 wave, 3 both
 int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int parts, const float *addU)
@@ -480,7 +661,7 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
         hipEvent_t ev;
         prof_begin(ctx, PROF_SPREAD, &ev);
         hipLaunchKernelGGL(spread_kernel, dim3(ctx->ntiles), dim3(ctx->tile * ctx->tile * ctx->tile), 0, ctx->stream,
-                           g, pos, ctx->d_porig, rec, ctx->d_tile_start, ctx->d_tile_list, ctx->d_grid, (int64_t)ctx->M);
+                           g, pos, ctx->d_porig, rec, ctx->d_ostart, ctx->d_olist, ctx->d_grid, (int64_t)ctx->M);
         FSD_HIP(hipGetLastError());
         prof_end(ctx, PROF_SPREAD, ev);
         prof_begin(ctx, PROF_FFT, &ev);
@@ -498,21 +679,8 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
                            ctx->d_gridk, (int64_t)ctx->Mh);
         FSD_HIP(hipGetLastError());
         prof_end(ctx, PROF_GREEN, ev);
-        prof_begin(ctx, PROF_FFT, &ev);
-        {
-            void *in[1] = {ctx->d_spec};
-            void *out[1] = {ctx->d_grid};
-            if (rocfft_execute(ctx->plan_inv, in, out, ctx->info_inv) != rocfft_status_success) {
-                ctx->err = "rocfft inverse failed";
-                return FSD_E_HIP;
-            }
-        }
-        prof_end(ctx, PROF_FFT, ev);
-        prof_begin(ctx, PROF_CONTRACT, &ev);
-        hipLaunchKernelGGL(contract_kernel, dim3(nblk((int64_t)N * 64, 256)), dim3(256), 0, ctx->stream, g, pos,
-                           ctx->d_porig, ctx->d_grid, (int64_t)ctx->M, N, ps.quadW * ps.prefac, ctx->d_wave);
-        FSD_HIP(hipGetLastError());
-        prof_end(ctx, PROF_CONTRACT, ev);
+        const int rcw = k_wave_contract(ctx, pos);
+        if (rcw) return rcw;
         wave = ctx->d_wave;
     }
     RealParams rp;

@@ -65,6 +65,70 @@ int scan_excl(fsd_ctx *ctx, const int *in, int *out, int n)
     return 0;
 }
 
+// multi-workgroup exclusive scan for long arrays: each workgroup scans
+// SCAN_CHUNK elements (4 per thread) and records its total; the totals are
+// scanned by one workgroup; a third pass adds each chunk's offset.
+#define SCAN_CHUNK 4096
+__global__ void __launch_bounds__(1024) scan_chunks_kernel(const int *in, int *out, int n, int *part)
+{
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int64_t base = (int64_t)blockIdx.x * SCAN_CHUNK + 4 * tid;
+    int v[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t i = base + k;
+        v[k] = (i < n) ? in[i] : 0;
+        s += v[k];
+    }
+    int x = s;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    if (wid == 0) {
+        int t = (lane < 16) ? wsum[lane] : 0;
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(t, o, 64);
+            if (lane >= o) t += y;
+        }
+        if (lane < 16) wsum[lane] = t;
+    }
+    __syncthreads();
+    int pre = ((wid > 0) ? wsum[wid - 1] : 0) + x - s;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int64_t i = base + k;
+        if (i < n) out[i] = pre;
+        pre += v[k];
+    }
+    if (tid == 0) part[blockIdx.x] = wsum[15];
+}
+
+__global__ void scan_add_kernel(int *out, int n, const int *part_scan, int nchunks)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += part_scan[i / SCAN_CHUNK];
+    if (i == 0) out[n] = part_scan[nchunks];
+}
+
+int scan_excl_chunks(int n) { return (n + SCAN_CHUNK - 1) / SCAN_CHUNK; }
+
+// out[0..n] = exclusive scan of in[0..n); tmp holds 2 (chunks + 1) ints
+int scan_excl_large(fsd_ctx *ctx, const int *in, int *out, int n, int *tmp)
+{
+    const int nc = scan_excl_chunks(n);
+    if (nc <= 1) return scan_excl(ctx, in, out, n);
+    int *part = tmp, *part_scan = tmp + nc + 1;
+    hipLaunchKernelGGL(scan_chunks_kernel, dim3(nc), dim3(1024), 0, ctx->stream, in, out, n, part);
+    hipLaunchKernelGGL(scan_excl_kernel, dim3(1), dim3(1024), 0, ctx->stream, part, part_scan, nc);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, out, n, part_scan, nc);
+    FSD_HIP(hipGetLastError());
+    return 0;
+}
+
 // ----------------------------------------------------------- cell list
 __global__ void cell_assign_kernel(const float4 *pos, int N, BoxF b, int cx, int cy, int cz, int *cell_of, int *count)
 {

@@ -5,6 +5,7 @@
 // (Stokes::integrateStepOne -> Stokes_StepOne), exported through fsd.h.
 #include "fsd_internal.h"
 #include "sweep_plan.h"
+#include "lanczos_host.h"
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -189,10 +190,13 @@ static int alloc_all(fsd_ctx *ctx)
     ctx->tiles[0] = (ps.Nx + 7) / 8; ctx->tiles[1] = (ps.Ny + 7) / 8; ctx->tiles[2] = (ps.Nz + 7) / 8;
     ctx->ntiles = ctx->tiles[0] * ctx->tiles[1] * ctx->tiles[2];
     CK(hipMalloc(&ctx->d_porig, sizeof(int) * 3 * N));
-    CK(hipMalloc(&ctx->d_tile_of, sizeof(int) * N));
-    CK(hipMalloc(&ctx->d_tile_list, sizeof(int) * N));
-    CK(hipMalloc(&ctx->d_tile_count, sizeof(int) * ctx->ntiles));
-    CK(hipMalloc(&ctx->d_tile_start, sizeof(int) * (ctx->ntiles + 1)));
+    CK(hipMalloc(&ctx->d_ocell, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_olist, sizeof(int) * N));
+    CK(hipMalloc(&ctx->d_ocount, sizeof(int) * (size_t)ctx->M));
+    CK(hipMalloc(&ctx->d_ostart, sizeof(int) * ((size_t)ctx->M + 1)));
+    CK(hipMalloc(&ctx->d_scan_tmp, sizeof(int) * 2 * (scan_excl_chunks(ctx->M) + 1)));
+    if (ctx->tile + ps.P - 1 > 32) { ctx->err = "spreading support P too large for the tile halo"; return FSD_E_PARAM; }
+    ctx->contract_lds = k_contract_lds(ctx);
     CK(hipMalloc(&ctx->d_wave, sizeof(float) * 11 * (size_t)N));
     CK(hipMalloc(&ctx->d_rec, sizeof(float4) * 3 * (size_t)N));
     {
@@ -291,18 +295,21 @@ static void free_all(fsd_ctx *ctx)
     void *ptrs[] = {ctx->d_nneigh, ctx->d_nlist, ctx->d_cell_of, ctx->d_cell_count, ctx->d_cell_start,
                     ctx->d_cell_list, ctx->d_flags, ctx->d_lub_n, ctx->d_lub_j, ctx->d_lub_a, ctx->d_lub_b,
                     ctx->d_lub_self, ctx->d_grid, ctx->d_spec, ctx->d_gridk, ctx->d_fftwork, ctx->d_porig,
-                    ctx->d_tile_of, ctx->d_tile_list, ctx->d_tile_count, ctx->d_tile_start, ctx->d_wave, ctx->d_rec,
+                    ctx->d_ocell, ctx->d_olist, ctx->d_ocount, ctx->d_ostart, ctx->d_scan_tmp, ctx->d_wave, ctx->d_rec,
                     ctx->d_ewald, ctx->d_res_dist, ctx->d_res_vals, ctx->d_pn, ctx->d_plist, ctx->d_pless,
                     ctx->d_perm, ctx->d_iperm, ctx->d_lrow, ctx->d_urow, ctx->d_lcol, ctx->d_ucol, ctx->d_upos,
                     ctx->d_Ablk, ctx->d_Lblk, ctx->d_Adiag, ctx->d_Ldiag, ctx->d_lev_f, ctx->d_lev_b, ctx->d_tmp6a,
                     ctx->d_tmp6b, ctx->d_g12, ctx->d_V, ctx->d_w, ctx->d_t, ctx->d_t2, ctx->d_part, ctx->d_rhs, ctx->d_sol,
                     ctx->d_force, ctx->d_mask1, ctx->d_mask2, ctx->d_sw_tab, ctx->d_sw_fcol, ctx->d_sw_bcol,
                     ctx->d_sw_fsrc, ctx->d_sw_bsrc, ctx->d_sw_fblk, ctx->d_sw_bblk, ctx->d_sw_fdinv, ctx->d_sw_bdinv,
-                    ctx->d_sw_forder, ctx->d_sw_border, ctx->d_sw_fin, ctx->d_sw_fyb, ctx->d_sw_bout, ctx->d_sw_work};
+                    ctx->d_sw_forder, ctx->d_sw_border, ctx->d_sw_fin, ctx->d_sw_fyb, ctx->d_sw_bout, ctx->d_sw_work,
+                    ctx->d_Z, ctx->d_lanV, ctx->d_lanw, ctx->d_lany0, ctx->d_lan_h, ctx->d_noise, ctx->d_posp,
+                    ctx->d_rfd_rhs, ctx->d_rfd_sol};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->h_flags) hipHostFree(ctx->h_flags);
     if (ctx->h_part) hipHostFree(ctx->h_part);
+    if (ctx->h_lan) hipHostFree(ctx->h_lan);
     if (ctx->plan_fwd) rocfft_plan_destroy(ctx->plan_fwd);
     if (ctx->plan_inv) rocfft_plan_destroy(ctx->plan_inv);
     if (ctx->info_fwd) rocfft_execution_info_destroy(ctx->info_fwd);
@@ -509,37 +516,45 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
 // Hessenberg, Givens rotations and the small triangular solve in FP64 on the host.
 static int true_residual(fsd_ctx *ctx, const float4 *pos, const float *b, const float *x, float *out);
 
-static int gmres_cusp(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
-                      fsd_solve_stats *st);
+static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
+                      bool right, fsd_solve_stats *st);
 
-// CUSP-semantics GMRES; with gmres_true_tol > 0 the solve is repeated (warm
-// started, tolerance lowered by the observed ratio) until the true relative
-// residual ||b - A x|| / ||b|| also meets gmres_true_tol.
+// Default: CUSP semantics (left preconditioning, converged when the
+// preconditioned residual estimate |s| <= tol ||b||, Solvers.cu:55-126).
+// With gmres_true_tol > 0 the solve is right-preconditioned instead, so its
+// residual estimate is the true residual ||b - A x||: it stops at
+// gmres_true_tol, and the true residual is then recomputed and the solve
+// continued (warm) if FP32 drift left it above the tolerance.
 static int gmres(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
                  fsd_solve_stats *st)
 {
     const float ttol = ctx->p.gmres_true_tol;
-    int rc = gmres_cusp(ctx, pos, b, x, tol, maxit, R, st);
-    if (rc < 0 || ttol <= 0.f) return rc;
-    int total = st ? st->iterations : 0, restarts = st ? st->restarts : 0;
-    float tres = 0.f;
+    if (ttol <= 0.f) return gmres_core(ctx, pos, b, x, tol, maxit, R, false, st);
+    int total = 0, restarts = 0, rc = 0;
+    float tres = 0.f, goal = ttol;
     for (int round = 0; round < 6; ++round) {
-        RC(true_residual(ctx, pos, b, x, &tres));
-        if (tres <= ttol || total >= maxit) break;
-        tol = tol * std::max(0.05f, 0.5f * ttol / tres);
-        rc = gmres_cusp(ctx, pos, b, x, tol, maxit - total, R, st);
+        rc = gmres_core(ctx, pos, b, x, goal, maxit - total, R, true, st);
         if (rc < 0) return rc;
         if (st) { total += st->iterations; restarts += st->restarts; }
+        RC(true_residual(ctx, pos, b, x, &tres));
+        if (tres <= ttol || total >= maxit) break;
+        goal = goal * std::max(0.1f, 0.9f * ttol / tres);
     }
     if (st) { st->iterations = total; st->restarts = restarts; st->true_residual = tres; }
     return (tres <= ttol) ? 0 : FSD_W_NOT_CONVERGED;
 }
 
-static int gmres_cusp(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
-                      fsd_solve_stats *st)
+// Restarted GMRES(R), classical Gram-Schmidt with one reorthogonalisation and
+// FP64 dot products; Hessenberg, Givens rotations and the triangular solve in
+// FP64 on the host.  left: Krylov space of M^-1 A, residual M^-1 (b - A x);
+// right: Krylov space of A M^-1 with Z = M^-1 V kept, residual b - A x.
+static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
+                      bool right, fsd_solve_stats *st)
 {
     const int64_t n = 17 * (int64_t)ctx->N;
     float *V = ctx->d_V, *w = ctx->d_w, *t = ctx->d_t;
+    if (right && !ctx->d_Z) CK(hipMalloc(&ctx->d_Z, sizeof(float) * n * R));
+    float *Z = ctx->d_Z;
     double *d_h = ctx->d_part + (int64_t)ctx->part_blocks * 64;   // 256 doubles of scratch after the partials
     std::vector<double> H((size_t)(R + 1) * R, 0.0), cs(R), sn(R), g(R + 1), yv(R + 1);
     auto hostsync = [&](int count) -> int {
@@ -554,15 +569,20 @@ static int gmres_cusp(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
     int iters = 0, restarts = 0;
     double resid = 0.0;
     for (;;) {
+        // r = A x - b (left: preconditioned); V_0 = -r / |r|
         RC(k_saddle_multiply(ctx, pos, x, t));
         RC(k_axpby(ctx, n, 1.0f, t, -1.0f, b, t));
-        RC(k_saddle_precond(ctx, t, w));
-        RC(k_nrm2(ctx, n, w, d_h));
+        float *r = t;
+        if (!right) {
+            RC(k_saddle_precond(ctx, t, w));
+            r = w;
+        }
+        RC(k_nrm2(ctx, n, r, d_h));
         RC(hostsync(1));
         const double beta = std::sqrt(ctx->h_part[0]);
         resid = beta;
         if (resid <= target || iters >= maxit || beta == 0.0) break;
-        RC(k_axpby(ctx, n, (float)(-1.0 / beta), w, 0.0f, nullptr, V));
+        RC(k_axpby(ctx, n, (float)(-1.0 / beta), r, 0.0f, nullptr, V));
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         int i = -1;
@@ -570,8 +590,14 @@ static int gmres_cusp(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
             ++i;
             ++iters;
             float *vi = V + (int64_t)i * n, *vn = V + (int64_t)(i + 1) * n;
-            RC(k_saddle_multiply(ctx, pos, vi, t));
-            RC(k_saddle_precond(ctx, t, w));
+            if (right) {
+                float *zi = Z + (int64_t)i * n;
+                RC(k_saddle_precond(ctx, vi, zi));
+                RC(k_saddle_multiply(ctx, pos, zi, w));
+            } else {
+                RC(k_saddle_multiply(ctx, pos, vi, t));
+                RC(k_saddle_precond(ctx, t, w));
+            }
             hipEvent_t ev;
             prof_begin(ctx, PROF_GMRES_VEC, &ev);
             // CGS2: two passes of h = V^T w, w -= V h
@@ -602,7 +628,7 @@ static int gmres_cusp(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
             g[i] = cs[i] * g[i];
             resid = std::fabs(g[i + 1]);
         } while (!(resid <= target) && iters < maxit && i < R - 1);
-        // y = H^-1 g ; x += V y
+        // y = H^-1 g ; x += V y (left) or Z y (right)
         for (int j = i; j >= 0; --j) {
             double s = g[j];
             for (int k = j + 1; k <= i; ++k) s -= H[(size_t)j * R + k] * yv[k];
@@ -610,7 +636,7 @@ static int gmres_cusp(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
         }
         for (int j = 0; j <= i; ++j) ctx->h_part[j] = -yv[j];
         CK(hipMemcpyAsync(d_h, ctx->h_part, sizeof(double) * (i + 1), hipMemcpyHostToDevice, ctx->stream));
-        RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h, x));
+        RC(k_multi_axpy(ctx, n, i + 1, right ? Z : V, n, d_h, x));
         CK(hipStreamSynchronize(ctx->stream));
         ++restarts;
         if (resid <= target || iters >= maxit) break;
@@ -621,6 +647,206 @@ static int gmres_cusp(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
         st->residual = (float)(resid / (bn > 0 ? bn : 1.0));
     }
     return (resid <= target || bn == 0.0) ? 0 : FSD_W_NOT_CONVERGED;
+}
+
+// ----------------------------------------------------------- Brownian motion
+// Noise streams of one step: far-field real space (11N), far-field grid (6M),
+// near field (6N), random finite difference (6N).
+enum { BRO_FF_REAL = 0, BRO_FF_GRID = 1, BRO_NF = 2, BRO_RFD = 3 };
+
+static int alloc_brownian(fsd_ctx *ctx)
+{
+    if (ctx->bro_alloc) return 0;
+    const int64_t N = ctx->N, n11 = 11 * N;
+    CK(hipMalloc(&ctx->d_lanV, sizeof(float) * (BRO_MMAX + 1) * n11));
+    CK(hipMalloc(&ctx->d_lanw, sizeof(float) * n11));
+    CK(hipMalloc(&ctx->d_lany0, sizeof(float) * n11));
+    CK(hipMalloc(&ctx->d_lan_h, sizeof(double) * 256));
+    CK(hipHostMalloc((void **)&ctx->h_lan, sizeof(double) * 256, hipHostMallocDefault));
+    CK(hipMalloc(&ctx->d_noise, sizeof(float) * (n11 + 6 * (int64_t)ctx->M + 12 * N)));
+    CK(hipMalloc(&ctx->d_posp, sizeof(float4) * N));
+    CK(hipMalloc(&ctx->d_rfd_rhs, sizeof(float) * 17 * N));
+    CK(hipMalloc(&ctx->d_rfd_sol, sizeof(float) * 17 * N));
+    ctx->bro_alloc = true;
+    return 0;
+}
+
+// unit normal noise of one stream: the caller's injected buffer, or Philox
+// normals keyed by (seed, timestep, stream)
+static int bro_noise(fsd_ctx *ctx, int stream, uint32_t timestep, const float **out)
+{
+    const int64_t N = ctx->N, n11 = 11 * N, nG = 6 * (int64_t)ctx->M;
+    const float *inj[4] = {ctx->inj_ff_real, ctx->inj_ff_grid, ctx->inj_nf, ctx->inj_rfd};
+    const int64_t off[4] = {0, n11, n11 + nG, n11 + nG + 6 * N};
+    const int64_t len[4] = {n11, nG, 6 * N, 6 * N};
+    if (inj[stream]) { *out = inj[stream]; return 0; }
+    float *dst = ctx->d_noise + off[stream];
+    RC(k_normals(ctx, len[stream], timestep, (uint32_t)stream, 1.0f, dst));
+    *out = dst;
+    return 0;
+}
+
+// y = |psi| V_m T_m^(1/2) e_1, Brownian_FarField_Chow_Saad /
+// Brownian_NearField_Chow_Saad (Brownian_FarField.cu:597-808,
+// Brownian_NearField.cu:174-389) after Chow and Saad (SIAM J. Sci. Comput.
+// 2014): the Krylov size starts at *m_io (the previous step's) and then grows
+// one vector at a time until the relative change of y drops below tol or the
+// basis holds BRO_MMAX - 1 vectors.  Each new vector is reorthogonalised
+// against the whole basis (two classical Gram-Schmidt passes with FP64 dot
+// products); T_m^(1/2) e_1 is formed in FP64 on the host.
+// op: 0 = real-space far-field mobility (11N), 1 = lubrication R_FU (6N).
+static int lanczos_sqrt(fsd_ctx *ctx, const float4 *pos, int op, const float *psi, float *y, int *m_io, float tol)
+{
+    const int64_t n = (op == 0 ? 11 : 6) * (int64_t)ctx->N;
+    float *V = ctx->d_lanV, *w = ctx->d_lanw, *y0 = ctx->d_lany0;
+    double *dh = ctx->d_lan_h, *hh = ctx->h_lan;
+    const double tol_beta = 1e-8;
+    auto fetch = [&](int cnt) -> int {
+        CK(hipMemcpyAsync(hh, dh, sizeof(double) * cnt, hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    };
+    auto apply = [&](const float *x, float *out) -> int {
+        if (op == 0) return k_mobility(ctx, pos, x, out, 1, nullptr);
+        return k_rfu_apply(ctx, x, out, nullptr, 1.0f);
+    };
+    RC(k_nrm2(ctx, n, psi, dh));
+    RC(fetch(1));
+    const double xnorm = std::sqrt(hh[0]);
+    if (xnorm == 0.0) {
+        RC(k_fill(ctx, n, 0.f, y));
+        return 0;
+    }
+    RC(k_axpby(ctx, n, (float)(1.0 / xnorm), psi, 0.0f, nullptr, V));
+    std::vector<double> alpha, off;
+    bool breakdown = false;
+    // appends alpha_j and (unless the Krylov space is exhausted) off_j and v_{j+1}
+    auto step = [&](int j) -> int {
+        float *vj = V + (int64_t)j * n;
+        RC(apply(vj, w));
+        double a = 0.0;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int c0 = 0; c0 <= j; c0 += 64) {
+                const int K = std::min(64, j + 1 - c0);
+                RC(k_multidot(ctx, n, K, V + (int64_t)c0 * n, n, w, dh));
+                RC(k_multi_axpy(ctx, n, K, V + (int64_t)c0 * n, n, dh, w));
+                if (j >= c0 && j < c0 + K) {
+                    RC(fetch(K));
+                    a += hh[j - c0];
+                }
+            }
+        RC(k_nrm2(ctx, n, w, dh));
+        RC(fetch(1));
+        const double b = std::sqrt(hh[0]);
+        alpha.push_back(a);
+        if (b < tol_beta) { breakdown = true; return 0; }
+        off.push_back(b);
+        RC(k_axpby(ctx, n, (float)(1.0 / b), w, 0.0f, nullptr, V + (int64_t)(j + 1) * n));
+        return 0;
+    };
+    // out = xnorm V_m T_m^(1/2) e_1
+    auto combine = [&](int m, float *out) -> int {
+        std::vector<double> t(m);
+        tridiag_sqrt_e1(m, alpha.data(), off.data(), t.data());
+        RC(k_fill(ctx, n, 0.f, out));
+        for (int c0 = 0; c0 < m; c0 += 64) {
+            const int K = std::min(64, m - c0);
+            for (int k = 0; k < K; ++k) hh[k] = -xnorm * t[c0 + k];
+            CK(hipMemcpyAsync(dh, hh, sizeof(double) * K, hipMemcpyHostToDevice, ctx->stream));
+            RC(k_multi_axpy(ctx, n, K, V + (int64_t)c0 * n, n, dh, out));
+            CK(hipStreamSynchronize(ctx->stream));
+        }
+        return 0;
+    };
+    int m = std::max(1, std::min(*m_io, BRO_MMAX - 2));
+    for (int j = 0; j < m; ++j) {
+        RC(step(j));
+        if (breakdown) { m = j + 1; break; }
+    }
+    RC(combine(m, y0));
+    double err = 1.0;
+    bool wrote = false;
+    while (!breakdown && err > tol && m < BRO_MMAX - 1) {
+        RC(step(m));
+        RC(combine(m + 1, y));
+        wrote = true;
+        RC(k_axpby(ctx, n, 1.0f, y0, -1.0f, y, y0));      // y0 <- y0 - y
+        RC(k_nrm2(ctx, n, y0, dh));
+        RC(k_nrm2(ctx, n, y, dh + 1));
+        RC(fetch(2));
+        err = std::sqrt(hh[0]) / std::max(std::sqrt(hh[1]), 1e-300);
+        RC(k_copy(ctx, n, y, y0));
+        ++m;
+    }
+    if (!wrote) RC(k_copy(ctx, n, y0, y));
+    *m_io = m - 1;
+    return 0;
+}
+
+// far-field Brownian slip U_B^ff (11N) into out: wave space from the grid
+// noise plus sqrt(2T/dt) M_real^(1/2) psi (Brownian_FarField_SlipVelocity,
+// Brownian_FarField.cu:829-1061)
+static int bro_farfield(fsd_ctx *ctx, const float4 *pos, uint32_t timestep, float *out)
+{
+    const float T = ctx->p.T, dt = ctx->p.dt;
+    const fsd_pse_info &ps = ctx->pse;
+    const float *psi = nullptr, *grid_noise = nullptr;
+    RC(bro_noise(ctx, BRO_FF_REAL, timestep, &psi));
+    RC(bro_noise(ctx, BRO_FF_GRID, timestep, &grid_noise));
+    RC(lanczos_sqrt(ctx, pos, 0, psi, out, &ctx->m_lanczos_ff, ctx->p.error));
+    ctx->last_lanczos_ff = ctx->m_lanczos_ff;
+    RC(k_axpby(ctx, 11 * (int64_t)ctx->N, std::sqrt(2.0f * T / dt), out, 0.0f, nullptr, out));
+    const float quadW = ps.gridh_x * ps.gridh_y * ps.gridh_z;
+    RC(k_ff_noise(ctx, grid_noise, std::sqrt(T / dt / quadW)));
+    RC(k_wave_contract(ctx, pos));
+    RC(k_wave_to_general(ctx, out));
+    return 0;
+}
+
+// near-field Brownian force F_B^nf = R_FU^(1/2) psi, psi ~ N(0, 2T/dt) (6N)
+// (Brownian_NearField_Force, Brownian_NearField.cu:407-464)
+static int bro_nearfield(fsd_ctx *ctx, const float4 *pos, uint32_t timestep, float *out)
+{
+    const float *psi = nullptr;
+    RC(bro_noise(ctx, BRO_NF, timestep, &psi));
+    RC(lanczos_sqrt(ctx, pos, 1, psi, out, &ctx->m_lanczos_nf, ctx->p.error));
+    ctx->last_lanczos_nf = ctx->m_lanczos_nf;
+    RC(k_axpby(ctx, 6 * (int64_t)ctx->N, std::sqrt(2.0f * ctx->p.T / ctx->p.dt), out, 0.0f, nullptr, out));
+    return 0;
+}
+
+// Integrator_RFD (Integrator.cu:549-718): drift (T / eps)(U+ - U-) (6N) with
+// U+- the velocities of saddle solves for the right-hand side [0; 0; -psi] at
+// positions wrap(x +- eps psi / 2), keeping this step's neighbour list and
+// preconditioner; the second solve starts from the first's solution.  The
+// lubrication pairs and the spreading set-up are rebuilt for each shifted
+// configuration and restored for x afterwards.
+static int bro_rfd(fsd_ctx *ctx, const float4 *pos, uint32_t timestep, float *drift)
+{
+    const int64_t N = ctx->N, n17 = 17 * N;
+    const float eps = ctx->p.error;
+    const float *psi = nullptr;
+    RC(bro_noise(ctx, BRO_RFD, timestep, &psi));
+    RC(k_fill(ctx, n17, 0.f, ctx->d_rfd_rhs));
+    RC(k_fill(ctx, n17, 0.f, ctx->d_rfd_sol));
+    RC(k_axpby(ctx, 6 * N, -1.0f, psi, 0.0f, nullptr, ctx->d_rfd_rhs + 11 * N));
+    const int maxit = ctx->p.gmres_maxiter, R = ctx->p.gmres_restart;
+    for (int side = 0; side < 2; ++side) {
+        RC(k_shift_wrap(ctx, pos, psi, side == 0 ? 0.5f * eps : -0.5f * eps, ctx->d_posp));
+        RC(k_build_lub_pairs(ctx, ctx->d_posp));
+        ctx->spread_valid = false;
+        fsd_solve_stats st;
+        memset(&st, 0, sizeof st);
+        int rc = gmres(ctx, ctx->d_posp, ctx->d_rfd_rhs, ctx->d_rfd_sol, ctx->p.gmres_tol, maxit, R, &st);
+        if (rc < 0) return rc;
+        if (side == 0) RC(k_copy(ctx, 6 * N, ctx->d_rfd_sol + 11 * N, drift));
+    }
+    const float fac = ctx->p.T / eps;
+    RC(k_axpby(ctx, 6 * N, fac, drift, -fac, ctx->d_rfd_sol + 11 * N, drift));
+    RC(k_build_lub_pairs(ctx, pos));
+    ctx->spread_valid = false;
+    RC(k_setup_spread(ctx, pos));
+    return 0;
 }
 
 static int true_residual(fsd_ctx *ctx, const float4 *pos, const float *b, const float *x, float *out)
@@ -773,6 +999,51 @@ static int ensure_lub(fsd_ctx *ctx, const float4 *pos)
 {
     if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
     if (!ctx->lub_valid) RC(k_build_lub_pairs(ctx, pos));
+    return 0;
+}
+
+int fsd_set_noise(fsd_ctx *ctx, const float *d_ff_real11, const float *d_ff_grid6M, const float *d_nf6,
+                  const float *d_rfd6)
+{
+    ctx->inj_ff_real = d_ff_real11;
+    ctx->inj_ff_grid = d_ff_grid6M;
+    ctx->inj_nf = d_nf6;
+    ctx->inj_rfd = d_rfd6;
+    return 0;
+}
+
+int fsd_brownian_farfield(fsd_ctx *ctx, const float *d_pos4, uint32_t timestep, float *d_out11)
+{
+    const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
+    if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    RC(alloc_brownian(ctx));
+    RC(k_gridk(ctx));
+    ctx->spread_valid = false;
+    RC(k_setup_spread(ctx, pos));
+    RC(bro_farfield(ctx, pos, timestep, d_out11));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_brownian_nearfield(fsd_ctx *ctx, const float *d_pos4, uint32_t timestep, float *d_out6)
+{
+    const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
+    ctx->lub_valid = false;
+    RC(ensure_lub(ctx, pos));
+    RC(alloc_brownian(ctx));
+    RC(bro_nearfield(ctx, pos, timestep, d_out6));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_rfd_drift(fsd_ctx *ctx, const float *d_pos4, uint32_t timestep, float *d_out6)
+{
+    const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
+    if (!ctx->precond_valid) { ctx->err = "preconditioner not built"; return FSD_E_STATE; }
+    RC(alloc_brownian(ctx));
+    RC(k_gridk(ctx));
+    RC(bro_rfd(ctx, pos, timestep, d_out6));
+    CK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 
@@ -1011,7 +1282,6 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     const int64_t n17 = 17 * (int64_t)N;
     float4 *pos = reinterpret_cast<float4 *>(d_pos4);
     if (!ctx->have_table) { ctx->err = "resistance table not set"; return FSD_E_STATE; }
-    if (ctx->p.T > 0.f) { ctx->err = "Brownian motion (T > 0) is not available in this build"; return FSD_E_PARAM; }
     // neighbour list (HOOMD NeighborList::compute) and sheared k vectors
     RC(k_build_nlist(ctx, pos));
     RC(k_gridk(ctx));
@@ -1027,10 +1297,17 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     // right-hand side (Integrator_ComputeVelocity, Integrator.cu:744-860)
     RC(k_fill(ctx, 11 * (int64_t)N, 0.f, d_vel11));
     RC(k_fill(ctx, n17, 0.f, ctx->d_rhs));
-    RC(k_strain_rhs(ctx, ctx->d_rhs, shear_rate, d_ori3, ctx->d_force));
-    RC(k_rfe(ctx, pos, shear_rate, d_ori3, ctx->d_rhs + 11 * (int64_t)N));
     RC(k_build_lub_pairs(ctx, pos));
     RC(k_setup_spread(ctx, pos));
+    if (ctx->p.T > 0.f) {
+        // rhs[0:11N] = U_B^ff, rhs[11N:17N] = F_B^nf, Velocity[0:6N] = RFD drift
+        RC(alloc_brownian(ctx));
+        RC(bro_farfield(ctx, pos, timestep, ctx->d_rhs));
+        RC(bro_nearfield(ctx, pos, timestep, ctx->d_rhs + 11 * (int64_t)N));
+        RC(bro_rfd(ctx, pos, timestep, d_vel11));
+    }
+    RC(k_strain_rhs(ctx, ctx->d_rhs, shear_rate, d_ori3, ctx->d_force));
+    RC(k_rfe(ctx, pos, shear_rate, d_ori3, ctx->d_rhs + 11 * (int64_t)N));
     // saddle point solve, warm-started from the previous step's solution
     fsd_solve_stats st;
     memset(&st, 0, sizeof st);
@@ -1055,8 +1332,14 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     st.lub_pairs = ctx->lub_total;
     st.lower_blocks = ctx->nlow;
     st.levels = ctx->nlev_f;
-    // relaxer bookkeeping (Stokes.cc:1666)
+    st.lanczos_ff = (ctx->p.T > 0.f) ? ctx->last_lanczos_ff : 0;
+    st.lanczos_nf = (ctx->p.T > 0.f) ? ctx->last_lanczos_nf : 0;
+    // relaxer and Krylov-size bookkeeping (Stokes.cc:1658-1666)
     ctx->ichol_relaxer = ((timestep % 5 == 0) || (ctx->ichol_relaxer > 1024.0f)) ? 1.0f : ctx->ichol_relaxer;
+    if (ctx->p.T > 0.f) {
+        ctx->m_lanczos_ff = ((timestep % 100 == 0) || (ctx->m_lanczos_ff > 50)) ? 3 : ctx->m_lanczos_ff;
+        ctx->m_lanczos_nf = ((timestep % 100 == 0) || (ctx->m_lanczos_nf > 50)) ? 3 : ctx->m_lanczos_nf;
+    }
     if (stats) *stats = st;
     return rc;
 }

@@ -252,6 +252,30 @@ class Stokes:
         self._check(self.lib.fsd_force_law(self.ctx, _ptr(pos), float(T_ext), _ptr(F)))
         return F
 
+    # ---- Brownian motion (T > 0)
+    def set_noise(self, ff_real=None, ff_grid=None, nf=None, rfd=None):
+        """inject unit-normal noise (device tensors; None = drawn by the library)"""
+        self._noise = (ff_real, ff_grid, nf, rfd)      # keep the buffers alive
+        self._check(self.lib.fsd_set_noise(self.ctx, _ptr(ff_real), _ptr(ff_grid), _ptr(nf), _ptr(rfd)))
+
+    def brownian_farfield(self, pos, timestep=0):
+        out = self._out(11 * self.N)
+        self._sync()
+        self._check(self.lib.fsd_brownian_farfield(self.ctx, _ptr(pos), int(timestep), _ptr(out)))
+        return out
+
+    def brownian_nearfield(self, pos, timestep=0):
+        out = self._out(6 * self.N)
+        self._sync()
+        self._check(self.lib.fsd_brownian_nearfield(self.ctx, _ptr(pos), int(timestep), _ptr(out)))
+        return out
+
+    def rfd_drift(self, pos, timestep=0):
+        out = self._out(6 * self.N)
+        self._sync()
+        self._check(self.lib.fsd_rfd_drift(self.ctx, _ptr(pos), int(timestep), _ptr(out)))
+        return out
+
     def get_ewald_table(self):
         info = self.pse_info()
         t = np.zeros(8 * (info["ewald_n"] + 1), dtype=np.float32)

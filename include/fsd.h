@@ -89,6 +89,8 @@ typedef struct fsd_solve_stats {
     int64_t lub_pairs;               /* lubrication pair records (r < rlub) */
     int64_t lower_blocks;            /* 6x6 blocks of the IC(0) factor below the diagonal */
     int32_t levels;                  /* level-schedule depth of the triangular solves */
+    int32_t lanczos_ff;              /* Krylov size of the far-field Brownian sampler (T > 0) */
+    int32_t lanczos_nf;              /* Krylov size of the near-field Brownian sampler (T > 0) */
 } fsd_solve_stats;
 
 /* ---- lifecycle: Stokes::Stokes + setParams + setSparseMath + AllocateWorkSpaces
@@ -171,6 +173,31 @@ int fsd_saddle_solve(fsd_ctx *ctx, const float *d_pos4, const float *d_rhs17, fl
 
 /* Stokes_SetForce_manually_kernel (Helper_Stokes.cu:65-172) */
 int fsd_force_law(fsd_ctx *ctx, const float *d_pos4, float T_ext, float *d_F6);
+
+/* ---- Brownian motion (T > 0) ----
+ * Noise: by default the library draws unit normals itself (counter-based
+ * Philox-4x32-10 keyed by seed, timestep and stream), where the reference
+ * draws cuRAND Philox streams seeded per step (Stokes.cc:1437-1441).
+ * fsd_set_noise injects the unit normals instead (device pointers kept until
+ * the next call; NULL = draw that stream again), which is how results are
+ * compared with other implementations:
+ *   d_ff_real11  11N  far-field real-space psi, generalised layout [F,T | S]
+ *   d_ff_grid6M  6M   far-field grid noise, read at [q, q+6) for grid point q
+ *                     (q = idx with parity = 1, the reference's reads; 6 idx otherwise)
+ *   d_nf6        6N   near-field psi (scaled by sqrt(2T/dt) inside)
+ *   d_rfd6       6N   random-finite-difference psi */
+int fsd_set_noise(fsd_ctx *ctx, const float *d_ff_real11, const float *d_ff_grid6M, const float *d_nf6,
+                  const float *d_rfd6);
+/* Brownian_FarField_SlipVelocity (Brownian_FarField.cu:829-1061): d_out11 = U_B^ff
+ * (wave space from the grid noise + sqrt(2T/dt) M_real^(1/2) psi by Lanczos);
+ * needs the neighbour list. */
+int fsd_brownian_farfield(fsd_ctx *ctx, const float *d_pos4, uint32_t timestep, float *d_out11);
+/* Brownian_NearField_Force (Brownian_NearField.cu:407-464): d_out6 = R_FU^(1/2) psi,
+ * psi ~ N(0, 2T/dt); needs the neighbour list. */
+int fsd_brownian_nearfield(fsd_ctx *ctx, const float *d_pos4, uint32_t timestep, float *d_out6);
+/* Integrator_RFD (Integrator.cu:549-718): d_out6 = (T/eps)(U(x + eps psi/2) - U(x - eps psi/2)),
+ * eps = error; needs the neighbour list and the preconditioner. */
+int fsd_rfd_drift(fsd_ctx *ctx, const float *d_pos4, uint32_t timestep, float *d_out6);
 /* Integrator_ExplicitEuler_Shear_kernel (Integrator.cu:126-228) */
 int fsd_euler_shear(fsd_ctx *ctx, float *d_pos4, float *d_ori3, float *d_pos_gb4, int32_t *d_image3,
                     const float *d_vel6, const float *d_mask1, const float *d_noise3,

@@ -506,17 +506,10 @@ static void real_space_i(int i, const float *pos4, const orc_box *b, const orc_p
 }
 
 /* PSE wave-space part (gpu_stokes_Mwave_wrap and kernels, Mobility.cu:131-1020). */
-static void wave_space(int N, const float *pos4, const orc_box *b, const orc_pse *p, const double *gk,
-                       const double *F3, const double *C8, double *u3, double *D8)
+/* per particle stencil origin, computed in float as the reference (Mobility.cu:203-225) */
+static int *stencil_origins(int N, const float *pos4, const orc_box *b, const orc_pse *p)
 {
     const int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz, P = p->P;
-    const long M = (long)Nx * Ny * Nz;
-    const double xisq = (double)p->xi * p->xi;
-    const double prefac = pow(2.0 * xisq / M_PI / p->eta, 1.5);
-    const double expfac = 2.0 * xisq / p->eta;
-    const double quadW = (double)p->gridh[0] * p->gridh[1] * p->gridh[2];
-    double complex *g = calloc((size_t)11 * M, sizeof(double complex));
-    /* per particle stencil: origin in float as the reference */
     int *org = malloc(sizeof(int) * 3 * N);
     for (int i = 0; i < N; ++i) {
         float f[3];
@@ -528,6 +521,22 @@ static void wave_space(int N, const float *pos4, const orc_box *b, const orc_pse
         org[3 * i + 1] = y - Pd2 + 1 - (P % 2) * ((fy - (float)y) < 0.5f);
         org[3 * i + 2] = z - Pd2 + 1 - (P % 2) * ((fz - (float)z) < 0.5f);
     }
+    return org;
+}
+
+static void contract(int N, const float *pos4, const orc_box *b, const orc_pse *p, const int *org,
+                     const double complex *g, double *u3, double *D8);
+
+static void wave_space(int N, const float *pos4, const orc_box *b, const orc_pse *p, const double *gk,
+                       const double *F3, const double *C8, double *u3, double *D8)
+{
+    const int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz, P = p->P;
+    const long M = (long)Nx * Ny * Nz;
+    const double xisq = (double)p->xi * p->xi;
+    const double prefac = pow(2.0 * xisq / M_PI / p->eta, 1.5);
+    const double expfac = 2.0 * xisq / p->eta;
+    double complex *g = calloc((size_t)11 * M, sizeof(double complex));
+    int *org = stencil_origins(N, pos4, b, p);
     const double Lx = b->Lx, Ly = b->Ly, Lz = b->Lz;
     /* spread (serial: grid accumulation) */
     for (int i = 0; i < N; ++i) {
@@ -600,7 +609,22 @@ static void wave_space(int N, const float *pos4, const orc_box *b, const orc_pse
         g[9 * M + id] = out_d[2][0]; g[10 * M + id] = out_d[2][1];
     }
     for (int c = 0; c < 11; ++c) fft3(g + c * M, Nx, Ny, Nz, +1);
-    /* contraction (Mobility_WaveSpace_ContractU/D, Mobility.cu:503-841) */
+    contract(N, pos4, b, p, org, g, u3, D8);
+    free(org);
+    free(g);
+}
+
+/* contraction of the 11 real parts (Mobility_WaveSpace_ContractU/D, Mobility.cu:503-841) */
+static void contract(int N, const float *pos4, const orc_box *b, const orc_pse *p, const int *org,
+                     const double complex *g, double *u3, double *D8)
+{
+    const int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz, P = p->P;
+    const long M = (long)Nx * Ny * Nz;
+    const double xisq = (double)p->xi * p->xi;
+    const double prefac = pow(2.0 * xisq / M_PI / p->eta, 1.5);
+    const double expfac = 2.0 * xisq / p->eta;
+    const double quadW = (double)p->gridh[0] * p->gridh[1] * p->gridh[2];
+    const double Lx = b->Lx, Ly = b->Ly, Lz = b->Lz;
 #pragma omp parallel for
     for (int i = 0; i < N; ++i) {
         double uu[3] = {0, 0, 0}, dd[8] = {0};
@@ -630,8 +654,77 @@ static void wave_space(int N, const float *pos4, const orc_box *b, const orc_pse
         D8[8 * i + 0] = dd[0]; D8[8 * i + 1] = dd[5]; D8[8 * i + 2] = dd[6]; D8[8 * i + 3] = dd[7];
         D8[8 * i + 4] = dd[4]; D8[8 * i + 5] = dd[1]; D8[8 * i + 6] = dd[2]; D8[8 * i + 7] = dd[3];
     }
-    free(org);
-    free(g);
+}
+
+/* Far-field Brownian slip, wave-space part (Brownian_FarField.cu:113-395, 940-1011):
+ * Grid1of2 places Gaussian noise on a Hermitian force grid (variance scaled by
+ * T / (dt quadW)), Grid2of2 applies the square root of the wave-space Green's
+ * function, then the 11 grids are inverse-transformed and contracted.  The
+ * reference writes the grid from all threads at once and races where a point
+ * and its conjugate both belong to the written half (the Nyquist planes);
+ * here the writes happen in ascending index order, so the later point wins and
+ * the grid is exactly Hermitian.  noise: 6M floats read at [q, q+6) with
+ * q = idx (parity: the reference's overlapping reads) or 6 idx. */
+void orc_bro_ff_wave(int N, const float *pos4, const orc_box *b, const orc_pse *p, const double *gk,
+                     const float *noise, int parity, double T, double dt, double *gU)
+{
+    const int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+    const long M = (long)Nx * Ny * Nz;
+    const double quadW = (double)p->gridh[0] * p->gridh[1] * p->gridh[2];
+    const double fac = sqrt(T / dt / quadW);
+    double complex *g = calloc((size_t)11 * M, sizeof(double complex));
+    for (long idx = 0; idx < M; ++idx) {
+        const int kk = (int)(idx % Nz), jj = (int)((idx / Nz) % Ny), ii = (int)(idx / ((long)Ny * Nz));
+        if (2 * kk >= Nz + 1) continue;
+        if (kk == 0 && 2 * jj >= Ny + 1) continue;
+        if (kk == 0 && jj == 0 && 2 * ii >= Nx + 1) continue;
+        if (kk == 0 && jj == 0 && ii == 0) continue;
+        const long q = parity ? idx : 6 * idx;
+        const double re[3] = {fac * noise[q], fac * noise[q + 1], fac * noise[q + 2]};
+        const double im[3] = {fac * noise[q + 3], fac * noise[q + 4], fac * noise[q + 5]};
+        const int in = (ii == Nx / 2) && (Nx % 2 == 0), jn = (jj == Ny / 2) && (Ny % 2 == 0),
+                  kn = (kk == Nz / 2) && (Nz % 2 == 0);
+        const int ic = (ii == 0 || in) ? ii : Nx - ii, jc = (jj == 0 || jn) ? jj : Ny - jj,
+                  kc = (kk == 0 || kn) ? kk : Nz - kk;
+        const long cidx = ((long)ic * Ny + jc) * Nz + kc;
+        const int special = (ii == 0 || in) && (jj == 0 || jn) && (kk == 0 || kn);
+        for (int a = 0; a < 3; ++a) {
+            if (special) {
+                g[a * M + idx] = sqrt(2.0) * re[a];
+            } else {
+                g[a * M + idx] = re[a] + I * im[a];
+                g[a * M + cidx] = re[a] - I * im[a];
+            }
+        }
+    }
+#pragma omp parallel for
+    for (long idx = 0; idx < M; ++idx) {
+        const double *tk = gk + 4 * idx;
+        const double ksq = tk[0] * tk[0] + tk[1] * tk[1] + tk[2] * tk[2];
+        const double k = sqrt(ksq);
+        const double B = (idx == 0) ? 0.0 : sqrt(tk[3]);
+        const double SU = (idx == 0) ? 0.0 : sin(k) / k;
+        const double SD = (idx == 0) ? 0.0 : 3.0 * (sin(k) - k * cos(k)) / (ksq * k);
+        double complex f[3] = {g[idx], g[M + idx], g[2 * M + idx]};
+        double complex kdF = (idx == 0) ? 0.0 : (tk[0] * f[0] + tk[1] * f[1] + tk[2] * f[2]) / ksq;
+        double complex W[3];
+        for (int a = 0; a < 3; ++a) W[a] = (f[a] - tk[a] * kdF) * B;
+        for (int a = 0; a < 3; ++a) g[a * M + idx] = SU * W[a];
+        /* gradient grids in the order XX XY XZ YZ YY YX ZX ZY: component (a, c) = i SD W_a k_c */
+        static const int sa[8] = {0, 0, 0, 1, 1, 1, 2, 2}, sc[8] = {0, 1, 2, 2, 1, 0, 0, 1};
+        for (int s2 = 0; s2 < 8; ++s2) g[(3 + s2) * M + idx] = I * SD * W[sa[s2]] * tk[sc[s2]];
+    }
+    for (int c = 0; c < 11; ++c) fft3(g + c * M, Nx, Ny, Nz, +1);
+    int *org = stencil_origins(N, pos4, b, p);
+    double *u3 = calloc(3 * (size_t)N, sizeof(double)), *D8 = calloc(8 * (size_t)N, sizeof(double));
+    contract(N, pos4, b, p, org, g, u3, D8);
+    for (int i = 0; i < N; ++i) {
+        double Wv[3], E5[5];
+        d2we(D8 + 8 * i, Wv, E5);
+        for (int a = 0; a < 3; ++a) { gU[6 * i + a] = u3[3 * i + a]; gU[6 * i + 3 + a] = Wv[a]; }
+        for (int c = 0; c < 5; ++c) gU[6 * (size_t)N + 5 * i + c] = E5[c];
+    }
+    free(org); free(u3); free(D8); free(g);
 }
 
 /* Mobility_GeneralizedMobility (Mobility.cu:1488-1542) */

@@ -72,6 +72,10 @@ void orc_rse(int N, const float *pos4, const orc_box *b, const int *nneigh, cons
 
 /* ---- far-field mobility (Mobility.cu) ---- */
 /* generalised mobility: gF 11N (F,T | S) -> gU 11N (U,W | E5) */
+/* far-field Brownian slip, wave-space part: noise 6M floats -> gU 11N
+ * (Brownian_FarField.cu:113-395, 940-1011; serial, deterministic Nyquist writes) */
+void orc_bro_ff_wave(int N, const float *pos4, const orc_box *b, const orc_pse *p, const double *gk,
+                     const float *noise, int parity, double T, double dt, double *gU);
 void orc_mobility(int N, const float *pos4, const orc_box *b, const orc_pse *p, const float *ewald_tab,
                   const double *gridk4, const int *nneigh, const int *head, const int *nbrs,
                   const double *gF, double *gU, int parts /* 1 real, 2 wave, 3 both */);

@@ -77,6 +77,8 @@ def load():
         lib.orc_saddle_precond.argtypes = [C.POINTER(System), _P, _P]
         lib.orc_gmres.argtypes = [C.POINTER(System), _P, _P, C.c_int, C.c_int, C.c_double, C.c_int,
                                   C.POINTER(C.c_double)]
+        lib.orc_bro_ff_wave.argtypes = [C.c_int, _P, C.POINTER(Box), C.POINTER(Pse), _P, _P, C.c_int, C.c_double,
+                                        C.c_double, _P]
         lib.orc_force_law.argtypes = [C.c_int, _P, C.POINTER(Box), _P, _P, _P, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double, _P]
         lib.orc_euler_shear.argtypes = [C.c_int, _P, _P, _P, _P, C.POINTER(Box), _P, C.c_double, _P, _P,
@@ -277,6 +279,57 @@ class Oracle:
                                 1 if use_precond else 0, C.byref(r))
         return x, it, r.value
 
+    # ---- Brownian motion
+    def bro_ff_wave(self, noise, T, dt, parity=True):
+        """wave-space far-field Brownian slip (11N) from grid noise (6M floats)"""
+        noise = np.ascontiguousarray(noise, dtype=np.float32)
+        gU = np.zeros(11 * self.N)
+        self.lib.orc_bro_ff_wave(self.N, _p(self.pos4), C.byref(self.box), C.byref(self.pse), _p(self.gridk),
+                                 _p(noise), 1 if parity else 0, T, dt, _p(gU))
+        return gU
+
+    def bro_ff_real(self, psi, T, dt, tol=1e-10):
+        """real-space far-field Brownian slip sqrt(2T/dt) M_real^(1/2) psi (11N),
+        converged FP64 Lanczos (Brownian_FarField.cu:493-808)"""
+        y, m = lanczos_sqrt(lambda v: self.mobility(v, parts=1), psi, tol)
+        return np.sqrt(2.0 * T / dt) * y
+
+    def bro_nf(self, psi, tol=1e-10):
+        """near-field Brownian force R_FU^(1/2) psi (6N); psi carries its
+        sqrt(2T/dt) (Brownian_NearField.cu:112-464, operator Lubrication_RFU_kernel)"""
+        y, m = lanczos_sqrt(self.rfu, psi, tol)
+        return y
+
+    def shifted(self, dpos3):
+        """a copy at positions wrapped(pos + dpos3) that keeps this configuration's
+        neighbour list and preconditioner (what Integrator_RFD evaluates)"""
+        import copy
+        o = copy.copy(self)
+        p4 = self.pos4.copy()
+        for i in range(self.N):
+            w = (C.c_float * 3)(*[float(p4[i, a] + dpos3[i, a]) for a in range(3)])
+            img = (C.c_int * 3)(0, 0, 0)
+            self.lib.orc_wrap(C.byref(self.box), w, img)
+            p4[i, :3] = list(w)
+        o.pos4 = np.ascontiguousarray(p4)
+        o.L = self.L
+        o.A = None          # the copy does not own the factor
+        o._owner = self
+        return o
+
+    def rfd_drift(self, psi6, T, eps, tol=1e-11):
+        """Integrator_RFD (Integrator.cu:549-718): (T/eps) [U(x + eps psi/2) - U(x - eps psi/2)]
+        for the saddle right-hand side [0; 0; -psi]"""
+        N = self.N
+        b = np.zeros(17 * N)
+        b[11 * N:] = -np.asarray(psi6, dtype=np.float64)
+        d3 = np.asarray(psi6, dtype=np.float64).reshape(N, 6)[:, :3]
+        up = self.shifted(0.5 * eps * d3)
+        xp, _, _ = up.gmres(b, tol=tol)
+        um = self.shifted(-0.5 * eps * d3)
+        xm, _, _ = um.gmres(b, x0=xp, tol=tol)
+        return (T / eps) * (xp[11 * N:] - xm[11 * N:])
+
     # ---- force / integration
     def force_law(self, ndsr=0.1, k_n=1e4, kappa=20.0, beta=0.0, epsq=1e-5, T_ext=0.0):
         F = np.zeros(6 * self.N)
@@ -294,6 +347,8 @@ class Oracle:
         return b
 
     def __del__(self):
+        if getattr(self, "_owner", None) is not None:
+            return          # a shifted copy: the factor belongs to its parent
         try:
             if self.L is not None:
                 self.lib.orc_csr_free(C.byref(self.L))
@@ -301,6 +356,38 @@ class Oracle:
                 self.lib.orc_csr_free(C.byref(self.A))
         except Exception:
             pass
+
+
+def lanczos_sqrt(apply, psi, tol=1e-10, mmax=400):
+    """|psi| V_m T_m^(1/2) e_1 by Lanczos with full FP64 reorthogonalisation
+    (Chow & Saad 2014), iterated until the relative change of the result is
+    below tol.  Returns (y, m)."""
+    psi = np.asarray(psi, dtype=np.float64)
+    nrm = np.linalg.norm(psi)
+    if nrm == 0:
+        return np.zeros_like(psi), 0
+    V = [psi / nrm]
+    alpha, beta = [], []
+    y_old = None
+    for m in range(1, mmax + 1):
+        w = apply(V[-1])
+        a = float(V[-1] @ w)
+        w = w - a * V[-1] - (beta[-1] * V[-2] if beta else 0.0)
+        for v in V:
+            w -= (v @ w) * v
+        alpha.append(a)
+        T = np.diag(alpha) + np.diag(beta, 1) + np.diag(beta, -1)
+        lam, Q = np.linalg.eigh(T)
+        y = nrm * (np.array(V).T @ (Q @ (np.sqrt(np.maximum(lam, 0.0)) * Q[0])))
+        b = np.linalg.norm(w)
+        if y_old is not None and np.linalg.norm(y - y_old) <= tol * np.linalg.norm(y):
+            return y, m
+        if b < 1e-14 * abs(a):
+            return y, m
+        y_old = y
+        beta.append(b)
+        V.append(w / b)
+    return y, mmax
 
 
 def pse_params(L, xi=0.5, error=1e-3, max_strain=0.5):

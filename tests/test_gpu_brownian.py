@@ -1,0 +1,116 @@
+"""GPU parity of the Brownian pieces (T > 0) against the FP64 oracle with the
+same injected noise: the far-field wave-space slip (exact up to FP32), the
+far-field real-space and near-field Lanczos square roots (to the sampler's
+tolerance, PSE error 1e-3), the random finite difference, and one full step.
+"""
+import numpy as np
+import pytest
+
+from tests._cases import packing, rel_err, rel_err2, forces
+
+pytestmark = pytest.mark.gpu
+
+N_, PHI, SEED = 343, 0.2, 11
+T, DT = 1.0, 1e-3
+
+
+def _setup(**kw):
+    import torch
+    from fsd_amd.stokes import Stokes
+    from oracle.oracle import Oracle
+    pos, L = packing(N_, PHI, SEED)
+    st = Stokes(N_, L, T=T, dt=DT, **kw)
+    p4 = np.zeros((N_, 4), dtype=np.float32)
+    p4[:, :3] = pos
+    dpos = torch.from_numpy(p4.reshape(-1)).cuda()
+    st.build_nlist(dpos)
+    return st, Oracle(pos, L), dpos, pos, L
+
+
+def _t(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
+
+
+def _noise(seed, orc):
+    rng = np.random.default_rng(seed)
+    M = orc.pse.Nx * orc.pse.Ny * orc.pse.Nz
+    return (rng.standard_normal(11 * N_).astype(np.float32), rng.standard_normal(6 * M).astype(np.float32),
+            rng.standard_normal(6 * N_).astype(np.float32), rng.standard_normal(6 * N_).astype(np.float32))
+
+
+def test_farfield_wave_space():
+    """grid noise -> sqrt(Green) -> inverse FFT -> contraction equals the oracle (FP32 vs FP64)"""
+    st, orc, dpos, _, _ = _setup()
+    ffr, ffg, nf, rfd = _noise(1, orc)
+    st.set_noise(_t(np.zeros_like(ffr)), _t(ffg), None, None)
+    u = st.brownian_farfield(dpos).cpu().numpy()
+    uo = orc.bro_ff_wave(ffg, T, DT, parity=True)
+    e = rel_err(u, uo)
+    print("wave-space Brownian slip rel err", e)
+    assert e < 1e-4
+
+
+def test_farfield_real_space_lanczos():
+    """sqrt(2T/dt) M_real^(1/2) psi by the GPU Lanczos sampler vs the converged FP64 oracle"""
+    st, orc, dpos, _, _ = _setup()
+    ffr, ffg, nf, rfd = _noise(2, orc)
+    st.set_noise(_t(ffr), _t(np.zeros_like(ffg)), None, None)
+    u = st.brownian_farfield(dpos).cpu().numpy()
+    uo = orc.bro_ff_real(ffr, T, DT)
+    e = rel_err2(u, uo)
+    print("real-space Brownian slip rel err", e)
+    assert e < 5e-3
+
+
+def test_nearfield_lanczos():
+    """R_FU^(1/2) psi (psi ~ N(0, 2T/dt)) vs the converged FP64 oracle"""
+    st, orc, dpos, _, _ = _setup()
+    ffr, ffg, nf, rfd = _noise(3, orc)
+    st.set_noise(None, None, _t(nf), None)
+    f = st.brownian_nearfield(dpos).cpu().numpy()
+    fo = orc.bro_nf(np.sqrt(2 * T / DT) * nf.astype(np.float64))
+    e = rel_err2(f, fo)
+    print("near-field Brownian force rel err", e)
+    assert e < 1e-2
+
+
+def test_rfd_drift():
+    """the random-finite-difference drift vs the oracle's two FP64 solves; the
+    difference of two FP32 solves over eps = 1e-3 limits the agreement"""
+    st, orc, dpos, _, _ = _setup(gmres_tol=1e-6)
+    st.build_precond(dpos)
+    orc.build_precond()
+    ffr, ffg, nf, rfd = _noise(4, orc)
+    st.set_noise(None, None, None, _t(rfd))
+    d = st.rfd_drift(dpos).cpu().numpy()
+    do = orc.rfd_drift(rfd.astype(np.float64), T, 1e-3)
+    e = rel_err2(d, do)
+    print("RFD drift rel err", e, "|drift|", np.linalg.norm(do))
+    assert e < 5e-2
+
+
+def test_brownian_step():
+    """one fsd_step at T = 1 with injected noise and forces: velocities equal the
+    oracle's composition of the same pieces (Integrator_ComputeVelocity)"""
+    import torch
+    from fsd_amd.stokes import ParticleState
+    st, orc, dpos, pos, L = _setup(gmres_tol=1e-6)
+    ffr, ffg, nf, rfd = _noise(5, orc)
+    st.set_noise(_t(ffr), _t(ffg), _t(nf), _t(rfd))
+    F = forces(N_, 23)
+    state = ParticleState(pos)
+    info = st.integrateStepOne(0, state, force6=torch.from_numpy(F).cuda(), shear_rate=0.0)
+    vel = state.velocity.cpu().numpy()[:6 * N_]
+    orc.build_precond()
+    b = np.zeros(17 * N_)
+    b[:11 * N_] = orc.bro_ff_wave(ffg, T, DT, parity=True) + orc.bro_ff_real(ffr, T, DT)
+    b[11 * N_:] = orc.bro_nf(np.sqrt(2 * T / DT) * nf.astype(np.float64))
+    b[11 * N_:] -= F
+    b[11 * N_:] = orc.rfe(0.0, F=b[11 * N_:])
+    x, it, res = orc.gmres(b, tol=1e-11)
+    ref = x[11 * N_:] + orc.rfd_drift(rfd.astype(np.float64), T, 1e-3)
+    e = rel_err2(vel, ref)
+    print("Brownian step", info, "velocity rel err", e)
+    assert info["converged"] and info["lanczos_ff"] > 0 and info["lanczos_nf"] > 0
+    assert e < 2e-2

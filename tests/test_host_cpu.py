@@ -73,6 +73,15 @@ def test_sweep_schedule_emulation(tmp_path, args):
     assert "unready reads=0, oob=0" in r.stdout
 
 
+def test_tridiagonal_square_root(tmp_path):
+    """FP64 T^(1/2) e_1 of the Lanczos sampler (lanczos_host.cpp) satisfies
+    t^T t = T_11 and t^T T t = (T^2)_11 up to m = 99"""
+    exe = str(tmp_path / "tridiag_sqrt_check")
+    _gxx(exe, os.path.join(CPU, "tridiag_sqrt_check.cpp"), os.path.join(CSRC, "lanczos_host.cpp"))
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
 # ------------------------------------------------------------ library surface
 def _header_symbols():
     txt = open(os.path.join(ROOT, "include", "fsd.h")).read()
