@@ -93,7 +93,20 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (diagnostics)")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MB",
+                    help="only run 5 device copies of MB megabytes (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        n = args.calib_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print("# copy calibration: 5 copies of %d bytes (read %d + write %d per copy)" % (4 * n, 4 * n, 4 * n),
+              file=sys.stderr)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU: the FSD step of one suspension is not sharded across GPUs "
               "in this build (see the design notes on sharding)", file=sys.stderr)

@@ -94,6 +94,7 @@ SIGNATURES = [
     ("fsd_get_pse_info", _I, [_P, C.POINTER(FsdPseInfo)]),
     ("fsd_set_resistance_table", _I, [_P, _P, _P]),
     ("fsd_set_friction", _I, [_P, C.c_char_p, _F, _P]),
+    ("fsd_get_resistance_table", _I, [_P, _P]),
     ("fsd_set_temperature", _I, [_P, _F]),
     ("fsd_set_box", _I, [_P, _F, _F, _F, _F, _F, _F]),
     ("fsd_get_ewald_table", _I, [_P, _P, _I]),

@@ -132,6 +132,7 @@ struct fsd_ctx {
     float *d_ewald = nullptr; int ewald_nR = 0;
     std::vector<float> h_ewald;
     float *d_res_dist = nullptr, *d_res_vals = nullptr; bool have_table = false;
+    std::vector<float> h_res_dist, h_res_vals;   // host copy (friction modifies the values)
     float table_min = 0.0001f, table_dr = 0.004305f;
 
     // neighbour list

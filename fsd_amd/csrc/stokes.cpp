@@ -930,6 +930,8 @@ int fsd_get_ewald_table(const fsd_ctx *ctx, float *h_tab, int32_t capacity)
 
 int fsd_set_resistance_table(fsd_ctx *ctx, const float *h_dist, const float *h_vals)
 {
+    ctx->h_res_dist.assign(h_dist, h_dist + 1000);
+    ctx->h_res_vals.assign(h_vals, h_vals + 22000);
     CK(hipMemcpy(ctx->d_res_dist, h_dist, sizeof(float) * 1000, hipMemcpyHostToDevice));
     CK(hipMemcpy(ctx->d_res_vals, h_vals, sizeof(float) * 22000, hipMemcpyHostToDevice));
     ctx->have_table = true;
@@ -938,14 +940,43 @@ int fsd_set_resistance_table(fsd_ctx *ctx, const float *h_dist, const float *h_v
     return 0;
 }
 
+// Stokes::setFriction (Stokes.cc:1132-1191).  TYPE1 adds, for gaps h <= h0,
+// alpha_k (2 h^2 / h0^3 - 3 h / h0^2 + 1 / h) to six tangential resistance
+// functions (zero at h = h0).  The reference's comments name YA11, YA12,
+// YB11, YB12, YC11, YC12 (columns 2, 3, 4, 5, 8, 9) but its code writes
+// columns 3, 4, 5, 6, 9, 10; parity = 1 keeps the columns the code writes.
 int fsd_set_friction(fsd_ctx *ctx, const char *type, float h0, const float *h_alpha6)
 {
-    (void)h0; (void)h_alpha6;
     std::string t = type ? type : "NONE";
     for (auto &c : t) c = (char)toupper(c);
     if (t == "NONE") return 0;
-    ctx->err = "friction type " + t + " is not supported";
-    return FSD_E_PARAM;
+    if (t != "TYPE1") { ctx->err = "Invalid Friction Type. Allowable types are: None, Type1"; return FSD_E_PARAM; }
+    if (!ctx->have_table) { ctx->err = "setFriction must follow setResistanceTable"; return FSD_E_STATE; }
+    static const int cols_code[6] = {3, 4, 5, 6, 9, 10}, cols_named[6] = {2, 3, 4, 5, 8, 9};
+    static const float sgn[6] = {1.f, -1.f, -1.f, 1.f, 1.f, 1.f};
+    const int *cols = ctx->p.parity ? cols_code : cols_named;
+    for (int ii = 0; ii < 1000; ++ii) {
+        const float h = (float)((double)ctx->h_res_dist[ii] - 2.0);
+        if (!(h <= h0)) continue;
+        const float h2 = h * h, h02 = h0 * h0, h03 = h0 * h02;
+        const double poly = 2.0 / h03 * h2 - 3.0 / h02 * h + 1.0 / h;
+        for (int k = 0; k < 6; ++k) {
+            const float add = (float)(h_alpha6[k] * poly);
+            ctx->h_res_vals[22 * ii + cols[k]] += sgn[k] * add;
+        }
+    }
+    CK(hipMemcpy(ctx->d_res_vals, ctx->h_res_vals.data(), sizeof(float) * 22000, hipMemcpyHostToDevice));
+    ctx->lub_valid = false;
+    ctx->precond_valid = false;
+    return 0;
+}
+
+// the resistance table in use (after friction), h_vals[22000]
+int fsd_get_resistance_table(const fsd_ctx *ctx, float *h_vals)
+{
+    if (!ctx->have_table) return FSD_E_STATE;
+    std::copy(ctx->h_res_vals.begin(), ctx->h_res_vals.end(), h_vals);
+    return 0;
 }
 
 int fsd_set_temperature(fsd_ctx *ctx, float T)

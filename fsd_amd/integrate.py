@@ -29,6 +29,8 @@ class PSEv3:
         self.system = group.system
         self.group = group
         self.period = int(period)
+        self.output_dir = extra.pop("output_dir", ".")
+        self._force_params = (ndsr, kappa, k_n, beta_AF, epsq)
         self.rcut = math.sqrt(-math.log(error)) / xi
         self._T = T
         T0 = T(self.system.timestep) if callable(T) else float(T)
@@ -71,13 +73,29 @@ class PSEv3:
             self.system.methods.remove(self)
 
     def integrate_step(self, timestep):
-        """one Stokes::integrateStepOne at `timestep`"""
+        """one Stokes::integrateStepOne at `timestep`, with the reference's output
+        before step 0 and after steps whose timestep + 1 is a multiple of period
+        (Stokes.cc:1349-1352, 1669-1672)"""
         if callable(self._T):
             self.cpp_method.setT(self._T(timestep))
+        if self.period > 0 and timestep == 0:
+            self.cpp_method.build_nlist(self.system.state.pos)
+            self._output(0)
         rate = self.function_form.get_shear_rate(timestep)
         info = self.cpp_method.integrateStepOne(timestep, self.system.state, shear_rate=rate)
         self.cpp_method.integrateStepTwo(timestep)
+        if self.period > 0 and (timestep + 1) % self.period == 0:
+            self._output(timestep + 1)
         return info
+
+    def _output(self, timestep):
+        from .output import write_output
+        s = self.system.state
+        nn, nb = self.cpp_method.get_nlist()
+        ndsr, kappa, k_n, beta, epsq = self._force_params
+        write_output(timestep, s.pos.view(-1, 4).cpu().numpy(), s.ori.cpu().numpy(),
+                     s.pos_gb.view(-1, 4).cpu().numpy(), s.velocity.cpu().numpy(), nn, nb, self.system.box, ndsr,
+                     kappa, k_n, beta, epsq, directory=self.output_dir)
 
 
 # the name some FSD scripts use for the same integrator

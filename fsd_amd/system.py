@@ -37,15 +37,57 @@ class System:
         self.dt = float(dt)
         self.timestep = int(timestep)
         self.methods = []
+        self.tilt_variant = None
         self.last_info = None
         _current = self
 
     def group_all(self):
         return ParticleGroup(self)
 
+    def box_resize(self, xy):
+        """deform the box each timestep: tilt factor xy = xy(timestep) (a
+        variant.shear_variant or any callable); particles keep their Cartesian
+        positions (the flow already advects them) and are re-wrapped"""
+        self.tilt_variant = xy
+
+    def set_tilt(self, xy):
+        """new xy tilt factor for the box, the integrators and the particle images"""
+        self.box = (self.box[0], self.box[1], self.box[2], float(xy), self.box[4], self.box[5])
+        for m in self.methods:
+            m.cpp_method.set_box_tilt(xy=self.box[3], xz=self.box[4], yz=self.box[5])
+        self._wrap()
+
+    def _wrap(self):
+        """HOOMD BoxDim::wrap of every particle into the current box (images updated)"""
+        import torch
+        Lx, Ly, Lz, xy, xz, yz = self.box
+        p = self.state.pos.view(-1, 4)
+        img = self.state.image.view(-1, 3)
+        x, y, z = p[:, 0].clone(), p[:, 1].clone(), p[:, 2].clone()
+        tilt_x = (xz - xy * yz) * z + xy * y
+        hi, lo = x >= 0.5 * Lx + tilt_x, x < -0.5 * Lx + tilt_x
+        x = torch.where(hi, x - Lx, torch.where(lo, x + Lx, x))
+        img[:, 0] += hi.int() - lo.int()
+        tilt_y = yz * z
+        hi, lo = y >= 0.5 * Ly + tilt_y, y < -0.5 * Ly + tilt_y
+        y = torch.where(hi, y - Ly, torch.where(lo, y + Ly, y))
+        x = torch.where(hi, x - Ly * xy, torch.where(lo, x + Ly * xy, x))
+        img[:, 1] += hi.int() - lo.int()
+        hi, lo = z >= 0.5 * Lz, z < -0.5 * Lz
+        z = torch.where(hi, z - Lz, torch.where(lo, z + Lz, z))
+        y = torch.where(hi, y - Lz * yz, torch.where(lo, y + Lz * yz, y))
+        x = torch.where(hi, x - Lz * xz, torch.where(lo, x + Lz * xz, x))
+        img[:, 2] += hi.int() - lo.int()
+        p[:, 0], p[:, 1], p[:, 2] = x, y, z
+
     def run(self, nsteps):
-        """advance nsteps timesteps with every registered integration method"""
+        """advance nsteps timesteps: box deformation first (as HOOMD's updaters),
+        then every registered integration method"""
         for _ in range(int(nsteps)):
+            if self.tilt_variant is not None:
+                xy = float(self.tilt_variant(self.timestep))
+                if xy != self.box[3]:
+                    self.set_tilt(xy)
             for m in self.methods:
                 self.last_info = m.integrate_step(self.timestep)
             self.timestep += 1

@@ -103,9 +103,13 @@ int fsd_get_pse_info(const fsd_ctx *ctx, fsd_pse_info *out);
 /* Stokes::setResistanceTable (Stokes_ResistanceTable.cc:34-23057):
  * h_dist[1000], h_vals[22000] in the reference's order. */
 int fsd_set_resistance_table(fsd_ctx *ctx, const float *h_dist, const float *h_vals);
-/* Stokes::setFriction (Stokes.cc:1132-1191); "NONE" (the default) is accepted,
- * other types return FSD_E_PARAM. */
+/* Stokes::setFriction (Stokes.cc:1132-1191): "NONE" (the default) or "TYPE1"
+ * (tangential friction added to the resistance table for gaps h <= h0, with
+ * coefficients h_alpha6[6]; parity = 1 writes the table columns the
+ * reference's code writes); other types return FSD_E_PARAM. */
 int fsd_set_friction(fsd_ctx *ctx, const char *type, float h0, const float *h_alpha6);
+/* the resistance values in use after setFriction, h_vals[22000] (Stokes::m_ResTable_vals) */
+int fsd_get_resistance_table(const fsd_ctx *ctx, float *h_vals);
 /* Stokes::setT / set_params (Stokes.h:64-67, integrate.py:139-149) */
 int fsd_set_temperature(fsd_ctx *ctx, float T);
 /* box tilt for a deforming box (HOOMD box update between steps) */

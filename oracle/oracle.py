@@ -358,6 +358,28 @@ class Oracle:
             pass
 
 
+def friction_type1(dist, vals, h0, alpha, parity=True):
+    """Stokes::setFriction TYPE1 (Stokes.cc:1132-1191) on a copy of the table
+    values: for gaps h <= h0 add alpha_k (2 h^2/h0^3 - 3 h/h0^2 + 1/h) with
+    signs (+, -, -, +, +, +) to columns 3, 4, 5, 6, 9, 10 (the columns the
+    reference's code writes; parity=False: the named YA11, YA12, YB11, YB12,
+    YC11, YC12 columns 2, 3, 4, 5, 8, 9)"""
+    v = np.array(vals, dtype=np.float32).reshape(1000, 22).copy()
+    cols = [3, 4, 5, 6, 9, 10] if parity else [2, 3, 4, 5, 8, 9]
+    sgn = [1.0, -1.0, -1.0, 1.0, 1.0, 1.0]
+    h0 = np.float32(h0)
+    for ii in range(1000):
+        h = np.float32(float(dist[ii]) - 2.0)
+        if not h <= h0:
+            continue
+        h2, h02 = np.float32(h * h), np.float32(h0 * h0)
+        h03 = np.float32(h0 * h02)
+        poly = 2.0 / float(h03) * float(h2) - 3.0 / float(h02) * float(h) + 1.0 / float(h)
+        for k in range(6):
+            v[ii, cols[k]] = np.float32(v[ii, cols[k]] + np.float32(sgn[k]) * np.float32(float(np.float32(alpha[k])) * poly))
+    return v.reshape(-1)
+
+
 def lanczos_sqrt(apply, psi, tol=1e-10, mmax=400):
     """|psi| V_m T_m^(1/2) e_1 by Lanczos with full FP64 reorthogonalisation
     (Chow & Saad 2014), iterated until the relative change of the result is

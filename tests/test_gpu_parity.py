@@ -224,3 +224,62 @@ def test_psev3_surface_matches_stokes():
     assert info["converged"]
     assert np.allclose(a, b, atol=1e-6)
     assert np.abs(a - pos).max() > 0
+
+
+def test_cpp_facade_runs(tmp_path):
+    """the C++ Stokes facade drives two sheared steps through the C ABI"""
+    import subprocess
+    import os
+    from tests.test_host_cpu import _build_facade_example
+    exe = str(tmp_path / "stokes_facade_example")
+    _build_facade_example(exe)
+    table = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fsd_amd", "data",
+                         "resistance_table.txt")
+    r = subprocess.run([exe, table], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+
+
+def test_friction_type1():
+    """setFriction TYPE1: the library's table equals the oracle's bit for bit,
+    and R_FU with friction matches the oracle"""
+    import ctypes as C
+    from fsd_amd.tables import load_resistance_table
+    from oracle.oracle import Oracle, friction_type1
+    st, orc, dpos, N = _setup("C1")
+    h0, alpha = 0.01, [0.5, 0.4, 0.3, 0.2, 0.1, 0.05]
+    st.setFriction("TYPE1", h0, alpha)
+    got = np.zeros(22000, dtype=np.float32)
+    assert st.lib.fsd_get_resistance_table(st.ctx, got.ctypes.data_as(C.c_void_p)) == 0
+    dist, vals = load_resistance_table()
+    want = friction_type1(dist, vals, h0, alpha, parity=True)
+    assert np.array_equal(got, want)
+    pos, L = packing(*CASES["C1"])
+    o2 = Oracle(pos, L, res_table=(dist, want))
+    U = random_vec(6 * N, 9).astype(np.float32)
+    e = rel_err(st.rfu(dpos, _t(U)).cpu().numpy(), o2.rfu(U))
+    print("RFU with friction rel err", e)
+    assert e < 1e-5
+
+
+def test_box_resize_shear_variant():
+    """box deformation with a wrapped strain (variant.shear_variant): every step
+    converges and the particles stay inside the tilted box"""
+    from fsd_amd import system, shear_function, integrate, variant
+    N, phi, seed = CASES["C1"]
+    pos, L = packing(N, phi, seed)
+    sysm = system.System(pos, L, dt=1e-2)
+    f = shear_function.steady(1e-2, 20.0)
+    sysm.box_resize(xy=variant.shear_variant(f, total_timestep=100, max_strain=0.5))
+    integrate.PSEv3(sysm.group_all(), T=0.0, function_form=f, gmres_tol=1e-5)
+    tilts = []
+    for _ in range(4):
+        info = sysm.run(1)
+        tilts.append(sysm.box[3])
+        assert info["converged"]
+    p = sysm.positions()
+    xy = sysm.box[3]
+    print("tilts", tilts)
+    assert min(tilts) < 0 < max(tilts)                       # the strain wrapped
+    assert np.all(np.abs(p[:, 1]) <= L / 2 + 1e-4) and np.all(np.abs(p[:, 2]) <= L / 2 + 1e-4)
+    assert np.all(np.abs(p[:, 0] - xy * p[:, 1]) <= L / 2 + 1e-4)

@@ -82,6 +82,19 @@ def test_tridiagonal_square_root(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
 
 
+def _build_facade_example(out):
+    cmd = ["g++", "-O2", "-std=c++17", "-Wno-unused-result", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I" + os.path.join(ROOT, "include"), os.path.join(CPU, "stokes_facade_example.cpp"),
+           "-L" + os.path.join(ROOT, "fsd_amd"), "-lfsd", "-L/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath,/opt/rocm/lib", "-Wl,-rpath," + os.path.join(ROOT, "fsd_amd"), "-o", out]
+    subprocess.run(cmd, check=True)
+
+
+def test_cpp_facade_compiles(tmp_path):
+    """the C++ Stokes facade (include/fsd_stokes.hpp) compiles and links against libfsd.so"""
+    _build_facade_example(str(tmp_path / "stokes_facade_example"))
+
+
 # ------------------------------------------------------------ library surface
 def _header_symbols():
     txt = open(os.path.join(ROOT, "include", "fsd.h")).read()

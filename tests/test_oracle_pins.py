@@ -214,3 +214,25 @@ def test_rcm_against_reference_build():
         ref = np.zeros(N, dtype=np.int32)
         lib.ref_genrcm(N, int(phead[N]), phead.ctypes.data, plist.ctypes.data, ref.ctypes.data)
         assert np.array_equal(_oracle_rcm(phead, plist), ref), (N, phi)
+
+
+def test_friction_type1_table():
+    """TYPE1 friction (Stokes.cc:1132-1191): the added term
+    alpha (2 h^2/h0^3 - 3 h/h0^2 + 1/h) vanishes at h = h0, rows with h > h0 and
+    all other columns are untouched, and parity moves the six columns"""
+    from fsd_amd.tables import load_resistance_table
+    from oracle.oracle import friction_type1
+    dist, vals = load_resistance_table()
+    k = 400
+    h0 = float(np.float32(float(dist[k]) - 2.0))
+    alpha = [1.0, 2.0, 3.0, 4.0, 5.0, 6.0]
+    for parity, cols in [(True, [3, 4, 5, 6, 9, 10]), (False, [2, 3, 4, 5, 8, 9])]:
+        v = friction_type1(dist, vals, h0, alpha, parity).reshape(1000, 22)
+        v0 = vals.reshape(1000, 22)
+        others = [c for c in range(22) if c not in cols]
+        assert np.array_equal(v[:, others], v0[:, others])
+        assert np.array_equal(v[k + 1:], v0[k + 1:])
+        assert np.allclose(v[k], v0[k], rtol=0, atol=1e-3 * np.abs(v0[k]).max())
+        d = v[:k, cols] - v0[:k, cols]
+        # sign pattern (+, -, -, +, +, +) with alpha > 0 and a positive polynomial below h0
+        assert np.all(d[:, [0, 3, 4, 5]] > 0) and np.all(d[:, [1, 2]] < 0)
