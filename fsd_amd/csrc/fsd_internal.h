@@ -83,6 +83,36 @@ __host__ __device__ inline void box_wrap(const BoxF &b, float &x, float &y, floa
     else if (z < -hz) { z += b.Lz; y += b.Lz * b.yz; x += b.Lz * b.xz; iz--; }
 }
 
+// Mobility_SetGridk_kernel (Helper_Mobility.cu:351-410) for the full-grid
+// index (i, j, k): sheared wave vector and the Hasimoto factor
+// w = 6 pi (1 + k^2/4xi^2) exp(-(1 - eta) k^2/4xi^2) / k^2 / (Nx Ny Nz)
+// evaluated in double, stored as float (w = 0 at the origin)
+__host__ __device__ inline float4 gridk_value(int i, int j, int k, int Nx, int Ny, int Nz, const BoxF &b, float xi,
+                                              float eta, float twopi_k)
+{
+    double kx = (i < (Nx + 1) / 2) ? i : i - Nx;
+    double ky = (((j < (Ny + 1) / 2) ? j : j - Ny) - (double)b.xy * kx * b.Ly / b.Lx) / b.Ly;
+    double kz = ((k < (Nz + 1) / 2) ? k : k - Nz) / (double)b.Lz;
+    kx = kx / b.Lx;
+    kx *= twopi_k; ky *= twopi_k; kz *= twopi_k;
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double xisq = (double)xi * xi;
+    double w = 0.0;
+    if (!(i == 0 && j == 0 && k == 0))
+        w = 6.0 * 3.1415926536 * (1.0 + k2 / 4.0 / xisq) * exp(-(1.0 - eta) * k2 / 4.0 / xisq) / k2
+            / (double)((int64_t)Nx * Ny * Nz);
+    return make_float4((float)kx, (float)ky, (float)kz, (float)w);
+}
+
+// True when the mode (i, j, k) has a Nyquist index: there the full grid's
+// conjugate mode carries the same Nyquist k component (not its negative), so
+// a scaled spectrum is not Hermitian and the real part of the reference's C2C
+// inverse equals the inverse of the Hermitian part (X(k) + conj X(k*)) / 2.
+__host__ __device__ inline bool has_nyquist(int i, int j, int k, int Nx, int Ny, int Nz)
+{
+    return (Nx % 2 == 0 && i == Nx / 2) || (Ny % 2 == 0 && j == Ny / 2) || (Nz % 2 == 0 && k == Nz / 2);
+}
+
 struct LubTable {
     const float *dist;   // 1000
     const float *vals;   // 22000
@@ -268,6 +298,7 @@ int k_setup_spread(fsd_ctx *ctx, const float4 *pos);
 int k_contract_lds(fsd_ctx *ctx);
 int scan_excl_chunks(int n);
 int k_gridk(fsd_ctx *ctx);
+float k_twopi(const fsd_ctx *ctx);
 int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int parts, const float *addU);
 int k_prune(fsd_ctx *ctx, const float4 *pos);
 int k_build_blocks(fsd_ctx *ctx, const float4 *pos);

@@ -70,11 +70,48 @@ __device__ __forceinline__ bool in_lower_half(int i, int j, int k, int Nx, int N
     return !(i == 0 && j == 0 && k == 0);
 }
 
-__global__ void ff_noise_kernel(const float *noise, int parity, float fac, const float4 *gk, int Nx, int Ny, int Nz,
-                                int64_t Mh, float2 *spec)
+// Grid2of2: square root of the wave-space Green's function on one mode
+__device__ inline void noise_scale(const float4 tk, bool origin, const float2 f[3], float2 o[11])
+{
+    const float ksq = tk.x * tk.x + tk.y * tk.y + tk.z * tk.z;
+    const float kk = sqrtf(ksq);
+    const float B = origin ? 0.f : sqrtf(tk.w);
+    const float SU = origin ? 0.f : sinf(kk) / kk;
+    const float SD = origin ? 0.f : 3.0f * (sinf(kk) - kk * cosf(kk)) / (ksq * kk);
+    float2 kdF = make_float2(0.f, 0.f);
+    if (!origin) {
+        kdF.x = (tk.x * f[0].x + tk.y * f[1].x + tk.z * f[2].x) / ksq;
+        kdF.y = (tk.x * f[0].y + tk.y * f[1].y + tk.z * f[2].y) / ksq;
+    }
+    const float kv[3] = {tk.x, tk.y, tk.z};
+    float2 W[3];
+    for (int a = 0; a < 3; ++a) {
+        W[a].x = (f[a].x - kv[a] * kdF.x) * B;
+        W[a].y = (f[a].y - kv[a] * kdF.y) * B;
+    }
+    for (int a = 0; a < 3; ++a) o[a] = make_float2(SU * W[a].x, SU * W[a].y);
+    // gradient fields XX XY XZ YZ YY YX ZX ZY: component (a, c) = i SD W_a k_c
+    const int sa[8] = {0, 0, 0, 1, 1, 1, 2, 2}, sc[8] = {0, 1, 2, 2, 1, 0, 0, 1};
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const float2 w = W[sa[s]];
+        const float kc_ = kv[sc[s]];
+        o[3 + s] = make_float2(-SD * w.y * kc_, SD * w.x * kc_);
+    }
+}
+
+struct NoiseGeom {
+    int Nx, Ny, Nz;
+    BoxF b;
+    float xi, eta, twopi_k;
+};
+
+__global__ void ff_noise_kernel(const float *noise, int parity, float fac, const float4 *gk, NoiseGeom g, int64_t Mh,
+                                float2 *spec)
 {
     const int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (h >= Mh) return;
+    const int Nx = g.Nx, Ny = g.Ny, Nz = g.Nz;
     const int Nzh = Nz / 2 + 1;
     const int k = (int)(h % Nzh), j = (int)((h / Nzh) % Ny), i = (int)(h / ((int64_t)Nzh * Ny));
     const bool in = (i == Nx / 2) && (Nx % 2 == 0), jn = (j == Ny / 2) && (Ny % 2 == 0),
@@ -94,41 +131,28 @@ __global__ void ff_noise_kernel(const float *noise, int parity, float fac, const
         const int64_t q = parity ? cidx : 6 * cidx;
         for (int a = 0; a < 3; ++a) f[a] = make_float2(fac * noise[q + a], -fac * noise[q + 3 + a]);
     }
-    // Grid2of2: square root of the wave-space Green's function
-    const float4 tk = gk[h];
-    const float ksq = tk.x * tk.x + tk.y * tk.y + tk.z * tk.z;
-    const float kk = sqrtf(ksq);
-    const bool origin = (h == 0);
-    const float B = origin ? 0.f : sqrtf(tk.w);
-    const float SU = origin ? 0.f : sinf(kk) / kk;
-    const float SD = origin ? 0.f : 3.0f * (sinf(kk) - kk * cosf(kk)) / (ksq * kk);
-    float2 kdF = make_float2(0.f, 0.f);
-    if (!origin) {
-        kdF.x = (tk.x * f[0].x + tk.y * f[1].x + tk.z * f[2].x) / ksq;
-        kdF.y = (tk.x * f[0].y + tk.y * f[1].y + tk.z * f[2].y) / ksq;
-    }
-    const float kv[3] = {tk.x, tk.y, tk.z};
-    float2 W[3];
-    for (int a = 0; a < 3; ++a) {
-        W[a].x = (f[a].x - kv[a] * kdF.x) * B;
-        W[a].y = (f[a].y - kv[a] * kdF.y) * B;
-    }
-    for (int a = 0; a < 3; ++a) spec[a * Mh + h] = make_float2(SU * W[a].x, SU * W[a].y);
-    // gradient fields XX XY XZ YZ YY YX ZX ZY: component (a, c) = i SD W_a k_c
-    const int sa[8] = {0, 0, 0, 1, 1, 1, 2, 2}, sc[8] = {0, 1, 2, 2, 1, 0, 0, 1};
+    float2 o[11];
+    noise_scale(gk[h], h == 0, f, o);
+    if (has_nyquist(i, j, k, Nx, Ny, Nz)) {
+        // the conjugate mode holds conj(f) scaled with its own wave vector
+        const float4 tkc = gridk_value(ic, jc, kc, Nx, Ny, Nz, g.b, g.xi, g.eta, g.twopi_k);
+        const float2 fc[3] = {make_float2(f[0].x, -f[0].y), make_float2(f[1].x, -f[1].y),
+                              make_float2(f[2].x, -f[2].y)};
+        float2 oc[11];
+        noise_scale(tkc, false, fc, oc);
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-        const float2 w = W[sa[s]];
-        const float kc_ = kv[sc[s]];
-        spec[(3 + s) * Mh + h] = make_float2(-SD * w.y * kc_, SD * w.x * kc_);
+        for (int c = 0; c < 11; ++c) o[c] = make_float2(0.5f * (o[c].x + oc[c].x), 0.5f * (o[c].y - oc[c].y));
     }
+#pragma unroll
+    for (int c = 0; c < 11; ++c) spec[c * Mh + h] = o[c];
 }
 
 int k_ff_noise(fsd_ctx *ctx, const float *noise, float fac)
 {
     const fsd_pse_info &ps = ctx->pse;
+    NoiseGeom g{ps.Nx, ps.Ny, ps.Nz, ctx->box, ctx->p.xi, ps.eta, k_twopi(ctx)};
     hipLaunchKernelGGL(ff_noise_kernel, dim3(nblk(ctx->Mh, 256)), dim3(256), 0, ctx->stream, noise, ctx->p.parity,
-                       fac, ctx->d_gridk, ps.Nx, ps.Ny, ps.Nz, (int64_t)ctx->Mh, ctx->d_spec);
+                       fac, ctx->d_gridk, g, (int64_t)ctx->Mh, ctx->d_spec);
     FSD_HIP(hipGetLastError());
     return 0;
 }

@@ -23,25 +23,20 @@ __global__ void gridk_kernel(float4 *gk, int Nx, int Ny, int Nz, BoxF b, float x
     int i = (int)(tid / ((int64_t)Ny * Nzh));
     int j = (int)((tid / Nzh) % Ny);
     int k = (int)(tid % Nzh);
-    double kx = (i < (Nx + 1) / 2) ? i : i - Nx;
-    double ky = (((j < (Ny + 1) / 2) ? j : j - Ny) - (double)b.xy * kx * b.Ly / b.Lx) / b.Ly;
-    double kz = ((k < (Nz + 1) / 2) ? k : k - Nz) / (double)b.Lz;
-    kx = kx / b.Lx;
-    kx *= twopi_k; ky *= twopi_k; kz *= twopi_k;
-    double k2 = kx * kx + ky * ky + kz * kz;
-    double xisq = (double)xi * xi;
-    double w = 0.0;
-    if (!(i == 0 && j == 0 && k == 0))
-        w = 6.0 * 3.1415926536 * (1.0 + k2 / 4.0 / xisq) * exp(-(1.0 - eta) * k2 / 4.0 / xisq) / k2
-            / (double)((int64_t)Nx * Ny * Nz);
-    gk[tid] = make_float4((float)kx, (float)ky, (float)kz, (float)w);
+    gk[tid] = gridk_value(i, j, k, Nx, Ny, Nz, b, xi, eta, twopi_k);
+}
+
+// 2 pi of the k vectors: the reference's constant 3.1416926536 with parity
+// (Helper_Mobility.cu:389-391)
+float k_twopi(const fsd_ctx *ctx)
+{
+    return (float)(2.0 * (ctx->p.parity ? 3.1416926536 : 3.14159265358979323846));
 }
 
 int k_gridk(fsd_ctx *ctx)
 {
-    const double twopi = 2.0 * (ctx->p.parity ? 3.1416926536 : 3.14159265358979323846);
     hipLaunchKernelGGL(gridk_kernel, dim3(nblk(ctx->Mh, 256)), dim3(256), 0, ctx->stream, ctx->d_gridk, ctx->pse.Nx,
-                       ctx->pse.Ny, ctx->pse.Nz, ctx->box, ctx->p.xi, ctx->pse.eta, (float)twopi);
+                       ctx->pse.Ny, ctx->pse.Nz, ctx->box, ctx->p.xi, ctx->pse.eta, k_twopi(ctx));
     FSD_HIP(hipGetLastError());
     return 0;
 }
@@ -268,66 +263,89 @@ __global__ void __launch_bounds__(512) spread_kernel(SpreadGeom g, const float4 
 // ------------------------------------------------------------ Green kernel
 // Mobility_WaveSpace_Green_kernel (Mobility.cu:315-479) on the half spectrum.
 // Field order: Fx Fy Fz | C_xx C_xy C_xz C_yz C_yy C_yx C_zx C_zy.
-__global__ void green_kernel(float2 *spec, const float4 *gk, int64_t Mh)
+__device__ inline void green_apply(const float4 tk, bool origin, const float2 v[11], float2 o[11])
+{
+#pragma unroll
+    for (int c = 0; c < 11; ++c) o[c] = make_float2(0.f, 0.f);
+    if (origin) return;
+    const float k[3] = {tk.x, tk.y, tk.z};
+    const float ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+    const float kk = sqrtf(ksq);
+    // couplet matrix rows (x, y, z)
+    const float2 Cm[3][3] = {{v[3], v[4], v[5]}, {v[8], v[7], v[6]}, {v[9], v[10], make_float2(-(v[3].x + v[7].x), -(v[3].y + v[7].y))}};
+    float2 kdF;
+    kdF.x = (k[0] * v[0].x + k[1] * v[1].x + k[2] * v[2].x) / ksq;
+    kdF.y = (k[0] * v[0].y + k[1] * v[1].y + k[2] * v[2].y) / ksq;
+    float2 Cdk[3];
+    for (int a = 0; a < 3; ++a) {
+        Cdk[a].x = Cm[a][0].x * k[0] + Cm[a][1].x * k[1] + Cm[a][2].x * k[2];
+        Cdk[a].y = Cm[a][0].y * k[0] + Cm[a][1].y * k[1] + Cm[a][2].y * k[2];
+    }
+    float2 kCk;
+    kCk.x = (Cdk[0].x * k[0] + Cdk[1].x * k[1] + Cdk[2].x * k[2]) / ksq;
+    kCk.y = (Cdk[0].y * k[0] + Cdk[1].y * k[1] + Cdk[2].y * k[2]) / ksq;
+    const float sk = sinf(kk), ck = cosf(kk);
+    const float j0 = sk / kk;
+    const float jt = 3.0f * (sk - kk * ck) / (ksq * kk);
+    const float Buf = tk.w * j0 * j0;
+    const float Buc = -tk.w * j0 * jt;        // UC (imaginary coupling)
+    const float Bdf = tk.w * j0 * jt;         // DF (imaginary coupling)
+    const float q = (sk - kk * ck) / (ksq * kk);
+    const float Bdc = tk.w * 9.0f * q * q;    // DC
+    for (int a = 0; a < 3; ++a) {
+        float2 ff = v[a];
+        float2 u;
+        u.x = (ff.x - k[a] * kdF.x) * Buf;
+        u.y = (ff.y - k[a] * kdF.y) * Buf;
+        // + i (Cdk_a - k_a kCk) Buc
+        u.x += -(Cdk[a].y - k[a] * kCk.y) * Buc;
+        u.y += (Cdk[a].x - k[a] * kCk.x) * Buc;
+        o[a] = u;
+    }
+    // gradient slots in field order: (x,x) (x,y) (x,z) (y,z) (y,y) (y,x) (z,x) (z,y)
+    const int sa[8] = {0, 0, 0, 1, 1, 1, 2, 2}, sc[8] = {0, 1, 2, 2, 1, 0, 0, 1};
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int a = sa[s], c = sc[s];
+        const float kk_ac = k[a] * k[c];
+        float2 z;
+        z.x = v[a].x * k[c] - kk_ac * kdF.x;
+        z.y = v[a].y * k[c] - kk_ac * kdF.y;
+        float2 d;
+        d.x = -z.y * Bdf + (Cdk[a].x * k[c] - kk_ac * kCk.x) * Bdc;
+        d.y = z.x * Bdf + (Cdk[a].y * k[c] - kk_ac * kCk.y) * Bdc;
+        o[3 + s] = d;
+    }
+}
+
+struct GreenGeom {
+    int Nx, Ny, Nz;
+    BoxF b;
+    float xi, eta, twopi_k;
+};
+
+// the spread grid is real, so its spectrum at the conjugate mode is conj(v);
+// modes with a Nyquist index get the Hermitian part of the two outputs
+__global__ void green_kernel(float2 *spec, const float4 *gk, int64_t Mh, GreenGeom g)
 {
     int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= Mh) return;
+    const int Nzh = g.Nz / 2 + 1;
+    const int k = (int)(id % Nzh), j = (int)((id / Nzh) % g.Ny), i = (int)(id / ((int64_t)Nzh * g.Ny));
     float2 v[11];
 #pragma unroll
     for (int c = 0; c < 11; ++c) v[c] = spec[c * Mh + id];
-    const float4 tk = gk[id];
     float2 o[11];
+    green_apply(gk[id], id == 0, v, o);
+    if (has_nyquist(i, j, k, g.Nx, g.Ny, g.Nz)) {
+        const int ic = (g.Nx - i) % g.Nx, jc = (g.Ny - j) % g.Ny, kc = (g.Nz - k) % g.Nz;
+        const float4 tkc = gridk_value(ic, jc, kc, g.Nx, g.Ny, g.Nz, g.b, g.xi, g.eta, g.twopi_k);
+        float2 vc[11], oc[11];
 #pragma unroll
-    for (int c = 0; c < 11; ++c) o[c] = make_float2(0.f, 0.f);
-    if (id != 0) {
-        const float k[3] = {tk.x, tk.y, tk.z};
-        const float ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
-        const float kk = sqrtf(ksq);
-        // couplet matrix rows (x, y, z)
-        const float2 Cm[3][3] = {{v[3], v[4], v[5]}, {v[8], v[7], v[6]}, {v[9], v[10], make_float2(-(v[3].x + v[7].x), -(v[3].y + v[7].y))}};
-        float2 kdF;
-        kdF.x = (k[0] * v[0].x + k[1] * v[1].x + k[2] * v[2].x) / ksq;
-        kdF.y = (k[0] * v[0].y + k[1] * v[1].y + k[2] * v[2].y) / ksq;
-        float2 Cdk[3];
-        for (int a = 0; a < 3; ++a) {
-            Cdk[a].x = Cm[a][0].x * k[0] + Cm[a][1].x * k[1] + Cm[a][2].x * k[2];
-            Cdk[a].y = Cm[a][0].y * k[0] + Cm[a][1].y * k[1] + Cm[a][2].y * k[2];
-        }
-        float2 kCk;
-        kCk.x = (Cdk[0].x * k[0] + Cdk[1].x * k[1] + Cdk[2].x * k[2]) / ksq;
-        kCk.y = (Cdk[0].y * k[0] + Cdk[1].y * k[1] + Cdk[2].y * k[2]) / ksq;
-        const float sk = sinf(kk), ck = cosf(kk);
-        const float j0 = sk / kk;
-        const float jt = 3.0f * (sk - kk * ck) / (ksq * kk);
-        const float Buf = tk.w * j0 * j0;
-        const float Buc = -tk.w * j0 * jt;        // UC (imaginary coupling)
-        const float Bdf = tk.w * j0 * jt;         // DF (imaginary coupling)
-        const float q = (sk - kk * ck) / (ksq * kk);
-        const float Bdc = tk.w * 9.0f * q * q;    // DC
-        for (int a = 0; a < 3; ++a) {
-            float2 ff = v[a];
-            float2 u;
-            u.x = (ff.x - k[a] * kdF.x) * Buf;
-            u.y = (ff.y - k[a] * kdF.y) * Buf;
-            // + i (Cdk_a - k_a kCk) Buc
-            u.x += -(Cdk[a].y - k[a] * kCk.y) * Buc;
-            u.y += (Cdk[a].x - k[a] * kCk.x) * Buc;
-            o[a] = u;
-        }
-        // gradient slots in field order: (x,x) (x,y) (x,z) (y,z) (y,y) (y,x) (z,x) (z,y)
-        const int sa[8] = {0, 0, 0, 1, 1, 1, 2, 2}, sc[8] = {0, 1, 2, 2, 1, 0, 0, 1};
+        for (int c = 0; c < 11; ++c) vc[c] = make_float2(v[c].x, -v[c].y);
+        green_apply(tkc, false, vc, oc);
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int a = sa[s], c = sc[s];
-            const float kk_ac = k[a] * k[c];
-            float2 z;
-            z.x = v[a].x * k[c] - kk_ac * kdF.x;
-            z.y = v[a].y * k[c] - kk_ac * kdF.y;
-            float2 d;
-            d.x = -z.y * Bdf + (Cdk[a].x * k[c] - kk_ac * kCk.x) * Bdc;
-            d.y = z.x * Bdf + (Cdk[a].y * k[c] - kk_ac * kCk.y) * Bdc;
-            o[3 + s] = d;
-        }
+        for (int c = 0; c < 11; ++c) o[c] = make_float2(0.5f * (o[c].x + oc[c].x), 0.5f * (o[c].y - oc[c].y));
     }
 #pragma unroll
     for (int c = 0; c < 11; ++c) spec[c * Mh + id] = o[c];
@@ -675,8 +693,9 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
         }
         prof_end(ctx, PROF_FFT, ev);
         prof_begin(ctx, PROF_GREEN, &ev);
+        GreenGeom gg{ps.Nx, ps.Ny, ps.Nz, ctx->box, ctx->p.xi, ps.eta, k_twopi(ctx)};
         hipLaunchKernelGGL(green_kernel, dim3(nblk(ctx->Mh, 256)), dim3(256), 0, ctx->stream, ctx->d_spec,
-                           ctx->d_gridk, (int64_t)ctx->Mh);
+                           ctx->d_gridk, (int64_t)ctx->Mh, gg);
         FSD_HIP(hipGetLastError());
         prof_end(ctx, PROF_GREEN, ev);
         const int rcw = k_wave_contract(ctx, pos);

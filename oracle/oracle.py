@@ -288,17 +288,21 @@ class Oracle:
                                  _p(noise), 1 if parity else 0, T, dt, _p(gU))
         return gU
 
-    def bro_ff_real(self, psi, T, dt, tol=1e-10):
-        """real-space far-field Brownian slip sqrt(2T/dt) M_real^(1/2) psi (11N),
-        converged FP64 Lanczos (Brownian_FarField.cu:493-808)"""
-        y, m = lanczos_sqrt(lambda v: self.mobility(v, parts=1), psi, tol)
+    def bro_ff_real(self, psi, T, dt, tol=1e-10, m0=None):
+        """real-space far-field Brownian slip sqrt(2T/dt) M_real^(1/2) psi (11N)
+        (Brownian_FarField.cu:493-808): converged FP64 Lanczos, or with m0 the
+        sampler's adaptive rule from m0 vectors at relative tolerance tol"""
+        op = lambda v: self.mobility(v, parts=1)
+        y = lanczos_sqrt(op, psi, tol)[0] if m0 is None else lanczos_sqrt_adaptive(op, psi, m0, tol)[0]
         return np.sqrt(2.0 * T / dt) * y
 
-    def bro_nf(self, psi, tol=1e-10):
+    def bro_nf(self, psi, tol=1e-10, m0=None):
         """near-field Brownian force R_FU^(1/2) psi (6N); psi carries its
-        sqrt(2T/dt) (Brownian_NearField.cu:112-464, operator Lubrication_RFU_kernel)"""
-        y, m = lanczos_sqrt(self.rfu, psi, tol)
-        return y
+        sqrt(2T/dt) (Brownian_NearField.cu:112-464, operator Lubrication_RFU_kernel);
+        m0 as in bro_ff_real"""
+        if m0 is None:
+            return lanczos_sqrt(self.rfu, psi, tol)[0]
+        return lanczos_sqrt_adaptive(self.rfu, psi, m0, tol)[0]
 
     def shifted(self, dpos3):
         """a copy at positions wrapped(pos + dpos3) that keeps this configuration's
@@ -410,6 +414,53 @@ def lanczos_sqrt(apply, psi, tol=1e-10, mmax=400):
         beta.append(b)
         V.append(w / b)
     return y, mmax
+
+
+def lanczos_sqrt_adaptive(apply, psi, m0, tol, mmax=100):
+    """the Chow-Saad sampler's stopping rule (Brownian_FarField.cu:597-808):
+    m0 Lanczos vectors, then one more at a time until the relative change of
+    |psi| V_m T_m^(1/2) e_1 is at most tol or the basis holds mmax - 1
+    vectors (FP64, full reorthogonalisation).  Returns (y, m - 1)."""
+    psi = np.asarray(psi, dtype=np.float64)
+    nrm = np.linalg.norm(psi)
+    V = [psi / nrm]
+    alpha, beta = [], []
+
+    def step():
+        w = apply(V[-1])
+        a = float(V[-1] @ w)
+        for _ in range(2):
+            for v in V:
+                w -= (v @ w) * v
+        alpha.append(a)
+        b = np.linalg.norm(w)
+        if b < 1e-8:
+            return False
+        beta.append(b)
+        V.append(w / b)
+        return True
+
+    def combine(m):
+        T = np.diag(alpha[:m]) + np.diag(beta[:m - 1], 1) + np.diag(beta[:m - 1], -1)
+        lam, Q = np.linalg.eigh(T)
+        return nrm * (np.array(V[:m]).T @ (Q @ (np.sqrt(np.maximum(lam, 0.0)) * Q[0])))
+
+    m = max(1, min(m0, mmax - 2))
+    ok = True
+    for j in range(m):
+        ok = step()
+        if not ok:
+            m = j + 1
+            break
+    y = combine(m)
+    err = 1.0
+    while ok and err > tol and m < mmax - 1:
+        ok = step()
+        y1 = combine(m + 1)
+        err = np.linalg.norm(y1 - y) / np.linalg.norm(y1)
+        y = y1
+        m += 1
+    return y, m - 1
 
 
 def pse_params(L, xi=0.5, error=1e-3, max_strain=0.5):

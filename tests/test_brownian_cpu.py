@@ -75,3 +75,17 @@ def test_rfd_even_in_psi():
     b = o.rfd_drift(-psi, 1.0, 1e-3)
     assert np.linalg.norm(a - b) <= 1e-6 * np.linalg.norm(a)
     assert np.linalg.norm(a) > 0
+
+
+def test_adaptive_lanczos_rule_converges_to_sqrt():
+    """the sampler's adaptive stopping rule (relative change <= tol) gives the
+    square root to about tol on the well-conditioned real-space mobility"""
+    from oracle.oracle import lanczos_sqrt_adaptive
+    o = Oracle(POS5, 12.0)
+    n = 11 * 5
+    M = _dense(lambda v: o.mobility(v, parts=1), n)
+    psi = np.random.default_rng(7).standard_normal(n)
+    y, m = lanczos_sqrt_adaptive(lambda v: o.mobility(v, parts=1), psi, 2, 1e-3)
+    ref = _sqrt_apply(M, psi)
+    assert np.linalg.norm(y - ref) <= 1e-3 * np.linalg.norm(ref)
+    assert 2 <= m < 99
