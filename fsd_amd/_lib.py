@@ -77,6 +77,7 @@ class FsdSolveStats(C.Structure):
         ("levels", C.c_int32),
         ("lanczos_ff", C.c_int32),
         ("lanczos_nf", C.c_int32),
+        ("sweep_steps", C.c_int32),
     ]
 
     def as_dict(self):

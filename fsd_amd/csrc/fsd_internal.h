@@ -22,8 +22,6 @@
 
 #define FSD_MAXP 14        // pruned neighbours incl. self (kissing number 12 + self + 1)
 #define FSD_PSTRIDE 16     // stride of the pruned list
-#define SW_THREADS 1024    // threads of the single-workgroup triangular sweeps
-#define SW_TAB_MAX 16384   // ints of sweep schedule held in LDS (64 KiB)
 #define BRO_MMAX 100       // Lanczos basis bound of the Brownian samplers (m_max, Brownian_FarField.cu:613)
 
 struct BoxF {
@@ -227,21 +225,18 @@ struct fsd_ctx {
     int *d_lev_f = nullptr, *d_lev_b = nullptr;
     float *d_tmp6a = nullptr, *d_tmp6b = nullptr;
     float *d_g12 = nullptr;       // G b_F and G b_U of the saddle preconditioner
-    // single-workgroup sweeps: schedule table (levels, jagged-diagonal records) and packed operands
-    struct SweepOffsets { int nlev, levrow, levjd, rec; } sw_f{}, sw_b{};
-    SweepPlan plan_f, plan_b;
-    std::vector<int> h_sw_tab;
-    bool sw_ok = false;
-    int sw_ntab = 0;
-    int *d_sw_tab = nullptr;
-    int *d_sw_fcol = nullptr, *d_sw_bcol = nullptr, *d_sw_fsrc = nullptr, *d_sw_bsrc = nullptr;  // nlow_cap
-    float *d_sw_fblk = nullptr, *d_sw_bblk = nullptr;                                            // 36 nlow_cap
-    float *d_sw_fdinv = nullptr, *d_sw_bdinv = nullptr;                                          // 36 N
-    int *d_sw_forder = nullptr, *d_sw_border = nullptr;                                          // N
-    int *d_sw_fin = nullptr;      // forward position -> particle (input gather)
-    int *d_sw_fyb = nullptr;      // forward position -> backward position (copy of y)
-    int *d_sw_bout = nullptr;     // backward position -> particle (output scatter)
-    float *d_sw_work = nullptr;   // 4 x 12N: gathered input, y (forward order), y (backward order), x
+    // single-workgroup sweeps: task plans (sweep_plan.h) and their device operands
+    TaskPlan tp_f, tp_b;
+    bool tp_ok = false;
+    int tp_R = 0;                                        // LDS ring capacity in rows
+    int64_t tp_cap_f = 0, tp_cap_b = 0;                  // lanes allocated per sweep
+    int4 *d_tp_finfo = nullptr, *d_tp_binfo = nullptr;   // per lane
+    int2 *d_tp_fbsrc = nullptr, *d_tp_bbsrc = nullptr;   // per lane
+    float4 *d_tp_fblk = nullptr, *d_tp_bblk = nullptr;   // 9 per lane
+    float4 *d_tp_finp = nullptr, *d_tp_binp = nullptr;   // 3 per lane
+    float4 *d_tp_fres = nullptr, *d_tp_bres = nullptr;   // 3 per lane
+    int *d_tp_fgtid = nullptr, *d_tp_bgtid = nullptr;    // N: owner lane of each permuted particle
+    float *d_tp_Z = nullptr;                             // 36 N: inverse diagonal blocks
 
     // saddle / GMRES
     float *d_V = nullptr, *d_w = nullptr, *d_t = nullptr, *d_t2 = nullptr;
@@ -303,6 +298,7 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
 int k_prune(fsd_ctx *ctx, const float4 *pos);
 int k_build_blocks(fsd_ctx *ctx, const float4 *pos);
 int k_ichol(fsd_ctx *ctx, float lambda, int *failed);
+int k_sweep_setup(fsd_ctx *ctx);
 int k_sweep_pack(fsd_ctx *ctx);
 int k_normals(fsd_ctx *ctx, int64_t n, uint32_t step, uint32_t stream, float sd, float *out);
 int k_ff_noise(fsd_ctx *ctx, const float *noise, float fac);

@@ -7,8 +7,10 @@
 // isolated particle, a diagonal block whose off-diagonal values are zero), so
 // the scalar recurrences of csric02 reduce to block Cholesky without fill.
 //
-// Factorisation and solves are level-scheduled on the permuted particle DAG
-// (one launch per level; the levels are built on the host with the RCM).
+// The factorisation is level-scheduled on the permuted particle DAG (one
+// launch per level; the levels are built on the host with the RCM).  The
+// solves run both sweeps in one single-workgroup launch on a static
+// schedule (sweep_plan.h), or one launch per level where LDS is too small.
 #include "fsd_internal.h"
 
 // ----------------------------------------------------------- pruning
@@ -319,221 +321,312 @@ __global__ void __launch_bounds__(64) trsv_bwd_level_kernel(const int *items, in
 }
 
 // ------------------------------------- single-workgroup sweeps (one launch)
-// Both sweeps of one application run in one workgroup of SW_THREADS threads;
-// __syncthreads orders the levels, so no workgroup ever waits on another.
-// Three lanes own one row (lane c: output components 2c and 2c+1) for both
-// right-hand sides; a wave holds 21 rows (lane 63 idle).  Inputs, blocks,
-// inverse diagonal blocks and dependency positions are stored in sweep order
-// (jagged-diagonal inside a level) and results are written in sweep order:
-// every load of a row is issued at once from its position, and the only
-// dependent loads are the dependencies' values, one batch per level.
-struct SweepDev {
-    int nlev, levrow, levjd, rec;   // offsets (ints) into the schedule table
-    const int *col;                 // entry -> position of the dependency
-    const float *blk;               // entry -> 6x6 block, row a feeds output component a
-    const float *dinv;              // position -> inverse diagonal block (row-major)
-    const float *in;                // 2 x 6N input in sweep order
-    float *res;                     // 2 x 6N results in sweep order
-    const int *scatter;             // position -> 6-vector index of the copy
-    float *copy0, *copy1;           // copies of the results (right-hand sides 0 and 1)
+// Both sweeps of one application run in one workgroup of 1024 threads, so
+// no workgroup ever waits on another.  The diagonal blocks are folded out of
+// the recurrences (Z_P = L_PP^-1, lower triangular):
+//   forward   y_P = Z_P b_P - sum_Q (Z_P L_PQ) y_Q
+//   backward  u_P = y_P - sum_Q (Z_Q L_QP)^T u_Q,   x_P = Z_P^T u_P
+// so a step is one 6x6 block product per lane, a sum over the row's lanes
+// and a subtraction; Z b and Z^T u are separate all-parallel kernels.
+//
+// The schedule (sweep_plan.h) gives each lane one dependency block per step.
+// Results of the last steps' rows are read from an LDS ring, older ones from
+// global memory; every operand of step s + 1 (blocks, inputs, the values of
+// dependencies old enough to be read from global memory) is fetched while
+// step s runs.  A workgroup barrier on gfx950 waits for LDS only (the memory
+// model orders global memory inside a workgroup without waiting for stores),
+// so stores stay in flight across steps.
+struct TaskDev {
+    int nsteps;
+    const int4 *info;       // per lane: src, slot, nd, dest
+    const float4 *blk;      // [step][9][1024]: the lane's 6x6 block, row-major
+    const float4 *inp;      // [step][3][1024]: the owner's input, right-hand sides 0 and 1 (6 floats each)
+    float4 *res;            // [step][3][1024]: the owner's result
+    float4 *copy;           // forward: the backward sweep's inp, at the row's backward lane; backward: null
 };
 
-#define SW_JC 8   // dependency positions loaded per batch
-#define SW_JB 4   // dependency values loaded per batch
-
-// 32-bit byte offsets from a uniform base: global loads in saddr form (the
-// host checks that every offset fits in 32 bits)
-template <typename T>
-__device__ __forceinline__ T ld_off(const void *base, uint32_t byte_off)
+__device__ __forceinline__ uint32_t tp_idx(uint32_t g, uint32_t i, uint32_t n)
 {
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
-}
-template <typename T>
-__device__ __forceinline__ void st_off(void *base, uint32_t byte_off, T v)
-{
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off) = v;
+    return ((g >> 10) * n + i) * (uint32_t)TP_LANES + (g & (TP_LANES - 1));
 }
 
+// operands of one step for this lane (values of 2 right-hand sides as 3 float4:
+// (a0 a1 a2 a3) (a4 a5 b0 b1) (b2 b3 b4 b5))
 template <int NRHS>
-__device__ __forceinline__ void sweep_levels(const int *s_tab, const SweepDev &S, uint32_t N6b, int slot, int nslot,
-                                             int c, int gbase, bool lane_ok)
+__device__ __forceinline__ void tp_fetch(const TaskDev &S, uint32_t s, uint32_t t, int4 inf, float *m, float4 *gx,
+                                         float4 *in)
 {
-    for (int l = 0; l < S.nlev; ++l) {
-        const int r0 = s_tab[S.levrow + l], r1 = s_tab[S.levrow + l + 1];
-        const int j0 = s_tab[S.levjd + l], nj = s_tab[S.levjd + l + 1] - j0;
-        for (int base = r0; base < r1; base += nslot) {
-            const int k = base + slot;
-            const bool act = lane_ok && k < r1;
-            const int kl = k - r0;
-            float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
-            float4 D0 = make_float4(0.f, 0.f, 0.f, 0.f), D1 = D0, D2 = D0;
-            const uint32_t kofs = 24u * (uint32_t)k + 8u * (uint32_t)c;   // bytes of components 2c, 2c+1 of row k
-            if (act) {
-                v0 = ld_off<float2>(S.in, kofs);
-                if (NRHS == 2) v1 = ld_off<float2>(S.in, N6b + kofs);
-                const uint32_t dofs = 144u * (uint32_t)k + 48u * (uint32_t)c;
-                D0 = ld_off<float4>(S.dinv, dofs);
-                D1 = ld_off<float4>(S.dinv, dofs + 16u);
-                D2 = ld_off<float4>(S.dinv, dofs + 32u);
-                for (int jb = 0; jb < nj; jb += SW_JC) {
-                    // dependency positions of up to SW_JC blocks at once, then their values SW_JB at a time
-                    uint32_t eo[SW_JC], qo[SW_JC];
-                    int n = 0;
+    if (inf.x >= 0) {
+        const float4 *b = S.blk + (size_t)s * 9 * TP_LANES + t;
 #pragma unroll
-                    for (int t = 0; t < SW_JC; ++t) {
-                        const int j = j0 + jb + t;
-                        const bool has = (jb + t < nj) && kl < s_tab[S.rec + 2 * j + 1];
-                        n += has;
-                        const uint32_t e = has ? (uint32_t)(s_tab[S.rec + 2 * j] + kl) : 0u;  // entry 0 exists
-                        eo[t] = 144u * e + 48u * (uint32_t)c;
-                        qo[t] = 24u * (uint32_t)ld_off<int>(S.col, 4u * e);
-                    }
-#pragma unroll
-                    for (int tb = 0; tb < SW_JC; tb += SW_JB) {
-                        if (tb >= n) break;
-#pragma unroll
-                        for (int t = tb; t < tb + SW_JB; ++t) {
-                            // row 2c: b0, b1.xy; row 2c+1: b1.zw, b2
-                            const float4 b0 = ld_off<float4>(S.blk, eo[t]);
-                            const float4 b1 = ld_off<float4>(S.blk, eo[t] + 16u);
-                            const float4 b2 = ld_off<float4>(S.blk, eo[t] + 32u);
-                            // slots t >= n read a valid row whose value may be stale: selected away below
-                            const float2 x01 = ld_off<float2>(S.res, qo[t]);
-                            const float2 x23 = ld_off<float2>(S.res, qo[t] + 8u);
-                            const float2 x45 = ld_off<float2>(S.res, qo[t] + 16u);
-                            const float p0 = b0.x * x01.x + b0.y * x01.y + b0.z * x23.x + b0.w * x23.y +
-                                             b1.x * x45.x + b1.y * x45.y;
-                            const float p1 = b1.z * x01.x + b1.w * x01.y + b2.x * x23.x + b2.y * x23.y +
-                                             b2.z * x45.x + b2.w * x45.y;
-                            v0.x -= (t < n) ? p0 : 0.f;
-                            v0.y -= (t < n) ? p1 : 0.f;
-                            if (NRHS == 2) {
-                                const float2 y01 = ld_off<float2>(S.res, N6b + qo[t]);
-                                const float2 y23 = ld_off<float2>(S.res, N6b + qo[t] + 8u);
-                                const float2 y45 = ld_off<float2>(S.res, N6b + qo[t] + 16u);
-                                const float p2 = b0.x * y01.x + b0.y * y01.y + b0.z * y23.x + b0.w * y23.y +
-                                                 b1.x * y45.x + b1.y * y45.y;
-                                const float p3 = b1.z * y01.x + b1.w * y01.y + b2.x * y23.x + b2.y * y23.y +
-                                                 b2.z * y45.x + b2.w * y45.y;
-                                v1.x -= (t < n) ? p2 : 0.f;
-                                v1.y -= (t < n) ? p3 : 0.f;
-                            }
-                        }
-                    }
-                }
-            }
-            // the row's six components from its three lanes
-            float u0[6], u1[6];
-#pragma unroll
-            for (int m = 0; m < 3; ++m) {
-                u0[2 * m] = __shfl(v0.x, gbase + m);
-                u0[2 * m + 1] = __shfl(v0.y, gbase + m);
-                if (NRHS == 2) {
-                    u1[2 * m] = __shfl(v1.x, gbase + m);
-                    u1[2 * m + 1] = __shfl(v1.y, gbase + m);
-                }
-            }
-            if (act) {
-                // D0, D1.xy: row 2c of the inverse; D1.zw, D2: row 2c+1
-                float2 w0, w1 = make_float2(0.f, 0.f);
-                w0.x = D0.x * u0[0] + D0.y * u0[1] + D0.z * u0[2] + D0.w * u0[3] + D1.x * u0[4] + D1.y * u0[5];
-                w0.y = D1.z * u0[0] + D1.w * u0[1] + D2.x * u0[2] + D2.y * u0[3] + D2.z * u0[4] + D2.w * u0[5];
-                if (NRHS == 2) {
-                    w1.x = D0.x * u1[0] + D0.y * u1[1] + D0.z * u1[2] + D0.w * u1[3] + D1.x * u1[4] + D1.y * u1[5];
-                    w1.y = D1.z * u1[0] + D1.w * u1[1] + D2.x * u1[2] + D2.y * u1[3] + D2.z * u1[4] + D2.w * u1[5];
-                }
-                st_off<float2>(S.res, kofs, w0);
-                if (NRHS == 2) st_off<float2>(S.res, N6b + kofs, w1);
-                const uint32_t o = 24u * (uint32_t)ld_off<int>(S.scatter, 4u * (uint32_t)k) + 8u * (uint32_t)c;
-                st_off<float2>(S.copy0, o, w0);
-                if (NRHS == 2) st_off<float2>(S.copy1, o, w1);
-            }
+        for (int i = 0; i < 9; ++i) {
+            const float4 v = b[i * TP_LANES];
+            m[4 * i] = v.x; m[4 * i + 1] = v.y; m[4 * i + 2] = v.z; m[4 * i + 3] = v.w;
         }
-        __syncthreads();
+        if (inf.x & TP_GLOBAL) {
+            const uint32_t h = (uint32_t)(inf.x & ~TP_GLOBAL);
+#pragma unroll
+            for (int i = 0; i < (NRHS == 2 ? 3 : 2); ++i) gx[i] = S.res[tp_idx(h, i, 3)];
+        }
+    }
+    if (inf.y >= 0) {
+#pragma unroll
+        for (int i = 0; i < (NRHS == 2 ? 3 : 2); ++i) in[i] = S.inp[tp_idx(s * TP_LANES + t, i, 3)];
     }
 }
 
 template <int NRHS>
-__global__ void __launch_bounds__(SW_THREADS) trsv_sweeps_kernel(const int *tab, int ntab, SweepDev F, SweepDev B,
-                                                                 uint32_t N6b)
+__device__ __forceinline__ void tp_unpack(const float4 *w, float *a, float *b)
 {
-    extern __shared__ int s_tab[];
-    for (int t = threadIdx.x; t < ntab; t += blockDim.x) s_tab[t] = tab[t];
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int g = lane / 3;
-    const bool lane_ok = lane < 63;
-    const int slot = (threadIdx.x >> 6) * 21 + g;
-    const int nslot = (blockDim.x >> 6) * 21;
-    const int c = lane - 3 * g;
-    // forward: y = L^-1 P b in forward order, copied to backward order
-    sweep_levels<NRHS>(s_tab, F, N6b, slot, nslot, c, 3 * g, lane_ok);
-    // backward: x = L^-T y in backward order, copied to the outputs in particle order
-    sweep_levels<NRHS>(s_tab, B, N6b, slot, nslot, c, 3 * g, lane_ok);
+    a[0] = w[0].x; a[1] = w[0].y; a[2] = w[0].z; a[3] = w[0].w; a[4] = w[1].x; a[5] = w[1].y;
+    if (NRHS == 2) { b[0] = w[1].z; b[1] = w[1].w; b[2] = w[2].x; b[3] = w[2].y; b[4] = w[2].z; b[5] = w[2].w; }
 }
 
-// inputs of the forward sweep: in[r][6k + a] = b_r[6 fin[k] + a]
-__global__ void sweep_gather_kernel(int N, const int *fin, const float *b0, const float *b1, int nrhs, float *in)
+template <int NRHS>
+__device__ __forceinline__ void tp_pack(const float *a, const float *b, float4 *w)
 {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int64_t N6 = 6 * (int64_t)N;
-    if (t >= N6) return;
-    const int64_t k = t / 6;
-    const int a = (int)(t - 6 * k);
-    const int64_t s = 6 * (int64_t)fin[k] + a;
-    in[t] = b0[s];
-    if (nrhs == 2) in[N6 + t] = b1[s];
+    w[0] = make_float4(a[0], a[1], a[2], a[3]);
+    w[1] = make_float4(a[4], a[5], NRHS == 2 ? b[0] : 0.f, NRHS == 2 ? b[1] : 0.f);
+    if (NRHS == 2) w[2] = make_float4(b[2], b[3], b[4], b[5]);
 }
 
-// blocks in sweep order: forward copies L_PQ, backward stores (L_QP)^T
-__global__ void sweep_pack_blocks_kernel(int64_t n, const int *src, const float *Lblk, float *dst, int transpose)
+template <int NRHS>
+__device__ __forceinline__ void tp_sweep(const TaskDev &S, float4 *ring, float4 *red, int R)
 {
-    const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (idx >= n * 36) return;
-    const int64_t e = idx / 36;
-    const int q = (int)(idx - e * 36);
-    const int a = q / 6, c = q - 6 * a;
-    const int64_t s = (int64_t)src[e] * 36;
-    dst[idx] = transpose ? Lblk[s + 6 * c + a] : Lblk[s + q];
+    constexpr int NW = NRHS == 2 ? 3 : 2;   // float4 per row value
+    const uint32_t t = threadIdx.x;
+    if (S.nsteps <= 0) return;
+    // positions are fetched two steps ahead; past the last step the last one is re-read (and not used)
+    const int last = S.nsteps - 1;
+    int4 cur = S.info[t];
+    int4 nxt = S.info[(size_t)min(1, last) * TP_LANES + t];
+    float m[36];
+    float4 gx[3], in[3];
+    tp_fetch<NRHS>(S, 0, t, cur, m, gx, in);
+    // the previous step's result of this lane (owner), stored during the next step
+    float4 pw[3];
+    bool pown = false;
+    uint32_t pg = 0, pdest = 0;
+    for (int s = 0; s < S.nsteps; ++s) {
+        // everything in flight is needed now: this step's operands, the next step's
+        // positions and (issued before them) the stores of two steps back
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        const bool has = cur.x >= 0, own = cur.y >= 0;
+        float pa[6], pb[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) { pa[a] = 0.f; pb[a] = 0.f; }
+        if (has) {
+            float4 xv[3];
+            if (cur.x & TP_GLOBAL) {
+#pragma unroll
+                for (int i = 0; i < NW; ++i) xv[i] = gx[i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < NW; ++i) xv[i] = ring[i * R + cur.x];
+            }
+            float xa[6], xb[6];
+            tp_unpack<NRHS>(xv, xa, xb);
+#pragma unroll
+            for (int a = 0; a < 6; ++a) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    pa[a] += m[6 * a + c] * xa[c];
+                    if (NRHS == 2) pb[a] += m[6 * a + c] * xb[c];
+                }
+            }
+        }
+        float wa[6], wb[6];
+        if (own) {
+            tp_unpack<NRHS>(in, wa, wb);
+#pragma unroll
+            for (int a = 0; a < 6; ++a) { wa[a] -= pa[a]; if (NRHS == 2) wb[a] -= pb[a]; }
+        }
+        const int slot = cur.y, nd = cur.z, dest = cur.w;
+        // global stores of the previous step, then the operands of step s + 1 (positions
+        // fetched one step earlier) and the positions of step s + 2: waiting for the
+        // operands never waits for a store issued after them
+        if (pown) {
+#pragma unroll
+            for (int i = 0; i < NW; ++i) {
+                S.res[tp_idx(pg, i, 3)] = pw[i];
+                if (S.copy) S.copy[tp_idx(pdest, i, 3)] = pw[i];
+            }
+        }
+        if (s + 1 < S.nsteps) tp_fetch<NRHS>(S, (uint32_t)s + 1, t, nxt, m, gx, in);
+        const int4 n2 = S.info[(size_t)min(s + 2, last) * TP_LANES + t];
+        // the row's other lanes hand their products to its owner (same wave)
+        if (has && !own) {
+            float4 w[3];
+            tp_pack<NRHS>(pa, pb, w);
+#pragma unroll
+            for (int i = 0; i < NW; ++i) red[i * TP_LANES + t] = w[i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        if (own) {
+            for (int d = 1; d < nd; ++d) {
+                float4 w[3];
+#pragma unroll
+                for (int i = 0; i < NW; ++i) w[i] = red[i * TP_LANES + t + d];
+                float qa[6], qb[6];
+                tp_unpack<NRHS>(w, qa, qb);
+#pragma unroll
+                for (int a = 0; a < 6; ++a) { wa[a] -= qa[a]; if (NRHS == 2) wb[a] -= qb[a]; }
+            }
+            tp_pack<NRHS>(wa, wb, pw);
+#pragma unroll
+            for (int i = 0; i < NW; ++i) ring[i * R + slot] = pw[i];
+        }
+        pown = own;
+        pg = (uint32_t)s * TP_LANES + t;
+        pdest = (uint32_t)dest;
+        __syncthreads();
+        cur = nxt;
+        nxt = n2;
+    }
+    if (pown) {
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            S.res[tp_idx(pg, i, 3)] = pw[i];
+            if (S.copy) S.copy[tp_idx(pdest, i, 3)] = pw[i];
+        }
+    }
 }
 
-// inverse Z = L_PP^-1 of every lower-triangular diagonal block: forward order
-// stores Z, backward order stores Z^T
-__global__ void sweep_pack_dinv_kernel(int N, const int *f_order, const int *b_order, const float *Ldiag, float *fd,
-                                       float *bd)
+template <int NRHS>
+__global__ void __launch_bounds__(TP_LANES) tp_sweeps_kernel(TaskDev F, TaskDev B, int R)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * N) return;
-    const bool fwd = t < N;
-    const int k = fwd ? t : t - N;
-    const int P = fwd ? f_order[k] : b_order[k];
-    float Lm[36], Z[36];
-    for (int q = 0; q < 36; ++q) { Lm[q] = Ldiag[(int64_t)P * 36 + q]; Z[q] = 0.f; }
+    extern __shared__ float4 tp_lds[];
+    float4 *ring = tp_lds;                   // [3][R]
+    float4 *red = tp_lds + 3 * R;            // [3][1024]
+    tp_sweep<NRHS>(F, ring, red, R);
+    __syncthreads();                         // the forward results (backward inputs) are in global memory
+    tp_sweep<NRHS>(B, ring, red, R);
+}
+
+// Z_P = L_PP^-1 of every lower-triangular diagonal block (row-major)
+__global__ void diag_inverse_kernel(int N, const float *Ldiag, float *Z)
+{
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= N) return;
+    float Lm[36], Zm[36];
+    for (int q = 0; q < 36; ++q) { Lm[q] = Ldiag[(int64_t)P * 36 + q]; Zm[q] = 0.f; }
     for (int c = 0; c < 6; ++c) {
-        Z[6 * c + c] = 1.0f / Lm[6 * c + c];
+        Zm[6 * c + c] = 1.0f / Lm[6 * c + c];
         for (int r = c + 1; r < 6; ++r) {
             float s = 0.f;
-            for (int q = c; q < r; ++q) s += Lm[6 * r + q] * Z[6 * q + c];
-            Z[6 * r + c] = -s / Lm[6 * r + r];
+            for (int q = c; q < r; ++q) s += Lm[6 * r + q] * Zm[6 * q + c];
+            Zm[6 * r + c] = -s / Lm[6 * r + r];
         }
     }
-    float *dst = (fwd ? fd : bd) + (int64_t)k * 36;
-    for (int a = 0; a < 6; ++a)
-        for (int c = 0; c < 6; ++c) dst[6 * a + c] = fwd ? Z[6 * a + c] : Z[6 * c + a];
+    for (int q = 0; q < 36; ++q) Z[(int64_t)P * 36 + q] = Zm[q];
 }
 
+// lane blocks: M = Z_d L_e (forward) or its transpose (backward)
+__global__ void tp_pack_kernel(int64_t lanes, const int2 *bsrc, const float *Lblk, const float *Z, float4 *blk,
+                               int transpose)
+{
+    const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (g >= lanes) return;
+    const int2 sr = bsrc[g];
+    if (sr.x < 0) return;
+    const float *L = Lblk + (int64_t)sr.x * 36;
+    const float *Zd = Z + (int64_t)sr.y * 36;
+    float M[36];
+    for (int a = 0; a < 6; ++a)
+        for (int c = 0; c < 6; ++c) {
+            float s = 0.f;
+            for (int q = 0; q <= a; ++q) s += Zd[6 * a + q] * L[6 * q + c];
+            if (transpose) M[6 * c + a] = s;
+            else M[6 * a + c] = s;
+        }
+    for (int i = 0; i < 9; ++i)
+        blk[tp_idx((uint32_t)g, i, 9)] = make_float4(M[4 * i], M[4 * i + 1], M[4 * i + 2], M[4 * i + 3]);
+}
+
+// forward inputs: inp[lane of P] = Z_P b_r[perm P]
+__global__ void tp_pre_kernel(int N, const int *gtid, const int *perm, const float *Z, const float *b0,
+                              const float *b1, int nrhs, float4 *inp)
+{
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= N) return;
+    const float *Zp = Z + (int64_t)P * 36;
+    const int64_t src = 6 * (int64_t)perm[P];
+    float a[6], b[6];
+    for (int r = 0; r < 6; ++r) {
+        float sa = 0.f, sb = 0.f;
+        for (int c = 0; c <= r; ++c) {
+            sa += Zp[6 * r + c] * b0[src + c];
+            if (nrhs == 2) sb += Zp[6 * r + c] * b1[src + c];
+        }
+        a[r] = sa;
+        b[r] = sb;
+    }
+    const uint32_t g = (uint32_t)gtid[P];
+    inp[tp_idx(g, 0, 3)] = make_float4(a[0], a[1], a[2], a[3]);
+    inp[tp_idx(g, 1, 3)] = make_float4(a[4], a[5], b[0], b[1]);
+    inp[tp_idx(g, 2, 3)] = make_float4(b[2], b[3], b[4], b[5]);
+}
+
+// outputs: y_r[6 perm P + a] = (Z_P^T u_P)_a
+__global__ void tp_post_kernel(int N, const int *gtid, const int *perm, const float *Z, const float4 *res,
+                               int nrhs, float *y0, float *y1)
+{
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= N) return;
+    const uint32_t g = (uint32_t)gtid[P];
+    const float4 w0 = res[tp_idx(g, 0, 3)], w1 = res[tp_idx(g, 1, 3)];
+    const float ua[6] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y};
+    float ub[6] = {w1.z, w1.w, 0.f, 0.f, 0.f, 0.f};
+    if (nrhs == 2) {
+        const float4 w2 = res[tp_idx(g, 2, 3)];
+        ub[2] = w2.x; ub[3] = w2.y; ub[4] = w2.z; ub[5] = w2.w;
+    }
+    const float *Zp = Z + (int64_t)P * 36;
+    const int64_t dst = 6 * (int64_t)perm[P];
+    for (int a = 0; a < 6; ++a) {
+        float sa = 0.f, sb = 0.f;
+        for (int c = a; c < 6; ++c) {
+            sa += Zp[6 * c + a] * ua[c];
+            sb += Zp[6 * c + a] * ub[c];
+        }
+        y0[dst + a] = sa;
+        if (nrhs == 2) y1[dst + a] = sb;
+    }
+}
+
+// ring capacity of the sweep kernel on this device (tp_R = 0: level by level)
+int k_sweep_setup(fsd_ctx *ctx)
+{
+    int dev = 0, lds = 0;
+    ctx->tp_R = 0;
+    FSD_HIP(hipGetDevice(&dev));
+    FSD_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const int R = (lds / (int)sizeof(float4) - 3 * TP_LANES) / 3;
+    if (R < 3 * TP_ROWS_MAX) return 0;
+    const int bytes = (int)sizeof(float4) * (3 * R + 3 * TP_LANES);
+    FSD_HIP(hipFuncSetAttribute((const void *)tp_sweeps_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                bytes));
+    FSD_HIP(hipFuncSetAttribute((const void *)tp_sweeps_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                bytes));
+    ctx->tp_R = R;
+    return 0;
+}
+
+// after IC(0): inverse diagonal blocks and the lane blocks of both sweeps
 int k_sweep_pack(fsd_ctx *ctx)
 {
     const int N = ctx->N;
-    const int64_t n = ctx->nlow;
-    if (n) {
-        hipLaunchKernelGGL(sweep_pack_blocks_kernel, dim3(nblk(n * 36, 256)), dim3(256), 0, ctx->stream, n,
-                           ctx->d_sw_fsrc, ctx->d_Lblk, ctx->d_sw_fblk, 0);
-        hipLaunchKernelGGL(sweep_pack_blocks_kernel, dim3(nblk(n * 36, 256)), dim3(256), 0, ctx->stream, n,
-                           ctx->d_sw_bsrc, ctx->d_Lblk, ctx->d_sw_bblk, 1);
-    }
-    hipLaunchKernelGGL(sweep_pack_dinv_kernel, dim3(nblk(2 * (int64_t)N, 128)), dim3(128), 0, ctx->stream, N,
-                       ctx->d_sw_forder, ctx->d_sw_border, ctx->d_Ldiag, ctx->d_sw_fdinv, ctx->d_sw_bdinv);
+    hipLaunchKernelGGL(diag_inverse_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_Ldiag,
+                       ctx->d_tp_Z);
+    const int64_t lf = (int64_t)ctx->tp_f.nsteps * TP_LANES, lb = (int64_t)ctx->tp_b.nsteps * TP_LANES;
+    if (lf)
+        hipLaunchKernelGGL(tp_pack_kernel, dim3(nblk(lf, 256)), dim3(256), 0, ctx->stream, lf, ctx->d_tp_fbsrc,
+                           ctx->d_Lblk, ctx->d_tp_Z, ctx->d_tp_fblk, 0);
+    if (lb)
+        hipLaunchKernelGGL(tp_pack_kernel, dim3(nblk(lb, 256)), dim3(256), 0, ctx->stream, lb, ctx->d_tp_bbsrc,
+                           ctx->d_Lblk, ctx->d_tp_Z, ctx->d_tp_bblk, 1);
     FSD_HIP(hipGetLastError());
     return 0;
 }
@@ -544,22 +637,19 @@ int k_precond_apply2(fsd_ctx *ctx, const float *b0, const float *b1, float *y0, 
     const int N = ctx->N;
     hipEvent_t ev;
     prof_begin(ctx, PROF_TRSV, &ev);
-    if (ctx->sw_ok) {
-        const int64_t N6 = 6 * (int64_t)N;
-        float *bin = ctx->d_sw_work, *yf = bin + 2 * N6, *yb = yf + 2 * N6, *x = yb + 2 * N6;
-        hipLaunchKernelGGL(sweep_gather_kernel, dim3(nblk(N6, 256)), dim3(256), 0, ctx->stream, N, ctx->d_sw_fin, b0,
-                           b1, n_rhs, bin);
-        SweepDev F{ctx->sw_f.nlev, ctx->sw_f.levrow, ctx->sw_f.levjd, ctx->sw_f.rec, ctx->d_sw_fcol, ctx->d_sw_fblk,
-                   ctx->d_sw_fdinv, bin, yf, ctx->d_sw_fyb, yb, yb + N6};
-        SweepDev B{ctx->sw_b.nlev, ctx->sw_b.levrow, ctx->sw_b.levjd, ctx->sw_b.rec, ctx->d_sw_bcol, ctx->d_sw_bblk,
-                   ctx->d_sw_bdinv, yb, x, ctx->d_sw_bout, y0, (n_rhs == 2) ? y1 : y0};
-        const size_t lds = sizeof(int) * (size_t)ctx->sw_ntab;
+    if (ctx->tp_ok) {
+        hipLaunchKernelGGL(tp_pre_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_fgtid,
+                           ctx->d_perm, ctx->d_tp_Z, b0, b1, n_rhs, ctx->d_tp_finp);
+        TaskDev F{ctx->tp_f.nsteps, ctx->d_tp_finfo, ctx->d_tp_fblk, ctx->d_tp_finp, ctx->d_tp_fres,
+                  ctx->d_tp_binp};
+        TaskDev B{ctx->tp_b.nsteps, ctx->d_tp_binfo, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres, nullptr};
+        const size_t lds = sizeof(float4) * (3 * (size_t)ctx->tp_R + 3 * TP_LANES);
         if (n_rhs == 2)
-            hipLaunchKernelGGL(trsv_sweeps_kernel<2>, dim3(1), dim3(SW_THREADS), lds, ctx->stream, ctx->d_sw_tab,
-                               ctx->sw_ntab, F, B, (uint32_t)(4 * N6));
+            hipLaunchKernelGGL(tp_sweeps_kernel<2>, dim3(1), dim3(TP_LANES), lds, ctx->stream, F, B, ctx->tp_R);
         else
-            hipLaunchKernelGGL(trsv_sweeps_kernel<1>, dim3(1), dim3(SW_THREADS), lds, ctx->stream, ctx->d_sw_tab,
-                               ctx->sw_ntab, F, B, (uint32_t)(4 * N6));
+            hipLaunchKernelGGL(tp_sweeps_kernel<1>, dim3(1), dim3(TP_LANES), lds, ctx->stream, F, B, ctx->tp_R);
+        hipLaunchKernelGGL(tp_post_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_bgtid,
+                           ctx->d_perm, ctx->d_tp_Z, ctx->d_tp_bres, n_rhs, y0, y1);
         FSD_HIP(hipGetLastError());
         prof_end(ctx, PROF_TRSV, ev);
         return 0;

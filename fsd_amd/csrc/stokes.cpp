@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -197,6 +198,12 @@ static int alloc_all(fsd_ctx *ctx)
     CK(hipMalloc(&ctx->d_scan_tmp, sizeof(int) * 2 * (scan_excl_chunks(ctx->M) + 1)));
     if (ctx->tile + ps.P - 1 > 32) { ctx->err = "spreading support P too large for the tile halo"; return FSD_E_PARAM; }
     ctx->contract_lds = k_contract_lds(ctx);
+    // FSD_TRSV_LEVELS=1 runs the triangular sweeps one launch per level
+    {
+        const char *lv = getenv("FSD_TRSV_LEVELS");
+        if (lv && lv[0] == '1') ctx->tp_R = 0;
+        else RC(k_sweep_setup(ctx));
+    }
     CK(hipMalloc(&ctx->d_wave, sizeof(float) * 11 * (size_t)N));
     CK(hipMalloc(&ctx->d_rec, sizeof(float4) * 3 * (size_t)N));
     {
@@ -248,18 +255,9 @@ static int alloc_all(fsd_ctx *ctx)
     CK(hipMalloc(&ctx->d_Ldiag, sizeof(float) * 36 * (size_t)N));
     CK(hipMalloc(&ctx->d_lev_f, sizeof(int) * N));
     CK(hipMalloc(&ctx->d_lev_b, sizeof(int) * N));
-    CK(hipMalloc(&ctx->d_sw_tab, sizeof(int) * SW_TAB_MAX));
-    CK(hipMalloc(&ctx->d_sw_fcol, sizeof(int) * ctx->nlow_cap));
-    CK(hipMalloc(&ctx->d_sw_bcol, sizeof(int) * ctx->nlow_cap));
-    CK(hipMalloc(&ctx->d_sw_fsrc, sizeof(int) * ctx->nlow_cap));
-    CK(hipMalloc(&ctx->d_sw_bsrc, sizeof(int) * ctx->nlow_cap));
-    CK(hipMalloc(&ctx->d_sw_fblk, sizeof(float) * 36 * ctx->nlow_cap));
-    CK(hipMalloc(&ctx->d_sw_bblk, sizeof(float) * 36 * ctx->nlow_cap));
-    CK(hipMalloc(&ctx->d_sw_fdinv, sizeof(float) * 36 * (size_t)N));
-    CK(hipMalloc(&ctx->d_sw_bdinv, sizeof(float) * 36 * (size_t)N));
-    for (int **pp : {&ctx->d_sw_forder, &ctx->d_sw_border, &ctx->d_sw_fin, &ctx->d_sw_fyb, &ctx->d_sw_bout})
-        CK(hipMalloc(pp, sizeof(int) * N));
-    CK(hipMalloc(&ctx->d_sw_work, sizeof(float) * 48 * (size_t)N));
+    CK(hipMalloc(&ctx->d_tp_Z, sizeof(float) * 36 * (size_t)N));
+    CK(hipMalloc(&ctx->d_tp_fgtid, sizeof(int) * (size_t)N));
+    CK(hipMalloc(&ctx->d_tp_bgtid, sizeof(int) * (size_t)N));
     CK(hipMalloc(&ctx->d_tmp6a, sizeof(float) * 12 * (size_t)N));
     CK(hipMalloc(&ctx->d_tmp6b, sizeof(float) * 12 * (size_t)N));
     CK(hipMalloc(&ctx->d_g12, sizeof(float) * 12 * (size_t)N));
@@ -300,9 +298,10 @@ static void free_all(fsd_ctx *ctx)
                     ctx->d_perm, ctx->d_iperm, ctx->d_lrow, ctx->d_urow, ctx->d_lcol, ctx->d_ucol, ctx->d_upos,
                     ctx->d_Ablk, ctx->d_Lblk, ctx->d_Adiag, ctx->d_Ldiag, ctx->d_lev_f, ctx->d_lev_b, ctx->d_tmp6a,
                     ctx->d_tmp6b, ctx->d_g12, ctx->d_V, ctx->d_w, ctx->d_t, ctx->d_t2, ctx->d_part, ctx->d_rhs, ctx->d_sol,
-                    ctx->d_force, ctx->d_mask1, ctx->d_mask2, ctx->d_sw_tab, ctx->d_sw_fcol, ctx->d_sw_bcol,
-                    ctx->d_sw_fsrc, ctx->d_sw_bsrc, ctx->d_sw_fblk, ctx->d_sw_bblk, ctx->d_sw_fdinv, ctx->d_sw_bdinv,
-                    ctx->d_sw_forder, ctx->d_sw_border, ctx->d_sw_fin, ctx->d_sw_fyb, ctx->d_sw_bout, ctx->d_sw_work,
+                    ctx->d_force, ctx->d_mask1, ctx->d_mask2, ctx->d_tp_finfo, ctx->d_tp_binfo,
+                    ctx->d_tp_fbsrc, ctx->d_tp_bbsrc, ctx->d_tp_fblk, ctx->d_tp_bblk,
+                    ctx->d_tp_finp, ctx->d_tp_binp, ctx->d_tp_fres, ctx->d_tp_bres, ctx->d_tp_fgtid, ctx->d_tp_bgtid,
+                    ctx->d_tp_Z,
                     ctx->d_Z, ctx->d_lanV, ctx->d_lanw, ctx->d_lany0, ctx->d_lan_h, ctx->d_noise, ctx->d_posp,
                     ctx->d_rfd_rhs, ctx->d_rfd_sol};
     for (void *p : ptrs)
@@ -324,16 +323,35 @@ static int grow_lower(fsd_ctx *ctx, int64_t need)
     if (need <= ctx->nlow_cap) return 0;
     const int64_t cap = need + need / 4 + 64;
     CK(hipStreamSynchronize(ctx->stream));
-    for (int *q : {ctx->d_lcol, ctx->d_ucol, ctx->d_upos, ctx->d_sw_fcol, ctx->d_sw_bcol, ctx->d_sw_fsrc,
-                   ctx->d_sw_bsrc})
-        hipFree(q);
-    for (float *q : {ctx->d_Ablk, ctx->d_Lblk, ctx->d_sw_fblk, ctx->d_sw_bblk}) hipFree(q);
-    for (int **pp : {&ctx->d_lcol, &ctx->d_ucol, &ctx->d_upos, &ctx->d_sw_fcol, &ctx->d_sw_bcol, &ctx->d_sw_fsrc,
-                     &ctx->d_sw_bsrc})
-        CK(hipMalloc(pp, sizeof(int) * cap));
-    for (float **pp : {&ctx->d_Ablk, &ctx->d_Lblk, &ctx->d_sw_fblk, &ctx->d_sw_bblk})
-        CK(hipMalloc(pp, sizeof(float) * 36 * cap));
+    for (int *q : {ctx->d_lcol, ctx->d_ucol, ctx->d_upos}) hipFree(q);
+    for (float *q : {ctx->d_Ablk, ctx->d_Lblk}) hipFree(q);
+    for (int **pp : {&ctx->d_lcol, &ctx->d_ucol, &ctx->d_upos}) CK(hipMalloc(pp, sizeof(int) * cap));
+    for (float **pp : {&ctx->d_Ablk, &ctx->d_Lblk}) CK(hipMalloc(pp, sizeof(float) * 36 * cap));
     ctx->nlow_cap = cap;
+    return 0;
+}
+
+// device operands of the sweep task plans (lanes grow with the number of steps)
+static int grow_tasks(fsd_ctx *ctx)
+{
+    const int64_t lf = (int64_t)ctx->tp_f.nsteps * TP_LANES, lb = (int64_t)ctx->tp_b.nsteps * TP_LANES;
+    auto grow = [ctx](int64_t need, int64_t &cap, int4 *&info, int2 *&bsrc, float4 *&blk, float4 *&inp,
+                      float4 *&res) -> int {
+        if (need <= cap) return 0;
+        const int64_t c = need + need / 4 + TP_LANES;
+        CK(hipStreamSynchronize(ctx->stream));
+        for (void *q : {(void *)info, (void *)bsrc, (void *)blk, (void *)inp, (void *)res})
+            if (q) hipFree(q);
+        CK(hipMalloc(&info, sizeof(int4) * c));
+        CK(hipMalloc(&bsrc, sizeof(int2) * c));
+        CK(hipMalloc(&blk, sizeof(float4) * 9 * c));
+        CK(hipMalloc(&inp, sizeof(float4) * 3 * c));
+        CK(hipMalloc(&res, sizeof(float4) * 3 * c));
+        cap = c;
+        return 0;
+    };
+    RC(grow(lf, ctx->tp_cap_f, ctx->d_tp_finfo, ctx->d_tp_fbsrc, ctx->d_tp_fblk, ctx->d_tp_finp, ctx->d_tp_fres));
+    RC(grow(lb, ctx->tp_cap_b, ctx->d_tp_binfo, ctx->d_tp_bbsrc, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres));
     return 0;
 }
 
@@ -433,52 +451,27 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
         CK(hipMemcpyAsync(ctx->d_lev_b, itb.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
         CK(hipStreamSynchronize(ctx->stream));
         // schedules of the single-workgroup sweeps
-        build_sweep_plan(N, lf.data(), ctx->nlev_f, lrow.data(), lcol.data(), nullptr, ctx->plan_f);
-        build_sweep_plan(N, lb.data(), ctx->nlev_b, urow.data(), ucol.data(), upos.data(), ctx->plan_b);
+        ctx->tp_ok = ctx->tp_R > 0 &&
+                     build_task_plan(N, lf.data(), ctx->nlev_f, lrow.data(), lcol.data(), nullptr, true, ctx->tp_R,
+                                     ctx->tp_f) &&
+                     build_task_plan(N, lb.data(), ctx->nlev_b, urow.data(), ucol.data(), upos.data(), false,
+                                     ctx->tp_R, ctx->tp_b);
     }
     RC(grow_lower(ctx, ctx->nlow + 1));
-    {
-        const SweepPlan &f = ctx->plan_f, &b = ctx->plan_b;
-        std::vector<int> &tab = ctx->h_sw_tab;
-        tab.clear();
-        auto put = [&tab](const SweepPlan &pl, fsd_ctx::SweepOffsets &o) {
-            o.nlev = pl.nlev;
-            o.levrow = (int)tab.size();
-            tab.insert(tab.end(), pl.levrow.begin(), pl.levrow.end());
-            o.levjd = (int)tab.size();
-            tab.insert(tab.end(), pl.levjd.begin(), pl.levjd.end());
-            o.rec = (int)tab.size();
-            tab.insert(tab.end(), pl.rec.begin(), pl.rec.end());
+    if (ctx->tp_ok) {
+        link_task_plans(N, ctx->tp_f, ctx->tp_b);
+        RC(grow_tasks(ctx));
+        auto up = [ctx](void *d, const void *h, size_t bytes) {
+            return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream);
         };
-        put(f, ctx->sw_f);
-        put(b, ctx->sw_b);
-        ctx->sw_ntab = (int)tab.size();
-        // LDS holds the schedule; the kernel addresses its operands with 32-bit byte offsets
-        ctx->sw_ok = ctx->sw_ntab <= SW_TAB_MAX && ctx->nlow < (int64_t)29000000 && N < 80000000;
-        if (ctx->sw_ok) {
-            std::vector<int> fin(N), fyb(N), bout(N);
-            for (int k = 0; k < N; ++k) {
-                fin[k] = ctx->h_perm[f.order[k]];
-                fyb[k] = b.pos[f.order[k]];
-                bout[k] = ctx->h_perm[b.order[k]];
-            }
-            auto up = [ctx](int *d, const int *h, size_t n) {
-                return hipMemcpyAsync(d, h, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream);
-            };
-            CK(up(ctx->d_sw_tab, tab.data(), tab.size()));
-            CK(up(ctx->d_sw_forder, f.order.data(), N));
-            CK(up(ctx->d_sw_border, b.order.data(), N));
-            CK(up(ctx->d_sw_fin, fin.data(), N));
-            CK(up(ctx->d_sw_fyb, fyb.data(), N));
-            CK(up(ctx->d_sw_bout, bout.data(), N));
-            if (ctx->nlow) {
-                CK(up(ctx->d_sw_fcol, f.ecol.data(), ctx->nlow));
-                CK(up(ctx->d_sw_fsrc, f.esrc.data(), ctx->nlow));
-                CK(up(ctx->d_sw_bcol, b.ecol.data(), ctx->nlow));
-                CK(up(ctx->d_sw_bsrc, b.esrc.data(), ctx->nlow));
-            }
-            CK(hipStreamSynchronize(ctx->stream));   // host vectors above go out of scope
-        }
+        const TaskPlan &f = ctx->tp_f, &b = ctx->tp_b;
+        CK(up(ctx->d_tp_finfo, f.info.data(), sizeof(int) * f.info.size()));
+        CK(up(ctx->d_tp_binfo, b.info.data(), sizeof(int) * b.info.size()));
+        CK(up(ctx->d_tp_fbsrc, f.bsrc.data(), sizeof(int) * f.bsrc.size()));
+        CK(up(ctx->d_tp_bbsrc, b.bsrc.data(), sizeof(int) * b.bsrc.size()));
+        CK(up(ctx->d_tp_fgtid, f.gtid.data(), sizeof(int) * (size_t)N));
+        CK(up(ctx->d_tp_bgtid, b.gtid.data(), sizeof(int) * (size_t)N));
+        CK(hipStreamSynchronize(ctx->stream));
     }
     CK(hipMemcpyAsync(ctx->d_perm, ctx->h_perm.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
     CK(hipMemcpyAsync(ctx->d_iperm, iperm.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
@@ -502,7 +495,7 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
         if (lambda > 1.0e7f) { ctx->err = "IC(0) did not succeed for any relaxer"; return FSD_E_ICHOL; }
     }
     ctx->ichol_relaxer = lambda;
-    if (ctx->sw_ok) RC(k_sweep_pack(ctx));
+    if (ctx->tp_ok) RC(k_sweep_pack(ctx));
     prof_end(ctx, PROF_PRECOND_BUILD, ev);
     ctx->precond_valid = true;
     return 0;
@@ -1128,6 +1121,9 @@ int fsd_build_precond(fsd_ctx *ctx, const float *d_pos4, fsd_solve_stats *stats)
         stats->ichol_relaxer = ctx->ichol_relaxer;
         stats->ichol_attempts = ctx->ichol_attempts;
         stats->nnz = ctx->nnz_ref;
+        stats->lower_blocks = ctx->nlow;
+        stats->levels = ctx->nlev_f;
+        stats->sweep_steps = ctx->tp_ok ? ctx->tp_f.nsteps + ctx->tp_b.nsteps : 0;
     }
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1282,6 +1278,7 @@ int fsd_saddle_solve(fsd_ctx *ctx, const float *d_pos4, const float *d_rhs17, fl
     st.lub_pairs = ctx->lub_total;
     st.lower_blocks = ctx->nlow;
     st.levels = ctx->nlev_f;
+    st.sweep_steps = ctx->tp_ok ? ctx->tp_f.nsteps + ctx->tp_b.nsteps : 0;
     if (stats) *stats = st;
     CK(hipStreamSynchronize(ctx->stream));
     return rc;
@@ -1363,6 +1360,7 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     st.lub_pairs = ctx->lub_total;
     st.lower_blocks = ctx->nlow;
     st.levels = ctx->nlev_f;
+    st.sweep_steps = ctx->tp_ok ? ctx->tp_f.nsteps + ctx->tp_b.nsteps : 0;
     st.lanczos_ff = (ctx->p.T > 0.f) ? ctx->last_lanczos_ff : 0;
     st.lanczos_nf = (ctx->p.T > 0.f) ? ctx->last_lanczos_nf : 0;
     // relaxer and Krylov-size bookkeeping (Stokes.cc:1658-1666)

@@ -1,58 +1,97 @@
-// Static schedules of the two block-triangular sweeps of G = P^T L^-T L^-1 P.
+// Static schedules of the two block-triangular sweeps (see sweep_plan.h).
 //
-// Rows (permuted particles) are grouped by level of the sweep's DAG; inside a
-// level they are ordered by descending number of off-diagonal blocks (ties by
-// particle id), and the blocks of the level are stored in jagged-diagonal
-// order: first the 0th block of every row that has one, then the 1st, ... .
-// A level's rows are therefore consecutive positions r0..r1-1 and the j-th
-// block of row r0+k sits at entry rec[j].first + k, so a solver thread finds
-// its blocks from two numbers per (level, j) with no per-row pointer loads.
+// Rows are taken level by level; inside a level by descending number of
+// dependencies (ties by particle id) and packed first-fit into the waves of
+// the current step, so busy waves are the low-numbered ones and spread over
+// the four SIMDs.  A level that does not fit in 1024 lanes or TP_ROWS_MAX rows
+// continues in the next step (its rows are independent, so the extra barrier
+// is harmless).
+
 #include "sweep_plan.h"
 
 #include <algorithm>
 
-void build_sweep_plan(int N, const int *lev, int nlev, const int *rowptr, const int *cols, const int *src,
-                      SweepPlan &pl)
+bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, const int *cols, const int *src,
+                     bool forward, int R, TaskPlan &tp)
 {
-    pl.nlev = nlev;
-    pl.levrow.assign(nlev + 1, 0);
-    for (int P = 0; P < N; ++P) pl.levrow[lev[P] + 1]++;
-    for (int l = 0; l < nlev; ++l) pl.levrow[l + 1] += pl.levrow[l];
-    pl.order.assign(N, 0);
-    {
-        std::vector<int> fill(pl.levrow.begin(), pl.levrow.end() - 1);
-        for (int P = 0; P < N; ++P) pl.order[fill[lev[P]]++] = P;   // ascending P inside a level
-    }
+    tp = TaskPlan{};
+    if (R < 3 * TP_ROWS_MAX) return false;
     auto cnt = [&](int P) { return rowptr[P + 1] - rowptr[P]; };
+    for (int P = 0; P < N; ++P)
+        if (cnt(P) > TP_WAVE) return false;
+    std::vector<int> lptr(nlev + 1, 0), order(N);
+    for (int P = 0; P < N; ++P) lptr[lev[P] + 1]++;
+    for (int l = 0; l < nlev; ++l) lptr[l + 1] += lptr[l];
+    {
+        std::vector<int> fill(lptr.begin(), lptr.end() - 1);
+        for (int P = 0; P < N; ++P) order[fill[lev[P]]++] = P;
+    }
     for (int l = 0; l < nlev; ++l)
-        std::stable_sort(pl.order.begin() + pl.levrow[l], pl.order.begin() + pl.levrow[l + 1],
+        std::stable_sort(order.begin() + lptr[l], order.begin() + lptr[l + 1],
                          [&](int a, int b) { return cnt(a) > cnt(b); });
-    pl.pos.assign(N, 0);
-    for (int k = 0; k < N; ++k) pl.pos[pl.order[k]] = k;
-    pl.levjd.assign(nlev + 1, 0);
-    pl.rec.clear();
-    pl.esrc.clear();
-    pl.ecol.clear();
-    const int64_t nent = rowptr[N];
-    pl.esrc.reserve(nent);
-    pl.ecol.reserve(nent);
+
+    // packing: lanes, sequence numbers and steps
+    constexpr int NWAVE = TP_LANES / TP_WAVE;
+    tp.gtid.assign(N, -1);
+    std::vector<int> seq(N, 0), rowstep(N, 0), send;
+    int nseq = 0, step = -1, rows = 0;
+    int fill[NWAVE];
+    auto open_step = [&]() {
+        ++step;
+        rows = 0;
+        std::fill(fill, fill + NWAVE, 0);
+    };
     for (int l = 0; l < nlev; ++l) {
-        pl.levjd[l] = (int)(pl.rec.size() / 2);
-        const int r0 = pl.levrow[l], r1 = pl.levrow[l + 1];
-        const int J = (r1 > r0) ? cnt(pl.order[r0]) : 0;   // rows sorted by descending count
-        for (int j = 0; j < J; ++j) {
-            int nr = 0;
-            while (r0 + nr < r1 && cnt(pl.order[r0 + nr]) > j) ++nr;
-            pl.rec.push_back((int)pl.esrc.size());
-            pl.rec.push_back(nr);
-            for (int k = 0; k < nr; ++k) {
-                const int P = pl.order[r0 + k];
-                const int e = rowptr[P] + j;
-                pl.esrc.push_back(src ? src[e] : e);
-                pl.ecol.push_back(cols[e]);              // particle id for now, position below
+        if (lptr[l + 1] == lptr[l]) continue;
+        open_step();
+        for (int k = lptr[l]; k < lptr[l + 1]; ++k) {
+            const int P = order[k];
+            const int need = std::max(cnt(P), 1);
+            int w = 0;
+            while (w < NWAVE && fill[w] + need > TP_WAVE) ++w;
+            if (w == NWAVE || rows == TP_ROWS_MAX) {
+                send.push_back(nseq);
+                open_step();
+                w = 0;
             }
+            tp.gtid[P] = step * TP_LANES + w * TP_WAVE + fill[w];
+            fill[w] += need;
+            ++rows;
+            rowstep[P] = step;
+            seq[P] = nseq++;
+        }
+        send.push_back(nseq);
+    }
+    tp.nsteps = step + 1;
+
+    auto in_ring = [&](int q, int S) { return q >= send[S] - R; };
+
+    // lanes
+    const size_t L = (size_t)tp.nsteps * TP_LANES;
+    tp.info.assign(4 * L, -1);
+    for (size_t g = 0; g < L; ++g) tp.info[4 * g + 2] = 0;
+    tp.bsrc.assign(2 * L, -1);
+    for (int P = 0; P < N; ++P) {
+        const int g = tp.gtid[P], S = rowstep[P], n = cnt(P);
+        tp.info[4 * (size_t)g + 1] = seq[P] % R;
+        tp.info[4 * (size_t)g + 2] = n;
+        for (int j = 0; j < n; ++j) {
+            const int e = rowptr[P] + j, Q = cols[e];
+            const size_t h = (size_t)g + j;
+            if (in_ring(seq[Q], S)) {
+                tp.info[4 * h] = seq[Q] % R;
+            } else {
+                if (rowstep[Q] > S - 3) return false;   // unreachable: three steps fit in the ring
+                tp.info[4 * h] = TP_GLOBAL | tp.gtid[Q];
+            }
+            tp.bsrc[2 * h] = src ? src[e] : e;
+            tp.bsrc[2 * h + 1] = forward ? P : Q;
         }
     }
-    pl.levjd[nlev] = (int)(pl.rec.size() / 2);
-    for (auto &c : pl.ecol) c = pl.pos[c];
+    return true;
+}
+
+void link_task_plans(int N, TaskPlan &fwd, const TaskPlan &bwd)
+{
+    for (int P = 0; P < N; ++P) fwd.info[4 * (size_t)fwd.gtid[P] + 3] = bwd.gtid[P];
 }

@@ -91,6 +91,7 @@ typedef struct fsd_solve_stats {
     int32_t levels;                  /* level-schedule depth of the triangular solves */
     int32_t lanczos_ff;              /* Krylov size of the far-field Brownian sampler (T > 0) */
     int32_t lanczos_nf;              /* Krylov size of the near-field Brownian sampler (T > 0) */
+    int32_t sweep_steps;             /* steps of the one-launch triangular sweeps (0: one launch per level) */
 } fsd_solve_stats;
 
 /* ---- lifecycle: Stokes::Stokes + setParams + setSparseMath + AllocateWorkSpaces

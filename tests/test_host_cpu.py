@@ -62,15 +62,19 @@ def test_host_rcm_matches_oracle(host_rcm):
 
 
 @pytest.mark.parametrize("args", [("1", "1", "5"), ("50", "5", "5"), ("3000", "40", "5"), ("20000", "200", "5"),
-                                  ("3000", "30", "16")])
+                                  ("3000", "30", "16"), ("20000", "300", "8", "30"), ("2000", "60", "70", "10"),
+                                  ("40000", "400", "6", "50", "2304")])
 def test_sweep_schedule_emulation(tmp_path, args):
-    """trsv_sweeps_kernel's index math replayed on the CPU: in bounds, no early
-    reads, equal to a direct block solve (rows with up to 15 blocks)"""
+    """tp_sweeps_kernel's schedule replayed on the CPU with a visibility model:
+    indices in bounds, ring and global reads find their producers after they
+    are ordered, reductions stay in a wave, and the result equals a direct
+    block solve (rows with up to 64 blocks, many isolated rows, the smallest
+    ring the planner accepts)"""
     exe = str(tmp_path / "sweep_emulation")
     _gxx(exe, os.path.join(CPU, "sweep_emulation.cpp"), os.path.join(CSRC, "sweep_plan.cpp"))
     r = subprocess.run([exe] + list(args), capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "unready reads=0, oob=0" in r.stdout
+    assert "hazards=0 oob=0" in r.stdout
 
 
 def test_tridiagonal_square_root(tmp_path):
