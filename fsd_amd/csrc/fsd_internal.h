@@ -228,14 +228,14 @@ struct fsd_ctx {
     // single-workgroup sweeps: task plans (sweep_plan.h) and their device operands
     TaskPlan tp_f, tp_b;
     bool tp_ok = false;
-    int tp_R = 0;                                        // LDS ring capacity in rows
+    int tp_R = 0;                                        // LDS ring capacity in rows (0: level path)
     int64_t tp_cap_f = 0, tp_cap_b = 0;                  // lanes allocated per sweep
-    int4 *d_tp_finfo = nullptr, *d_tp_binfo = nullptr;   // per lane
+    int2 *d_tp_finfo = nullptr, *d_tp_binfo = nullptr;   // per lane
     int2 *d_tp_fbsrc = nullptr, *d_tp_bbsrc = nullptr;   // per lane
     float4 *d_tp_fblk = nullptr, *d_tp_bblk = nullptr;   // 9 per lane
-    float4 *d_tp_finp = nullptr, *d_tp_binp = nullptr;   // 3 per lane
-    float4 *d_tp_fres = nullptr, *d_tp_bres = nullptr;   // 3 per lane
-    int *d_tp_fgtid = nullptr, *d_tp_bgtid = nullptr;    // N: owner lane of each permuted particle
+    float4 *d_tp_finp = nullptr, *d_tp_binp = nullptr;   // 3 N: per-row inputs
+    float4 *d_tp_fres = nullptr, *d_tp_bres = nullptr;   // 3 N: per-row results
+    int *d_tp_fseq = nullptr, *d_tp_bseq = nullptr;      // N: sequence number of each permuted particle
     float *d_tp_Z = nullptr;                             // 36 N: inverse diagonal blocks
 
     // saddle / GMRES

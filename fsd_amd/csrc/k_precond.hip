@@ -320,14 +320,15 @@ __global__ void __launch_bounds__(64) trsv_bwd_level_kernel(const int *items, in
     }
 }
 
-// ------------------------------------- single-workgroup sweeps (one launch)
-// Both sweeps of one application run in one workgroup of 1024 threads, so
-// no workgroup ever waits on another.  The diagonal blocks are folded out of
+// ------------------------------------- single-workgroup sweeps
+// Each sweep of one application runs in one workgroup of 1024 threads, so no
+// workgroup ever waits on another.  The diagonal blocks are folded out of
 // the recurrences (Z_P = L_PP^-1, lower triangular):
 //   forward   y_P = Z_P b_P - sum_Q (Z_P L_PQ) y_Q
 //   backward  u_P = y_P - sum_Q (Z_Q L_QP)^T u_Q,   x_P = Z_P^T u_P
 // so a step is one 6x6 block product per lane, a sum over the row's lanes
-// and a subtraction; Z b and Z^T u are separate all-parallel kernels.
+// and a subtraction; Z b, the reordering of y between the sweeps and Z^T u
+// are separate all-parallel kernels.
 //
 // The schedule (sweep_plan.h) gives each lane one dependency block per step.
 // Results of the last steps' rows are read from an LDS ring, older ones from
@@ -338,11 +339,11 @@ __global__ void __launch_bounds__(64) trsv_bwd_level_kernel(const int *items, in
 // so stores stay in flight across steps.
 struct TaskDev {
     int nsteps;
-    const int4 *info;       // per lane: src, slot, nd, dest
+    int nrows;              // rows (= particles): stride of the per-row arrays
+    const int2 *info;       // per lane: src, own
     const float4 *blk;      // [step][9][1024]: the lane's 6x6 block, row-major
-    const float4 *inp;      // [step][3][1024]: the owner's input, right-hand sides 0 and 1 (6 floats each)
-    float4 *res;            // [step][3][1024]: the owner's result
-    float4 *copy;           // forward: the backward sweep's inp, at the row's backward lane; backward: null
+    const float4 *inp;      // [3][nrows]: the rows' inputs, right-hand sides 0 and 1 (6 floats each)
+    float4 *res;            // [3][nrows]: the rows' results
 };
 
 __device__ __forceinline__ uint32_t tp_idx(uint32_t g, uint32_t i, uint32_t n)
@@ -353,9 +354,11 @@ __device__ __forceinline__ uint32_t tp_idx(uint32_t g, uint32_t i, uint32_t n)
 // operands of one step for this lane (values of 2 right-hand sides as 3 float4:
 // (a0 a1 a2 a3) (a4 a5 b0 b1) (b2 b3 b4 b5))
 template <int NRHS>
-__device__ __forceinline__ void tp_fetch(const TaskDev &S, uint32_t s, uint32_t t, int4 inf, float *m, float4 *gx,
+__device__ __forceinline__ void tp_fetch(const TaskDev &S, uint32_t s, uint32_t t, int2 inf, float *m, float4 *gx,
                                          float4 *in)
 {
+    constexpr int NW = NRHS == 2 ? 3 : 2;
+    const uint32_t nr = (uint32_t)S.nrows;
     if (inf.x >= 0) {
         const float4 *b = S.blk + (size_t)s * 9 * TP_LANES + t;
 #pragma unroll
@@ -364,14 +367,15 @@ __device__ __forceinline__ void tp_fetch(const TaskDev &S, uint32_t s, uint32_t 
             m[4 * i] = v.x; m[4 * i + 1] = v.y; m[4 * i + 2] = v.z; m[4 * i + 3] = v.w;
         }
         if (inf.x & TP_GLOBAL) {
-            const uint32_t h = (uint32_t)(inf.x & ~TP_GLOBAL);
+            const uint32_t k = (uint32_t)(inf.x & ~TP_GLOBAL);
 #pragma unroll
-            for (int i = 0; i < (NRHS == 2 ? 3 : 2); ++i) gx[i] = S.res[tp_idx(h, i, 3)];
+            for (int i = 0; i < NW; ++i) gx[i] = S.res[i * nr + k];
         }
     }
     if (inf.y >= 0) {
+        const uint32_t k = (uint32_t)(inf.y & ((1 << TP_ND_SHIFT) - 1));
 #pragma unroll
-        for (int i = 0; i < (NRHS == 2 ? 3 : 2); ++i) in[i] = S.inp[tp_idx(s * TP_LANES + t, i, 3)];
+        for (int i = 0; i < NW; ++i) in[i] = S.inp[i * nr + k];
     }
 }
 
@@ -390,23 +394,28 @@ __device__ __forceinline__ void tp_pack(const float *a, const float *b, float4 *
     if (NRHS == 2) w[2] = make_float4(b[2], b[3], b[4], b[5]);
 }
 
+// one sweep in one workgroup of 1024 threads; dynamic LDS: ring [3][TP_RING]
+// and the owners' partial sums [3][1024] (float4)
 template <int NRHS>
-__device__ __forceinline__ void tp_sweep(const TaskDev &S, float4 *ring, float4 *red, int R)
+__global__ void __launch_bounds__(TP_LANES) tp_sweep_kernel(TaskDev S)
 {
     constexpr int NW = NRHS == 2 ? 3 : 2;   // float4 per row value
+    extern __shared__ float4 tp_lds[];
+    float4 *ring = tp_lds;
+    float4 *red = tp_lds + 3 * TP_RING;
     const uint32_t t = threadIdx.x;
+    const uint32_t nr = (uint32_t)S.nrows;
     if (S.nsteps <= 0) return;
     // positions are fetched two steps ahead; past the last step the last one is re-read (and not used)
     const int last = S.nsteps - 1;
-    int4 cur = S.info[t];
-    int4 nxt = S.info[(size_t)min(1, last) * TP_LANES + t];
+    int2 cur = S.info[t];
+    int2 nxt = S.info[(size_t)min(1, last) * TP_LANES + t];
     float m[36];
     float4 gx[3], in[3];
     tp_fetch<NRHS>(S, 0, t, cur, m, gx, in);
-    // the previous step's result of this lane (owner), stored during the next step
+    // this lane's result of the previous step (owner), stored during the next step
     float4 pw[3];
-    bool pown = false;
-    uint32_t pg = 0, pdest = 0;
+    int pk = -1;
     for (int s = 0; s < S.nsteps; ++s) {
         // everything in flight is needed now: this step's operands, the next step's
         // positions and (issued before them) the stores of two steps back
@@ -422,7 +431,7 @@ __device__ __forceinline__ void tp_sweep(const TaskDev &S, float4 *ring, float4 
                 for (int i = 0; i < NW; ++i) xv[i] = gx[i];
             } else {
 #pragma unroll
-                for (int i = 0; i < NW; ++i) xv[i] = ring[i * R + cur.x];
+                for (int i = 0; i < NW; ++i) xv[i] = ring[i * TP_RING + cur.x];
             }
             float xa[6], xb[6];
             tp_unpack<NRHS>(xv, xa, xb);
@@ -441,19 +450,16 @@ __device__ __forceinline__ void tp_sweep(const TaskDev &S, float4 *ring, float4 
 #pragma unroll
             for (int a = 0; a < 6; ++a) { wa[a] -= pa[a]; if (NRHS == 2) wb[a] -= pb[a]; }
         }
-        const int slot = cur.y, nd = cur.z, dest = cur.w;
+        const int k = cur.y & ((1 << TP_ND_SHIFT) - 1), nd = cur.y >> TP_ND_SHIFT;
         // global stores of the previous step, then the operands of step s + 1 (positions
         // fetched one step earlier) and the positions of step s + 2: waiting for the
         // operands never waits for a store issued after them
-        if (pown) {
+        if (pk >= 0) {
 #pragma unroll
-            for (int i = 0; i < NW; ++i) {
-                S.res[tp_idx(pg, i, 3)] = pw[i];
-                if (S.copy) S.copy[tp_idx(pdest, i, 3)] = pw[i];
-            }
+            for (int i = 0; i < NW; ++i) S.res[i * nr + (uint32_t)pk] = pw[i];
         }
         if (s + 1 < S.nsteps) tp_fetch<NRHS>(S, (uint32_t)s + 1, t, nxt, m, gx, in);
-        const int4 n2 = S.info[(size_t)min(s + 2, last) * TP_LANES + t];
+        const int2 n2 = S.info[(size_t)min(s + 2, last) * TP_LANES + t];
         // the row's other lanes hand their products to its owner (same wave)
         if (has && !own) {
             float4 w[3];
@@ -465,44 +471,38 @@ __device__ __forceinline__ void tp_sweep(const TaskDev &S, float4 *ring, float4 
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
         if (own) {
-            for (int d = 1; d < nd; ++d) {
-                float4 w[3];
+            // partial sums of the row's other lanes, four lanes per batch of loads
+            for (int d0 = 1; d0 < nd; d0 += 4) {
+                float4 w[4][3];
 #pragma unroll
-                for (int i = 0; i < NW; ++i) w[i] = red[i * TP_LANES + t + d];
-                float qa[6], qb[6];
-                tp_unpack<NRHS>(w, qa, qb);
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t lu = t + (uint32_t)min(d0 + u, nd - 1);   // in-row lane (repeats are masked below)
 #pragma unroll
-                for (int a = 0; a < 6; ++a) { wa[a] -= qa[a]; if (NRHS == 2) wb[a] -= qb[a]; }
+                    for (int i = 0; i < NW; ++i) w[u][i] = red[i * TP_LANES + lu];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (d0 + u < nd) {
+                        float qa[6], qb[6];
+                        tp_unpack<NRHS>(w[u], qa, qb);
+#pragma unroll
+                        for (int a = 0; a < 6; ++a) { wa[a] -= qa[a]; if (NRHS == 2) wb[a] -= qb[a]; }
+                    }
+                }
             }
             tp_pack<NRHS>(wa, wb, pw);
 #pragma unroll
-            for (int i = 0; i < NW; ++i) ring[i * R + slot] = pw[i];
+            for (int i = 0; i < NW; ++i) ring[i * TP_RING + (k & (TP_RING - 1))] = pw[i];
         }
-        pown = own;
-        pg = (uint32_t)s * TP_LANES + t;
-        pdest = (uint32_t)dest;
+        pk = own ? k : -1;
         __syncthreads();
         cur = nxt;
         nxt = n2;
     }
-    if (pown) {
+    if (pk >= 0) {
 #pragma unroll
-        for (int i = 0; i < NW; ++i) {
-            S.res[tp_idx(pg, i, 3)] = pw[i];
-            if (S.copy) S.copy[tp_idx(pdest, i, 3)] = pw[i];
-        }
+        for (int i = 0; i < NW; ++i) S.res[i * nr + (uint32_t)pk] = pw[i];
     }
-}
-
-template <int NRHS>
-__global__ void __launch_bounds__(TP_LANES) tp_sweeps_kernel(TaskDev F, TaskDev B, int R)
-{
-    extern __shared__ float4 tp_lds[];
-    float4 *ring = tp_lds;                   // [3][R]
-    float4 *red = tp_lds + 3 * R;            // [3][1024]
-    tp_sweep<NRHS>(F, ring, red, R);
-    __syncthreads();                         // the forward results (backward inputs) are in global memory
-    tp_sweep<NRHS>(B, ring, red, R);
 }
 
 // Z_P = L_PP^-1 of every lower-triangular diagonal block (row-major)
@@ -545,8 +545,8 @@ __global__ void tp_pack_kernel(int64_t lanes, const int2 *bsrc, const float *Lbl
         blk[tp_idx((uint32_t)g, i, 9)] = make_float4(M[4 * i], M[4 * i + 1], M[4 * i + 2], M[4 * i + 3]);
 }
 
-// forward inputs: inp[lane of P] = Z_P b_r[perm P]
-__global__ void tp_pre_kernel(int N, const int *gtid, const int *perm, const float *Z, const float *b0,
+// forward inputs: inp[k_f(P)] = Z_P b_r[perm P]
+__global__ void tp_pre_kernel(int N, const int *seq, const int *perm, const float *Z, const float *b0,
                               const float *b1, int nrhs, float4 *inp)
 {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
@@ -563,24 +563,33 @@ __global__ void tp_pre_kernel(int N, const int *gtid, const int *perm, const flo
         a[r] = sa;
         b[r] = sb;
     }
-    const uint32_t g = (uint32_t)gtid[P];
-    inp[tp_idx(g, 0, 3)] = make_float4(a[0], a[1], a[2], a[3]);
-    inp[tp_idx(g, 1, 3)] = make_float4(a[4], a[5], b[0], b[1]);
-    inp[tp_idx(g, 2, 3)] = make_float4(b[2], b[3], b[4], b[5]);
+    const int k = seq[P];
+    inp[k] = make_float4(a[0], a[1], a[2], a[3]);
+    inp[N + k] = make_float4(a[4], a[5], b[0], b[1]);
+    inp[2 * N + k] = make_float4(b[2], b[3], b[4], b[5]);
 }
 
-// outputs: y_r[6 perm P + a] = (Z_P^T u_P)_a
-__global__ void tp_post_kernel(int N, const int *gtid, const int *perm, const float *Z, const float4 *res,
-                               int nrhs, float *y0, float *y1)
+// backward inputs from forward results: inp_b[k_b(P)] = res_f[k_f(P)]
+__global__ void tp_mid_kernel(int N, const int *seq_f, const int *seq_b, const float4 *res_f, float4 *inp_b)
 {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= N) return;
-    const uint32_t g = (uint32_t)gtid[P];
-    const float4 w0 = res[tp_idx(g, 0, 3)], w1 = res[tp_idx(g, 1, 3)];
+    const int kf = seq_f[P], kb = seq_b[P];
+    for (int i = 0; i < 3; ++i) inp_b[i * N + kb] = res_f[i * N + kf];
+}
+
+// outputs: y_r[6 perm P + a] = (Z_P^T u_P)_a with u_P = res_b[k_b(P)]
+__global__ void tp_post_kernel(int N, const int *seq, const int *perm, const float *Z, const float4 *res, int nrhs,
+                               float *y0, float *y1)
+{
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= N) return;
+    const int k = seq[P];
+    const float4 w0 = res[k], w1 = res[N + k];
     const float ua[6] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y};
     float ub[6] = {w1.z, w1.w, 0.f, 0.f, 0.f, 0.f};
     if (nrhs == 2) {
-        const float4 w2 = res[tp_idx(g, 2, 3)];
+        const float4 w2 = res[2 * N + k];
         ub[2] = w2.x; ub[3] = w2.y; ub[4] = w2.z; ub[5] = w2.w;
     }
     const float *Zp = Z + (int64_t)P * 36;
@@ -596,21 +605,20 @@ __global__ void tp_post_kernel(int N, const int *gtid, const int *perm, const fl
     }
 }
 
-// ring capacity of the sweep kernel on this device (tp_R = 0: level by level)
+// whether the sweep kernel's LDS fits on this device (tp_R = 0: level by level)
 int k_sweep_setup(fsd_ctx *ctx)
 {
     int dev = 0, lds = 0;
     ctx->tp_R = 0;
     FSD_HIP(hipGetDevice(&dev));
     FSD_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    const int R = (lds / (int)sizeof(float4) - 3 * TP_LANES) / 3;
-    if (R < 3 * TP_ROWS_MAX) return 0;
-    const int bytes = (int)sizeof(float4) * (3 * R + 3 * TP_LANES);
-    FSD_HIP(hipFuncSetAttribute((const void *)tp_sweeps_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    const int bytes = (int)sizeof(float4) * 3 * (TP_RING + TP_LANES);
+    if (lds < bytes) return 0;
+    FSD_HIP(hipFuncSetAttribute((const void *)tp_sweep_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 bytes));
-    FSD_HIP(hipFuncSetAttribute((const void *)tp_sweeps_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    FSD_HIP(hipFuncSetAttribute((const void *)tp_sweep_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 bytes));
-    ctx->tp_R = R;
+    ctx->tp_R = TP_RING;
     return 0;
 }
 
@@ -638,17 +646,18 @@ int k_precond_apply2(fsd_ctx *ctx, const float *b0, const float *b1, float *y0, 
     hipEvent_t ev;
     prof_begin(ctx, PROF_TRSV, &ev);
     if (ctx->tp_ok) {
-        hipLaunchKernelGGL(tp_pre_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_fgtid,
+        const size_t lds = sizeof(float4) * 3 * (TP_RING + TP_LANES);
+        TaskDev F{ctx->tp_f.nsteps, N, ctx->d_tp_finfo, ctx->d_tp_fblk, ctx->d_tp_finp, ctx->d_tp_fres};
+        TaskDev B{ctx->tp_b.nsteps, N, ctx->d_tp_binfo, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres};
+        hipLaunchKernelGGL(tp_pre_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_fseq,
                            ctx->d_perm, ctx->d_tp_Z, b0, b1, n_rhs, ctx->d_tp_finp);
-        TaskDev F{ctx->tp_f.nsteps, ctx->d_tp_finfo, ctx->d_tp_fblk, ctx->d_tp_finp, ctx->d_tp_fres,
-                  ctx->d_tp_binp};
-        TaskDev B{ctx->tp_b.nsteps, ctx->d_tp_binfo, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres, nullptr};
-        const size_t lds = sizeof(float4) * (3 * (size_t)ctx->tp_R + 3 * TP_LANES);
-        if (n_rhs == 2)
-            hipLaunchKernelGGL(tp_sweeps_kernel<2>, dim3(1), dim3(TP_LANES), lds, ctx->stream, F, B, ctx->tp_R);
-        else
-            hipLaunchKernelGGL(tp_sweeps_kernel<1>, dim3(1), dim3(TP_LANES), lds, ctx->stream, F, B, ctx->tp_R);
-        hipLaunchKernelGGL(tp_post_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_bgtid,
+        if (n_rhs == 2) hipLaunchKernelGGL(tp_sweep_kernel<2>, dim3(1), dim3(TP_LANES), lds, ctx->stream, F);
+        else hipLaunchKernelGGL(tp_sweep_kernel<1>, dim3(1), dim3(TP_LANES), lds, ctx->stream, F);
+        hipLaunchKernelGGL(tp_mid_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->d_tp_fseq,
+                           ctx->d_tp_bseq, ctx->d_tp_fres, ctx->d_tp_binp);
+        if (n_rhs == 2) hipLaunchKernelGGL(tp_sweep_kernel<2>, dim3(1), dim3(TP_LANES), lds, ctx->stream, B);
+        else hipLaunchKernelGGL(tp_sweep_kernel<1>, dim3(1), dim3(TP_LANES), lds, ctx->stream, B);
+        hipLaunchKernelGGL(tp_post_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_bseq,
                            ctx->d_perm, ctx->d_tp_Z, ctx->d_tp_bres, n_rhs, y0, y1);
         FSD_HIP(hipGetLastError());
         prof_end(ctx, PROF_TRSV, ev);

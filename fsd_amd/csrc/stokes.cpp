@@ -256,8 +256,10 @@ static int alloc_all(fsd_ctx *ctx)
     CK(hipMalloc(&ctx->d_lev_f, sizeof(int) * N));
     CK(hipMalloc(&ctx->d_lev_b, sizeof(int) * N));
     CK(hipMalloc(&ctx->d_tp_Z, sizeof(float) * 36 * (size_t)N));
-    CK(hipMalloc(&ctx->d_tp_fgtid, sizeof(int) * (size_t)N));
-    CK(hipMalloc(&ctx->d_tp_bgtid, sizeof(int) * (size_t)N));
+    CK(hipMalloc(&ctx->d_tp_fseq, sizeof(int) * (size_t)N));
+    CK(hipMalloc(&ctx->d_tp_bseq, sizeof(int) * (size_t)N));
+    for (float4 **pp : {&ctx->d_tp_finp, &ctx->d_tp_binp, &ctx->d_tp_fres, &ctx->d_tp_bres})
+        CK(hipMalloc(pp, sizeof(float4) * 3 * (size_t)N));
     CK(hipMalloc(&ctx->d_tmp6a, sizeof(float) * 12 * (size_t)N));
     CK(hipMalloc(&ctx->d_tmp6b, sizeof(float) * 12 * (size_t)N));
     CK(hipMalloc(&ctx->d_g12, sizeof(float) * 12 * (size_t)N));
@@ -300,7 +302,7 @@ static void free_all(fsd_ctx *ctx)
                     ctx->d_tmp6b, ctx->d_g12, ctx->d_V, ctx->d_w, ctx->d_t, ctx->d_t2, ctx->d_part, ctx->d_rhs, ctx->d_sol,
                     ctx->d_force, ctx->d_mask1, ctx->d_mask2, ctx->d_tp_finfo, ctx->d_tp_binfo,
                     ctx->d_tp_fbsrc, ctx->d_tp_bbsrc, ctx->d_tp_fblk, ctx->d_tp_bblk,
-                    ctx->d_tp_finp, ctx->d_tp_binp, ctx->d_tp_fres, ctx->d_tp_bres, ctx->d_tp_fgtid, ctx->d_tp_bgtid,
+                    ctx->d_tp_finp, ctx->d_tp_binp, ctx->d_tp_fres, ctx->d_tp_bres, ctx->d_tp_fseq, ctx->d_tp_bseq,
                     ctx->d_tp_Z,
                     ctx->d_Z, ctx->d_lanV, ctx->d_lanw, ctx->d_lany0, ctx->d_lan_h, ctx->d_noise, ctx->d_posp,
                     ctx->d_rfd_rhs, ctx->d_rfd_sol};
@@ -335,23 +337,20 @@ static int grow_lower(fsd_ctx *ctx, int64_t need)
 static int grow_tasks(fsd_ctx *ctx)
 {
     const int64_t lf = (int64_t)ctx->tp_f.nsteps * TP_LANES, lb = (int64_t)ctx->tp_b.nsteps * TP_LANES;
-    auto grow = [ctx](int64_t need, int64_t &cap, int4 *&info, int2 *&bsrc, float4 *&blk, float4 *&inp,
-                      float4 *&res) -> int {
+    auto grow = [ctx](int64_t need, int64_t &cap, int2 *&info, int2 *&bsrc, float4 *&blk) -> int {
         if (need <= cap) return 0;
         const int64_t c = need + need / 4 + TP_LANES;
         CK(hipStreamSynchronize(ctx->stream));
-        for (void *q : {(void *)info, (void *)bsrc, (void *)blk, (void *)inp, (void *)res})
+        for (void *q : {(void *)info, (void *)bsrc, (void *)blk})
             if (q) hipFree(q);
-        CK(hipMalloc(&info, sizeof(int4) * c));
+        CK(hipMalloc(&info, sizeof(int2) * c));
         CK(hipMalloc(&bsrc, sizeof(int2) * c));
         CK(hipMalloc(&blk, sizeof(float4) * 9 * c));
-        CK(hipMalloc(&inp, sizeof(float4) * 3 * c));
-        CK(hipMalloc(&res, sizeof(float4) * 3 * c));
         cap = c;
         return 0;
     };
-    RC(grow(lf, ctx->tp_cap_f, ctx->d_tp_finfo, ctx->d_tp_fbsrc, ctx->d_tp_fblk, ctx->d_tp_finp, ctx->d_tp_fres));
-    RC(grow(lb, ctx->tp_cap_b, ctx->d_tp_binfo, ctx->d_tp_bbsrc, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres));
+    RC(grow(lf, ctx->tp_cap_f, ctx->d_tp_finfo, ctx->d_tp_fbsrc, ctx->d_tp_fblk));
+    RC(grow(lb, ctx->tp_cap_b, ctx->d_tp_binfo, ctx->d_tp_bbsrc, ctx->d_tp_bblk));
     return 0;
 }
 
@@ -459,7 +458,6 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
     }
     RC(grow_lower(ctx, ctx->nlow + 1));
     if (ctx->tp_ok) {
-        link_task_plans(N, ctx->tp_f, ctx->tp_b);
         RC(grow_tasks(ctx));
         auto up = [ctx](void *d, const void *h, size_t bytes) {
             return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream);
@@ -469,8 +467,8 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
         CK(up(ctx->d_tp_binfo, b.info.data(), sizeof(int) * b.info.size()));
         CK(up(ctx->d_tp_fbsrc, f.bsrc.data(), sizeof(int) * f.bsrc.size()));
         CK(up(ctx->d_tp_bbsrc, b.bsrc.data(), sizeof(int) * b.bsrc.size()));
-        CK(up(ctx->d_tp_fgtid, f.gtid.data(), sizeof(int) * (size_t)N));
-        CK(up(ctx->d_tp_bgtid, b.gtid.data(), sizeof(int) * (size_t)N));
+        CK(up(ctx->d_tp_fseq, f.seq.data(), sizeof(int) * (size_t)N));
+        CK(up(ctx->d_tp_bseq, b.seq.data(), sizeof(int) * (size_t)N));
         CK(hipStreamSynchronize(ctx->stream));
     }
     CK(hipMemcpyAsync(ctx->d_perm, ctx->h_perm.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));

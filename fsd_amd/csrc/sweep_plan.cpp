@@ -5,8 +5,9 @@
 // the current step, so busy waves are the low-numbered ones and spread over
 // the four SIMDs.  A level that does not fit in 1024 lanes or TP_ROWS_MAX rows
 // continues in the next step (its rows are independent, so the extra barrier
-// is harmless).
-
+// is harmless).  Sequence numbers follow the packing order, so a wave's
+// owners hold nearly consecutive numbers and their per-row loads and stores
+// coalesce.
 #include "sweep_plan.h"
 
 #include <algorithm>
@@ -15,7 +16,7 @@ bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, const i
                      bool forward, int R, TaskPlan &tp)
 {
     tp = TaskPlan{};
-    if (R < 3 * TP_ROWS_MAX) return false;
+    if (R < 3 * TP_ROWS_MAX || (R & (R - 1)) != 0 || N >= (1 << TP_ND_SHIFT)) return false;
     auto cnt = [&](int P) { return rowptr[P + 1] - rowptr[P]; };
     for (int P = 0; P < N; ++P)
         if (cnt(P) > TP_WAVE) return false;
@@ -32,8 +33,8 @@ bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, const i
 
     // packing: lanes, sequence numbers and steps
     constexpr int NWAVE = TP_LANES / TP_WAVE;
-    tp.gtid.assign(N, -1);
-    std::vector<int> seq(N, 0), rowstep(N, 0), send;
+    std::vector<int> gtid(N, -1), rowstep(N, 0), send;
+    tp.seq.assign(N, 0);
     int nseq = 0, step = -1, rows = 0;
     int fill[NWAVE];
     auto open_step = [&]() {
@@ -44,54 +45,55 @@ bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, const i
     for (int l = 0; l < nlev; ++l) {
         if (lptr[l + 1] == lptr[l]) continue;
         open_step();
+        std::vector<std::vector<int>> wrows(NWAVE);
+        auto flush = [&]() {
+            // sequence numbers wave by wave, in lane order inside a wave
+            for (int w = 0; w < NWAVE; ++w) {
+                for (int P : wrows[w]) tp.seq[P] = nseq++;
+                wrows[w].clear();
+            }
+            send.push_back(nseq);
+        };
         for (int k = lptr[l]; k < lptr[l + 1]; ++k) {
             const int P = order[k];
             const int need = std::max(cnt(P), 1);
             int w = 0;
             while (w < NWAVE && fill[w] + need > TP_WAVE) ++w;
             if (w == NWAVE || rows == TP_ROWS_MAX) {
-                send.push_back(nseq);
+                flush();
                 open_step();
                 w = 0;
             }
-            tp.gtid[P] = step * TP_LANES + w * TP_WAVE + fill[w];
+            gtid[P] = step * TP_LANES + w * TP_WAVE + fill[w];
+            wrows[w].push_back(P);
             fill[w] += need;
             ++rows;
             rowstep[P] = step;
-            seq[P] = nseq++;
         }
-        send.push_back(nseq);
+        flush();
     }
     tp.nsteps = step + 1;
 
-    auto in_ring = [&](int q, int S) { return q >= send[S] - R; };
-
     // lanes
+    auto in_ring = [&](int q, int S) { return q >= send[S] - R; };
     const size_t L = (size_t)tp.nsteps * TP_LANES;
-    tp.info.assign(4 * L, -1);
-    for (size_t g = 0; g < L; ++g) tp.info[4 * g + 2] = 0;
+    tp.info.assign(2 * L, -1);
     tp.bsrc.assign(2 * L, -1);
     for (int P = 0; P < N; ++P) {
-        const int g = tp.gtid[P], S = rowstep[P], n = cnt(P);
-        tp.info[4 * (size_t)g + 1] = seq[P] % R;
-        tp.info[4 * (size_t)g + 2] = n;
+        const int g = gtid[P], S = rowstep[P], n = cnt(P);
+        tp.info[2 * (size_t)g + 1] = tp.seq[P] | (n << TP_ND_SHIFT);
         for (int j = 0; j < n; ++j) {
             const int e = rowptr[P] + j, Q = cols[e];
             const size_t h = (size_t)g + j;
-            if (in_ring(seq[Q], S)) {
-                tp.info[4 * h] = seq[Q] % R;
+            if (in_ring(tp.seq[Q], S)) {
+                tp.info[2 * h] = tp.seq[Q] & (R - 1);
             } else {
                 if (rowstep[Q] > S - 3) return false;   // unreachable: three steps fit in the ring
-                tp.info[4 * h] = TP_GLOBAL | tp.gtid[Q];
+                tp.info[2 * h] = TP_GLOBAL | tp.seq[Q];
             }
             tp.bsrc[2 * h] = src ? src[e] : e;
             tp.bsrc[2 * h + 1] = forward ? P : Q;
         }
     }
     return true;
-}
-
-void link_task_plans(int N, TaskPlan &fwd, const TaskPlan &bwd)
-{
-    for (int P = 0; P < N; ++P) fwd.info[4 * (size_t)fwd.gtid[P] + 3] = bwd.gtid[P];
 }

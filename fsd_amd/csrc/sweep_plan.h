@@ -7,52 +7,50 @@
 // Lanes of the sweep workgroup: 16 waves of 64.
 constexpr int TP_LANES = 1024;
 constexpr int TP_WAVE = 64;
+// LDS ring of recent results (rows; a power of two) and rows per step, so
+// that any three consecutive steps fit in the ring
+constexpr int TP_RING = 2048;
+constexpr int TP_ROWS_MAX = 640;
 // src field of a lane: dependency read from the global result array (the
-// rest of the word is the producer's lane id) instead of the LDS ring
+// rest of the word is the producer's sequence number) instead of the ring
 constexpr int TP_GLOBAL = 0x40000000;
-// rows per step, so that any three consecutive steps fit in the ring
-constexpr int TP_ROWS_MAX = 768;
+constexpr int TP_ND_SHIFT = 24;   // own field: sequence number | lanes << TP_ND_SHIFT
 
 // One sweep, cut into steps.  A step holds rows of a single level of the
 // sweep's DAG, packed into the 1024 lanes: a row with nd dependencies takes
 // nd consecutive lanes of one wave (one dependency block each, the row's
 // first lane owns the row), a row without dependencies takes one lane.
-// Lane g = step * 1024 + threadIdx.x ("gtid").  Per lane:
-//   info[4g + 0] src   -1: no block; r < TP_GLOBAL: ring slot r of the
-//                      dependency; TP_GLOBAL | h: lane h holds it in the
-//                      global result array
-//   info[4g + 1] slot  owner: ring slot of the row, else -1
-//   info[4g + 2] nd    owner: lanes of the row (0 when it has no block)
-//   info[4g + 3] dest  owner: lane of the row in the other sweep (the
-//                      forward sweep writes its result there as the
-//                      backward sweep's input), else -1
+// Rows are numbered in step order (the sequence number k); every per-row
+// value (input, result) lives at index k.  Lane g = step * 1024 + threadIdx.x.
+//   info[2g]     src   -1: no block; r < TP_GLOBAL: the dependency's result
+//                      is in ring slot r; TP_GLOBAL | k: it is result k in
+//                      global memory
+//   info[2g + 1] own   owner: k | nd << TP_ND_SHIFT (nd = 0 for a row
+//                      without block), else -1
 //   bsrc[2g], bsrc[2g + 1]: factor block index and the particle whose
 //                      diagonal block scales it (-1 when no block)
-// A row's result enters the LDS ring at slot (its sequence number mod R)
-// during its step, and global memory during the next step (its store is
-// issued just before the next step's operands are fetched); every step ends
-// with a workgroup barrier.  The operands of step S are fetched during step
-// S - 1, after the barrier that ends step S - 2, which orders the global
-// stores of results of steps up to S - 3.  So a dependency of step S comes
-// from the ring when its producer is in steps S - 2 or S - 1 (three steps of
-// at most TP_ROWS_MAX rows fit in the ring) or is otherwise still there, and
-// from global memory when it is older.
+// A row's result enters the LDS ring at slot (k mod TP_RING) during its
+// step, and global memory during the next step (its store is issued just
+// before the next step's operands are fetched); every step ends with a
+// workgroup barrier.  The operands of step S are fetched during step S - 1,
+// after the barrier that ends step S - 2, which orders the global stores of
+// results of steps up to S - 3.  So a dependency of step S comes from the
+// ring when its producer is in steps S - 2 or S - 1 (three steps of at most
+// TP_ROWS_MAX rows fit in the ring) or is otherwise still there, and from
+// global memory when it is older.
 struct TaskPlan {
     int nsteps = 0;
-    std::vector<int> info;      // 4 per lane
+    std::vector<int> info;      // 2 per lane
     std::vector<int> bsrc;      // 2 per lane
-    std::vector<int> gtid;      // permuted particle -> owner lane
+    std::vector<int> seq;       // permuted particle -> sequence number
 };
 
 // lev[P] in [0, nlev): level of permuted particle P in the sweep's DAG;
 // rowptr/cols: the sweep's dependencies of each P (cols[e] = particle that
 // row P depends on); src[e]: factor block of dependency e (nullptr: e);
 // forward: the block is scaled by the consumer's diagonal block (forward
-// sweep) or by the producer's (backward sweep).  R: ring capacity in rows
-// (at least 3 * TP_ROWS_MAX).  Returns false when a row has more than TP_WAVE
-// dependencies or R is too small.
+// sweep) or by the producer's (backward sweep).  R: ring capacity in rows (a
+// power of two, at least 3 * TP_ROWS_MAX; the kernel uses TP_RING).  Returns
+// false when a row has more than TP_WAVE dependencies or R is unusable.
 bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, const int *cols, const int *src,
                      bool forward, int R, TaskPlan &tp);
-
-// dest fields of the forward plan from the backward plan's owner lanes
-void link_task_plans(int N, TaskPlan &fwd, const TaskPlan &bwd);

@@ -1,4 +1,4 @@
-// CPU emulation of tp_sweeps_kernel and its pre/post/pack kernels
+// CPU emulation of tp_sweep_kernel and its pre/mid/post/pack kernels
 // (fsd_amd/csrc/k_precond.hip) on the schedules of sweep_plan.cpp.
 //
 // The kernel's data flow is replayed step by step with a model of when each
@@ -11,7 +11,7 @@
 // producer's lane, stored before the barrier the fetch follows; every
 // reduction stays inside one wave and reads lanes of the same row; the
 // result equals a direct block solve.
-// usage: sweep_emulation N band max_lower_degree [isolated_percent] [R]
+// usage: sweep_emulation N band max_lower_degree [isolated_percent] [R (power of two)]
 #include "../../fsd_amd/csrc/sweep_plan.h"
 #include <algorithm>
 #include <cmath>
@@ -39,7 +39,7 @@ int main(int argc, char **argv)
 {
     const int N = argc > 1 ? atoi(argv[1]) : 3000, band = argc > 2 ? atoi(argv[2]) : 40;
     const int maxdeg = argc > 3 ? atoi(argv[3]) : 5, iso = argc > 4 ? atoi(argv[4]) : 0;
-    const int R = argc > 5 ? atoi(argv[5]) : (163840 / 16 - 3 * TP_LANES) / 3;
+    const int R = argc > 5 ? atoi(argv[5]) : TP_RING;
     std::mt19937 rng(5);
     std::vector<std::set<int>> adj(N);
     for (int P = 0; P < N; ++P) {
@@ -75,7 +75,6 @@ int main(int argc, char **argv)
         printf("plan failed\n");
         return 1;
     }
-    link_task_plans(N, tf, tb);
 
     // random factor (well conditioned), permutation and right-hand sides
     std::uniform_real_distribution<double> U(-0.3, 0.3);
@@ -126,23 +125,40 @@ int main(int argc, char **argv)
     struct Dev {
         const TaskPlan *tp;
         uint64_t lanes;
-        std::vector<double> blk, inp, res;   // float4 slots x 4
+        std::vector<double> blk;        // float4 slots x 4, [step][9][1024]
+        std::vector<double> inp, res;   // [3][N] float4 -> 12 doubles per row, slot i at 4 (i N + k)
         std::vector<Lane> lane;
     } D[2];
     const int *rowp[2] = {lrow.data(), urow.data()}, *colp[2] = {lcol.data(), ucol.data()};
+    const int KM = (1 << TP_ND_SHIFT) - 1;
     for (int sw = 0; sw < 2; ++sw) {
         Dev &d = D[sw];
         d.tp = sw ? &tb : &tf;
         d.lanes = (uint64_t)d.tp->nsteps * TP_LANES;
         d.blk.assign(d.lanes * 9 * 4, NAN);
-        d.inp.assign(d.lanes * 3 * 4, NAN);
-        d.res.assign(d.lanes * 3 * 4, NAN);
+        d.inp.assign(12 * (size_t)N, NAN);
+        d.res.assign(12 * (size_t)N, NAN);
         d.lane.assign(d.lanes, Lane{});
-        for (int P = 0; P < N; ++P) {
-            const int g = d.tp->gtid[P], n = rowp[sw][P + 1] - rowp[sw][P];
-            CHECK(g >= 0 && (uint64_t)g + std::max(n, 1) <= d.lanes, "owner lane out of range");
-            d.lane[g] = Lane{P, n ? 0 : -1};
-            for (int j = 1; j < n; ++j) d.lane[g + j] = Lane{P, j};
+        std::vector<int> seen(N, 0), inv(N, -1);
+        for (int Q = 0; Q < N; ++Q) {
+            const int k = d.tp->seq[Q];
+            CHECK(k >= 0 && k < N && inv[k] < 0, "sequence numbers are not a permutation");
+            if (k >= 0 && k < N) inv[k] = Q;
+        }
+        for (uint64_t g = 0; g < d.lanes; ++g) {
+            const int own = d.tp->info[2 * g + 1];
+            if (own < 0) continue;
+            const int k = own & KM, nd = own >> TP_ND_SHIFT;
+            CHECK(k >= 0 && k < N, "sequence number out of range");
+            if (k < 0 || k >= N) continue;
+            const int P = inv[k];
+            CHECK(P >= 0 && !seen[P], "row owned twice");
+            if (P < 0) continue;
+            seen[P] = 1;
+            CHECK(nd == rowp[sw][P + 1] - rowp[sw][P], "owner's lane count");
+            CHECK(g + std::max(nd, 1) <= d.lanes, "row lanes out of range");
+            d.lane[g] = Lane{P, nd ? 0 : -1};
+            for (int j = 1; j < nd; ++j) d.lane[g + j] = Lane{P, j};
         }
         // tp_pack_kernel
         for (uint64_t g = 0; g < d.lanes; ++g) {
@@ -163,6 +179,12 @@ int main(int argc, char **argv)
             }
         }
     }
+    auto rowget = [&](const std::vector<double> &a, int k, double *w) {
+        for (int i = 0; i < 3; ++i) for (int q = 0; q < 4; ++q) w[4 * i + q] = a[4 * ((size_t)i * N + k) + q];
+    };
+    auto rowput = [&](std::vector<double> &a, int k, const double *w) {
+        for (int i = 0; i < 3; ++i) for (int q = 0; q < 4; ++q) a[4 * ((size_t)i * N + k) + q] = w[4 * i + q];
+    };
     // tp_pre_kernel
     for (int P = 0; P < N; ++P) {
         double w[12];
@@ -171,50 +193,30 @@ int main(int argc, char **argv)
             for (int c = 0; c <= r; ++c) { sa += Z[36 * P + 6 * r + c] * bvec[0][6 * perm[P] + c]; sb += Z[36 * P + 6 * r + c] * bvec[1][6 * perm[P] + c]; }
             w[r] = sa; w[6 + r] = sb;
         }
-        for (int i = 0; i < 3; ++i) {
-            const uint64_t k = tp_idx(tf.gtid[P], i, 3);
-            if (k >= D[0].lanes * 3) { g_oob++; continue; }
-            for (int q = 0; q < 4; ++q) D[0].inp[4 * k + q] = w[4 * i + q];
-        }
+        rowput(D[0].inp, tf.seq[P], w);
     }
 
-    // the sweeps
+    // the sweeps (one launch each), with tp_mid_kernel in between
     long nring = 0, nglob = 0;
-    std::vector<int> cissued(D[1].lanes, -1);   // step during which the copy into a backward input is issued
     for (int sw = 0; sw < 2; ++sw) {
         Dev &d = D[sw];
         const TaskPlan &tp = *d.tp;
+        if (sw) {   // tp_mid_kernel
+            for (int P = 0; P < N; ++P) { double w[12]; rowget(D[0].res, tf.seq[P], w); rowput(D[1].inp, tb.seq[P], w); }
+        }
         std::vector<double> ring(12 * (size_t)R, NAN), red(12 * TP_LANES, NAN);
         std::vector<int> ring_row(R, -1), ring_step(R, -1);
-        std::vector<int> issued(d.lanes, -1);                  // step during which the lane's result store is issued
-        auto lanevec = [&](const std::vector<double> &a, uint64_t g, double *w) {
-            for (int i = 0; i < 3; ++i) {
-                const uint64_t k = tp_idx(g, i, 3);
-                if (k >= a.size() / 4) { g_oob++; return; }
-                for (int q = 0; q < 4; ++q) w[4 * i + q] = a[4 * k + q];
-            }
-        };
-        std::vector<std::pair<uint64_t, std::vector<double>>> pend;   // results of the previous step
+        std::vector<int> issued(N, -1);   // step during which the store of result k is issued
+        std::vector<std::pair<int, std::vector<double>>> pend;   // results of the previous step
         for (int S = 0; S < tp.nsteps; ++S) {
-            // stores of the previous step, issued during this step
-            for (auto &pw : pend) {
-                const uint64_t g = pw.first;
-                for (int i = 0; i < 3; ++i) for (int q = 0; q < 4; ++q) d.res[4 * tp_idx(g, i, 3) + q] = pw.second[4 * i + q];
-                issued[g] = S;
-                if (!sw) {
-                    const int dest = tp.info[4 * g + 3];
-                    if (dest < 0 || (uint64_t)dest >= D[1].lanes) { g_oob++; continue; }
-                    for (int i = 0; i < 3; ++i) for (int q = 0; q < 4; ++q) D[1].inp[4 * tp_idx(dest, i, 3) + q] = pw.second[4 * i + q];
-                    cissued[dest] = S;
-                }
-            }
+            for (auto &pw : pend) { rowput(d.res, pw.first, pw.second.data()); issued[pw.first] = S; }
             pend.clear();
             std::vector<double> part(12 * TP_LANES, 0.0);
             std::set<int> read_slots;
-            std::vector<std::pair<int, std::vector<double>>> writes;   // ring writes of this step
+            std::vector<std::pair<int, std::vector<double>>> writes;
             for (int t = 0; t < TP_LANES; ++t) {
                 const uint64_t g = (uint64_t)S * TP_LANES + t;
-                const int src = tp.info[4 * g], slot = tp.info[4 * g + 1];
+                const int src = tp.info[2 * g], own = tp.info[2 * g + 1];
                 const Lane ln = d.lane[g];
                 if (src >= 0) {
                     CHECK(ln.P >= 0 && ln.j >= 0, "block on a lane without dependency");
@@ -222,11 +224,11 @@ int main(int argc, char **argv)
                     const int Q = colp[sw][rowp[sw][ln.P] + ln.j];
                     double x[12];
                     if (src & TP_GLOBAL) {
-                        const int h = src & ~TP_GLOBAL;
-                        CHECK(h == tp.gtid[Q], "global read of the wrong lane");
+                        const int q = src & ~TP_GLOBAL;
+                        CHECK(q == tp.seq[Q], "global read of the wrong row");
                         // fetched during step S - 1, after the barrier ending step S - 2
-                        CHECK(issued[h] >= 0 && issued[h] <= S - 2, "global read before the store is ordered");
-                        lanevec(d.res, h, x);
+                        CHECK(issued[q] >= 0 && issued[q] <= S - 2, "global read before the store is ordered");
+                        rowget(d.res, q, x);
                         ++nglob;
                     } else {
                         CHECK(src < R, "ring slot out of range");
@@ -241,33 +243,32 @@ int main(int argc, char **argv)
                     for (int a = 0; a < 6; ++a)
                         for (int c = 0; c < 6; ++c) { part[12 * t + a] += M[6 * a + c] * x[c]; part[12 * t + 6 + a] += M[6 * a + c] * x[6 + c]; }
                 }
-                if (src >= 0 && slot < 0)
+                if (src >= 0 && own < 0)
                     for (int q = 0; q < 12; ++q) red[12 * t + q] = part[12 * t + q];
             }
             for (int t = 0; t < TP_LANES; ++t) {
                 const uint64_t g = (uint64_t)S * TP_LANES + t;
-                const int src = tp.info[4 * g], slot = tp.info[4 * g + 1], nd = tp.info[4 * g + 2];
-                if (slot < 0) continue;
+                const int src = tp.info[2 * g], own = tp.info[2 * g + 1];
+                if (own < 0) continue;
                 const Lane ln = d.lane[g];
-                CHECK(ln.P >= 0 && ln.j <= 0, "owner lane does not own a row");
                 if (ln.P < 0) continue;
-                CHECK(nd == rowp[sw][ln.P + 1] - rowp[sw][ln.P], "owner's lane count");
+                const int k = own & KM, nd = own >> TP_ND_SHIFT;
                 double in[12];
-                if (sw) CHECK(cissued[g] >= 0, "backward input not written");   // ordered by the barrier between sweeps
-                lanevec(d.inp, g, in);
+                rowget(d.inp, k, in);
                 std::vector<double> w(12);
                 for (int q = 0; q < 12; ++q) w[q] = in[q] - (src >= 0 ? part[12 * t + q] : 0.0);
                 for (int dd = 1; dd < nd; ++dd) {
                     const int u = t + dd;
                     CHECK(u < TP_LANES && u / TP_WAVE == t / TP_WAVE, "reduction leaves the wave");
                     if (u >= TP_LANES) { g_oob++; continue; }
-                    const Lane lu = d.lane[(uint64_t)S * TP_LANES + u];
-                    CHECK(lu.P == ln.P && lu.j == dd && tp.info[4 * ((uint64_t)S * TP_LANES + u) + 1] < 0, "reduction reads a lane of another row");
+                    const uint64_t gu = (uint64_t)S * TP_LANES + u;
+                    const Lane lu = d.lane[gu];
+                    CHECK(lu.P == ln.P && lu.j == dd && tp.info[2 * gu + 1] < 0, "reduction reads a lane of another row");
                     for (int q = 0; q < 12; ++q) w[q] -= red[12 * u + q];
                 }
-                CHECK(slot < R, "owner slot out of range");
+                const int slot = k & (R - 1);
                 writes.push_back({slot, w});
-                pend.push_back({g, w});
+                pend.push_back({k, w});
                 ring_step[slot] = S;
                 ring_row[slot] = ln.P;
             }
@@ -276,20 +277,7 @@ int main(int argc, char **argv)
                 for (int q = 0; q < 12; ++q) ring[12 * wr.first + q] = wr.second[q];
             }
         }
-        // stores of the last step after the loop, then the barrier between the sweeps / kernel end
-        for (auto &pw : pend) {
-            const uint64_t g = pw.first;
-            for (int i = 0; i < 3; ++i) for (int q = 0; q < 4; ++q) d.res[4 * tp_idx(g, i, 3) + q] = pw.second[4 * i + q];
-            if (!sw) {
-                const int dest = tp.info[4 * g + 3];
-                if (dest < 0 || (uint64_t)dest >= D[1].lanes) { g_oob++; continue; }
-                for (int i = 0; i < 3; ++i) for (int q = 0; q < 4; ++q) D[1].inp[4 * tp_idx(dest, i, 3) + q] = pw.second[4 * i + q];
-                cissued[dest] = tp.nsteps;
-            }
-        }
-        if (!sw)   // every backward input is written before the barrier between the sweeps
-            for (uint64_t g = 0; g < D[1].lanes; ++g)
-                if (tb.info[4 * g + 1] >= 0) CHECK(cissued[g] >= 0, "backward input never written");
+        for (auto &pw : pend) rowput(d.res, pw.first, pw.second.data());   // after the loop; kernel end orders them
     }
     // tp_post_kernel
     std::vector<double> out[2];
@@ -297,7 +285,7 @@ int main(int argc, char **argv)
     out[1].assign(6 * (size_t)N, NAN);
     for (int P = 0; P < N; ++P) {
         double u[12];
-        for (int i = 0; i < 3; ++i) for (int q = 0; q < 4; ++q) u[4 * i + q] = D[1].res[4 * tp_idx(tb.gtid[P], i, 3) + q];
+        rowget(D[1].res, tb.seq[P], u);
         for (int a = 0; a < 6; ++a) {
             double sa = 0, sb = 0;
             for (int c = a; c < 6; ++c) { sa += Z[36 * P + 6 * c + a] * u[c]; sb += Z[36 * P + 6 * c + a] * u[6 + c]; }

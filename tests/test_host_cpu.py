@@ -63,9 +63,9 @@ def test_host_rcm_matches_oracle(host_rcm):
 
 @pytest.mark.parametrize("args", [("1", "1", "5"), ("50", "5", "5"), ("3000", "40", "5"), ("20000", "200", "5"),
                                   ("3000", "30", "16"), ("20000", "300", "8", "30"), ("2000", "60", "70", "10"),
-                                  ("40000", "400", "6", "50", "2304")])
+                                  ("40000", "400", "6", "50", "2048")])
 def test_sweep_schedule_emulation(tmp_path, args):
-    """tp_sweeps_kernel's schedule replayed on the CPU with a visibility model:
+    """tp_sweep_kernel's schedule replayed on the CPU with a visibility model:
     indices in bounds, ring and global reads find their producers after they
     are ordered, reductions stay in a wave, and the result equals a direct
     block solve (rows with up to 64 blocks, many isolated rows, the smallest
