@@ -111,6 +111,19 @@ __host__ __device__ inline bool has_nyquist(int i, int j, int k, int Nx, int Ny,
     return (Nx % 2 == 0 && i == Nx / 2) || (Ny % 2 == 0 && j == Ny / 2) || (Nz % 2 == 0 && k == Nz / 2);
 }
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin (workgroup
+// w runs on XCD w mod 8), and each XCD has its own L2.  xcd_remap maps
+// workgroup w of a grid of n to a work item so that every XCD gets one
+// contiguous range of items: neighbouring items (which share halo data)
+// then share an L2.  It is a bijection of [0, n) whatever the dispatch.
+constexpr int FSD_NXCD = 8;
+__device__ inline int xcd_remap(int w, int n)
+{
+    const int q = n / FSD_NXCD, r = n % FSD_NXCD;
+    const int x = w % FSD_NXCD, k = w / FSD_NXCD;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 struct LubTable {
     const float *dist;   // 1000
     const float *vals;   // 22000
@@ -198,6 +211,7 @@ struct fsd_ctx {
     int *d_ostart = nullptr;      // M+1: first position of each origin cell in d_olist
     int *d_scan_tmp = nullptr;    // chunk totals of the multi-workgroup scan
     int contract_lds = 0;         // LDS bytes of the tile contraction (0: per-particle contraction)
+    bool contract_split = false;  // the two-workgroup-per-tile contraction (P = 6, tile 8)
     float *d_wave = nullptr;      // 11 per particle: wave-space u (3) and gradient (8)
     float4 *d_rec = nullptr;      // 3 per particle: force + couplet records of the current apply
     bool spread_valid = false;
@@ -280,7 +294,7 @@ static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 void prof_begin(fsd_ctx *ctx, int slot, hipEvent_t *ev);
 void prof_end(fsd_ctx *ctx, int slot, hipEvent_t ev);
 enum { PROF_SPREAD = 0, PROF_GREEN, PROF_CONTRACT, PROF_FFT, PROF_REAL, PROF_RFU, PROF_TRSV, PROF_ICHOL,
-       PROF_GMRES_VEC, PROF_NLIST, PROF_PRECOND_BUILD, PROF_COUNT };
+       PROF_GMRES_VEC, PROF_NLIST, PROF_PRECOND_BUILD, PROF_SWEEP, PROF_COUNT };
 
 // kernel groups (launchers live with their kernels)
 int k_build_nlist(fsd_ctx *ctx, const float4 *pos);

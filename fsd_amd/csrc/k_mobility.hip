@@ -260,6 +260,152 @@ __global__ void __launch_bounds__(512) spread_kernel(SpreadGeom g, const float4 
     }
 }
 
+// Spreading for the usual stencil (P = 6, tile 8) with the work of a grid
+// point limited to its own stencil window.  The tile's H^3 halo of origin
+// cells (H = tile + P - 1) is counted and scanned in LDS, its particles are
+// staged in LDS in (x, y, z) halo-cell order, and a point at local (lx, ly,
+// lz) visits, for each of its 36 halo columns (lx + a, ly + b), the
+// consecutive staged particles of cells lz .. lz + P - 1: exactly the
+// particles whose stencil covers it.  A tile whose halo holds more than
+// SP2_CAP particles reads them from global memory instead.  Same weights and
+// accumulation as spread_kernel (each point sums its particles in the same
+// cell order, and writes once).
+#define SP2_CAP 640
+// contribution of one particle (origin o, position p, records r) to grid point (gx, gy, gz)
+__device__ __forceinline__ void spread_point_add(const SpreadGeom &g, int gx, int gy, int gz, int o0, int o1, int o2,
+                                                 float p0, float p1, float p2, float4 r0, float4 r1, float4 r2,
+                                                 float *acc)
+{
+    int tx = gx - o0; tx += (tx < 0) ? g.Nx : 0; tx -= (tx >= g.Nx) ? g.Nx : 0;
+    int ty = gy - o1; ty += (ty < 0) ? g.Ny : 0; ty -= (ty >= g.Ny) ? g.Ny : 0;
+    int tz = gz - o2; tz += (tz < 0) ? g.Nz : 0; tz -= (tz >= g.Nz) ? g.Nz : 0;
+    const float uy = g.hy * (float)(o1 + ty) - g.Ly2;
+    const float dx = g.hx * (float)(o0 + tx) - g.Lx2 + g.xy * uy - p0;
+    const float dy = uy - p1;
+    const float dz = g.hz * (float)(o2 + tz) - g.Lz2 - p2;
+    const float w = g.prefac * __expf(-g.expfac * (dx * dx + dy * dy + dz * dz));
+    acc[0] += w * r0.x; acc[1] += w * r0.y; acc[2] += w * r0.z;
+    acc[3] += w * r0.w; acc[4] += w * r1.x; acc[5] += w * r1.y; acc[6] += w * r1.z;
+    acc[7] += w * r1.w; acc[8] += w * r2.x; acc[9] += w * r2.y; acc[10] += w * r2.z;
+}
+
+template <int TILE, int PP>
+__global__ void __launch_bounds__(512) spread_tile2_kernel(SpreadGeom g, const float4 *pos, const int *orig,
+                                                           const float4 *rec, const int *ostart, const int *olist,
+                                                           float *grid, int64_t M)
+{
+    constexpr int H = TILE + PP - 1;
+    constexpr int NQ = H * H * H;                // halo cells
+    constexpr int PER = (NQ + 511) / 512;
+    __shared__ int s_beg[NQ];                    // first olist entry of each halo cell
+    __shared__ int s_base[NQ + 1];               // exclusive scan of the cells' counts
+    __shared__ int s_wsum[8];
+    __shared__ int s_o[SP2_CAP][3];
+    __shared__ float s_p[SP2_CAP][3];
+    __shared__ float4 s_r[SP2_CAP][3];
+
+    const int tile_id = xcd_remap(blockIdx.x, gridDim.x);
+    const int tz_i = tile_id % g.tnz, ty_i = (tile_id / g.tnz) % g.tny, tx_i = tile_id / (g.tnz * g.tny);
+    const int x0 = tx_i * TILE, y0 = ty_i * TILE, z0 = tz_i * TILE;
+    const int lt = threadIdx.x;
+    // counts of the halo cells, PER consecutive cells per thread
+    int cnt[PER];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int q = lt * PER + k;
+        cnt[k] = 0;
+        if (q < NQ) {
+            const int hx = q / (H * H), hy = (q / H) % H, hz = q % H;
+            int x = x0 - (PP - 1) + hx, y = y0 - (PP - 1) + hy, z = z0 - (PP - 1) + hz;
+            x += (x < 0) ? g.Nx : 0; x -= (x >= g.Nx) ? g.Nx : 0; x -= (x >= g.Nx) ? g.Nx : 0;
+            y += (y < 0) ? g.Ny : 0; y -= (y >= g.Ny) ? g.Ny : 0; y -= (y >= g.Ny) ? g.Ny : 0;
+            z += (z < 0) ? g.Nz : 0; z -= (z >= g.Nz) ? g.Nz : 0; z -= (z >= g.Nz) ? g.Nz : 0;
+            const int c = (x * g.Ny + y) * g.Nz + z;
+            const int b = ostart[c];
+            s_beg[q] = b;
+            cnt[k] = ostart[c + 1] - b;
+        }
+        sum += cnt[k];
+    }
+    // exclusive scan over the threads: wave inclusive scan, then wave offsets
+    const int lane = lt & 63, wv = lt >> 6;
+    int inc = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+    }
+    if (lane == 63) s_wsum[wv] = inc;
+    __syncthreads();
+    int woff = 0;
+    for (int w = 0; w < wv; ++w) woff += s_wsum[w];
+    int run = woff + inc - sum;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int q = lt * PER + k;
+        if (q < NQ) s_base[q] = run;
+        run += cnt[k];
+    }
+    int total = 0;
+    for (int w = 0; w < 8; ++w) total += s_wsum[w];
+    if (lt == 0) s_base[NQ] = total;
+    const bool staged = total <= SP2_CAP;
+    __syncthreads();
+    if (staged) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int q = lt * PER + k;
+            if (q < NQ) {
+                const int b = s_beg[q], o = s_base[q];
+                for (int m = 0; m < cnt[k]; ++m) {
+                    const int i = olist[b + m];
+                    s_o[o + m][0] = orig[3 * i]; s_o[o + m][1] = orig[3 * i + 1]; s_o[o + m][2] = orig[3 * i + 2];
+                    const float4 p = pos[i];
+                    s_p[o + m][0] = p.x; s_p[o + m][1] = p.y; s_p[o + m][2] = p.z;
+                    s_r[o + m][0] = rec[3 * (int64_t)i]; s_r[o + m][1] = rec[3 * (int64_t)i + 1];
+                    s_r[o + m][2] = rec[3 * (int64_t)i + 2];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int lx = lt / (TILE * TILE), ly = (lt / TILE) % TILE, lz = lt % TILE;
+    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    if (lt >= TILE * TILE * TILE || gx >= g.Nx || gy >= g.Ny || gz >= g.Nz) return;
+    float acc[11];
+#pragma unroll
+    for (int c = 0; c < 11; ++c) acc[c] = 0.f;
+    // one loop per source (separate code, so the staged loop reads LDS only)
+    if (staged) {
+        for (int a = 0; a < PP; ++a)
+            for (int b = 0; b < PP; ++b) {
+                const int q0 = ((lx + a) * H + (ly + b)) * H + lz;
+                const int s1 = s_base[q0 + PP];
+                for (int sl = s_base[q0]; sl < s1; ++sl)
+                    spread_point_add(g, gx, gy, gz, s_o[sl][0], s_o[sl][1], s_o[sl][2], s_p[sl][0], s_p[sl][1],
+                                     s_p[sl][2], s_r[sl][0], s_r[sl][1], s_r[sl][2], acc);
+            }
+    } else {
+        for (int a = 0; a < PP; ++a)
+            for (int b = 0; b < PP; ++b) {
+                const int q0 = ((lx + a) * H + (ly + b)) * H + lz;
+                for (int q = q0; q < q0 + PP; ++q) {
+                    const int beg = s_beg[q], n = s_base[q + 1] - s_base[q];
+                    for (int m = 0; m < n; ++m) {
+                        const int i = olist[beg + m];
+                        const float4 p = pos[i];
+                        spread_point_add(g, gx, gy, gz, orig[3 * i], orig[3 * i + 1], orig[3 * i + 2], p.x, p.y, p.z,
+                                         rec[3 * (int64_t)i], rec[3 * (int64_t)i + 1], rec[3 * (int64_t)i + 2], acc);
+                    }
+                }
+            }
+    }
+    const int64_t id = ((int64_t)gx * g.Ny + gy) * g.Nz + gz;
+#pragma unroll
+    for (int c = 0; c < 11; ++c) grid[c * M + id] = acc[c];
+}
+
 // ------------------------------------------------------------ Green kernel
 // Mobility_WaveSpace_Green_kernel (Mobility.cu:315-479) on the half spectrum.
 // Field order: Fx Fy Fz | C_xx C_xy C_xz C_yz C_yy C_yx C_zx C_zy.
@@ -412,7 +558,7 @@ __global__ void __launch_bounds__(512) contract_tile_kernel(SpreadGeom g, const 
     __shared__ int s_run_beg[64], s_run_off[65];
     const int S = g.tile + g.P - 1;
     const int S3 = S * S * S;
-    const int tile_id = blockIdx.x;
+    const int tile_id = xcd_remap(blockIdx.x, gridDim.x);   // a slab of tiles per XCD: halos shared in its L2
     const int tz_i = tile_id % g.tnz, ty_i = (tile_id / g.tnz) % g.tny, tx_i = tile_id / (g.tnz * g.tny);
     const int x0 = tx_i * g.tile, y0 = ty_i * g.tile, z0 = tz_i * g.tile;
     const int lt = threadIdx.x;
@@ -488,11 +634,117 @@ __global__ void __launch_bounds__(512) contract_tile_kernel(SpreadGeom g, const 
     }
 }
 
+// The tile contraction for the usual stencil (P = 6, tile 8, S = 13) with
+// compile-time sizes, the 11 fields split over two workgroups per tile
+// (fields [0, 6) and [6, 11)) so that three workgroups share a CU and one's
+// halo load overlaps another's arithmetic.  Same arithmetic per output as
+// contract_tile_kernel.
+#define CT_NF 6
+template <int S, int PP>
+__global__ void __launch_bounds__(512) contract_tile2_kernel(SpreadGeom g, const float4 *pos, const int *orig,
+                                                             const int *ostart, const int *olist, const float *grid,
+                                                             int64_t M, float quadw_prefac, float *wave)
+{
+    constexpr int S3 = S * S * S;
+    constexpr int TILE = S - PP + 1;
+    extern __shared__ float s_g[];           // [CT_NF][S][S][S]
+    __shared__ int s_run_beg[TILE * TILE], s_run_off[TILE * TILE + 1];
+    __shared__ int s_w[3][S];                // wrapped grid index of each halo offset
+    const int item = xcd_remap(blockIdx.x, gridDim.x);   // a slab of tiles per XCD: halos shared in its L2
+    const int tile_id = item >> 1, grp = item & 1;
+    const int f0 = grp ? CT_NF : 0, nf = grp ? 11 - CT_NF : CT_NF;
+    const int tz_i = tile_id % g.tnz, ty_i = (tile_id / g.tnz) % g.tny, tx_i = tile_id / (g.tnz * g.tny);
+    const int x0 = tx_i * TILE, y0 = ty_i * TILE, z0 = tz_i * TILE;
+    const int lt = threadIdx.x;
+    const int xe = min(x0 + TILE, g.Nx), ye = min(y0 + TILE, g.Ny), z1 = min(z0 + TILE, g.Nz);
+    const int ncx = xe - x0, ncy = ye - y0;
+    if (lt < ncx * ncy) {
+        const int x = x0 + lt / ncy, y = y0 + lt % ncy;
+        const int cb = (x * g.Ny + y) * g.Nz;
+        const int b0 = ostart[cb + z0], b1 = ostart[cb + z1];
+        s_run_beg[lt] = b0;
+        s_run_off[lt + 1] = b1 - b0;
+    }
+    if (lt < 3 * S) {
+        const int ax = lt / S, o = lt - ax * S;
+        const int n = ax == 0 ? g.Nx : (ax == 1 ? g.Ny : g.Nz);
+        int v = (ax == 0 ? x0 : (ax == 1 ? y0 : z0)) + o;
+        v -= (v >= n) ? n : 0;
+        v -= (v >= n) ? n : 0;
+        s_w[ax][o] = v;
+    }
+    __syncthreads();
+    // grid block of this workgroup's fields, wrapped
+    for (int idx = lt; idx < nf * S3; idx += blockDim.x) {
+        const int c = idx / S3, r = idx - c * S3;
+        const int ax = r / (S * S), ay = (r / S) % S, az = r % S;
+        s_g[idx] = grid[(f0 + c) * M + ((int64_t)s_w[0][ax] * g.Ny + s_w[1][ay]) * g.Nz + s_w[2][az]];
+    }
+    if (lt == 0) {
+        s_run_off[0] = 0;
+        for (int q = 0; q < ncx * ncy; ++q) s_run_off[q + 1] += s_run_off[q];
+    }
+    __syncthreads();
+    const int nruns = ncx * ncy;
+    const int total = s_run_off[nruns];
+    const int wv = lt >> 6, lane = lt & 63, nw = blockDim.x >> 6;
+    // fields XX XY XZ YZ YY YX ZX ZY (grid order 3..10) -> slots {XX, YX, ZX, ZY, YY, XY, XZ, YZ}
+    constexpr int slot_of[11] = {0, 1, 2, 3, 8, 9, 10, 7, 4, 5, 6};
+    for (int q = wv; q < total; q += nw) {
+        int lo = 0, hi = nruns - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_run_off[mid] <= q) lo = mid; else hi = mid - 1;
+        }
+        const int i = olist[s_run_beg[lo] + (q - s_run_off[lo])];
+        const int ox = orig[3 * i], oy = orig[3 * i + 1], oz = orig[3 * i + 2];
+        const float4 p = pos[i];
+        const int bx = (((ox % g.Nx) + g.Nx) % g.Nx) - x0;
+        const int by = (((oy % g.Ny) + g.Ny) % g.Ny) - y0;
+        const int bz = (((oz % g.Nz) + g.Nz) % g.Nz) - z0;
+        float acc[CT_NF];
+#pragma unroll
+        for (int c = 0; c < CT_NF; ++c) acc[c] = 0.f;
+        if (lane < PP * PP) {
+            const int ty = lane / PP, tz = lane - PP * ty;
+            const float uy = g.hy * (float)(oy + ty) - g.Ly2;
+            const float dz = g.hz * (float)(oz + tz) - g.Lz2 - p.z;
+            const float dy = uy - p.y;
+#pragma unroll
+            for (int tx = 0; tx < PP; ++tx) {
+                const float dx = g.hx * (float)(ox + tx) - g.Lx2 + g.xy * uy - p.x;
+                const float w = quadw_prefac * __expf(-g.expfac * (dx * dx + dy * dy + dz * dz));
+                const int r = ((bx + tx) * S + (by + ty)) * S + (bz + tz);
+#pragma unroll
+                for (int c = 0; c < CT_NF; ++c)
+                    if (c < nf) acc[c] += w * s_g[c * S3 + r];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CT_NF; ++c)
+            for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
+        if (lane == 0) {
+            float *w = wave + 11 * (int64_t)i;
+#pragma unroll
+            for (int c = 0; c < CT_NF; ++c)
+                if (c < nf) w[slot_of[f0 + c]] = acc[c];
+        }
+    }
+}
+
 // LDS bytes of the tile contraction, 0 when it does not fit (then the
 // per-particle contraction runs); sets the launch attribute once
 int k_contract_lds(fsd_ctx *ctx)
 {
     const int S = ctx->tile + ctx->pse.P - 1;
+    if (S == 13 && ctx->pse.P == 6) {
+        const size_t b2 = sizeof(float) * CT_NF * 13 * 13 * 13;
+        if (hipFuncSetAttribute((const void *)contract_tile2_kernel<13, 6>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)b2) == hipSuccess) {
+            ctx->contract_split = true;
+            return (int)b2;
+        }
+    }
     const size_t bytes = sizeof(float) * 11 * (size_t)S * S * S;
     int dev = 0, maxlds = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -643,7 +895,11 @@ int k_wave_contract(fsd_ctx *ctx, const float4 *pos)
     }
     prof_end(ctx, PROF_FFT, ev);
     prof_begin(ctx, PROF_CONTRACT, &ev);
-    if (ctx->contract_lds)
+    if (ctx->contract_lds && ctx->contract_split)
+        hipLaunchKernelGGL((contract_tile2_kernel<13, 6>), dim3(2 * ctx->ntiles), dim3(512), ctx->contract_lds,
+                           ctx->stream, g, pos, ctx->d_porig, ctx->d_ostart, ctx->d_olist, ctx->d_grid, (int64_t)ctx->M,
+                           ps.quadW * ps.prefac, ctx->d_wave);
+    else if (ctx->contract_lds)
         hipLaunchKernelGGL(contract_tile_kernel, dim3(ctx->ntiles), dim3(512), ctx->contract_lds, ctx->stream, g, pos,
                            ctx->d_porig, ctx->d_ostart, ctx->d_olist, ctx->d_grid, (int64_t)ctx->M,
                            ps.quadW * ps.prefac, ctx->d_wave);
@@ -678,8 +934,13 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
         }
         hipEvent_t ev;
         prof_begin(ctx, PROF_SPREAD, &ev);
-        hipLaunchKernelGGL(spread_kernel, dim3(ctx->ntiles), dim3(ctx->tile * ctx->tile * ctx->tile), 0, ctx->stream,
-                           g, pos, ctx->d_porig, rec, ctx->d_ostart, ctx->d_olist, ctx->d_grid, (int64_t)ctx->M);
+        if (ctx->tile == 8 && ctx->pse.P == 6)
+            hipLaunchKernelGGL((spread_tile2_kernel<8, 6>), dim3(ctx->ntiles), dim3(512), 0, ctx->stream, g, pos,
+                               ctx->d_porig, rec, ctx->d_ostart, ctx->d_olist, ctx->d_grid, (int64_t)ctx->M);
+        else
+            hipLaunchKernelGGL(spread_kernel, dim3(ctx->ntiles), dim3(ctx->tile * ctx->tile * ctx->tile), 0,
+                               ctx->stream, g, pos, ctx->d_porig, rec, ctx->d_ostart, ctx->d_olist, ctx->d_grid,
+                               (int64_t)ctx->M);
         FSD_HIP(hipGetLastError());
         prof_end(ctx, PROF_SPREAD, ev);
         prof_begin(ctx, PROF_FFT, &ev);

@@ -651,12 +651,17 @@ int k_precond_apply2(fsd_ctx *ctx, const float *b0, const float *b1, float *y0, 
         TaskDev B{ctx->tp_b.nsteps, N, ctx->d_tp_binfo, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres};
         hipLaunchKernelGGL(tp_pre_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_fseq,
                            ctx->d_perm, ctx->d_tp_Z, b0, b1, n_rhs, ctx->d_tp_finp);
+        hipEvent_t es;
+        prof_begin(ctx, PROF_SWEEP, &es);
         if (n_rhs == 2) hipLaunchKernelGGL(tp_sweep_kernel<2>, dim3(1), dim3(TP_LANES), lds, ctx->stream, F);
         else hipLaunchKernelGGL(tp_sweep_kernel<1>, dim3(1), dim3(TP_LANES), lds, ctx->stream, F);
+        prof_end(ctx, PROF_SWEEP, es);
         hipLaunchKernelGGL(tp_mid_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->d_tp_fseq,
                            ctx->d_tp_bseq, ctx->d_tp_fres, ctx->d_tp_binp);
+        prof_begin(ctx, PROF_SWEEP, &es);
         if (n_rhs == 2) hipLaunchKernelGGL(tp_sweep_kernel<2>, dim3(1), dim3(TP_LANES), lds, ctx->stream, B);
         else hipLaunchKernelGGL(tp_sweep_kernel<1>, dim3(1), dim3(TP_LANES), lds, ctx->stream, B);
+        prof_end(ctx, PROF_SWEEP, es);
         hipLaunchKernelGGL(tp_post_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_bseq,
                            ctx->d_perm, ctx->d_tp_Z, ctx->d_tp_bres, n_rhs, y0, y1);
         FSD_HIP(hipGetLastError());

@@ -21,7 +21,7 @@ void fsd_ewald_self(float xi, float self_out[2]);
 
 // ------------------------------------------------------------ profiling
 static const char *kProfNames[PROF_COUNT] = {"spread", "green", "contract", "fft", "realspace", "rfu",
-                                             "trsv", "ichol", "gmres_vec", "nlist", "precond_build"};
+                                             "trsv", "ichol", "gmres_vec", "nlist", "precond_build", "sweep"};
 
 void prof_begin(fsd_ctx *ctx, int slot, hipEvent_t *ev)
 {
