@@ -98,6 +98,7 @@ def main():
     ap.add_argument("--phi", type=float, default=0.45)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--restart", type=int, default=100, help="GMRES restart length")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (diagnostics)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MB",
                     help="only run 5 device copies of MB megabytes (calibrates the HBM byte counters) and exit")
@@ -130,7 +131,7 @@ def main():
     print("# workload generated in %.1f s: N=%d phi=%.2f L=%.3f" % (time.time() - t, N, args.phi, L),
           file=sys.stderr, flush=True)
     shear_rate = 1.0
-    st = Stokes(N, L, xi=0.5, error=1e-3, gmres_tol=1e-4, gmres_true_tol=1e-4, dt=1e-3)
+    st = Stokes(N, L, xi=0.5, error=1e-3, gmres_tol=1e-4, gmres_true_tol=1e-4, dt=1e-3, gmres_restart=args.restart)
     pse = st.pse_info()
     state = ParticleState(pos)
     dF = torch.from_numpy(F).cuda()
@@ -204,7 +205,7 @@ def main():
         "data": "synthetic (seeded packing and forces)",
         "config": {
             "workload": "N=32768 phi=0.45 periodic suspension, shear rate 1, random forces, T=0, xi=0.5, "
-                        "PSE error 1e-3, IC(0)-preconditioned GMRES to true residual 1e-4",
+                        "PSE error 1e-3, IC(0)-preconditioned GMRES(%d) to true residual 1e-4" % args.restart,
             "N": N, "phi": args.phi, "grid": [pse["Nx"], pse["Ny"], pse["Nz"]], "P": pse["P"],
             "preconditioner": "ichol",
             "gmres_iterations_mean": float(np.mean(iters)),

@@ -68,7 +68,7 @@ def build(verbose=False):
                 print(o)
     tmp = LIB + ".tmp"
     _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs +
-         ["-L/opt/rocm/lib", "-lrocfft", "-lquadmath", "-Wl,-rpath,/opt/rocm/lib"])
+         ["-L/opt/rocm/lib", "-lrocfft", "-lquadmath", "-lpthread", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

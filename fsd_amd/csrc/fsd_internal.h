@@ -124,6 +124,13 @@ __device__ inline int xcd_remap(int w, int n)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
+// host arrays of the preconditioner build
+struct PrecondHost {
+    std::vector<int> iperm;                    // particle -> permuted index
+    std::vector<int> lrow, lcol, urow, ucol, upos;   // block adjacency below / above the diagonal
+    std::vector<int> itf, itb;                 // particles by forward / backward level
+};
+
 struct LubTable {
     const float *dist;   // 1000
     const float *vals;   // 22000
@@ -239,6 +246,10 @@ struct fsd_ctx {
     int *d_lev_f = nullptr, *d_lev_b = nullptr;
     float *d_tmp6a = nullptr, *d_tmp6b = nullptr;
     float *d_g12 = nullptr;       // G b_F and G b_U of the saddle preconditioner
+    // host phase of the preconditioner build (kept between steps to reuse capacity)
+    std::vector<int> h_plstride;                         // pruned lists, FSD_PSTRIDE per particle
+    PrecondHost pch;
+    hipEvent_t pc_ev = nullptr;                          // profiling scope of the build
     // single-workgroup sweeps: task plans (sweep_plan.h) and their device operands
     TaskPlan tp_f, tp_b;
     bool tp_ok = false;

@@ -113,8 +113,13 @@ void host_rcm(int N, const int *phead, const int *plist, int *perm)
                 const int w = g.adj[e];
                 if (mask[w]) { mask[w] = 0; out[n_out++] = w; }
             }
-            if (n_out - first > 2)
-                std::stable_sort(out + first + 1, out + n_out, [&](int a, int b) { return deg[a] < deg[b]; });
+            // stable insertion sort by increasing degree of all children but the first (a few at most)
+            for (int k = first + 2; k < n_out; ++k) {
+                const int v2 = out[k], d2 = deg[v2];
+                int m = k;
+                while (m > first + 1 && deg[out[m - 1]] > d2) { out[m] = out[m - 1]; --m; }
+                out[m] = v2;
+            }
         }
         std::reverse(out, out + n_out);
         placed += n_out;

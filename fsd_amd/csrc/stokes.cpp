@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 extern "C" {
@@ -354,50 +355,63 @@ static int grow_tasks(fsd_ctx *ctx)
     return 0;
 }
 
-// Precondition_Wrap (Precondition.cu:1326-1497)
-static int build_precond(fsd_ctx *ctx, const float4 *pos)
+// Precondition_Wrap (Precondition.cu:1326-1497) in three phases: fetch (the
+// pruned lists on the GPU, copied to the host), plan (host only: RCM, block
+// adjacency, level schedules, sweep plans; it touches no HIP state, so a step
+// runs it on a second thread while independent kernels are enqueued) and
+// finish (uploads, block values, IC(0), packed sweep operands).
+static int precond_fetch(fsd_ctx *ctx, const float4 *pos)
 {
     const int N = ctx->N;
     if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
     if (!ctx->have_table) { ctx->err = "resistance table not set"; return FSD_E_STATE; }
-    hipEvent_t ev;
-    prof_begin(ctx, PROF_PRECOND_BUILD, &ev);
+    prof_begin(ctx, PROF_PRECOND_BUILD, &ctx->pc_ev);
     RC(k_prune(ctx, pos));
     ctx->h_pn.resize(N);
-    std::vector<int> pl_stride((size_t)N * FSD_PSTRIDE);
+    ctx->h_plstride.resize((size_t)N * FSD_PSTRIDE);
     CK(hipMemcpyAsync(ctx->h_pn.data(), ctx->d_pn, sizeof(int) * N, hipMemcpyDeviceToHost, ctx->stream));
-    CK(hipMemcpyAsync(pl_stride.data(), ctx->d_plist, sizeof(int) * pl_stride.size(), hipMemcpyDeviceToHost,
-                      ctx->stream));
+    CK(hipMemcpyAsync(ctx->h_plstride.data(), ctx->d_plist, sizeof(int) * ctx->h_plstride.size(),
+                      hipMemcpyDeviceToHost, ctx->stream));
     CK(hipMemcpyAsync(ctx->h_flags + 2, ctx->d_flags + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
     if (ctx->h_flags[2]) { ctx->err = "more than 15 neighbours inside the preconditioner cutoff"; return FSD_E_CAPACITY; }
+    return 0;
+}
+
+static void precond_plan(fsd_ctx *ctx)
+{
+    const int N = ctx->N;
+    PrecondHost &h = ctx->pch;
     // packed adjacency (self loops included), as Precondition_Reorder passes to genrcm
     std::vector<int> phead(N + 1);
     phead[0] = 0;
     for (int i = 0; i < N; ++i) phead[i + 1] = phead[i] + ctx->h_pn[i];
     ctx->h_plist.resize(phead[N]);
     for (int i = 0; i < N; ++i)
-        for (int s = 0; s < ctx->h_pn[i]; ++s) ctx->h_plist[phead[i] + s] = pl_stride[(size_t)i * FSD_PSTRIDE + s];
+        for (int s = 0; s < ctx->h_pn[i]; ++s) ctx->h_plist[phead[i] + s] = ctx->h_plstride[(size_t)i * FSD_PSTRIDE + s];
     ctx->h_perm.resize(N);
     host_rcm(N, phead.data(), ctx->h_plist.data(), ctx->h_perm.data());
-    std::vector<int> iperm(N);
-    for (int P = 0; P < N; ++P) iperm[ctx->h_perm[P]] = P;
+    h.iperm.resize(N);
+    for (int P = 0; P < N; ++P) h.iperm[ctx->h_perm[P]] = P;
     // lower and upper block adjacency on the permuted graph
-    std::vector<int> lrow(N + 1), lcol, urow(N + 1, 0), ucol, upos;
-    lcol.reserve(2 * (size_t)N);
-    std::vector<int> tmp;
+    std::vector<int> &lrow = h.lrow, &lcol = h.lcol, &urow = h.urow, &ucol = h.ucol, &upos = h.upos;
+    lrow.assign(N + 1, 0);
+    urow.assign(N + 1, 0);
+    lcol.clear();
+    lcol.reserve(3 * (size_t)N);
+    int tmp[FSD_PSTRIDE];
     int64_t nnz_ref = 0;
     for (int P = 0; P < N; ++P) {
         const int i = ctx->h_perm[P];
         lrow[P] = (int)lcol.size();
-        tmp.clear();
+        int nt = 0;
         for (int s = phead[i]; s < phead[i + 1]; ++s) {
-            const int Q = iperm[ctx->h_plist[s]];
-            if (Q < P) tmp.push_back(Q);
+            const int Q = h.iperm[ctx->h_plist[s]];
+            if (Q < P) tmp[nt++] = Q;
         }
-        std::sort(tmp.begin(), tmp.end());
-        lcol.insert(lcol.end(), tmp.begin(), tmp.end());
-        for (int Q : tmp) urow[Q + 1]++;
+        std::sort(tmp, tmp + nt);
+        lcol.insert(lcol.end(), tmp, tmp + nt);
+        for (int k = 0; k < nt; ++k) urow[tmp[k] + 1]++;
         nnz_ref += (ctx->h_pn[i] > 1) ? 36 * (int64_t)ctx->h_pn[i] : 6;
     }
     lrow[N] = (int)lcol.size();
@@ -418,50 +432,58 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
     ctx->nnz_ref = nnz_ref;
     // level schedules: forward level = 1 + max level of the lower neighbours,
     // backward level = 1 + max level of the upper neighbours (counted from N-1)
-    {
-        std::vector<int> lf(N, 0), lb(N, 0);
-        int mf = 0, mb = 0;
-        for (int P = 0; P < N; ++P) {
-            int l = 0;
-            for (int e = lrow[P]; e < lrow[P + 1]; ++e) l = std::max(l, lf[lcol[e]] + 1);
-            lf[P] = l;
-            mf = std::max(mf, l);
-        }
-        for (int P = N - 1; P >= 0; --P) {
-            int l = 0;
-            for (int e = urow[P]; e < urow[P + 1]; ++e) l = std::max(l, lb[ucol[e]] + 1);
-            lb[P] = l;
-            mb = std::max(mb, l);
-        }
-        auto bucket = [N](const std::vector<int> &lev, int nl, std::vector<int> &ptr, std::vector<int> &items) {
-            ptr.assign(nl + 1, 0);
-            for (int P = 0; P < N; ++P) ptr[lev[P] + 1]++;
-            for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
-            items.resize(N);
-            std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-            for (int P = 0; P < N; ++P) items[fill[lev[P]]++] = P;
-        };
-        std::vector<int> itf, itb;
-        ctx->nlev_f = mf + 1;
-        ctx->nlev_b = mb + 1;
-        bucket(lf, ctx->nlev_f, ctx->h_lev_f_ptr, itf);
-        bucket(lb, ctx->nlev_b, ctx->h_lev_b_ptr, itb);
-        CK(hipMemcpyAsync(ctx->d_lev_f, itf.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
-        CK(hipMemcpyAsync(ctx->d_lev_b, itb.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
-        CK(hipStreamSynchronize(ctx->stream));
-        // schedules of the single-workgroup sweeps
-        ctx->tp_ok = ctx->tp_R > 0 &&
-                     build_task_plan(N, lf.data(), ctx->nlev_f, lrow.data(), lcol.data(), nullptr, true, ctx->tp_R,
-                                     ctx->tp_f) &&
-                     build_task_plan(N, lb.data(), ctx->nlev_b, urow.data(), ucol.data(), upos.data(), false,
-                                     ctx->tp_R, ctx->tp_b);
+    std::vector<int> lf(N, 0), lb(N, 0);
+    int mf = 0, mb = 0;
+    for (int P = 0; P < N; ++P) {
+        int l = 0;
+        for (int e = lrow[P]; e < lrow[P + 1]; ++e) l = std::max(l, lf[lcol[e]] + 1);
+        lf[P] = l;
+        mf = std::max(mf, l);
     }
+    for (int P = N - 1; P >= 0; --P) {
+        int l = 0;
+        for (int e = urow[P]; e < urow[P + 1]; ++e) l = std::max(l, lb[ucol[e]] + 1);
+        lb[P] = l;
+        mb = std::max(mb, l);
+    }
+    auto bucket = [N](const std::vector<int> &lev, int nl, std::vector<int> &ptr, std::vector<int> &items) {
+        ptr.assign(nl + 1, 0);
+        for (int P = 0; P < N; ++P) ptr[lev[P] + 1]++;
+        for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+        items.resize(N);
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (int P = 0; P < N; ++P) items[fill[lev[P]]++] = P;
+    };
+    ctx->nlev_f = mf + 1;
+    ctx->nlev_b = mb + 1;
+    bucket(lf, ctx->nlev_f, ctx->h_lev_f_ptr, h.itf);
+    bucket(lb, ctx->nlev_b, ctx->h_lev_b_ptr, h.itb);
+    // schedules of the single-workgroup sweeps; the two plans are independent: the
+    // backward one on a second host thread
+    ctx->tp_ok = false;
+    if (ctx->tp_R > 0) {
+        bool okb = false;
+        std::thread tb([&]() {
+            okb = build_task_plan(N, lb.data(), ctx->nlev_b, urow.data(), ucol.data(), upos.data(), false, ctx->tp_R,
+                                  ctx->tp_b);
+        });
+        const bool okf = build_task_plan(N, lf.data(), ctx->nlev_f, lrow.data(), lcol.data(), nullptr, true,
+                                         ctx->tp_R, ctx->tp_f);
+        tb.join();
+        ctx->tp_ok = okf && okb;
+    }
+}
+
+static int precond_finish(fsd_ctx *ctx, const float4 *pos)
+{
+    const int N = ctx->N;
+    PrecondHost &h = ctx->pch;
+    auto up = [ctx](void *d, const void *src, size_t bytes) {
+        return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    };
     RC(grow_lower(ctx, ctx->nlow + 1));
     if (ctx->tp_ok) {
         RC(grow_tasks(ctx));
-        auto up = [ctx](void *d, const void *h, size_t bytes) {
-            return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream);
-        };
         const TaskPlan &f = ctx->tp_f, &b = ctx->tp_b;
         CK(up(ctx->d_tp_finfo, f.info.data(), sizeof(int) * f.info.size()));
         CK(up(ctx->d_tp_binfo, b.info.data(), sizeof(int) * b.info.size()));
@@ -469,16 +491,17 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
         CK(up(ctx->d_tp_bbsrc, b.bsrc.data(), sizeof(int) * b.bsrc.size()));
         CK(up(ctx->d_tp_fseq, f.seq.data(), sizeof(int) * (size_t)N));
         CK(up(ctx->d_tp_bseq, b.seq.data(), sizeof(int) * (size_t)N));
-        CK(hipStreamSynchronize(ctx->stream));
     }
-    CK(hipMemcpyAsync(ctx->d_perm, ctx->h_perm.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
-    CK(hipMemcpyAsync(ctx->d_iperm, iperm.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream));
-    CK(hipMemcpyAsync(ctx->d_lrow, lrow.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
-    CK(hipMemcpyAsync(ctx->d_urow, urow.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice, ctx->stream));
+    CK(up(ctx->d_lev_f, h.itf.data(), sizeof(int) * N));
+    CK(up(ctx->d_lev_b, h.itb.data(), sizeof(int) * N));
+    CK(up(ctx->d_perm, ctx->h_perm.data(), sizeof(int) * N));
+    CK(up(ctx->d_iperm, h.iperm.data(), sizeof(int) * N));
+    CK(up(ctx->d_lrow, h.lrow.data(), sizeof(int) * (N + 1)));
+    CK(up(ctx->d_urow, h.urow.data(), sizeof(int) * (N + 1)));
     if (ctx->nlow) {
-        CK(hipMemcpyAsync(ctx->d_lcol, lcol.data(), sizeof(int) * ctx->nlow, hipMemcpyHostToDevice, ctx->stream));
-        CK(hipMemcpyAsync(ctx->d_ucol, ucol.data(), sizeof(int) * ctx->nlow, hipMemcpyHostToDevice, ctx->stream));
-        CK(hipMemcpyAsync(ctx->d_upos, upos.data(), sizeof(int) * ctx->nlow, hipMemcpyHostToDevice, ctx->stream));
+        CK(up(ctx->d_lcol, h.lcol.data(), sizeof(int) * ctx->nlow));
+        CK(up(ctx->d_ucol, h.ucol.data(), sizeof(int) * ctx->nlow));
+        CK(up(ctx->d_upos, h.upos.data(), sizeof(int) * ctx->nlow));
     }
     RC(k_build_blocks(ctx, pos));
     // IC(0) with the relaxer doubling on a non-positive pivot (Precondition.cu:1274-1288, 1455-1489)
@@ -494,9 +517,16 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
     }
     ctx->ichol_relaxer = lambda;
     if (ctx->tp_ok) RC(k_sweep_pack(ctx));
-    prof_end(ctx, PROF_PRECOND_BUILD, ev);
+    prof_end(ctx, PROF_PRECOND_BUILD, ctx->pc_ev);
     ctx->precond_valid = true;
     return 0;
+}
+
+static int build_precond(fsd_ctx *ctx, const float4 *pos)
+{
+    RC(precond_fetch(ctx, pos));
+    precond_plan(ctx);
+    return precond_finish(ctx, pos);
 }
 
 // ------------------------------------------------------------ GMRES
@@ -1311,20 +1341,29 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     // neighbour list (HOOMD NeighborList::compute) and sheared k vectors
     RC(k_build_nlist(ctx, pos));
     RC(k_gridk(ctx));
-    // preconditioner (Precondition_Wrap)
-    RC(build_precond(ctx, pos));
-    // applied force: injected, or the reference force law with the external torque
-    if (d_force6) {
-        RC(k_copy(ctx, 6 * (int64_t)N, d_force6, ctx->d_force));
-    } else {
-        const float T_ext = (float)(ctx->p.T_ext * std::cos(ctx->p.omega_ext * (double)timestep * ctx->p.dt));
-        RC(k_force_law(ctx, pos, T_ext, ctx->d_force));
-    }
-    // right-hand side (Integrator_ComputeVelocity, Integrator.cu:744-860)
-    RC(k_fill(ctx, 11 * (int64_t)N, 0.f, d_vel11));
-    RC(k_fill(ctx, n17, 0.f, ctx->d_rhs));
-    RC(k_build_lub_pairs(ctx, pos));
-    RC(k_setup_spread(ctx, pos));
+    // preconditioner (Precondition_Wrap): its host phase runs on a second thread while the
+    // kernels that do not depend on it are enqueued
+    RC(precond_fetch(ctx, pos));
+    std::thread planner(precond_plan, ctx);
+    auto independent = [&]() -> int {
+        // applied force: injected, or the reference force law with the external torque
+        if (d_force6) {
+            RC(k_copy(ctx, 6 * (int64_t)N, d_force6, ctx->d_force));
+        } else {
+            const float T_ext = (float)(ctx->p.T_ext * std::cos(ctx->p.omega_ext * (double)timestep * ctx->p.dt));
+            RC(k_force_law(ctx, pos, T_ext, ctx->d_force));
+        }
+        // right-hand side (Integrator_ComputeVelocity, Integrator.cu:744-860)
+        RC(k_fill(ctx, 11 * (int64_t)N, 0.f, d_vel11));
+        RC(k_fill(ctx, n17, 0.f, ctx->d_rhs));
+        RC(k_build_lub_pairs(ctx, pos));
+        RC(k_setup_spread(ctx, pos));
+        return 0;
+    };
+    const int rci = independent();
+    planner.join();
+    if (rci < 0) return rci;
+    RC(precond_finish(ctx, pos));
     if (ctx->p.T > 0.f) {
         // rhs[0:11N] = U_B^ff, rhs[11N:17N] = F_B^nf, Velocity[0:6N] = RFD drift
         RC(alloc_brownian(ctx));
