@@ -254,6 +254,7 @@ struct fsd_ctx {
     TaskPlan tp_f, tp_b;
     bool tp_ok = false;
     int tp_R = 0;                                        // LDS ring capacity in rows (0: level path)
+    bool tp_joint = false;                               // both right-hand sides in one workgroup
     int64_t tp_cap_f = 0, tp_cap_b = 0;                  // lanes allocated per sweep
     int2 *d_tp_finfo = nullptr, *d_tp_binfo = nullptr;   // per lane
     int2 *d_tp_fbsrc = nullptr, *d_tp_bbsrc = nullptr;   // per lane

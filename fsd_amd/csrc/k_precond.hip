@@ -505,6 +505,140 @@ __global__ void __launch_bounds__(TP_LANES) tp_sweep_kernel(TaskDev S)
     }
 }
 
+// One right-hand side per workgroup: the launch has one workgroup per
+// right-hand side (r = blockIdx.x), on different CUs, so each CU does half the
+// products and reductions of the joint kernel above and reads the same
+// blocks.  Row values of right-hand side r are the 6 floats (a0..a5) of the
+// [3][nrows] float4 layout: r = 0 at (f4[0], f4[1].xy), r = 1 at
+// (f4[1].zw, f4[2]).
+__device__ __forceinline__ void row_ld(const float4 *A, uint32_t nr, uint32_t k, int r, float *v)
+{
+    if (r == 0) {
+        const float4 a = A[k];
+        const float2 b = reinterpret_cast<const float2 *>(A + nr + k)[0];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y;
+    } else {
+        const float2 a = reinterpret_cast<const float2 *>(A + nr + k)[1];
+        const float4 b = A[2 * nr + k];
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y; v[4] = b.z; v[5] = b.w;
+    }
+}
+
+__device__ __forceinline__ void row_st(float4 *A, uint32_t nr, uint32_t k, int r, const float *v)
+{
+    if (r == 0) {
+        A[k] = make_float4(v[0], v[1], v[2], v[3]);
+        reinterpret_cast<float2 *>(A + nr + k)[0] = make_float2(v[4], v[5]);
+    } else {
+        reinterpret_cast<float2 *>(A + nr + k)[1] = make_float2(v[0], v[1]);
+        A[2 * nr + k] = make_float4(v[2], v[3], v[4], v[5]);
+    }
+}
+
+// LDS: ring [TP_RING] float4 + [TP_RING] float2, partial sums [1024] float4 + [1024] float2
+#define TP1_LDS (TP_RING * 24 + TP_LANES * 24)
+__global__ void __launch_bounds__(TP_LANES) tp_sweep1_kernel(TaskDev S)
+{
+    extern __shared__ float4 tp1_lds[];
+    float4 *ring4 = tp1_lds;
+    float2 *ring2 = reinterpret_cast<float2 *>(tp1_lds + TP_RING);
+    float4 *red4 = reinterpret_cast<float4 *>(ring2 + TP_RING);
+    float2 *red2 = reinterpret_cast<float2 *>(red4 + TP_LANES);
+    const int r = blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    const uint32_t nr = (uint32_t)S.nrows;
+    if (S.nsteps <= 0) return;
+    const int last = S.nsteps - 1;
+    int2 cur = S.info[t];
+    int2 nxt = S.info[(size_t)min(1, last) * TP_LANES + t];
+    float m[36], gx[6], in[6];
+    auto fetch = [&](uint32_t s, int2 inf) {
+        if (inf.x >= 0) {
+            const float4 *b = S.blk + (size_t)s * 9 * TP_LANES + t;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                const float4 v = b[i * TP_LANES];
+                m[4 * i] = v.x; m[4 * i + 1] = v.y; m[4 * i + 2] = v.z; m[4 * i + 3] = v.w;
+            }
+            if (inf.x & TP_GLOBAL) row_ld(S.res, nr, (uint32_t)(inf.x & ~TP_GLOBAL), r, gx);
+        }
+        if (inf.y >= 0) row_ld(S.inp, nr, (uint32_t)(inf.y & ((1 << TP_ND_SHIFT) - 1)), r, in);
+    };
+    fetch(0, cur);
+    float pw[6];
+    int pk = -1;
+    for (int s = 0; s < S.nsteps; ++s) {
+        // everything in flight is needed now: this step's operands, the next step's
+        // positions and (issued before them) the stores of two steps back
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
+        const bool has = cur.x >= 0, own = cur.y >= 0;
+        float p[6];
+#pragma unroll
+        for (int a = 0; a < 6; ++a) p[a] = 0.f;
+        if (has) {
+            float x[6];
+            if (cur.x & TP_GLOBAL) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) x[c] = gx[c];
+            } else {
+                const float4 u = ring4[cur.x];
+                const float2 v = ring2[cur.x];
+                x[0] = u.x; x[1] = u.y; x[2] = u.z; x[3] = u.w; x[4] = v.x; x[5] = v.y;
+            }
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) p[a] += m[6 * a + c] * x[c];
+        }
+        float w[6];
+        if (own) {
+#pragma unroll
+            for (int a = 0; a < 6; ++a) w[a] = in[a] - p[a];
+        }
+        const int k = cur.y & ((1 << TP_ND_SHIFT) - 1), nd = cur.y >> TP_ND_SHIFT;
+        // global store of the previous step, then the operands of step s + 1 and positions of s + 2
+        if (pk >= 0) row_st(S.res, nr, (uint32_t)pk, r, pw);
+        if (s + 1 < S.nsteps) fetch((uint32_t)s + 1, nxt);
+        const int2 n2 = S.info[(size_t)min(s + 2, last) * TP_LANES + t];
+        if (has && !own) {
+            red4[t] = make_float4(p[0], p[1], p[2], p[3]);
+            red2[t] = make_float2(p[4], p[5]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+        if (own) {
+            for (int d0 = 1; d0 < nd; d0 += 4) {
+                float4 q4[4];
+                float2 q2[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t lu = t + (uint32_t)min(d0 + u, nd - 1);   // in-row lane (repeats masked below)
+                    q4[u] = red4[lu];
+                    q2[u] = red2[lu];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (d0 + u < nd) {
+                        w[0] -= q4[u].x; w[1] -= q4[u].y; w[2] -= q4[u].z; w[3] -= q4[u].w;
+                        w[4] -= q2[u].x; w[5] -= q2[u].y;
+                    }
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 6; ++a) pw[a] = w[a];
+            const uint32_t sl = (uint32_t)k & (TP_RING - 1);
+            ring4[sl] = make_float4(w[0], w[1], w[2], w[3]);
+            ring2[sl] = make_float2(w[4], w[5]);
+        }
+        pk = own ? k : -1;
+        __syncthreads();
+        cur = nxt;
+        nxt = n2;
+    }
+    if (pk >= 0) row_st(S.res, nr, (uint32_t)pk, r, pw);
+}
+
 // Z_P = L_PP^-1 of every lower-triangular diagonal block (row-major)
 __global__ void diag_inverse_kernel(int N, const float *Ldiag, float *Z)
 {
@@ -614,10 +748,10 @@ int k_sweep_setup(fsd_ctx *ctx)
     FSD_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     const int bytes = (int)sizeof(float4) * 3 * (TP_RING + TP_LANES);
     if (lds < bytes) return 0;
-    FSD_HIP(hipFuncSetAttribute((const void *)tp_sweep_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                bytes));
     FSD_HIP(hipFuncSetAttribute((const void *)tp_sweep_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 bytes));
+    FSD_HIP(hipFuncSetAttribute((const void *)tp_sweep1_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                TP1_LDS));
     ctx->tp_R = TP_RING;
     return 0;
 }
@@ -653,14 +787,18 @@ int k_precond_apply2(fsd_ctx *ctx, const float *b0, const float *b1, float *y0, 
                            ctx->d_perm, ctx->d_tp_Z, b0, b1, n_rhs, ctx->d_tp_finp);
         hipEvent_t es;
         prof_begin(ctx, PROF_SWEEP, &es);
-        if (n_rhs == 2) hipLaunchKernelGGL(tp_sweep_kernel<2>, dim3(1), dim3(TP_LANES), lds, ctx->stream, F);
-        else hipLaunchKernelGGL(tp_sweep_kernel<1>, dim3(1), dim3(TP_LANES), lds, ctx->stream, F);
+        if (n_rhs == 2 && ctx->tp_joint)
+            hipLaunchKernelGGL(tp_sweep_kernel<2>, dim3(1), dim3(TP_LANES), lds, ctx->stream, F);
+        else
+            hipLaunchKernelGGL(tp_sweep1_kernel, dim3(n_rhs), dim3(TP_LANES), TP1_LDS, ctx->stream, F);
         prof_end(ctx, PROF_SWEEP, es);
         hipLaunchKernelGGL(tp_mid_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->d_tp_fseq,
                            ctx->d_tp_bseq, ctx->d_tp_fres, ctx->d_tp_binp);
         prof_begin(ctx, PROF_SWEEP, &es);
-        if (n_rhs == 2) hipLaunchKernelGGL(tp_sweep_kernel<2>, dim3(1), dim3(TP_LANES), lds, ctx->stream, B);
-        else hipLaunchKernelGGL(tp_sweep_kernel<1>, dim3(1), dim3(TP_LANES), lds, ctx->stream, B);
+        if (n_rhs == 2 && ctx->tp_joint)
+            hipLaunchKernelGGL(tp_sweep_kernel<2>, dim3(1), dim3(TP_LANES), lds, ctx->stream, B);
+        else
+            hipLaunchKernelGGL(tp_sweep1_kernel, dim3(n_rhs), dim3(TP_LANES), TP1_LDS, ctx->stream, B);
         prof_end(ctx, PROF_SWEEP, es);
         hipLaunchKernelGGL(tp_post_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_bseq,
                            ctx->d_perm, ctx->d_tp_Z, ctx->d_tp_bres, n_rhs, y0, y1);

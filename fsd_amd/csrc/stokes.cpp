@@ -204,6 +204,9 @@ static int alloc_all(fsd_ctx *ctx)
         const char *lv = getenv("FSD_TRSV_LEVELS");
         if (lv && lv[0] == '1') ctx->tp_R = 0;
         else RC(k_sweep_setup(ctx));
+        // FSD_SWEEP_JOINT=1 runs both right-hand sides of a sweep in one workgroup
+        const char *jt = getenv("FSD_SWEEP_JOINT");
+        ctx->tp_joint = jt && jt[0] == '1';
     }
     CK(hipMalloc(&ctx->d_wave, sizeof(float) * 11 * (size_t)N));
     CK(hipMalloc(&ctx->d_rec, sizeof(float4) * 3 * (size_t)N));

@@ -288,8 +288,9 @@ def test_box_resize_shear_variant():
 @pytest.mark.parametrize("N,phi,seed", [(4096, 0.30, 12), (8192, 0.45, 3)])
 def test_precond_one_launch_matches_level_path(N, phi, seed, monkeypatch):
     """the single-workgroup sweeps (schedule of sweep_plan.cpp, LDS ring,
-    folded diagonal blocks) give G v of the one-launch-per-level sweeps, for
-    one and two right-hand sides (FP32, different rounding order)"""
+    folded diagonal blocks; one workgroup per right-hand side by default, or
+    both in one) give G v of the one-launch-per-level sweeps, for one and two
+    right-hand sides (FP32, different rounding order)"""
     import torch
     from fsd_amd.stokes import Stokes
     pos, L = packing(N, phi, seed)
@@ -298,19 +299,20 @@ def test_precond_one_launch_matches_level_path(N, phi, seed, monkeypatch):
     p4[:, :3] = pos
     dpos = torch.from_numpy(p4.reshape(-1)).cuda()
     sts, infos = [], []
-    for levels in ("0", "1"):
+    for levels, joint in (("0", "0"), ("0", "1"), ("1", "0")):
         monkeypatch.setenv("FSD_TRSV_LEVELS", levels)
+        monkeypatch.setenv("FSD_SWEEP_JOINT", joint)
         st = Stokes(N, L)
         st.build_nlist(dpos)
         infos.append(st.build_precond(dpos))
         sts.append(st)
-    print("one-launch", infos[0], "level path", infos[1])
-    assert infos[0]["sweep_steps"] > infos[0]["levels"] and infos[1]["sweep_steps"] == 0
+    print("one-launch", infos[0], "level path", infos[2])
+    assert infos[0]["sweep_steps"] > infos[0]["levels"] and infos[2]["sweep_steps"] == 0
     x = random_vec(6 * N, 5).astype(np.float32)
     y = [s.precond_apply(_t(x)).cpu().numpy() for s in sts]
-    e1 = rel_err(y[0], y[1])
     v = random_vec(17 * N, 6).astype(np.float32)
     z = [s.saddle_precond(_t(v)).cpu().numpy() for s in sts]
-    e2 = rel_err(z[0], z[1])
-    print("G v one-launch vs levels rel err", e1, "saddle precond", e2)
-    assert e1 < 1e-5 and e2 < 1e-5
+    for k in (0, 1):
+        e1, e2 = rel_err(y[k], y[2]), rel_err(z[k], z[2])
+        print("variant", k, "G v vs levels rel err", e1, "saddle precond", e2)
+        assert e1 < 1e-5 and e2 < 1e-5
