@@ -99,6 +99,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--restart", type=int, default=100, help="GMRES restart length")
+    ap.add_argument("--guess", default="extrapolate", choices=["previous", "extrapolate"],
+                    help="initial guess of each solve: the previous solution (the reference) or its extrapolation")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (diagnostics)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MB",
                     help="only run 5 device copies of MB megabytes (calibrates the HBM byte counters) and exit")
@@ -131,7 +133,8 @@ def main():
     print("# workload generated in %.1f s: N=%d phi=%.2f L=%.3f" % (time.time() - t, N, args.phi, L),
           file=sys.stderr, flush=True)
     shear_rate = 1.0
-    st = Stokes(N, L, xi=0.5, error=1e-3, gmres_tol=1e-4, gmres_true_tol=1e-4, dt=1e-3, gmres_restart=args.restart)
+    st = Stokes(N, L, xi=0.5, error=1e-3, gmres_tol=1e-4, gmres_true_tol=1e-4, dt=1e-3, gmres_restart=args.restart,
+                guess=args.guess)
     pse = st.pse_info()
     state = ParticleState(pos)
     dF = torch.from_numpy(F).cuda()
@@ -205,9 +208,12 @@ def main():
         "data": "synthetic (seeded packing and forces)",
         "config": {
             "workload": "N=32768 phi=0.45 periodic suspension, shear rate 1, random forces, T=0, xi=0.5, "
-                        "PSE error 1e-3, IC(0)-preconditioned GMRES(%d) to true residual 1e-4" % args.restart,
+                        "PSE error 1e-3, IC(0)-preconditioned GMRES(%d) to true residual 1e-4, initial guess: %s"
+                        % (args.restart, "linear extrapolation of the last two solutions" if args.guess == "extrapolate"
+                           else "previous solution"),
             "N": N, "phi": args.phi, "grid": [pse["Nx"], pse["Ny"], pse["Nz"]], "P": pse["P"],
             "preconditioner": "ichol",
+            "gmres_restart": args.restart, "initial_guess": args.guess,
             "gmres_iterations_mean": float(np.mean(iters)),
             "true_residual_last": last.get("true_residual"),
             "nnz": last.get("nnz"), "lub_pairs": last.get("lub_pairs"), "levels": last.get("levels"),

@@ -48,6 +48,7 @@ class FsdParams(C.Structure):
         ("r_buff", C.c_float),
         ("parity", C.c_int32),
         ("gmres_true_tol", C.c_float),
+        ("guess", C.c_int32),
     ]
 
 
@@ -78,6 +79,7 @@ class FsdSolveStats(C.Structure):
         ("lanczos_ff", C.c_int32),
         ("lanczos_nf", C.c_int32),
         ("sweep_steps", C.c_int32),
+        ("initial_residual", C.c_float),
     ]
 
     def as_dict(self):

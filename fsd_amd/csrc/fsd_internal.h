@@ -268,7 +268,10 @@ struct fsd_ctx {
     float *d_V = nullptr, *d_w = nullptr, *d_t = nullptr, *d_t2 = nullptr;
     float *d_Z = nullptr;         // M^-1 V of the right-preconditioned solve (allocated on first use)
     double *d_part = nullptr; int part_blocks = 0; double *h_part = nullptr;
+    int kmax = 64;                // most vectors of one multidot (restart + 1, at least 64)
     float *d_rhs = nullptr, *d_sol = nullptr;
+    float *d_sol_prev = nullptr;  // x_{n-2} when the initial guess extrapolates (guess = 1)
+    int64_t n_solved = 0;         // steps solved by this context
     float *d_force = nullptr;
     float *d_mask1 = nullptr, *d_mask2 = nullptr;
     float *d_ori_zero = nullptr;

@@ -86,8 +86,8 @@ int k_multidot(fsd_ctx *ctx, int64_t n, int K, const float *V, int64_t ldv, cons
 // w -= sum_k h[k] V_k
 __global__ void multi_axpy_kernel(int64_t n, int K, const float *V, int64_t ldv, const double *h, float *w)
 {
-    __shared__ float hs[64];
-    if (threadIdx.x < K) hs[threadIdx.x] = (float)h[threadIdx.x];
+    __shared__ float hs[256];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) hs[k] = (float)h[k];
     __syncthreads();
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -98,6 +98,7 @@ __global__ void multi_axpy_kernel(int64_t n, int K, const float *V, int64_t ldv,
 
 int k_multi_axpy(fsd_ctx *ctx, int64_t n, int K, const float *V, int64_t ldv, const double *h, float *w)
 {
+    if (K > 256) { ctx->err = "multi_axpy: more than 256 vectors"; return FSD_E_PARAM; }
     hipLaunchKernelGGL(multi_axpy_kernel, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, K, V, ldv, h, w);
     FSD_HIP(hipGetLastError());
     return 0;

@@ -77,31 +77,16 @@ void host_rcm(int N, const int *phead, const int *plist, int *perm)
 {
     Graph g{phead, plist};
     std::vector<char> mask(N, 1);
-    std::vector<int> deg(N, 0), order, lstart, comp;
-    std::vector<char> seen(N, 0);
+    std::vector<int> deg(N, 0), order, lstart;
+    for (int v = 0; v < N; ++v) deg[v] = phead[v + 1] - phead[v];
     order.reserve(N);
     int placed = 0;
     for (int s = 0; s < N && placed < N; ++s) {
         if (!mask[s]) continue;
         const int root = pseudo_peripheral(g, s, mask, order, lstart);
-        // degrees inside the component (unmasked neighbours, self loop included)
-        comp.clear();
-        comp.push_back(root);
-        seen[root] = 1;
-        for (size_t q = 0; q < comp.size(); ++q) {
-            const int v = comp[q];
-            int d = 0;
-            for (int e = g.xadj[v]; e < g.xadj[v + 1]; ++e) {
-                const int w = g.adj[e];
-                if (mask[w]) {
-                    ++d;
-                    if (!seen[w]) { seen[w] = 1; comp.push_back(w); }
-                }
-            }
-            deg[v] = d;
-        }
-        for (int v : comp) seen[v] = 0;
-        // Cuthill-McKee: children of each node appended in increasing degree
+        // Cuthill-McKee: children of each node appended in increasing degree.  The degree counts
+        // unmasked neighbours (self loop included); a component is unnumbered while it is ordered and
+        // has no neighbour outside it, so that is the full degree of the packed list (deg above).
         int *out = perm + placed;
         int n_out = 1;
         out[0] = root;

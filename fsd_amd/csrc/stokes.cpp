@@ -273,12 +273,18 @@ static int alloc_all(fsd_ctx *ctx)
     CK(hipMalloc(&ctx->d_w, sizeof(float) * n17));
     CK(hipMalloc(&ctx->d_t, sizeof(float) * n17));
     CK(hipMalloc(&ctx->d_t2, sizeof(float) * n17));
+    // dot partials of up to kmax vectors, then 2 kmax + 2 doubles of results (two projection passes, a norm)
     ctx->part_blocks = (int)((n17 + 4095) / 4096);
-    CK(hipMalloc(&ctx->d_part, sizeof(double) * ctx->part_blocks * 64 + sizeof(double) * 256));
-    CK(hipHostMalloc((void **)&ctx->h_part, sizeof(double) * 256, hipHostMallocDefault));
+    ctx->kmax = std::max(64, ctx->p.gmres_restart + 1);
+    CK(hipMalloc(&ctx->d_part, sizeof(double) * ((size_t)ctx->part_blocks * ctx->kmax + 2 * ctx->kmax + 2)));
+    CK(hipHostMalloc((void **)&ctx->h_part, sizeof(double) * (2 * ctx->kmax + 2), hipHostMallocDefault));
     CK(hipMalloc(&ctx->d_rhs, sizeof(float) * n17));
     CK(hipMalloc(&ctx->d_sol, sizeof(float) * n17));
     CK(hipMemset(ctx->d_sol, 0, sizeof(float) * n17));
+    if (ctx->p.guess == 1) {
+        CK(hipMalloc(&ctx->d_sol_prev, sizeof(float) * n17));
+        CK(hipMemset(ctx->d_sol_prev, 0, sizeof(float) * n17));
+    }
     CK(hipMalloc(&ctx->d_force, sizeof(float) * 6 * (size_t)N));
     // squirmer masks (Stokes.cc:1318-1346, index == tag)
     std::vector<float> m1(N), m2(N);
@@ -303,7 +309,7 @@ static void free_all(fsd_ctx *ctx)
                     ctx->d_ewald, ctx->d_res_dist, ctx->d_res_vals, ctx->d_pn, ctx->d_plist, ctx->d_pless,
                     ctx->d_perm, ctx->d_iperm, ctx->d_lrow, ctx->d_urow, ctx->d_lcol, ctx->d_ucol, ctx->d_upos,
                     ctx->d_Ablk, ctx->d_Lblk, ctx->d_Adiag, ctx->d_Ldiag, ctx->d_lev_f, ctx->d_lev_b, ctx->d_tmp6a,
-                    ctx->d_tmp6b, ctx->d_g12, ctx->d_V, ctx->d_w, ctx->d_t, ctx->d_t2, ctx->d_part, ctx->d_rhs, ctx->d_sol,
+                    ctx->d_tmp6b, ctx->d_g12, ctx->d_V, ctx->d_w, ctx->d_t, ctx->d_t2, ctx->d_part, ctx->d_rhs, ctx->d_sol, ctx->d_sol_prev,
                     ctx->d_force, ctx->d_mask1, ctx->d_mask2, ctx->d_tp_finfo, ctx->d_tp_binfo,
                     ctx->d_tp_fbsrc, ctx->d_tp_bbsrc, ctx->d_tp_fblk, ctx->d_tp_bblk,
                     ctx->d_tp_finp, ctx->d_tp_binp, ctx->d_tp_fres, ctx->d_tp_bres, ctx->d_tp_fseq, ctx->d_tp_bseq,
@@ -396,28 +402,46 @@ static void precond_plan(fsd_ctx *ctx)
     host_rcm(N, phead.data(), ctx->h_plist.data(), ctx->h_perm.data());
     h.iperm.resize(N);
     for (int P = 0; P < N; ++P) h.iperm[ctx->h_perm[P]] = P;
-    // lower and upper block adjacency on the permuted graph
+    // lower and upper block adjacency on the permuted graph: lower counts, then the rows filled
+    // (sorted) in parallel over contiguous ranges of P, then the transpose
     std::vector<int> &lrow = h.lrow, &lcol = h.lcol, &urow = h.urow, &ucol = h.ucol, &upos = h.upos;
     lrow.assign(N + 1, 0);
     urow.assign(N + 1, 0);
-    lcol.clear();
-    lcol.reserve(3 * (size_t)N);
-    int tmp[FSD_PSTRIDE];
+    const int nthr = N >= 8192 ? 4 : 1;
+    auto ranges = [&](auto &&fn) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nthr; ++t) th.emplace_back(fn, (int)((int64_t)N * t / nthr), (int)((int64_t)N * (t + 1) / nthr));
+        fn(0, (int)((int64_t)N / nthr));
+        for (auto &x : th) x.join();
+    };
+    ranges([&](int p0, int p1) {
+        for (int P = p0; P < p1; ++P) {
+            const int i = ctx->h_perm[P];
+            int nt = 0;
+            for (int s = phead[i]; s < phead[i + 1]; ++s) nt += h.iperm[ctx->h_plist[s]] < P;
+            lrow[P + 1] = nt;
+        }
+    });
+    for (int P = 0; P < N; ++P) lrow[P + 1] += lrow[P];
+    lcol.resize(lrow[N]);
+    ranges([&](int p0, int p1) {
+        for (int P = p0; P < p1; ++P) {
+            const int i = ctx->h_perm[P];
+            int *row = lcol.data() + lrow[P];
+            int nt = 0;
+            for (int s = phead[i]; s < phead[i + 1]; ++s) {
+                const int Q = h.iperm[ctx->h_plist[s]];
+                if (Q < P) row[nt++] = Q;
+            }
+            std::sort(row, row + nt);
+        }
+    });
     int64_t nnz_ref = 0;
     for (int P = 0; P < N; ++P) {
         const int i = ctx->h_perm[P];
-        lrow[P] = (int)lcol.size();
-        int nt = 0;
-        for (int s = phead[i]; s < phead[i + 1]; ++s) {
-            const int Q = h.iperm[ctx->h_plist[s]];
-            if (Q < P) tmp[nt++] = Q;
-        }
-        std::sort(tmp, tmp + nt);
-        lcol.insert(lcol.end(), tmp, tmp + nt);
-        for (int k = 0; k < nt; ++k) urow[tmp[k] + 1]++;
         nnz_ref += (ctx->h_pn[i] > 1) ? 36 * (int64_t)ctx->h_pn[i] : 6;
+        for (int e = lrow[P]; e < lrow[P + 1]; ++e) urow[lcol[e] + 1]++;
     }
-    lrow[N] = (int)lcol.size();
     for (int P = 0; P < N; ++P) urow[P + 1] += urow[P];
     ucol.resize(lcol.size());
     upos.resize(lcol.size());
@@ -555,16 +579,17 @@ static int gmres(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, floa
     const float ttol = ctx->p.gmres_true_tol;
     if (ttol <= 0.f) return gmres_core(ctx, pos, b, x, tol, maxit, R, false, st);
     int total = 0, restarts = 0, rc = 0;
-    float tres = 0.f, goal = ttol;
+    float tres = 0.f, goal = ttol, r0 = -1.f;
     for (int round = 0; round < 6; ++round) {
         rc = gmres_core(ctx, pos, b, x, goal, maxit - total, R, true, st);
         if (rc < 0) return rc;
+        if (st && round == 0) r0 = st->initial_residual;
         if (st) { total += st->iterations; restarts += st->restarts; }
         RC(true_residual(ctx, pos, b, x, &tres));
         if (tres <= ttol || total >= maxit) break;
         goal = goal * std::max(0.1f, 0.9f * ttol / tres);
     }
-    if (st) { st->iterations = total; st->restarts = restarts; st->true_residual = tres; }
+    if (st) { st->iterations = total; st->restarts = restarts; st->true_residual = tres; st->initial_residual = r0; }
     return (tres <= ttol) ? 0 : FSD_W_NOT_CONVERGED;
 }
 
@@ -579,7 +604,8 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
     float *V = ctx->d_V, *w = ctx->d_w, *t = ctx->d_t;
     if (right && !ctx->d_Z) CK(hipMalloc(&ctx->d_Z, sizeof(float) * n * R));
     float *Z = ctx->d_Z;
-    double *d_h = ctx->d_part + (int64_t)ctx->part_blocks * 64;   // 256 doubles of scratch after the partials
+    const int KM = ctx->kmax;
+    double *d_h = ctx->d_part + (int64_t)ctx->part_blocks * KM;   // 2 kmax + 2 doubles after the partials
     std::vector<double> H((size_t)(R + 1) * R, 0.0), cs(R), sn(R), g(R + 1), yv(R + 1);
     auto hostsync = [&](int count) -> int {
         CK(hipMemcpyAsync(ctx->h_part, d_h, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
@@ -605,6 +631,7 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
         RC(hostsync(1));
         const double beta = std::sqrt(ctx->h_part[0]);
         resid = beta;
+        if (st && iters == 0 && restarts == 0) st->initial_residual = (float)(beta / bn);
         if (resid <= target || iters >= maxit || beta == 0.0) break;
         RC(k_axpby(ctx, n, (float)(-1.0 / beta), r, 0.0f, nullptr, V));
         std::fill(g.begin(), g.end(), 0.0);
@@ -627,14 +654,14 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
             // CGS2: two passes of h = V^T w, w -= V h
             RC(k_multidot(ctx, n, i + 1, V, n, w, d_h));
             RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h, w));
-            RC(k_multidot(ctx, n, i + 1, V, n, w, d_h + 64));
-            RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h + 64, w));
-            RC(k_nrm2(ctx, n, w, d_h + 128));
+            RC(k_multidot(ctx, n, i + 1, V, n, w, d_h + KM));
+            RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h + KM, w));
+            RC(k_nrm2(ctx, n, w, d_h + 2 * KM));
             prof_end(ctx, PROF_GMRES_VEC, ev);
-            CK(hipMemcpyAsync(ctx->h_part, d_h, sizeof(double) * 129, hipMemcpyDeviceToHost, ctx->stream));
+            CK(hipMemcpyAsync(ctx->h_part, d_h, sizeof(double) * (2 * KM + 1), hipMemcpyDeviceToHost, ctx->stream));
             CK(hipStreamSynchronize(ctx->stream));
-            const double hn = std::sqrt(ctx->h_part[128]);
-            for (int k = 0; k <= i; ++k) H[(size_t)k * R + i] = ctx->h_part[k] + ctx->h_part[64 + k];
+            const double hn = std::sqrt(ctx->h_part[2 * KM]);
+            for (int k = 0; k <= i; ++k) H[(size_t)k * R + i] = ctx->h_part[k] + ctx->h_part[KM + k];
             H[(size_t)(i + 1) * R + i] = hn;
             RC(k_axpby(ctx, n, hn > 0.0 ? (float)(1.0 / hn) : 0.0f, w, 0.0f, nullptr, vn));
             for (int k = 0; k < i; ++k) {
@@ -876,7 +903,7 @@ static int bro_rfd(fsd_ctx *ctx, const float4 *pos, uint32_t timestep, float *dr
 static int true_residual(fsd_ctx *ctx, const float4 *pos, const float *b, const float *x, float *out)
 {
     const int64_t n = 17 * (int64_t)ctx->N;
-    double *d_h = ctx->d_part + (int64_t)ctx->part_blocks * 64;
+    double *d_h = ctx->d_part + (int64_t)ctx->part_blocks * ctx->kmax;
     RC(k_saddle_multiply(ctx, pos, x, ctx->d_t));
     RC(k_axpby(ctx, n, 1.0f, b, -1.0f, ctx->d_t, ctx->d_t));
     RC(k_nrm2(ctx, n, ctx->d_t, d_h));
@@ -898,7 +925,7 @@ int fsd_create(const fsd_params *p, fsd_ctx **out)
     fsd_params &q = ctx->p;
     if (q.gmres_tol <= 0.f) q.gmres_tol = q.error;
     if (q.gmres_restart <= 0) q.gmres_restart = 50;
-    if (q.gmres_restart > 63) q.gmres_restart = 63;
+    if (q.gmres_restart > 255) q.gmres_restart = 255;
     if (q.gmres_maxiter <= 0) q.gmres_maxiter = 1000;
     if (q.rlub <= 0.f) q.rlub = 4.0f;
     if (q.rp <= 0.f) q.rp = 2.1f;
@@ -1380,7 +1407,18 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     fsd_solve_stats st;
     memset(&st, 0, sizeof st);
     st.true_residual = -1.f;
+    // initial guess: the previous step's solution (Integrator.cu:777), or its linear extrapolation
+    if (ctx->p.guess == 1) {
+        if (ctx->n_solved >= 2) {   // d_sol = x_{n-1}, d_sol_prev = x_{n-2}
+            RC(k_copy(ctx, n17, ctx->d_sol, ctx->d_t));
+            RC(k_axpby(ctx, n17, 2.0f, ctx->d_sol, -1.0f, ctx->d_sol_prev, ctx->d_sol));
+            RC(k_copy(ctx, n17, ctx->d_t, ctx->d_sol_prev));
+        } else {
+            RC(k_copy(ctx, n17, ctx->d_sol, ctx->d_sol_prev));
+        }
+    }
     int rc = gmres(ctx, pos, ctx->d_rhs, ctx->d_sol, ctx->p.gmres_tol, ctx->p.gmres_maxiter, ctx->p.gmres_restart, &st);
+    ctx->n_solved++;
     if (rc < 0) return rc;
     RC(k_axpby(ctx, 6 * (int64_t)N, 1.0f, ctx->d_sol + 11 * (int64_t)N, 1.0f, d_vel11, d_vel11));
     if (ctx->p.period > 0 && (int)(timestep + 1) % ctx->p.period == 0) {

@@ -37,7 +37,7 @@ class Stokes:
                  ndsr=1e-1, kappa=1 / 0.05, k_n=1e4, beta_AF=0.0, epsq=1e-5, sqm_B1=0.0, sqm_B2=0.0,
                  N_mix=1, coef_B1_mask=1.0, coef_B2_mask=1.0, rot_diff=0.0, T_ext=0.0, omega_ext=0.0,
                  dt=1e-3, max_strain=0.5, gmres_tol=None, gmres_restart=50, gmres_maxiter=1000,
-                 parity=True, gmres_true_tol=0.0):
+                 parity=True, gmres_true_tol=0.0, guess="previous"):
         self.lib = _lib.load()
         self.torch = _torch()
         L = box if not np.isscalar(box) else (box, box, box, 0.0, 0.0, 0.0)
@@ -58,6 +58,9 @@ class Stokes:
         p.rlub = 4.0; p.rp = 2.1; p.r_buff = 0.4
         p.parity = 1 if parity else 0
         p.gmres_true_tol = float(gmres_true_tol)
+        if guess not in ("previous", "extrapolate"):
+            raise ValueError("guess must be 'previous' (the reference) or 'extrapolate'")
+        p.guess = 1 if guess == "extrapolate" else 0
         self.params = p
         self.N = int(N)
         self.fileprefix = fileprefix

@@ -66,6 +66,9 @@ typedef struct fsd_params {
     float r_buff;                    /* < 0: 0.4 */
     int32_t parity;                  /* 1: reproduce the reference's k-vector constant (Helper_Mobility.cu:389-391) */
     float gmres_true_tol;            /* > 0: also require ||b - A x|| / ||b|| <= this (extra mat-vecs) */
+    int32_t guess;                   /* initial guess of a step's solve: 0 = the previous step's solution
+                                        (the reference, Integrator.cu:777); 1 = linear extrapolation
+                                        2 x_{n-1} - x_{n-2} of the last two steps' solutions */
 } fsd_params;
 
 /* Derived PSE parameters (Stokes.cc:131-261). */
@@ -92,6 +95,7 @@ typedef struct fsd_solve_stats {
     int32_t lanczos_ff;              /* Krylov size of the far-field Brownian sampler (T > 0) */
     int32_t lanczos_nf;              /* Krylov size of the near-field Brownian sampler (T > 0) */
     int32_t sweep_steps;             /* steps of the one-launch triangular sweeps (0: one launch per level) */
+    float initial_residual;          /* ||b - A x0|| / ||b|| (true residual) or its preconditioned form */
 } fsd_solve_stats;
 
 /* ---- lifecycle: Stokes::Stokes + setParams + setSparseMath + AllocateWorkSpaces
