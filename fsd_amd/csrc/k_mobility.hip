@@ -767,15 +767,19 @@ struct RealParams {
 // Mobility_RealSpace_kernel (Mobility.cu:1046-1226) + LinearCombination/Add4
 // + Mobility_D2WE_kernel + Saddle_MakeGeneralizedU_kernel (+ the saddle's
 // x_U term), G = 16 lanes per particle.
+// Particles are taken in cell order (`order`, the neighbour list's cell-sorted
+// particles) with each XCD given one contiguous range of workgroups, so the
+// neighbours gathered by concurrent particles share that XCD's L2.
 template <int G>
 __global__ void __launch_bounds__(256) realspace_kernel(RealParams rp, const float4 *pos, const float4 *rec,
                                                         const float4 *tab, const int *nneigh, const int *nlist,
                                                         int maxn, int N, const float *wave, const float *addU,
-                                                        int use_real, float *gU)
+                                                        int use_real, float *gU, const int *order)
 {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = gid / G, lane = gid % G;
-    const bool active = i < N;
+    const int gid = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int slot = gid / G, lane = gid % G;
+    const bool active = slot < N;
+    const int i = (active && order) ? order[slot] : slot;
     float u[3] = {0.f, 0.f, 0.f};
     float D[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (active && use_real) {
@@ -966,11 +970,14 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
     RealParams rp;
     rp.b = ctx->box; rp.rcut = ps.rcut; rp.dr = 0.001f; rp.self_x = ps.self_x; rp.self_y = ps.self_y;
     rp.ewald_n = ps.ewald_n;
+    // the processing order is the neighbour list's cell list (identity without a list: wave part only)
+    if (!ctx->nlist_valid && (parts & 1)) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    const int *order = ctx->nlist_valid ? ctx->d_cell_list : nullptr;
     hipEvent_t ev;
     prof_begin(ctx, PROF_REAL, &ev);
     hipLaunchKernelGGL(realspace_kernel<16>, dim3(nblk((int64_t)N * 16, 256)), dim3(256), 0, ctx->stream, rp, pos,
                        rec, reinterpret_cast<const float4 *>(ctx->d_ewald), ctx->d_nneigh, ctx->d_nlist, ctx->maxn, N,
-                       wave, addU, (parts & 1) ? 1 : 0, gU);
+                       wave, addU, (parts & 1) ? 1 : 0, gU, order);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_REAL, ev);
     return 0;
