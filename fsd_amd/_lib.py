@@ -118,6 +118,8 @@ SIGNATURES = [
     ("fsd_saddle_solve", _I, [_P, _P, _P, _P, _I, C.POINTER(FsdSolveStats)]),
     ("fsd_force_law", _I, [_P, _P, _F, _P]),
     ("fsd_euler_shear", _I, [_P, _P, _P, _P, _P, _P, _P, _P, _F, _F]),
+    ("fsd_rk2_stage1", _I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _F]),
+    ("fsd_rk2_stage2", _I, [_P, _F, _P, _P, _F, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F, _F, _F]),
     ("fsd_set_noise", _I, [_P, _P, _P, _P, _P]),
     ("fsd_brownian_farfield", _I, [_P, _P, C.c_uint32, _P]),
     ("fsd_brownian_nearfield", _I, [_P, _P, C.c_uint32, _P]),

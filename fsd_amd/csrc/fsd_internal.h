@@ -339,6 +339,11 @@ int k_export_csr(fsd_ctx *ctx, const float4 *pos, int32_t *h_rowptr, int32_t *h_
                  int32_t *h_prow, int32_t *h_pcol, float *h_L, int64_t cap);
 int k_force_law(fsd_ctx *ctx, const float4 *pos, float T_ext, float *F);
 int k_strain_rhs(fsd_ctx *ctx, float *rhs, float shear_rate, const float *ori, const float *force);
+int k_rk2_stage1(fsd_ctx *ctx, const float4 *pos_in, const float *ori_in, float4 *pos_out, float *ori_out,
+                 float4 *pos_gb, const float *vel, const float *mask1, const float *noise, float dt, float shear_rate);
+int k_rk2_stage2(fsd_ctx *ctx, float c1, const float4 *pos1, const float *ori1, float c2, const float4 *pos2,
+                 const float *ori2, float4 *pos_out, float *ori_out, float4 *pos_gb, const float *vel,
+                 const float *mask1, const float *noise, int *image, float c3, float dt, float shear_rate);
 int k_euler_shear(fsd_ctx *ctx, float4 *pos, float *ori, float4 *pos_gb, int *image, const float *vel,
                   const float *mask1, const float *noise, float dt, float shear_rate, float B1);
 // vector helpers

@@ -227,6 +227,117 @@ __global__ void euler_shear_kernel(int N, float4 *pos, float *ori, float4 *pos_g
     }
 }
 
+// Director rotation pv <- q pv q* with the quaternion of angular velocity w over
+// time tau (Integrator.cu:278-286, 312-321), renormalised
+__device__ inline void rotate_director(float wx, float wy, float wz, float tau, float &ox, float &oy, float &oz)
+{
+    float q0 = 1.f, qx = 0.f, qy = 0.f, qz = 0.f;
+    const float aw = sqrtf(wx * wx + wy * wy + wz * wz);
+    if (aw > 1e-6f) {
+        const float ang = aw * tau;
+        q0 = cosf(ang / 2.f);
+        const float s = sinf(ang / 2.f) / aw;
+        qx = wx * s; qy = wy * s; qz = wz * s;
+    }
+    const float qq = qx * qx + qy * qy + qz * qz, qp = qx * ox + qy * oy + qz * oz;
+    const float cx = qy * oz - qz * oy, cy = qz * ox - qx * oz, cz = qx * oy - qy * ox;
+    const float nx = (q0 * q0 - qq) * ox + 2.f * qp * qx + 2.f * q0 * cx;
+    const float ny = (q0 * q0 - qq) * oy + 2.f * qp * qy + 2.f * q0 * cy;
+    const float nz = (q0 * q0 - qq) * oz + 2.f * qp * qz + 2.f * q0 * cz;
+    const float m = sqrtf(nx * nx + ny * ny + nz * nz);
+    ox = nx / m; oy = ny / m; oz = nz / m;
+}
+
+// Integrator_ExplicitEuler1_Shear_kernel (Integrator.cu:232-326): first stage of
+// the second-order Runge-Kutta scheme.  The unwrapped position advances half a
+// step, the periodic position a full step without wrapping (the second stage
+// averages it), pos.w is kept.
+__global__ void rk2_stage1_kernel(int N, const float4 *pos_in, const float *ori_in, float4 *pos_out, float *ori_out,
+                                  float4 *pos_gb, const float *vel, float B1, const float *mask1, const float *noise,
+                                  float dt, float shear_rate)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float4 p4 = pos_in[i];
+    float px = p4.x, py = p4.y, pz = p4.z;
+    float vx = vel[6 * (int64_t)i], vy = vel[6 * (int64_t)i + 1], vz = vel[6 * (int64_t)i + 2];
+    float wx = vel[6 * (int64_t)i + 3], wy = vel[6 * (int64_t)i + 4], wz = vel[6 * (int64_t)i + 5];
+    vx += shear_rate * py;
+    wz -= shear_rate / 2.f;
+    if (noise) { wx += noise[3 * i]; wy += noise[3 * i + 1]; wz += noise[3 * i + 2]; }
+    float ox = ori_in[3 * i], oy = ori_in[3 * i + 1], oz = ori_in[3 * i + 2];
+    const float act = -2.f / 3.f * B1 * (mask1 ? mask1[i] : 1.f);
+    vx -= act * ox; vy -= act * oy; vz -= act * oz;
+    float4 g = pos_gb[i];
+    g.x += (vx - shear_rate * py + shear_rate * g.y) * 0.5f * dt;
+    g.y += vy * 0.5f * dt;
+    g.z += vz * 0.5f * dt;
+    pos_gb[i] = g;
+    px += vx * dt; py += vy * dt; pz += vz * dt;
+    pos_out[i] = make_float4(px, py, pz, p4.w);
+    rotate_director(wx, wy, wz, dt, ox, oy, oz);
+    ori_out[3 * i] = ox; ori_out[3 * i + 1] = oy; ori_out[3 * i + 2] = oz;
+}
+
+// Integrator_RK_Shear_kernel (Integrator.cu:329-438): second stage.  Position
+// and director are the combinations c1 * state1 + c2 * state2 (the director
+// renormalised), moved by c3 * dt and wrapped; the unwrapped position advances
+// half a step; pos.w is written as 0 as the reference does.
+__global__ void rk2_stage2_kernel(int N, float c1, const float4 *pos1, const float *ori1, float c2, const float4 *pos2,
+                                  const float *ori2, float4 *pos_out, float *ori_out, float4 *pos_gb, const float *vel,
+                                  float B1, const float *mask1, const float *noise, int *image, BoxF b, float c3,
+                                  float dt, float shear_rate)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float4 a4 = pos1[i], b4 = pos2[i];
+    float px = c1 * a4.x + c2 * b4.x, py = c1 * a4.y + c2 * b4.y, pz = c1 * a4.z + c2 * b4.z;
+    float vx = vel[6 * (int64_t)i], vy = vel[6 * (int64_t)i + 1], vz = vel[6 * (int64_t)i + 2];
+    float wx = vel[6 * (int64_t)i + 3], wy = vel[6 * (int64_t)i + 4], wz = vel[6 * (int64_t)i + 5];
+    vx += shear_rate * py;
+    wz -= shear_rate / 2.f;
+    if (noise) { wx += noise[3 * i]; wy += noise[3 * i + 1]; wz += noise[3 * i + 2]; }
+    float ox = ori1[3 * i] * c1 + ori2[3 * i] * c2;
+    float oy = ori1[3 * i + 1] * c1 + ori2[3 * i + 1] * c2;
+    float oz = ori1[3 * i + 2] * c1 + ori2[3 * i + 2] * c2;
+    const float m0 = sqrtf(ox * ox + oy * oy + oz * oz);
+    ox /= m0; oy /= m0; oz /= m0;
+    const float act = -2.f / 3.f * B1 * (mask1 ? mask1[i] : 1.f);
+    vx -= act * ox; vy -= act * oy; vz -= act * oz;
+    float4 g = pos_gb[i];
+    g.x += (vx - shear_rate * py + shear_rate * g.y) * 0.5f * dt;
+    g.y += vy * 0.5f * dt;
+    g.z += vz * 0.5f * dt;
+    pos_gb[i] = g;
+    px += vx * dt * c3; py += vy * dt * c3; pz += vz * dt * c3;
+    int ix = image[3 * i], iy = image[3 * i + 1], iz = image[3 * i + 2];
+    box_wrap(b, px, py, pz, ix, iy, iz);
+    pos_out[i] = make_float4(px, py, pz, 0.f);
+    image[3 * i] = ix; image[3 * i + 1] = iy; image[3 * i + 2] = iz;
+    rotate_director(wx, wy, wz, dt * c3, ox, oy, oz);
+    ori_out[3 * i] = ox; ori_out[3 * i + 1] = oy; ori_out[3 * i + 2] = oz;
+}
+
+int k_rk2_stage1(fsd_ctx *ctx, const float4 *pos_in, const float *ori_in, float4 *pos_out, float *ori_out,
+                 float4 *pos_gb, const float *vel, const float *mask1, const float *noise, float dt, float shear_rate)
+{
+    hipLaunchKernelGGL(rk2_stage1_kernel, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, pos_in, ori_in,
+                       pos_out, ori_out, pos_gb, vel, ctx->p.sqm_B1, mask1, noise, dt, shear_rate);
+    FSD_HIP(hipGetLastError());
+    return 0;
+}
+
+int k_rk2_stage2(fsd_ctx *ctx, float c1, const float4 *pos1, const float *ori1, float c2, const float4 *pos2,
+                 const float *ori2, float4 *pos_out, float *ori_out, float4 *pos_gb, const float *vel,
+                 const float *mask1, const float *noise, int *image, float c3, float dt, float shear_rate)
+{
+    hipLaunchKernelGGL(rk2_stage2_kernel, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, ctx->N, c1, pos1, ori1,
+                       c2, pos2, ori2, pos_out, ori_out, pos_gb, vel, ctx->p.sqm_B1, mask1, noise, image, ctx->box, c3,
+                       dt, shear_rate);
+    FSD_HIP(hipGetLastError());
+    return 0;
+}
+
 int k_euler_shear(fsd_ctx *ctx, float4 *pos, float *ori, float4 *pos_gb, int *image, const float *vel,
                   const float *mask1, const float *noise, float dt, float shear_rate, float B1)
 {

@@ -1359,6 +1359,31 @@ int fsd_euler_shear(fsd_ctx *ctx, float *d_pos4, float *d_ori3, float *d_pos_gb4
     return 0;
 }
 
+// The second-order Runge-Kutta stages (Integrator.cu:232-438; never called by the reference's step)
+int fsd_rk2_stage1(fsd_ctx *ctx, const float *d_pos_in4, const float *d_ori_in3, float *d_pos_out4, float *d_ori_out3,
+                   float *d_pos_gb4, const float *d_vel6, const float *d_mask1, const float *d_noise3, float dt,
+                   float shear_rate)
+{
+    RC(k_rk2_stage1(ctx, reinterpret_cast<const float4 *>(d_pos_in4), d_ori_in3, reinterpret_cast<float4 *>(d_pos_out4),
+                    d_ori_out3, reinterpret_cast<float4 *>(d_pos_gb4), d_vel6, d_mask1 ? d_mask1 : ctx->d_mask1,
+                    d_noise3, dt, shear_rate));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int fsd_rk2_stage2(fsd_ctx *ctx, float coef1, const float *d_pos_in1_4, const float *d_ori_in1_3, float coef2,
+                   const float *d_pos_in2_4, const float *d_ori_in2_3, float *d_pos_out4, float *d_ori_out3,
+                   float *d_pos_gb4, const float *d_vel6, const float *d_mask1, const float *d_noise3,
+                   int32_t *d_image3, float coef3, float dt, float shear_rate)
+{
+    RC(k_rk2_stage2(ctx, coef1, reinterpret_cast<const float4 *>(d_pos_in1_4), d_ori_in1_3, coef2,
+                    reinterpret_cast<const float4 *>(d_pos_in2_4), d_ori_in2_3, reinterpret_cast<float4 *>(d_pos_out4),
+                    d_ori_out3, reinterpret_cast<float4 *>(d_pos_gb4), d_vel6, d_mask1 ? d_mask1 : ctx->d_mask1,
+                    d_noise3, d_image3, coef3, dt, shear_rate));
+    CK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 // Stokes::integrateStepOne -> Stokes_StepOne (Stokes.cc:1197-1683, Stokes.cu:119-273)
 int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, float *d_ori3, float *d_pos_gb4,
              int32_t *d_image3, const float *d_force6, const float *d_noise_ang3, float *d_vel11,

@@ -163,6 +163,26 @@ class Stokes:
     def _out(self, n):
         return self.torch.zeros(n, dtype=self.torch.float32, device="cuda")
 
+    # ---- second-order Runge-Kutta stages (Integrator.cu:232-438; the reference's step never calls them)
+    def rk2_stage1(self, pos4, ori3, pos_gb4, vel6, dt, shear_rate, noise3=None):
+        """ExplicitEuler1: returns (pos_out4, ori_out3); pos_gb4 is advanced in place"""
+        po = self._out(4 * self.N)
+        oo = self._out(3 * self.N)
+        self._sync()
+        self._check(self.lib.fsd_rk2_stage1(self.ctx, _ptr(pos4), _ptr(ori3), _ptr(po), _ptr(oo), _ptr(pos_gb4),
+                                            _ptr(vel6), None, _ptr(noise3), float(dt), float(shear_rate)))
+        return po, oo
+
+    def rk2_stage2(self, c1, pos1, ori1, c2, pos2, ori2, pos_gb4, vel6, image3, c3, dt, shear_rate, noise3=None):
+        """RK_Shear: returns (pos_out4, ori_out3); pos_gb4 and image3 are updated in place"""
+        po = self._out(4 * self.N)
+        oo = self._out(3 * self.N)
+        self._sync()
+        self._check(self.lib.fsd_rk2_stage2(self.ctx, float(c1), _ptr(pos1), _ptr(ori1), float(c2), _ptr(pos2),
+                                            _ptr(ori2), _ptr(po), _ptr(oo), _ptr(pos_gb4), _ptr(vel6), None,
+                                            _ptr(noise3), _ptr(image3), float(c3), float(dt), float(shear_rate)))
+        return po, oo
+
     def rfu(self, pos, U):
         F = self._out(6 * self.N)
         self._sync()

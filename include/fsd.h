@@ -212,12 +212,28 @@ int fsd_euler_shear(fsd_ctx *ctx, float *d_pos4, float *d_ori3, float *d_pos_gb4
                     const float *d_vel6, const float *d_mask1, const float *d_noise3,
                     float dt, float shear_rate);
 
+/* Second-order Runge-Kutta stages, present in the reference but never called by
+ * its step.  Stage 1 = Integrator_ExplicitEuler1_Shear_kernel (Integrator.cu:232-326):
+ * pos_out = pos_in + u dt (no wrap, pos.w kept), director rotated over dt,
+ * pos_gb advanced by u dt / 2.  Stage 2 = Integrator_RK_Shear_kernel
+ * (Integrator.cu:329-438): position and director = coef1 * state1 + coef2 *
+ * state2 (director renormalised), moved over coef3 * dt and wrapped (image
+ * updated, pos.w written 0), pos_gb advanced by u dt / 2.  d_mask1 nullable (the
+ * context's B1 mask), d_noise3 nullable. */
+int fsd_rk2_stage1(fsd_ctx *ctx, const float *d_pos_in4, const float *d_ori_in3, float *d_pos_out4, float *d_ori_out3,
+                   float *d_pos_gb4, const float *d_vel6, const float *d_mask1, const float *d_noise3, float dt,
+                   float shear_rate);
+int fsd_rk2_stage2(fsd_ctx *ctx, float coef1, const float *d_pos_in1_4, const float *d_ori_in1_3, float coef2,
+                   const float *d_pos_in2_4, const float *d_ori_in2_3, float *d_pos_out4, float *d_ori_out3,
+                   float *d_pos_gb4, const float *d_vel6, const float *d_mask1, const float *d_noise3,
+                   int32_t *d_image3, float coef3, float dt, float shear_rate);
+
 /* ---- timing of the hot kernels (HIP events on the context's stream) ---- */
 /* enable: bit mask of kernel classes to time (bit k = k-th name below, -1 = all, 0 = off);
  * an event pair is recorded around every launch of a selected class */
 int fsd_profile_enable(fsd_ctx *ctx, int32_t enable);
 /* name: "spread", "green", "contract", "fft", "realspace", "rfu", "trsv", "ichol",
- * "gmres_vec", "nlist", "precond_build"; returns launches and total milliseconds */
+ * "gmres_vec", "nlist", "precond_build", "sweep"; returns launches and total milliseconds */
 int fsd_profile_get(fsd_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
 int fsd_profile_reset(fsd_ctx *ctx);
 

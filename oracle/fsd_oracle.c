@@ -1260,3 +1260,79 @@ void orc_euler_shear(int N, float *pos4, float *ori3, float *pos_gb4, int *image
         for (int a = 0; a < 3; ++a) ori3[3 * i + a] = (float)(pn[a] / m);
     }
 }
+
+/* director rotation pv <- q pv q* over angle |w| tau (Integrator.cu:278-286, 312-321) */
+static void orc_rotate(const double w[3], double tau, double pv[3])
+{
+    double q0 = 1.0, qv[3] = {0, 0, 0};
+    const double aw = sqrt(dot3(w, w));
+    if (aw > 1e-6) {
+        const double ang = aw * tau;
+        q0 = cos(ang / 2.);
+        for (int a = 0; a < 3; ++a) qv[a] = w[a] / aw * sin(ang / 2.);
+    }
+    const double qq = dot3(qv, qv), qp = dot3(qv, pv);
+    const double cr[3] = {qv[1] * pv[2] - qv[2] * pv[1], qv[2] * pv[0] - qv[0] * pv[2], qv[0] * pv[1] - qv[1] * pv[0]};
+    double pn[3];
+    for (int a = 0; a < 3; ++a) pn[a] = (q0 * q0 - qq) * pv[a] + 2. * qp * qv[a] + 2. * q0 * cr[a];
+    const double m = sqrt(dot3(pn, pn));
+    for (int a = 0; a < 3; ++a) pv[a] = pn[a] / m;
+}
+
+/* Integrator_ExplicitEuler1_Shear_kernel (Integrator.cu:232-326) */
+void orc_rk2_stage1(int N, const float *pos_in4, const float *ori_in3, float *pos_out4, float *ori_out3,
+                    float *pos_gb4, const double *vel6, double B1, const float *mask1, const float *noise3,
+                    double dt, double shear_rate)
+{
+    for (int i = 0; i < N; ++i) {
+        double p[3] = {pos_in4[4 * i], pos_in4[4 * i + 1], pos_in4[4 * i + 2]};
+        double v[3] = {vel6[6 * i], vel6[6 * i + 1], vel6[6 * i + 2]};
+        double w[3] = {vel6[6 * i + 3], vel6[6 * i + 4], vel6[6 * i + 5]};
+        v[0] += shear_rate * p[1];
+        w[2] -= shear_rate / 2.;
+        if (noise3) for (int a = 0; a < 3; ++a) w[a] += noise3[3 * i + a];
+        double pv[3] = {ori_in3[3 * i], ori_in3[3 * i + 1], ori_in3[3 * i + 2]};
+        const double m1 = mask1 ? mask1[i] : 1.0;
+        for (int a = 0; a < 3; ++a) v[a] -= (-2. / 3. * B1 * m1) * pv[a];
+        pos_gb4[4 * i] += (float)((v[0] - shear_rate * p[1] + shear_rate * pos_gb4[4 * i + 1]) * 0.5 * dt);
+        pos_gb4[4 * i + 1] += (float)(v[1] * 0.5 * dt);
+        pos_gb4[4 * i + 2] += (float)(v[2] * 0.5 * dt);
+        for (int a = 0; a < 3; ++a) pos_out4[4 * i + a] = (float)(p[a] + v[a] * dt);
+        pos_out4[4 * i + 3] = pos_in4[4 * i + 3];
+        orc_rotate(w, dt, pv);
+        for (int a = 0; a < 3; ++a) ori_out3[3 * i + a] = (float)pv[a];
+    }
+}
+
+/* Integrator_RK_Shear_kernel (Integrator.cu:329-438) */
+void orc_rk2_stage2(int N, double c1, const float *pos1, const float *ori1, double c2, const float *pos2,
+                    const float *ori2, float *pos_out4, float *ori_out3, float *pos_gb4, const double *vel6,
+                    double B1, const float *mask1, const float *noise3, int *image3, const orc_box *b, double c3,
+                    double dt, double shear_rate)
+{
+    for (int i = 0; i < N; ++i) {
+        double p[3], pv[3];
+        for (int a = 0; a < 3; ++a) {
+            p[a] = c1 * pos1[4 * i + a] + c2 * pos2[4 * i + a];
+            pv[a] = ori1[3 * i + a] * c1 + ori2[3 * i + a] * c2;
+        }
+        const double m0 = sqrt(dot3(pv, pv));
+        for (int a = 0; a < 3; ++a) pv[a] /= m0;
+        double v[3] = {vel6[6 * i], vel6[6 * i + 1], vel6[6 * i + 2]};
+        double w[3] = {vel6[6 * i + 3], vel6[6 * i + 4], vel6[6 * i + 5]};
+        v[0] += shear_rate * p[1];
+        w[2] -= shear_rate / 2.;
+        if (noise3) for (int a = 0; a < 3; ++a) w[a] += noise3[3 * i + a];
+        const double m1 = mask1 ? mask1[i] : 1.0;
+        for (int a = 0; a < 3; ++a) v[a] -= (-2. / 3. * B1 * m1) * pv[a];
+        pos_gb4[4 * i] += (float)((v[0] - shear_rate * p[1] + shear_rate * pos_gb4[4 * i + 1]) * 0.5 * dt);
+        pos_gb4[4 * i + 1] += (float)(v[1] * 0.5 * dt);
+        pos_gb4[4 * i + 2] += (float)(v[2] * 0.5 * dt);
+        float np[3] = {(float)(p[0] + v[0] * dt * c3), (float)(p[1] + v[1] * dt * c3), (float)(p[2] + v[2] * dt * c3)};
+        orc_wrap(b, np, image3 + 3 * i);
+        for (int a = 0; a < 3; ++a) pos_out4[4 * i + a] = np[a];
+        pos_out4[4 * i + 3] = 0.f;
+        orc_rotate(w, dt * c3, pv);
+        for (int a = 0; a < 3; ++a) ori_out3[3 * i + a] = (float)pv[a];
+    }
+}

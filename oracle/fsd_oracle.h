@@ -129,6 +129,13 @@ void orc_force_law(int N, const float *pos4, const orc_box *b, const int *nneigh
 void orc_euler_shear(int N, float *pos4, float *ori3, float *pos_gb4, int *image3, const orc_box *b,
                      const double *vel6, double B1, const float *mask1, const float *noise3,
                      double dt, double shear_rate);
+void orc_rk2_stage1(int N, const float *pos_in4, const float *ori_in3, float *pos_out4, float *ori_out3,
+                    float *pos_gb4, const double *vel6, double B1, const float *mask1, const float *noise3,
+                    double dt, double shear_rate);
+void orc_rk2_stage2(int N, double c1, const float *pos1, const float *ori1, double c2, const float *pos2,
+                    const float *ori2, float *pos_out4, float *ori_out3, float *pos_gb4, const double *vel6,
+                    double B1, const float *mask1, const float *noise3, int *image3, const orc_box *b, double c3,
+                    double dt, double shear_rate);
 
 #ifdef __cplusplus
 }

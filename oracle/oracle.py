@@ -83,6 +83,9 @@ def load():
                                       C.c_double, C.c_double, C.c_double, C.c_double, _P]
         lib.orc_euler_shear.argtypes = [C.c_int, _P, _P, _P, _P, C.POINTER(Box), _P, C.c_double, _P, _P,
                                         C.c_double, C.c_double]
+        lib.orc_rk2_stage1.argtypes = [C.c_int, _P, _P, _P, _P, _P, _P, C.c_double, _P, _P, C.c_double, C.c_double]
+        lib.orc_rk2_stage2.argtypes = [C.c_int, C.c_double, _P, _P, C.c_double, _P, _P, _P, _P, _P, _P, C.c_double,
+                                       _P, _P, _P, C.POINTER(Box), C.c_double, C.c_double, C.c_double]
         lib.fsd_ewald_table_fill.argtypes = [C.c_float, C.c_int, _P]
         lib.fsd_ewald_functions.argtypes = [C.c_double, C.c_double, _P]
         lib.fsd_ewald_self.argtypes = [C.c_float, _P]
@@ -360,6 +363,58 @@ class Oracle:
                 self.lib.orc_csr_free(C.byref(self.A))
         except Exception:
             pass
+
+
+def _f32(a, shape):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(shape))
+
+
+def euler_shear(pos4, ori3, pos_gb4, image3, vel6, L, dt, shear_rate, B1=0.0, mask1=None, noise3=None, xy=0.0):
+    """Integrator_ExplicitEuler_Shear_kernel (Integrator.cu:126-228); returns new
+    (pos4, ori3, pos_gb4, image3) (FP64 arithmetic, FP32 state as the reference stores it)"""
+    N = len(pos4)
+    p, o, g = _f32(pos4, (N, 4)).copy(), _f32(ori3, (N, 3)).copy(), _f32(pos_gb4, (N, 4)).copy()
+    im = np.ascontiguousarray(np.asarray(image3, dtype=np.int32).reshape(N, 3)).copy()
+    v = np.ascontiguousarray(np.asarray(vel6, dtype=np.float64).reshape(N, 6))
+    m = None if mask1 is None else _f32(mask1, (N,))
+    nz = None if noise3 is None else _f32(noise3, (N, 3))
+    box = make_box(L, xy)
+    load().orc_euler_shear(N, _p(p), _p(o), _p(g), _p(im), C.byref(box), _p(v), B1,
+                           None if m is None else _p(m), None if nz is None else _p(nz), dt, shear_rate)
+    return p, o, g, im
+
+
+def rk2_stage1(pos4, ori3, pos_gb4, vel6, dt, shear_rate, B1=0.0, mask1=None, noise3=None):
+    """Integrator_ExplicitEuler1_Shear_kernel (Integrator.cu:232-326); returns
+    (pos_out4, ori_out3, pos_gb4)"""
+    N = len(pos4)
+    p, o = _f32(pos4, (N, 4)), _f32(ori3, (N, 3))
+    g = _f32(pos_gb4, (N, 4)).copy()
+    po, oo = np.zeros((N, 4), np.float32), np.zeros((N, 3), np.float32)
+    v = np.ascontiguousarray(np.asarray(vel6, dtype=np.float64).reshape(N, 6))
+    m = None if mask1 is None else _f32(mask1, (N,))
+    nz = None if noise3 is None else _f32(noise3, (N, 3))
+    load().orc_rk2_stage1(N, _p(p), _p(o), _p(po), _p(oo), _p(g), _p(v), B1, None if m is None else _p(m),
+                          None if nz is None else _p(nz), dt, shear_rate)
+    return po, oo, g
+
+
+def rk2_stage2(c1, pos1, ori1, c2, pos2, ori2, pos_gb4, vel6, image3, L, c3, dt, shear_rate, B1=0.0, mask1=None,
+               noise3=None, xy=0.0):
+    """Integrator_RK_Shear_kernel (Integrator.cu:329-438); returns (pos_out4, ori_out3, pos_gb4, image3)"""
+    N = len(pos1)
+    a, oa, b, ob = _f32(pos1, (N, 4)), _f32(ori1, (N, 3)), _f32(pos2, (N, 4)), _f32(ori2, (N, 3))
+    g = _f32(pos_gb4, (N, 4)).copy()
+    im = np.ascontiguousarray(np.asarray(image3, dtype=np.int32).reshape(N, 3)).copy()
+    po, oo = np.zeros((N, 4), np.float32), np.zeros((N, 3), np.float32)
+    v = np.ascontiguousarray(np.asarray(vel6, dtype=np.float64).reshape(N, 6))
+    m = None if mask1 is None else _f32(mask1, (N,))
+    nz = None if noise3 is None else _f32(noise3, (N, 3))
+    box = make_box(L, xy)
+    load().orc_rk2_stage2(N, c1, _p(a), _p(oa), c2, _p(b), _p(ob), _p(po), _p(oo), _p(g), _p(v), B1,
+                          None if m is None else _p(m), None if nz is None else _p(nz), _p(im), C.byref(box), c3, dt,
+                          shear_rate)
+    return po, oo, g, im
 
 
 def friction_type1(dist, vals, h0, alpha, parity=True):

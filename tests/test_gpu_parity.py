@@ -316,3 +316,38 @@ def test_precond_one_launch_matches_level_path(N, phi, seed, monkeypatch):
         e1, e2 = rel_err(y[k], y[2]), rel_err(z[k], z[2])
         print("variant", k, "G v vs levels rel err", e1, "saddle precond", e2)
         assert e1 < 1e-5 and e2 < 1e-5
+
+
+def test_rk2_stages_match_oracle():
+    """the two Runge-Kutta stages (Integrator.cu:232-438) through the C-ABI equal the
+    oracle's FP64 restatement, with shear, squirmer slip B1, angular noise and a wrap"""
+    import torch
+    from fsd_amd.stokes import Stokes
+    from oracle.oracle import rk2_stage1, rk2_stage2
+    N, L, dt, g, B1 = 343, 19.3, 0.05, 0.7, 0.4
+    rng = np.random.default_rng(31)
+    pos = np.zeros((N, 4), np.float32)
+    pos[:, :3] = rng.uniform(-L / 2, L / 2, (N, 3))
+    pos[:, 3] = 1.0
+    ori = rng.standard_normal((N, 3)).astype(np.float32)
+    ori /= np.linalg.norm(ori, axis=1, keepdims=True)
+    gb = pos.copy()
+    img = np.zeros((N, 3), np.int32)
+    u = rng.uniform(-20, 20, (N, 6)).astype(np.float32)
+    nz = rng.standard_normal((N, 3)).astype(np.float32)
+    st = Stokes(N, L, sqm_B1=B1)
+    d = lambda a: torch.from_numpy(np.ascontiguousarray(a).reshape(-1)).cuda()
+    dgb = d(gb)
+    po, oo = st.rk2_stage1(d(pos), d(ori), dgb, d(u), dt, g, noise3=d(nz))
+    p1, o1, g1 = rk2_stage1(pos, ori, gb, u.astype(np.float64), dt, g, B1=B1, noise3=nz)
+    po, oo, gbg = po.cpu().numpy().reshape(N, 4), oo.cpu().numpy().reshape(N, 3), dgb.cpu().numpy().reshape(N, 4)
+    e1 = max(np.abs(po - p1).max(), np.abs(oo - o1).max(), np.abs(gbg - g1).max())
+    dimg = d(img)
+    pq, oq = st.rk2_stage2(0.5, d(pos), d(ori), 0.5, d(p1), d(o1), d(g1), d(u), dimg, 0.5, dt, g, noise3=d(nz))
+    p2, o2, g2, i2 = rk2_stage2(0.5, pos, ori, 0.5, p1, o1, g1, u.astype(np.float64), img, L, 0.5, dt, g, B1=B1,
+                                noise3=nz)
+    pq, oq = pq.cpu().numpy().reshape(N, 4), oq.cpu().numpy().reshape(N, 3)
+    e2 = max(np.abs(pq - p2).max(), np.abs(oq - o2).max())
+    print("stage 1 max abs err", e1, "stage 2", e2)
+    assert e1 < 2e-5 and e2 < 2e-5
+    assert np.array_equal(dimg.cpu().numpy().reshape(N, 3), i2) and np.any(i2 != 0)
