@@ -27,13 +27,15 @@ sys.path.insert(0, ROOT)
 
 METRIC = "steps/s at N=32k phi=0.45: learned vs IChol precond at equal residual 1e-4"
 HBM_PEAK_GBS = 8000.0
+GUESS_TEXT = {"previous": "previous solution", "extrapolate": "linear extrapolation of the last two solutions",
+              "quadratic": "quadratic extrapolation of the last three solutions"}
 # kernel-class name -> kernel name in rocprofv3 traces
-KERNEL_NAMES = {"spread": "spread_kernel", "green": "green_kernel", "contract": "contract_tile_kernel",
-                "realspace": "realspace_kernel<16>", "rfu": "rfu_apply_kernel<8>", "sweep": "tp_sweep_kernel<2>",
+KERNEL_NAMES = {"spread": "spread_tile2_kernel<8, 6>", "green": "green_kernel", "contract": "contract_tile2_kernel<13, 6>",
+                "realspace": "realspace_kernel<16>", "rfu": "rfu_apply_kernel<8>", "sweep": "tp_sweep1_kernel",
                 "ichol": "ichol_level_kernel", "gmres_vec": "multidot_partial_kernel"}
 
 
-def algorithmic_bytes(cls, N, info, pse, st):
+def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5):
     """bytes a launch of kernel class `cls` must move at minimum (each input
     read once, each output written once), per the layouts in HBM"""
     M = pse["Nx"] * pse["Ny"] * pse["Nz"]
@@ -48,16 +50,18 @@ def algorithmic_bytes(cls, N, info, pse, st):
         return 4 * st["nlist_total"] + (16 + 48 + 44 + 24 + 44) * N
     if cls == "rfu":         # per pair: index + 2 float4; per particle: self block, U, x_F, out
         return 36 * st["lub_pairs"] + (144 + 24 + 24 + 24) * N
-    if cls == "sweep":       # one triangular sweep for 2 right-hand sides: every block of the factor below
-        # the diagonal once (6x6 FP32, 144 B), the rows' inputs in (48 B) and results out (48 B)
-        return 144 * st["lower_blocks"] + (48 + 48) * N
-    if cls == "trsv":        # one application G b for 2 right-hand sides (five kernels): both sweeps as above;
-        # Z b (b in 48 B, Z 144 B, out 48 B), reorder (96 B), Z^T u (Z 144 B, u in 48 B, out 48 B) per particle
-        return 2 * (144 * st["lower_blocks"] + 96 * N) + (48 + 144 + 48 + 96 + 144 + 48 + 48) * N
+    if cls == "sweep":       # one triangular sweep, one right-hand side (the saddle preconditioner's
+        # d = b_F - b_U): every block of the factor below the diagonal once (6x6 FP32, 144 B), the rows'
+        # inputs in (24 B) and results out (24 B)
+        return 144 * st["lower_blocks"] + (24 + 24) * N
+    if cls == "trsv":        # one saddle preconditioner application (five kernels): both sweeps as above;
+        # Z d (b_F, b_U in 48 B, Z 144 B, out 24 B), reorder (48 B), Z^T u and the combination
+        # (Z 144 B, u 24 B, b_F and b_S 44 B in, x_F, x_S, x_U 68 B out) per particle
+        return 2 * (144 * st["lower_blocks"] + 48 * N) + (48 + 144 + 24 + 48 + 144 + 24 + 44 + 68) * N
     if cls == "ichol":       # one factorisation: A blocks in, L blocks out (lower and diagonal)
         return 144 * 2 * (st["lower_blocks"] + N)
     if cls == "gmres_vec":   # CGS2 on 17N-float vectors with k basis vectors: (4k + 7) vector passes,
-        k = 25.5             # k averaged over a restart cycle of 50
+        k = kbar             # k averaged over the solve's iterations
         return 4 * 17 * N * (4 * k + 7)
     return None
 
@@ -99,7 +103,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--restart", type=int, default=100, help="GMRES restart length")
-    ap.add_argument("--guess", default="extrapolate", choices=["previous", "extrapolate"],
+    ap.add_argument("--guess", default="extrapolate", choices=["previous", "extrapolate", "quadratic"],
                     help="initial guess of each solve: the previous solution (the reference) or its extrapolation")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (diagnostics)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MB",
@@ -167,7 +171,9 @@ def main():
     launches, kms = st.profile_get(dom)
     last = infos[-1]
     avg_ms = kms / max(launches, 1)
-    bytes_per_launch = algorithmic_bytes(dom, N, None, pse, last)
+    it_mean = float(np.mean(iters))
+    kbar = 0.5 * (min(it_mean, args.restart) + 1.0)
+    bytes_per_launch = algorithmic_bytes(dom, N, None, pse, last, kbar)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_launch else None
     # HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
     # (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a plain copy)
@@ -209,8 +215,7 @@ def main():
         "config": {
             "workload": "N=32768 phi=0.45 periodic suspension, shear rate 1, random forces, T=0, xi=0.5, "
                         "PSE error 1e-3, IC(0)-preconditioned GMRES(%d) to true residual 1e-4, initial guess: %s"
-                        % (args.restart, "linear extrapolation of the last two solutions" if args.guess == "extrapolate"
-                           else "previous solution"),
+                        % (args.restart, GUESS_TEXT[args.guess]),
             "N": N, "phi": args.phi, "grid": [pse["Nx"], pse["Ny"], pse["Nz"]], "P": pse["P"],
             "preconditioner": "ichol",
             "gmres_restart": args.restart, "initial_guess": args.guess,

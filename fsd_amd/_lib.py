@@ -19,7 +19,7 @@ FSD_W_NOT_CONVERGED = 1
 
 # kernel classes timed by fsd_profile_* (bit k of the enable mask)
 PROF_NAMES = ["spread", "green", "contract", "fft", "realspace", "rfu", "trsv", "ichol", "gmres_vec", "nlist",
-              "precond_build", "sweep"]
+              "precond_build", "sweep", "plan_host", "finish_host"]
 
 
 class FsdParams(C.Structure):

@@ -270,7 +270,8 @@ struct fsd_ctx {
     double *d_part = nullptr; int part_blocks = 0; double *h_part = nullptr;
     int kmax = 64;                // most vectors of one multidot (restart + 1, at least 64)
     float *d_rhs = nullptr, *d_sol = nullptr;
-    float *d_sol_prev = nullptr;  // x_{n-2} when the initial guess extrapolates (guess = 1)
+    float *d_sol_prev = nullptr;  // x_{n-2} when the initial guess extrapolates (guess >= 1)
+    float *d_sol_prev2 = nullptr; // x_{n-3} for the quadratic extrapolation (guess = 2)
     int64_t n_solved = 0;         // steps solved by this context
     float *d_force = nullptr;
     float *d_mask1 = nullptr, *d_mask2 = nullptr;
@@ -309,7 +310,9 @@ static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 void prof_begin(fsd_ctx *ctx, int slot, hipEvent_t *ev);
 void prof_end(fsd_ctx *ctx, int slot, hipEvent_t ev);
 enum { PROF_SPREAD = 0, PROF_GREEN, PROF_CONTRACT, PROF_FFT, PROF_REAL, PROF_RFU, PROF_TRSV, PROF_ICHOL,
-       PROF_GMRES_VEC, PROF_NLIST, PROF_PRECOND_BUILD, PROF_SWEEP, PROF_COUNT };
+       PROF_GMRES_VEC, PROF_NLIST, PROF_PRECOND_BUILD, PROF_SWEEP, PROF_PLAN_HOST, PROF_FINISH_HOST, PROF_COUNT };
+// host wall time of a phase, added to a slot when that slot is enabled
+void prof_host(fsd_ctx *ctx, int slot, double ms);
 
 // kernel groups (launchers live with their kernels)
 int k_build_nlist(fsd_ctx *ctx, const float4 *pos);

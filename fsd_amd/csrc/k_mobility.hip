@@ -970,9 +970,10 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
     RealParams rp;
     rp.b = ctx->box; rp.rcut = ps.rcut; rp.dr = 0.001f; rp.self_x = ps.self_x; rp.self_y = ps.self_y;
     rp.ewald_n = ps.ewald_n;
-    // the processing order is the neighbour list's cell list (identity without a list: wave part only)
+    // the processing order is the neighbour list's cell list (identity without a list, wave part
+    // only, or for a box of fewer than three cells per side, whose list is built over all pairs)
     if (!ctx->nlist_valid && (parts & 1)) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
-    const int *order = ctx->nlist_valid ? ctx->d_cell_list : nullptr;
+    const int *order = (ctx->nlist_valid && ctx->use_cells) ? ctx->d_cell_list : nullptr;
     hipEvent_t ev;
     prof_begin(ctx, PROF_REAL, &ev);
     hipLaunchKernelGGL(realspace_kernel<16>, dim3(nblk((int64_t)N * 16, 256)), dim3(256), 0, ctx->stream, rp, pos,

@@ -367,7 +367,7 @@ __global__ void __launch_bounds__(256) rfu_apply_kernel(int N, const float *U, c
     const int gid = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int slot = gid / G, lane = gid % G;
     const bool active = slot < N;
-    const int i = active ? order[slot] : slot;
+    const int i = (active && order) ? order[slot] : slot;
     float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (active) {
         const int n = lub_n[i];
@@ -414,14 +414,16 @@ __global__ void __launch_bounds__(256) rfu_apply_kernel(int N, const float *U, c
 
 int k_rfu_apply(fsd_ctx *ctx, const float *U, float *F, const float *xF, float sign_out)
 {
-    // the processing order is the neighbour list's cell list
+    // the processing order is the neighbour list's cell list; a box of fewer than three cells per
+    // side builds its list over all pairs and has no cell list (identity order)
     if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    const int *order = ctx->use_cells ? ctx->d_cell_list : nullptr;
     const int N = ctx->N;
     hipEvent_t ev;
     prof_begin(ctx, PROF_RFU, &ev);
     hipLaunchKernelGGL(rfu_apply_kernel<8>, dim3(nblk((int64_t)N * 8, 256)), dim3(256), 0, ctx->stream, N, U,
                        ctx->d_lub_n, ctx->d_lub_j, ctx->d_lub_a, ctx->d_lub_b, ctx->d_lub_self, ctx->maxlub, xF,
-                       sign_out, F, ctx->d_cell_list);
+                       sign_out, F, order);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_RFU, ev);
     return 0;

@@ -739,6 +739,89 @@ __global__ void tp_post_kernel(int N, const int *seq, const int *perm, const flo
     }
 }
 
+// Saddle preconditioner with one right-hand side.  Saddle_Preconditioner
+// (Saddle.cu:171-264) forms G b_F and G b_U and combines them as
+// x_F = b_F - G b_F + G b_U, x_U = G b_F - G b_U; G is linear, so with
+// d = b_F - b_U this is x_F = b_F - G d, x_U = G d, and one sweep pair
+// suffices.  Forward inputs inp[k_f(P)] = Z_P d[perm P] (right-hand side 0).
+__global__ void tp_pre_diff_kernel(int N, const int *seq, const int *perm, const float *Z, const float *bF,
+                                   const float *bU, float4 *inp)
+{
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= N) return;
+    const float *Zp = Z + (int64_t)P * 36;
+    const int64_t src = 6 * (int64_t)perm[P];
+    float d[6], a[6];
+    for (int c = 0; c < 6; ++c) d[c] = bF[src + c] - bU[src + c];
+    for (int r = 0; r < 6; ++r) {
+        float sa = 0.f;
+        for (int c = 0; c <= r; ++c) sa += Zp[6 * r + c] * d[c];
+        a[r] = sa;
+    }
+    const int k = seq[P];
+    inp[k] = make_float4(a[0], a[1], a[2], a[3]);
+    reinterpret_cast<float2 *>(inp + N + k)[0] = make_float2(a[4], a[5]);
+}
+
+// backward inputs of right-hand side 0 from its forward results
+__global__ void tp_mid1_kernel(int N, const int *seq_f, const int *seq_b, const float4 *res_f, float4 *inp_b)
+{
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= N) return;
+    const int kf = seq_f[P], kb = seq_b[P];
+    inp_b[kb] = res_f[kf];
+    reinterpret_cast<float2 *>(inp_b + N + kb)[0] = reinterpret_cast<const float2 *>(res_f + N + kf)[0];
+}
+
+// g = Z_P^T u_P (u: right-hand side 0 of the backward results), then
+// x_F = b_F - g, x_S = b_S, x_U = g for particle perm[P]
+__global__ void tp_post_saddle_kernel(int N, const int *seq, const int *perm, const float *Z, const float4 *res,
+                                      const float *b, float *x)
+{
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= N) return;
+    const int k = seq[P];
+    const float4 w0 = res[k];
+    const float2 w1 = reinterpret_cast<const float2 *>(res + N + k)[0];
+    const float u[6] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y};
+    const float *Zp = Z + (int64_t)P * 36;
+    const int64_t i = perm[P], n6 = 6 * (int64_t)N, n11 = 11 * (int64_t)N;
+    for (int a = 0; a < 6; ++a) {
+        float g = 0.f;
+        for (int c = a; c < 6; ++c) g += Zp[6 * c + a] * u[c];
+        x[6 * i + a] = b[6 * i + a] - g;
+        x[n11 + 6 * i + a] = g;
+    }
+    for (int j = 0; j < 5; ++j) x[n6 + 5 * i + j] = b[n6 + 5 * i + j];
+}
+
+// x = saddle preconditioner of b through the one-launch sweeps; returns 1 (nothing done)
+// when the sweeps run level by level
+int k_saddle_precond_tp(fsd_ctx *ctx, const float *b, float *x)
+{
+    if (!ctx->tp_ok) return 1;
+    const int N = ctx->N;
+    hipEvent_t ev, es;
+    prof_begin(ctx, PROF_TRSV, &ev);
+    TaskDev F{ctx->tp_f.nsteps, N, ctx->d_tp_finfo, ctx->d_tp_fblk, ctx->d_tp_finp, ctx->d_tp_fres};
+    TaskDev B{ctx->tp_b.nsteps, N, ctx->d_tp_binfo, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres};
+    hipLaunchKernelGGL(tp_pre_diff_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_fseq,
+                       ctx->d_perm, ctx->d_tp_Z, b, b + 11 * (int64_t)N, ctx->d_tp_finp);
+    prof_begin(ctx, PROF_SWEEP, &es);
+    hipLaunchKernelGGL(tp_sweep1_kernel, dim3(1), dim3(TP_LANES), TP1_LDS, ctx->stream, F);
+    prof_end(ctx, PROF_SWEEP, es);
+    hipLaunchKernelGGL(tp_mid1_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->d_tp_fseq,
+                       ctx->d_tp_bseq, ctx->d_tp_fres, ctx->d_tp_binp);
+    prof_begin(ctx, PROF_SWEEP, &es);
+    hipLaunchKernelGGL(tp_sweep1_kernel, dim3(1), dim3(TP_LANES), TP1_LDS, ctx->stream, B);
+    prof_end(ctx, PROF_SWEEP, es);
+    hipLaunchKernelGGL(tp_post_saddle_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_bseq,
+                       ctx->d_perm, ctx->d_tp_Z, ctx->d_tp_bres, b, x);
+    FSD_HIP(hipGetLastError());
+    prof_end(ctx, PROF_TRSV, ev);
+    return 0;
+}
+
 // whether the sweep kernel's LDS fits on this device (tp_R = 0: level by level)
 int k_sweep_setup(fsd_ctx *ctx)
 {

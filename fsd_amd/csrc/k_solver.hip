@@ -126,9 +126,13 @@ __global__ void saddle_combine_kernel(int N, const float *b, const float *gF, co
 }
 
 int k_precond_apply2(fsd_ctx *ctx, const float *b0, const float *b1, float *y0, float *y1, int n_rhs);
+int k_saddle_precond_tp(fsd_ctx *ctx, const float *b, float *x);
 
 int k_saddle_precond(fsd_ctx *ctx, const float *b, float *x)
 {
+    // one sweep pair on d = b_F - b_U when the one-launch sweeps are built
+    const int tp = k_saddle_precond_tp(ctx, b, x);
+    if (tp <= 0) return tp;
     const int N = ctx->N;
     float *gF = ctx->d_g12, *gU = ctx->d_g12 + 6 * (int64_t)N;
     int rc = k_precond_apply2(ctx, b, b + 11 * (int64_t)N, gF, gU, 2);

@@ -7,6 +7,7 @@
 #include "sweep_plan.h"
 #include "lanczos_host.h"
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -22,7 +23,15 @@ void fsd_ewald_self(float xi, float self_out[2]);
 
 // ------------------------------------------------------------ profiling
 static const char *kProfNames[PROF_COUNT] = {"spread", "green", "contract", "fft", "realspace", "rfu",
-                                             "trsv", "ichol", "gmres_vec", "nlist", "precond_build", "sweep"};
+                                             "trsv", "ichol", "gmres_vec", "nlist", "precond_build", "sweep",
+                                             "plan_host", "finish_host"};
+
+void prof_host(fsd_ctx *ctx, int slot, double ms)
+{
+    if (!(ctx->prof & (1u << slot))) return;
+    ctx->slots[slot].ms += ms;
+    ctx->slots[slot].launches += 1;
+}
 
 void prof_begin(fsd_ctx *ctx, int slot, hipEvent_t *ev)
 {
@@ -281,9 +290,13 @@ static int alloc_all(fsd_ctx *ctx)
     CK(hipMalloc(&ctx->d_rhs, sizeof(float) * n17));
     CK(hipMalloc(&ctx->d_sol, sizeof(float) * n17));
     CK(hipMemset(ctx->d_sol, 0, sizeof(float) * n17));
-    if (ctx->p.guess == 1) {
+    if (ctx->p.guess >= 1) {
         CK(hipMalloc(&ctx->d_sol_prev, sizeof(float) * n17));
         CK(hipMemset(ctx->d_sol_prev, 0, sizeof(float) * n17));
+    }
+    if (ctx->p.guess >= 2) {
+        CK(hipMalloc(&ctx->d_sol_prev2, sizeof(float) * n17));
+        CK(hipMemset(ctx->d_sol_prev2, 0, sizeof(float) * n17));
     }
     CK(hipMalloc(&ctx->d_force, sizeof(float) * 6 * (size_t)N));
     // squirmer masks (Stokes.cc:1318-1346, index == tag)
@@ -309,7 +322,7 @@ static void free_all(fsd_ctx *ctx)
                     ctx->d_ewald, ctx->d_res_dist, ctx->d_res_vals, ctx->d_pn, ctx->d_plist, ctx->d_pless,
                     ctx->d_perm, ctx->d_iperm, ctx->d_lrow, ctx->d_urow, ctx->d_lcol, ctx->d_ucol, ctx->d_upos,
                     ctx->d_Ablk, ctx->d_Lblk, ctx->d_Adiag, ctx->d_Ldiag, ctx->d_lev_f, ctx->d_lev_b, ctx->d_tmp6a,
-                    ctx->d_tmp6b, ctx->d_g12, ctx->d_V, ctx->d_w, ctx->d_t, ctx->d_t2, ctx->d_part, ctx->d_rhs, ctx->d_sol, ctx->d_sol_prev,
+                    ctx->d_tmp6b, ctx->d_g12, ctx->d_V, ctx->d_w, ctx->d_t, ctx->d_t2, ctx->d_part, ctx->d_rhs, ctx->d_sol, ctx->d_sol_prev, ctx->d_sol_prev2,
                     ctx->d_force, ctx->d_mask1, ctx->d_mask2, ctx->d_tp_finfo, ctx->d_tp_binfo,
                     ctx->d_tp_fbsrc, ctx->d_tp_bbsrc, ctx->d_tp_fblk, ctx->d_tp_bblk,
                     ctx->d_tp_finp, ctx->d_tp_binp, ctx->d_tp_fres, ctx->d_tp_bres, ctx->d_tp_fseq, ctx->d_tp_bseq,
@@ -387,8 +400,14 @@ static int precond_fetch(fsd_ctx *ctx, const float4 *pos)
     return 0;
 }
 
+static double wall_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
 static void precond_plan(fsd_ctx *ctx)
 {
+    const double t0 = wall_ms();
     const int N = ctx->N;
     PrecondHost &h = ctx->pch;
     // packed adjacency (self loops included), as Precondition_Reorder passes to genrcm
@@ -499,10 +518,12 @@ static void precond_plan(fsd_ctx *ctx)
         tb.join();
         ctx->tp_ok = okf && okb;
     }
+    prof_host(ctx, PROF_PLAN_HOST, wall_ms() - t0);
 }
 
 static int precond_finish(fsd_ctx *ctx, const float4 *pos)
 {
+    const double t0 = wall_ms();
     const int N = ctx->N;
     PrecondHost &h = ctx->pch;
     auto up = [ctx](void *d, const void *src, size_t bytes) {
@@ -544,6 +565,7 @@ static int precond_finish(fsd_ctx *ctx, const float4 *pos)
     }
     ctx->ichol_relaxer = lambda;
     if (ctx->tp_ok) RC(k_sweep_pack(ctx));
+    prof_host(ctx, PROF_FINISH_HOST, wall_ms() - t0);
     prof_end(ctx, PROF_PRECOND_BUILD, ctx->pc_ev);
     ctx->precond_valid = true;
     return 0;
@@ -1432,15 +1454,20 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     fsd_solve_stats st;
     memset(&st, 0, sizeof st);
     st.true_residual = -1.f;
-    // initial guess: the previous step's solution (Integrator.cu:777), or its linear extrapolation
-    if (ctx->p.guess == 1) {
-        if (ctx->n_solved >= 2) {   // d_sol = x_{n-1}, d_sol_prev = x_{n-2}
-            RC(k_copy(ctx, n17, ctx->d_sol, ctx->d_t));
+    // initial guess: the previous step's solution (Integrator.cu:777), or the linear (guess 1) or
+    // quadratic (guess 2) extrapolation of the last solutions, of the highest order they allow.
+    // d_sol = x_{n-1}, d_sol_prev = x_{n-2}, d_sol_prev2 = x_{n-3}
+    if (ctx->p.guess >= 1) {
+        const int order = std::min((int)ctx->p.guess, (int)ctx->n_solved - 1);
+        if (order >= 1) RC(k_copy(ctx, n17, ctx->d_sol, ctx->d_t));
+        if (order == 1) {
             RC(k_axpby(ctx, n17, 2.0f, ctx->d_sol, -1.0f, ctx->d_sol_prev, ctx->d_sol));
-            RC(k_copy(ctx, n17, ctx->d_t, ctx->d_sol_prev));
-        } else {
-            RC(k_copy(ctx, n17, ctx->d_sol, ctx->d_sol_prev));
+        } else if (order == 2) {
+            RC(k_axpby(ctx, n17, 3.0f, ctx->d_sol, -3.0f, ctx->d_sol_prev, ctx->d_sol));
+            RC(k_axpby(ctx, n17, 1.0f, ctx->d_sol, 1.0f, ctx->d_sol_prev2, ctx->d_sol));
         }
+        if (ctx->p.guess >= 2) std::swap(ctx->d_sol_prev, ctx->d_sol_prev2);   // x_{n-2} becomes the older one
+        RC(k_copy(ctx, n17, order >= 1 ? ctx->d_t : ctx->d_sol, ctx->d_sol_prev));
     }
     int rc = gmres(ctx, pos, ctx->d_rhs, ctx->d_sol, ctx->p.gmres_tol, ctx->p.gmres_maxiter, ctx->p.gmres_restart, &st);
     ctx->n_solved++;

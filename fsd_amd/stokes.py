@@ -58,9 +58,10 @@ class Stokes:
         p.rlub = 4.0; p.rp = 2.1; p.r_buff = 0.4
         p.parity = 1 if parity else 0
         p.gmres_true_tol = float(gmres_true_tol)
-        if guess not in ("previous", "extrapolate"):
-            raise ValueError("guess must be 'previous' (the reference) or 'extrapolate'")
-        p.guess = 1 if guess == "extrapolate" else 0
+        guesses = {"previous": 0, "extrapolate": 1, "quadratic": 2}
+        if guess not in guesses:
+            raise ValueError("guess must be 'previous' (the reference), 'extrapolate' (linear) or 'quadratic'")
+        p.guess = guesses[guess]
         self.params = p
         self.N = int(N)
         self.fileprefix = fileprefix

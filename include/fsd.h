@@ -68,7 +68,9 @@ typedef struct fsd_params {
     float gmres_true_tol;            /* > 0: also require ||b - A x|| / ||b|| <= this (extra mat-vecs) */
     int32_t guess;                   /* initial guess of a step's solve: 0 = the previous step's solution
                                         (the reference, Integrator.cu:777); 1 = linear extrapolation
-                                        2 x_{n-1} - x_{n-2} of the last two steps' solutions */
+                                        2 x_{n-1} - x_{n-2} of the last two steps' solutions; 2 = quadratic
+                                        extrapolation 3 x_{n-1} - 3 x_{n-2} + x_{n-3} (lower orders while
+                                        fewer solutions exist) */
 } fsd_params;
 
 /* Derived PSE parameters (Stokes.cc:131-261). */
@@ -233,7 +235,9 @@ int fsd_rk2_stage2(fsd_ctx *ctx, float coef1, const float *d_pos_in1_4, const fl
  * an event pair is recorded around every launch of a selected class */
 int fsd_profile_enable(fsd_ctx *ctx, int32_t enable);
 /* name: "spread", "green", "contract", "fft", "realspace", "rfu", "trsv", "ichol",
- * "gmres_vec", "nlist", "precond_build", "sweep"; returns launches and total milliseconds */
+ * "gmres_vec", "nlist", "precond_build", "sweep", and the host wall time of the preconditioner's
+ * host plan ("plan_host") and of its upload/build phase ("finish_host"); returns launches (phases)
+ * and total milliseconds */
 int fsd_profile_get(fsd_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
 int fsd_profile_reset(fsd_ctx *ctx);
 

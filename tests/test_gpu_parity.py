@@ -13,6 +13,9 @@ from tests._cases import packing, rel_err, rel_err2, forces, random_vec, sorted_
 pytestmark = pytest.mark.gpu
 
 CASES = {
+    # C0: a box of fewer than three neighbour-list cells per side (list built over all
+    # pairs, no cell order for the real-space and lubrication gathers)
+    "C0": (64, 0.14, 10),
     "C1": (343, 0.20, 11),
     "C2": (4096, 0.30, 12),
 }
@@ -39,7 +42,7 @@ def _t(x):
     return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
 
 
-@pytest.mark.parametrize("case", ["C1", "C2"])
+@pytest.mark.parametrize("case", ["C0", "C1", "C2"])
 def test_nlist_bitexact(case):
     st, orc, dpos, N = _setup(case)
     nn, nb = st.get_nlist()
@@ -48,9 +51,12 @@ def test_nlist_bitexact(case):
     assert got == ref
 
 
-@pytest.mark.parametrize("case", ["C1", "C2"])
+@pytest.mark.parametrize("case", ["C0", "C1", "C2"])
 def test_rfu_parity(case):
     st, orc, dpos, N = _setup(case)
+    if case == "C0":   # the case is meant to exercise the all-pairs list
+        L = packing(*CASES[case])[1]
+        assert L / (st.pse_info()["rcut"] + 0.4) < 3
     U = random_vec(6 * N, 1)
     F = st.rfu(dpos, _t(U)).cpu().numpy()
     Fo = orc.rfu(U.astype(np.float32))
@@ -59,7 +65,7 @@ def test_rfu_parity(case):
     assert e < 1e-5
 
 
-@pytest.mark.parametrize("case", ["C1", "C2"])
+@pytest.mark.parametrize("case", ["C0", "C1", "C2"])
 def test_rfe_rsu_rse_parity(case):
     st, orc, dpos, N = _setup(case)
     F = st.rfe(dpos, 1.0).cpu().numpy()
@@ -71,7 +77,7 @@ def test_rfe_rsu_rse_parity(case):
     assert rel_err(S, orc.rse(1.0)) < 1e-5
 
 
-@pytest.mark.parametrize("case", ["C1", "C2"])
+@pytest.mark.parametrize("case", ["C0", "C1", "C2"])
 @pytest.mark.parametrize("parts", [1, 2, 3])
 def test_mobility_parity(case, parts):
     st, orc, dpos, N = _setup(case)
