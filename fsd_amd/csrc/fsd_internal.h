@@ -18,7 +18,7 @@
 #include <string>
 #include <vector>
 #include "fsd.h"
-#include "sweep_plan.h"
+#include "precond_plan.h"
 
 #define FSD_MAXP 14        // pruned neighbours incl. self (kissing number 12 + self + 1)
 #define FSD_PSTRIDE 16     // stride of the pruned list
@@ -124,13 +124,6 @@ __device__ inline int xcd_remap(int w, int n)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
-// host arrays of the preconditioner build
-struct PrecondHost {
-    std::vector<int> iperm;                    // particle -> permuted index
-    std::vector<int> lrow, lcol, urow, ucol, upos;   // block adjacency below / above the diagonal
-    std::vector<int> itf, itb;                 // particles by forward / backward level
-};
-
 struct LubTable {
     const float *dist;   // 1000
     const float *vals;   // 22000
@@ -227,7 +220,7 @@ struct fsd_ctx {
     // preconditioner (blocks on the permuted particle graph)
     int *d_pn = nullptr, *d_plist = nullptr, *d_pless = nullptr;
     int *d_perm = nullptr, *d_iperm = nullptr;
-    std::vector<int> h_pn, h_plist, h_perm;
+    std::vector<int> h_pn;
     int *d_lrow = nullptr;        // N+1: start of each permuted particle's lower blocks
     int *d_lcol = nullptr;        // lower block columns (permuted ids), ascending
     int *d_urow = nullptr, *d_ucol = nullptr, *d_upos = nullptr;  // upper adjacency for L^T solves
@@ -235,24 +228,18 @@ struct fsd_ctx {
     float *d_Adiag = nullptr;     // 36 per particle (A diagonal block, full)
     float *d_Lblk = nullptr, *d_Ldiag = nullptr;
     int *d_isolated = nullptr;    // per permuted particle
-    int64_t nlow = 0, nlow_cap = 0;
-    int64_t nnz_ref = 0;          // reference CSR nonzeros
+    int64_t nlow_cap = 0;         // lower blocks allocated
     float ichol_relaxer = 1.0f;
     int ichol_attempts = 0;
     bool precond_valid = false;
-    // level schedules of the forward (lower) and backward (upper) particle DAGs
-    int nlev_f = 0, nlev_b = 0;
-    std::vector<int> h_lev_f_ptr, h_lev_b_ptr;
     int *d_lev_f = nullptr, *d_lev_b = nullptr;
     float *d_tmp6a = nullptr, *d_tmp6b = nullptr;
     float *d_g12 = nullptr;       // G b_F and G b_U of the saddle preconditioner
     // host phase of the preconditioner build (kept between steps to reuse capacity)
     std::vector<int> h_plstride;                         // pruned lists, FSD_PSTRIDE per particle
-    PrecondHost pch;
+    PrecondHost pch;                                     // RCM, pattern, levels, sweep plans
     hipEvent_t pc_ev = nullptr;                          // profiling scope of the build
     // single-workgroup sweeps: task plans (sweep_plan.h) and their device operands
-    TaskPlan tp_f, tp_b;
-    bool tp_ok = false;
     int tp_R = 0;                                        // LDS ring capacity in rows (0: level path)
     bool tp_joint = false;                               // both right-hand sides in one workgroup
     int64_t tp_cap_f = 0, tp_cap_b = 0;                  // lanes allocated per sweep
@@ -262,6 +249,7 @@ struct fsd_ctx {
     float4 *d_tp_finp = nullptr, *d_tp_binp = nullptr;   // 3 N: per-row inputs
     float4 *d_tp_fres = nullptr, *d_tp_bres = nullptr;   // 3 N: per-row results
     int *d_tp_fseq = nullptr, *d_tp_bseq = nullptr;      // N: sequence number of each permuted particle
+    int *d_tp_fplace = nullptr, *d_tp_bplace = nullptr;  // 3 N: first lane and step of each row, step ends
     float *d_tp_Z = nullptr;                             // 36 N: inverse diagonal blocks
 
     // saddle / GMRES
@@ -359,5 +347,3 @@ int k_nrm2(fsd_ctx *ctx, int64_t n, const float *x, double *out);
 int k_saddle_precond(fsd_ctx *ctx, const float *b, float *x);
 int k_saddle_multiply(fsd_ctx *ctx, const float4 *pos, const float *x, float *y);
 
-// host RCM (George & Liu GENRCM); phead[N+1], plist packed with self loops
-void host_rcm(int N, const int *phead, const int *plist, int *perm);

@@ -225,8 +225,8 @@ int k_ichol(fsd_ctx *ctx, float lambda, int *failed)
     hipEvent_t ev;
     prof_begin(ctx, PROF_ICHOL, &ev);
     FSD_HIP(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->stream));
-    for (int l = 0; l < ctx->nlev_f; ++l) {
-        const int b = ctx->h_lev_f_ptr[l], n = ctx->h_lev_f_ptr[l + 1] - b;
+    for (int l = 0; l < ctx->pch.nlev_f; ++l) {
+        const int b = ctx->pch.lev_f_ptr[l], n = ctx->pch.lev_f_ptr[l + 1] - b;
         hipLaunchKernelGGL(ichol_level_kernel, dim3(n), dim3(64), 0, ctx->stream, ctx->d_lev_f + b, ctx->d_lrow,
                            ctx->d_lcol, ctx->d_Ablk, ctx->d_Adiag, lambda, ctx->d_Lblk, ctx->d_Ldiag,
                            ctx->d_flags + 2);
@@ -657,6 +657,28 @@ __global__ void diag_inverse_kernel(int N, const float *Ldiag, float *Z)
     for (int q = 0; q < 36; ++q) Z[(int64_t)P * 36 + q] = Zm[q];
 }
 
+// Lane arrays of a sweep from its row placement (the rules of task_plan_lanes):
+// a row's first lane holds (src of its first dependency, k | nd << TP_ND_SHIFT),
+// its other lanes (src of dependency j, -1); bsrc = (factor block of the
+// dependency, particle whose diagonal block scales it).  Lanes no row uses
+// keep the -1 they were cleared to.  place = [lane: N][step: N][send: nsteps].
+__global__ void tp_lanes_kernel(int N, const int *rowptr, const int *cols, const int *src, int forward, int R,
+                                const int *seq, const int *place, int2 *info, int2 *bsrc)
+{
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= N) return;
+    const int g = place[P], S = place[N + P], r0 = rowptr[P], n = rowptr[P + 1] - r0;
+    const int first = place[2 * N + S] - R;   // lowest sequence number still in the LDS ring at step S
+    const int own = seq[P] | (n << TP_ND_SHIFT);
+    if (n == 0) info[g] = make_int2(-1, own);
+    for (int j = 0; j < n; ++j) {
+        const int e = r0 + j, Q = cols[e], kq = seq[Q];
+        const int s = kq >= first ? (kq & (R - 1)) : (TP_GLOBAL | kq);
+        info[g + j] = make_int2(s, j == 0 ? own : -1);
+        bsrc[g + j] = make_int2(src ? src[e] : e, forward ? P : Q);
+    }
+}
+
 // lane blocks: M = Z_d L_e (forward) or its transpose (backward)
 __global__ void tp_pack_kernel(int64_t lanes, const int2 *bsrc, const float *Lblk, const float *Z, float4 *blk,
                                int transpose)
@@ -799,12 +821,12 @@ __global__ void tp_post_saddle_kernel(int N, const int *seq, const int *perm, co
 // when the sweeps run level by level
 int k_saddle_precond_tp(fsd_ctx *ctx, const float *b, float *x)
 {
-    if (!ctx->tp_ok) return 1;
+    if (!ctx->pch.tp_ok) return 1;
     const int N = ctx->N;
     hipEvent_t ev, es;
     prof_begin(ctx, PROF_TRSV, &ev);
-    TaskDev F{ctx->tp_f.nsteps, N, ctx->d_tp_finfo, ctx->d_tp_fblk, ctx->d_tp_finp, ctx->d_tp_fres};
-    TaskDev B{ctx->tp_b.nsteps, N, ctx->d_tp_binfo, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres};
+    TaskDev F{ctx->pch.tp_f.nsteps, N, ctx->d_tp_finfo, ctx->d_tp_fblk, ctx->d_tp_finp, ctx->d_tp_fres};
+    TaskDev B{ctx->pch.tp_b.nsteps, N, ctx->d_tp_binfo, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres};
     hipLaunchKernelGGL(tp_pre_diff_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_fseq,
                        ctx->d_perm, ctx->d_tp_Z, b, b + 11 * (int64_t)N, ctx->d_tp_finp);
     prof_begin(ctx, PROF_SWEEP, &es);
@@ -845,7 +867,17 @@ int k_sweep_pack(fsd_ctx *ctx)
     const int N = ctx->N;
     hipLaunchKernelGGL(diag_inverse_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_Ldiag,
                        ctx->d_tp_Z);
-    const int64_t lf = (int64_t)ctx->tp_f.nsteps * TP_LANES, lb = (int64_t)ctx->tp_b.nsteps * TP_LANES;
+    const int64_t lf = (int64_t)ctx->pch.tp_f.nsteps * TP_LANES, lb = (int64_t)ctx->pch.tp_b.nsteps * TP_LANES;
+    // lane arrays from the row placements
+    FSD_HIP(hipMemsetAsync(ctx->d_tp_finfo, 0xFF, sizeof(int2) * lf, ctx->stream));
+    FSD_HIP(hipMemsetAsync(ctx->d_tp_fbsrc, 0xFF, sizeof(int2) * lf, ctx->stream));
+    FSD_HIP(hipMemsetAsync(ctx->d_tp_binfo, 0xFF, sizeof(int2) * lb, ctx->stream));
+    FSD_HIP(hipMemsetAsync(ctx->d_tp_bbsrc, 0xFF, sizeof(int2) * lb, ctx->stream));
+    hipLaunchKernelGGL(tp_lanes_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->d_lrow, ctx->d_lcol,
+                       (const int *)nullptr, 1, ctx->tp_R, ctx->d_tp_fseq, ctx->d_tp_fplace, ctx->d_tp_finfo,
+                       ctx->d_tp_fbsrc);
+    hipLaunchKernelGGL(tp_lanes_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->d_urow, ctx->d_ucol,
+                       ctx->d_upos, 0, ctx->tp_R, ctx->d_tp_bseq, ctx->d_tp_bplace, ctx->d_tp_binfo, ctx->d_tp_bbsrc);
     if (lf)
         hipLaunchKernelGGL(tp_pack_kernel, dim3(nblk(lf, 256)), dim3(256), 0, ctx->stream, lf, ctx->d_tp_fbsrc,
                            ctx->d_Lblk, ctx->d_tp_Z, ctx->d_tp_fblk, 0);
@@ -862,10 +894,10 @@ int k_precond_apply2(fsd_ctx *ctx, const float *b0, const float *b1, float *y0, 
     const int N = ctx->N;
     hipEvent_t ev;
     prof_begin(ctx, PROF_TRSV, &ev);
-    if (ctx->tp_ok) {
+    if (ctx->pch.tp_ok) {
         const size_t lds = sizeof(float4) * 3 * (TP_RING + TP_LANES);
-        TaskDev F{ctx->tp_f.nsteps, N, ctx->d_tp_finfo, ctx->d_tp_fblk, ctx->d_tp_finp, ctx->d_tp_fres};
-        TaskDev B{ctx->tp_b.nsteps, N, ctx->d_tp_binfo, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres};
+        TaskDev F{ctx->pch.tp_f.nsteps, N, ctx->d_tp_finfo, ctx->d_tp_fblk, ctx->d_tp_finp, ctx->d_tp_fres};
+        TaskDev B{ctx->pch.tp_b.nsteps, N, ctx->d_tp_binfo, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres};
         hipLaunchKernelGGL(tp_pre_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_fseq,
                            ctx->d_perm, ctx->d_tp_Z, b0, b1, n_rhs, ctx->d_tp_finp);
         hipEvent_t es;
@@ -889,13 +921,13 @@ int k_precond_apply2(fsd_ctx *ctx, const float *b0, const float *b1, float *y0, 
         prof_end(ctx, PROF_TRSV, ev);
         return 0;
     }
-    for (int l = 0; l < ctx->nlev_f; ++l) {
-        const int b = ctx->h_lev_f_ptr[l], n = ctx->h_lev_f_ptr[l + 1] - b;
+    for (int l = 0; l < ctx->pch.nlev_f; ++l) {
+        const int b = ctx->pch.lev_f_ptr[l], n = ctx->pch.lev_f_ptr[l + 1] - b;
         hipLaunchKernelGGL(trsv_fwd_level_kernel, dim3(n), dim3(64), 0, ctx->stream, ctx->d_lev_f + b, N, n_rhs,
                            ctx->d_lrow, ctx->d_lcol, ctx->d_Lblk, ctx->d_Ldiag, ctx->d_perm, b0, b1, ctx->d_tmp6a);
     }
-    for (int l = 0; l < ctx->nlev_b; ++l) {
-        const int b = ctx->h_lev_b_ptr[l], n = ctx->h_lev_b_ptr[l + 1] - b;
+    for (int l = 0; l < ctx->pch.nlev_b; ++l) {
+        const int b = ctx->pch.lev_b_ptr[l], n = ctx->pch.lev_b_ptr[l + 1] - b;
         hipLaunchKernelGGL(trsv_bwd_level_kernel, dim3(n), dim3(64), 0, ctx->stream, ctx->d_lev_b + b, N, n_rhs,
                            ctx->d_urow, ctx->d_ucol, ctx->d_upos, ctx->d_Lblk, ctx->d_Ldiag, ctx->d_perm,
                            ctx->d_tmp6a, ctx->d_tmp6b, y0, y1);

@@ -271,6 +271,8 @@ static int alloc_all(fsd_ctx *ctx)
     CK(hipMalloc(&ctx->d_tp_Z, sizeof(float) * 36 * (size_t)N));
     CK(hipMalloc(&ctx->d_tp_fseq, sizeof(int) * (size_t)N));
     CK(hipMalloc(&ctx->d_tp_bseq, sizeof(int) * (size_t)N));
+    CK(hipMalloc(&ctx->d_tp_fplace, sizeof(int) * 3 * (size_t)N));   // a step holds one row at least
+    CK(hipMalloc(&ctx->d_tp_bplace, sizeof(int) * 3 * (size_t)N));
     for (float4 **pp : {&ctx->d_tp_finp, &ctx->d_tp_binp, &ctx->d_tp_fres, &ctx->d_tp_bres})
         CK(hipMalloc(pp, sizeof(float4) * 3 * (size_t)N));
     CK(hipMalloc(&ctx->d_tmp6a, sizeof(float) * 12 * (size_t)N));
@@ -325,7 +327,7 @@ static void free_all(fsd_ctx *ctx)
                     ctx->d_tmp6b, ctx->d_g12, ctx->d_V, ctx->d_w, ctx->d_t, ctx->d_t2, ctx->d_part, ctx->d_rhs, ctx->d_sol, ctx->d_sol_prev, ctx->d_sol_prev2,
                     ctx->d_force, ctx->d_mask1, ctx->d_mask2, ctx->d_tp_finfo, ctx->d_tp_binfo,
                     ctx->d_tp_fbsrc, ctx->d_tp_bbsrc, ctx->d_tp_fblk, ctx->d_tp_bblk,
-                    ctx->d_tp_finp, ctx->d_tp_binp, ctx->d_tp_fres, ctx->d_tp_bres, ctx->d_tp_fseq, ctx->d_tp_bseq,
+                    ctx->d_tp_finp, ctx->d_tp_binp, ctx->d_tp_fres, ctx->d_tp_bres, ctx->d_tp_fseq, ctx->d_tp_bseq, ctx->d_tp_fplace, ctx->d_tp_bplace,
                     ctx->d_tp_Z,
                     ctx->d_Z, ctx->d_lanV, ctx->d_lanw, ctx->d_lany0, ctx->d_lan_h, ctx->d_noise, ctx->d_posp,
                     ctx->d_rfd_rhs, ctx->d_rfd_sol};
@@ -359,7 +361,7 @@ static int grow_lower(fsd_ctx *ctx, int64_t need)
 // device operands of the sweep task plans (lanes grow with the number of steps)
 static int grow_tasks(fsd_ctx *ctx)
 {
-    const int64_t lf = (int64_t)ctx->tp_f.nsteps * TP_LANES, lb = (int64_t)ctx->tp_b.nsteps * TP_LANES;
+    const int64_t lf = (int64_t)ctx->pch.tp_f.nsteps * TP_LANES, lb = (int64_t)ctx->pch.tp_b.nsteps * TP_LANES;
     auto grow = [ctx](int64_t need, int64_t &cap, int2 *&info, int2 *&bsrc, float4 *&blk) -> int {
         if (need <= cap) return 0;
         const int64_t c = need + need / 4 + TP_LANES;
@@ -408,116 +410,7 @@ static double wall_ms()
 static void precond_plan(fsd_ctx *ctx)
 {
     const double t0 = wall_ms();
-    const int N = ctx->N;
-    PrecondHost &h = ctx->pch;
-    // packed adjacency (self loops included), as Precondition_Reorder passes to genrcm
-    std::vector<int> phead(N + 1);
-    phead[0] = 0;
-    for (int i = 0; i < N; ++i) phead[i + 1] = phead[i] + ctx->h_pn[i];
-    ctx->h_plist.resize(phead[N]);
-    for (int i = 0; i < N; ++i)
-        for (int s = 0; s < ctx->h_pn[i]; ++s) ctx->h_plist[phead[i] + s] = ctx->h_plstride[(size_t)i * FSD_PSTRIDE + s];
-    ctx->h_perm.resize(N);
-    host_rcm(N, phead.data(), ctx->h_plist.data(), ctx->h_perm.data());
-    h.iperm.resize(N);
-    for (int P = 0; P < N; ++P) h.iperm[ctx->h_perm[P]] = P;
-    // lower and upper block adjacency on the permuted graph: lower counts, then the rows filled
-    // (sorted) in parallel over contiguous ranges of P, then the transpose
-    std::vector<int> &lrow = h.lrow, &lcol = h.lcol, &urow = h.urow, &ucol = h.ucol, &upos = h.upos;
-    lrow.assign(N + 1, 0);
-    urow.assign(N + 1, 0);
-    const int nthr = N >= 8192 ? 4 : 1;
-    auto ranges = [&](auto &&fn) {
-        std::vector<std::thread> th;
-        for (int t = 1; t < nthr; ++t) th.emplace_back(fn, (int)((int64_t)N * t / nthr), (int)((int64_t)N * (t + 1) / nthr));
-        fn(0, (int)((int64_t)N / nthr));
-        for (auto &x : th) x.join();
-    };
-    ranges([&](int p0, int p1) {
-        for (int P = p0; P < p1; ++P) {
-            const int i = ctx->h_perm[P];
-            int nt = 0;
-            for (int s = phead[i]; s < phead[i + 1]; ++s) nt += h.iperm[ctx->h_plist[s]] < P;
-            lrow[P + 1] = nt;
-        }
-    });
-    for (int P = 0; P < N; ++P) lrow[P + 1] += lrow[P];
-    lcol.resize(lrow[N]);
-    ranges([&](int p0, int p1) {
-        for (int P = p0; P < p1; ++P) {
-            const int i = ctx->h_perm[P];
-            int *row = lcol.data() + lrow[P];
-            int nt = 0;
-            for (int s = phead[i]; s < phead[i + 1]; ++s) {
-                const int Q = h.iperm[ctx->h_plist[s]];
-                if (Q < P) row[nt++] = Q;
-            }
-            std::sort(row, row + nt);
-        }
-    });
-    int64_t nnz_ref = 0;
-    for (int P = 0; P < N; ++P) {
-        const int i = ctx->h_perm[P];
-        nnz_ref += (ctx->h_pn[i] > 1) ? 36 * (int64_t)ctx->h_pn[i] : 6;
-        for (int e = lrow[P]; e < lrow[P + 1]; ++e) urow[lcol[e] + 1]++;
-    }
-    for (int P = 0; P < N; ++P) urow[P + 1] += urow[P];
-    ucol.resize(lcol.size());
-    upos.resize(lcol.size());
-    {
-        std::vector<int> fill(urow.begin(), urow.end() - 1);
-        for (int P = 0; P < N; ++P)
-            for (int e = lrow[P]; e < lrow[P + 1]; ++e) {
-                const int Q = lcol[e];          // block (P, Q) with Q < P -> upper entry of row Q
-                ucol[fill[Q]] = P;
-                upos[fill[Q]] = e;
-                fill[Q]++;
-            }
-    }
-    ctx->nlow = (int64_t)lcol.size();
-    ctx->nnz_ref = nnz_ref;
-    // level schedules: forward level = 1 + max level of the lower neighbours,
-    // backward level = 1 + max level of the upper neighbours (counted from N-1)
-    std::vector<int> lf(N, 0), lb(N, 0);
-    int mf = 0, mb = 0;
-    for (int P = 0; P < N; ++P) {
-        int l = 0;
-        for (int e = lrow[P]; e < lrow[P + 1]; ++e) l = std::max(l, lf[lcol[e]] + 1);
-        lf[P] = l;
-        mf = std::max(mf, l);
-    }
-    for (int P = N - 1; P >= 0; --P) {
-        int l = 0;
-        for (int e = urow[P]; e < urow[P + 1]; ++e) l = std::max(l, lb[ucol[e]] + 1);
-        lb[P] = l;
-        mb = std::max(mb, l);
-    }
-    auto bucket = [N](const std::vector<int> &lev, int nl, std::vector<int> &ptr, std::vector<int> &items) {
-        ptr.assign(nl + 1, 0);
-        for (int P = 0; P < N; ++P) ptr[lev[P] + 1]++;
-        for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
-        items.resize(N);
-        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-        for (int P = 0; P < N; ++P) items[fill[lev[P]]++] = P;
-    };
-    ctx->nlev_f = mf + 1;
-    ctx->nlev_b = mb + 1;
-    bucket(lf, ctx->nlev_f, ctx->h_lev_f_ptr, h.itf);
-    bucket(lb, ctx->nlev_b, ctx->h_lev_b_ptr, h.itb);
-    // schedules of the single-workgroup sweeps; the two plans are independent: the
-    // backward one on a second host thread
-    ctx->tp_ok = false;
-    if (ctx->tp_R > 0) {
-        bool okb = false;
-        std::thread tb([&]() {
-            okb = build_task_plan(N, lb.data(), ctx->nlev_b, urow.data(), ucol.data(), upos.data(), false, ctx->tp_R,
-                                  ctx->tp_b);
-        });
-        const bool okf = build_task_plan(N, lf.data(), ctx->nlev_f, lrow.data(), lcol.data(), nullptr, true,
-                                         ctx->tp_R, ctx->tp_f);
-        tb.join();
-        ctx->tp_ok = okf && okb;
-    }
+    plan_precond(ctx->N, ctx->h_pn.data(), ctx->h_plstride.data(), FSD_PSTRIDE, ctx->tp_R, 4, ctx->pch);
     prof_host(ctx, PROF_PLAN_HOST, wall_ms() - t0);
 }
 
@@ -529,27 +422,30 @@ static int precond_finish(fsd_ctx *ctx, const float4 *pos)
     auto up = [ctx](void *d, const void *src, size_t bytes) {
         return hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, ctx->stream);
     };
-    RC(grow_lower(ctx, ctx->nlow + 1));
-    if (ctx->tp_ok) {
+    RC(grow_lower(ctx, ctx->pch.nlow + 1));
+    if (ctx->pch.tp_ok) {
         RC(grow_tasks(ctx));
-        const TaskPlan &f = ctx->tp_f, &b = ctx->tp_b;
-        CK(up(ctx->d_tp_finfo, f.info.data(), sizeof(int) * f.info.size()));
-        CK(up(ctx->d_tp_binfo, b.info.data(), sizeof(int) * b.info.size()));
-        CK(up(ctx->d_tp_fbsrc, f.bsrc.data(), sizeof(int) * f.bsrc.size()));
-        CK(up(ctx->d_tp_bbsrc, b.bsrc.data(), sizeof(int) * b.bsrc.size()));
+        // row placements (the lane arrays are built from them on the device)
+        const TaskPlan &f = ctx->pch.tp_f, &b = ctx->pch.tp_b;
         CK(up(ctx->d_tp_fseq, f.seq.data(), sizeof(int) * (size_t)N));
         CK(up(ctx->d_tp_bseq, b.seq.data(), sizeof(int) * (size_t)N));
+        CK(up(ctx->d_tp_fplace, f.lane.data(), sizeof(int) * (size_t)N));
+        CK(up(ctx->d_tp_fplace + N, f.step.data(), sizeof(int) * (size_t)N));
+        CK(up(ctx->d_tp_fplace + 2 * (size_t)N, f.send.data(), sizeof(int) * f.send.size()));
+        CK(up(ctx->d_tp_bplace, b.lane.data(), sizeof(int) * (size_t)N));
+        CK(up(ctx->d_tp_bplace + N, b.step.data(), sizeof(int) * (size_t)N));
+        CK(up(ctx->d_tp_bplace + 2 * (size_t)N, b.send.data(), sizeof(int) * b.send.size()));
     }
     CK(up(ctx->d_lev_f, h.itf.data(), sizeof(int) * N));
     CK(up(ctx->d_lev_b, h.itb.data(), sizeof(int) * N));
-    CK(up(ctx->d_perm, ctx->h_perm.data(), sizeof(int) * N));
+    CK(up(ctx->d_perm, ctx->pch.perm.data(), sizeof(int) * N));
     CK(up(ctx->d_iperm, h.iperm.data(), sizeof(int) * N));
     CK(up(ctx->d_lrow, h.lrow.data(), sizeof(int) * (N + 1)));
     CK(up(ctx->d_urow, h.urow.data(), sizeof(int) * (N + 1)));
-    if (ctx->nlow) {
-        CK(up(ctx->d_lcol, h.lcol.data(), sizeof(int) * ctx->nlow));
-        CK(up(ctx->d_ucol, h.ucol.data(), sizeof(int) * ctx->nlow));
-        CK(up(ctx->d_upos, h.upos.data(), sizeof(int) * ctx->nlow));
+    if (ctx->pch.nlow) {
+        CK(up(ctx->d_lcol, h.lcol.data(), sizeof(int) * ctx->pch.nlow));
+        CK(up(ctx->d_ucol, h.ucol.data(), sizeof(int) * ctx->pch.nlow));
+        CK(up(ctx->d_upos, h.upos.data(), sizeof(int) * ctx->pch.nlow));
     }
     RC(k_build_blocks(ctx, pos));
     // IC(0) with the relaxer doubling on a non-positive pivot (Precondition.cu:1274-1288, 1455-1489)
@@ -564,7 +460,7 @@ static int precond_finish(fsd_ctx *ctx, const float4 *pos)
         if (lambda > 1.0e7f) { ctx->err = "IC(0) did not succeed for any relaxer"; return FSD_E_ICHOL; }
     }
     ctx->ichol_relaxer = lambda;
-    if (ctx->tp_ok) RC(k_sweep_pack(ctx));
+    if (ctx->pch.tp_ok) RC(k_sweep_pack(ctx));
     prof_host(ctx, PROF_FINISH_HOST, wall_ms() - t0);
     prof_end(ctx, PROF_PRECOND_BUILD, ctx->pc_ev);
     ctx->precond_valid = true;
@@ -1200,10 +1096,10 @@ int fsd_build_precond(fsd_ctx *ctx, const float *d_pos4, fsd_solve_stats *stats)
     if (stats) {
         stats->ichol_relaxer = ctx->ichol_relaxer;
         stats->ichol_attempts = ctx->ichol_attempts;
-        stats->nnz = ctx->nnz_ref;
-        stats->lower_blocks = ctx->nlow;
-        stats->levels = ctx->nlev_f;
-        stats->sweep_steps = ctx->tp_ok ? ctx->tp_f.nsteps + ctx->tp_b.nsteps : 0;
+        stats->nnz = ctx->pch.nnz_ref;
+        stats->lower_blocks = ctx->pch.nlow;
+        stats->levels = ctx->pch.nlev_f;
+        stats->sweep_steps = ctx->pch.tp_ok ? ctx->pch.tp_f.nsteps + ctx->pch.tp_b.nsteps : 0;
     }
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1225,25 +1121,26 @@ int fsd_get_precond(fsd_ctx *ctx, int32_t *h_pn, int32_t *h_plist, int64_t plist
 {
     if (!ctx->precond_valid) { ctx->err = "preconditioner not built"; return FSD_E_STATE; }
     const int N = ctx->N;
-    const std::vector<int> &pn = ctx->h_pn, &perm = ctx->h_perm;
+    const std::vector<int> &pn = ctx->h_pn, &perm = ctx->pch.perm;
     std::vector<int> phead(N + 1, 0);
     for (int i = 0; i < N; ++i) phead[i + 1] = phead[i] + pn[i];
     if (h_pn) memcpy(h_pn, pn.data(), sizeof(int) * N);
     if (h_plist) {
-        if ((int64_t)ctx->h_plist.size() > plist_cap) return FSD_E_CAPACITY;
-        memcpy(h_plist, ctx->h_plist.data(), sizeof(int) * ctx->h_plist.size());
+        if ((int64_t)phead[N] > plist_cap) return FSD_E_CAPACITY;
+        for (int i = 0; i < N; ++i)
+            memcpy(h_plist + phead[i], ctx->h_plstride.data() + (size_t)i * FSD_PSTRIDE, sizeof(int) * pn[i]);
     }
     if (h_perm) memcpy(h_perm, perm.data(), sizeof(int) * N);
-    if (nnz_out) *nnz_out = ctx->nnz_ref;
+    if (nnz_out) *nnz_out = ctx->pch.nnz_ref;
     if (!h_rowptr && !h_col && !h_val && !h_prow && !h_pcol && !h_L) return 0;
-    if (ctx->nnz_ref > nnz_cap) return FSD_E_CAPACITY;
+    if (ctx->pch.nnz_ref > nnz_cap) return FSD_E_CAPACITY;
     std::vector<int> iperm(N);
     for (int P = 0; P < N; ++P) iperm[perm[P]] = P;
-    std::vector<int> lrow(N + 1), lcol(ctx->nlow);
-    std::vector<float> Ab(36 * (size_t)ctx->nlow), Ad(36 * (size_t)N), Lb(36 * (size_t)ctx->nlow), Ld(36 * (size_t)N);
+    std::vector<int> lrow(N + 1), lcol(ctx->pch.nlow);
+    std::vector<float> Ab(36 * (size_t)ctx->pch.nlow), Ad(36 * (size_t)N), Lb(36 * (size_t)ctx->pch.nlow), Ld(36 * (size_t)N);
     CK(hipMemcpy(lrow.data(), ctx->d_lrow, sizeof(int) * (N + 1), hipMemcpyDeviceToHost));
-    if (ctx->nlow) {
-        CK(hipMemcpy(lcol.data(), ctx->d_lcol, sizeof(int) * ctx->nlow, hipMemcpyDeviceToHost));
+    if (ctx->pch.nlow) {
+        CK(hipMemcpy(lcol.data(), ctx->d_lcol, sizeof(int) * ctx->pch.nlow, hipMemcpyDeviceToHost));
         CK(hipMemcpy(Ab.data(), ctx->d_Ablk, sizeof(float) * Ab.size(), hipMemcpyDeviceToHost));
         CK(hipMemcpy(Lb.data(), ctx->d_Lblk, sizeof(float) * Lb.size(), hipMemcpyDeviceToHost));
     }
@@ -1274,7 +1171,7 @@ int fsd_get_precond(fsd_ctx *ctx, int32_t *h_pn, int32_t *h_plist, int64_t plist
                     continue;
                 }
                 for (int s = phead[i]; s < phead[i + 1]; ++s) {
-                    const int j = ctx->h_plist[s];
+                    const int j = ctx->h_plstride[(size_t)i * FSD_PSTRIDE + (s - phead[i])];
                     for (int c = 0; c < 6; ++c) {
                         if (h_col) h_col[k] = 6 * j + c;
                         if (h_val) h_val[k] = blockval(Ab, Ad, iperm[i], iperm[j], a, c);
@@ -1293,7 +1190,7 @@ int fsd_get_precond(fsd_ctx *ctx, int32_t *h_pn, int32_t *h_plist, int64_t plist
             const int i = perm[P];
             const bool iso = pn[i] <= 1;
             cols.clear();
-            for (int s = phead[i]; s < phead[i + 1]; ++s) cols.push_back(iperm[ctx->h_plist[s]]);
+            for (int s = 0; s < pn[i]; ++s) cols.push_back(iperm[ctx->h_plstride[(size_t)i * FSD_PSTRIDE + s]]);
             std::sort(cols.begin(), cols.end());
             for (int a = 0; a < 6; ++a) {
                 const int row = 6 * P + a;
@@ -1353,12 +1250,12 @@ int fsd_saddle_solve(fsd_ctx *ctx, const float *d_pos4, const float *d_rhs17, fl
     if (want_true_residual) RC(true_residual(ctx, pos, d_rhs17, d_sol17, &st.true_residual));
     st.ichol_relaxer = ctx->ichol_relaxer;
     st.ichol_attempts = ctx->ichol_attempts;
-    st.nnz = ctx->nnz_ref;
+    st.nnz = ctx->pch.nnz_ref;
     st.nlist_total = ctx->nlist_total;
     st.lub_pairs = ctx->lub_total;
-    st.lower_blocks = ctx->nlow;
-    st.levels = ctx->nlev_f;
-    st.sweep_steps = ctx->tp_ok ? ctx->tp_f.nsteps + ctx->tp_b.nsteps : 0;
+    st.lower_blocks = ctx->pch.nlow;
+    st.levels = ctx->pch.nlev_f;
+    st.sweep_steps = ctx->pch.tp_ok ? ctx->pch.tp_f.nsteps + ctx->pch.tp_b.nsteps : 0;
     if (stats) *stats = st;
     CK(hipStreamSynchronize(ctx->stream));
     return rc;
@@ -1485,12 +1382,12 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     CK(hipStreamSynchronize(ctx->stream));
     st.ichol_relaxer = ctx->ichol_relaxer;
     st.ichol_attempts = ctx->ichol_attempts;
-    st.nnz = ctx->nnz_ref;
+    st.nnz = ctx->pch.nnz_ref;
     st.nlist_total = ctx->nlist_total;
     st.lub_pairs = ctx->lub_total;
-    st.lower_blocks = ctx->nlow;
-    st.levels = ctx->nlev_f;
-    st.sweep_steps = ctx->tp_ok ? ctx->tp_f.nsteps + ctx->tp_b.nsteps : 0;
+    st.lower_blocks = ctx->pch.nlow;
+    st.levels = ctx->pch.nlev_f;
+    st.sweep_steps = ctx->pch.tp_ok ? ctx->pch.tp_f.nsteps + ctx->pch.tp_b.nsteps : 0;
     st.lanczos_ff = (ctx->p.T > 0.f) ? ctx->last_lanczos_ff : 0;
     st.lanczos_nf = (ctx->p.T > 0.f) ? ctx->last_lanczos_nf : 0;
     // relaxer and Krylov-size bookkeeping (Stokes.cc:1658-1666)

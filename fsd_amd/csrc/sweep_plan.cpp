@@ -12,8 +12,7 @@
 
 #include <algorithm>
 
-bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, const int *cols, const int *src,
-                     bool forward, int R, TaskPlan &tp)
+bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, int R, TaskPlan &tp)
 {
     tp.nsteps = 0;   // the vectors keep their capacity from the previous build (no page faults)
     if (R < 3 * TP_ROWS_MAX || (R & (R - 1)) != 0 || N >= (1 << TP_ND_SHIFT)) return false;
@@ -23,7 +22,10 @@ bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, const i
     // rows by level, then by descending count, then by particle: one counting sort on
     // the key level * (TP_WAVE + 1) + (TP_WAVE - count)
     const size_t nkey = (size_t)nlev * (TP_WAVE + 1);
-    std::vector<int> kptr(nkey + 1, 0), order(N), lptr(nlev + 1, 0);
+    std::vector<int> &kptr = tp.kptr, &order = tp.order, &lptr = tp.lptr;
+    kptr.assign(nkey + 1, 0);
+    order.resize(N);
+    lptr.assign(nlev + 1, 0);
     for (int P = 0; P < N; ++P) kptr[(size_t)lev[P] * (TP_WAVE + 1) + (TP_WAVE - cnt(P)) + 1]++;
     for (size_t k = 0; k < nkey; ++k) kptr[k + 1] += kptr[k];
     for (int l = 0; l <= nlev; ++l) lptr[l] = kptr[(size_t)l * (TP_WAVE + 1)];
@@ -31,8 +33,11 @@ bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, const i
 
     // packing: lanes, sequence numbers and steps
     constexpr int NWAVE = TP_LANES / TP_WAVE;
-    std::vector<int> gtid(N, -1), rowstep(N, 0), send;
-    tp.seq.assign(N, 0);
+    std::vector<int> &gtid = tp.lane, &rowstep = tp.step, &send = tp.send;
+    gtid.resize(N);
+    rowstep.resize(N);
+    send.clear();
+    tp.seq.resize(N);
     int nseq = 0, step = -1, rows = 0;
     int fill[NWAVE];
     auto open_step = [&]() {
@@ -71,14 +76,18 @@ bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, const i
         flush();
     }
     tp.nsteps = step + 1;
+    return true;
+}
 
-    // lanes
+bool task_plan_lanes(int N, const int *rowptr, const int *cols, const int *src, bool forward, int R, TaskPlan &tp)
+{
+    const std::vector<int> &send = tp.send;
     auto in_ring = [&](int q, int S) { return q >= send[S] - R; };
     const size_t L = (size_t)tp.nsteps * TP_LANES;
     tp.info.assign(2 * L, -1);   // reuses the capacity of the previous build
     tp.bsrc.assign(2 * L, -1);
     for (int P = 0; P < N; ++P) {
-        const int g = gtid[P], S = rowstep[P], n = cnt(P);
+        const int g = tp.lane[P], S = tp.step[P], n = rowptr[P + 1] - rowptr[P];
         tp.info[2 * (size_t)g + 1] = tp.seq[P] | (n << TP_ND_SHIFT);
         for (int j = 0; j < n; ++j) {
             const int e = rowptr[P] + j, Q = cols[e];
@@ -86,7 +95,7 @@ bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, const i
             if (in_ring(tp.seq[Q], S)) {
                 tp.info[2 * h] = tp.seq[Q] & (R - 1);
             } else {
-                if (rowstep[Q] > S - 3) return false;   // unreachable: three steps fit in the ring
+                if (tp.step[Q] > S - 3) return false;
                 tp.info[2 * h] = TP_GLOBAL | tp.seq[Q];
             }
             tp.bsrc[2 * h] = src ? src[e] : e;

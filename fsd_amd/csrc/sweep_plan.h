@@ -38,19 +38,30 @@ constexpr int TP_ND_SHIFT = 24;   // own field: sequence number | lanes << TP_ND
 // ring when its producer is in steps S - 2 or S - 1 (three steps of at most
 // TP_ROWS_MAX rows fit in the ring) or is otherwise still there, and from
 // global memory when it is older.
+// build_task_plan places the rows (seq, lane, step of every row; the end of
+// every step); task_plan_lanes derives the lane arrays from the placement
+// (the library does that on the GPU, tp_lanes_kernel, from the same rules).
 struct TaskPlan {
     int nsteps = 0;
-    std::vector<int> info;      // 2 per lane
-    std::vector<int> bsrc;      // 2 per lane
     std::vector<int> seq;       // permuted particle -> sequence number
+    std::vector<int> lane;      // permuted particle -> its first lane (step * TP_LANES + lane in the step)
+    std::vector<int> step;      // permuted particle -> its step
+    std::vector<int> send;      // step -> one past the last sequence number of the step
+    std::vector<int> info;      // 2 per lane (task_plan_lanes)
+    std::vector<int> bsrc;      // 2 per lane (task_plan_lanes)
+    std::vector<int> kptr, order, lptr;   // scratch of the placement
 };
 
 // lev[P] in [0, nlev): level of permuted particle P in the sweep's DAG;
-// rowptr/cols: the sweep's dependencies of each P (cols[e] = particle that
-// row P depends on); src[e]: factor block of dependency e (nullptr: e);
-// forward: the block is scaled by the consumer's diagonal block (forward
-// sweep) or by the producer's (backward sweep).  R: ring capacity in rows (a
-// power of two, at least 3 * TP_ROWS_MAX; the kernel uses TP_RING).  Returns
-// false when a row has more than TP_WAVE dependencies or R is unusable.
-bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, const int *cols, const int *src,
-                     bool forward, int R, TaskPlan &tp);
+// rowptr: the start of each P's dependencies in the sweep's dependency list
+// (cols[e] below: the particle that row P depends on); R: ring capacity in rows (a power of two, at least
+// 3 * TP_ROWS_MAX; the kernel uses TP_RING).  Returns false when a row has
+// more than TP_WAVE dependencies or R is unusable.
+bool build_task_plan(int N, const int *lev, int nlev, const int *rowptr, int R, TaskPlan &tp);
+
+// The lane arrays of a placed plan: src[e]: factor block of dependency e
+// (nullptr: e); forward: the block is scaled by the consumer's diagonal block
+// (forward sweep) or by the producer's (backward sweep).  Returns false if a
+// dependency outside the ring is not at least three steps old (it cannot be,
+// as three steps fit in the ring).
+bool task_plan_lanes(int N, const int *rowptr, const int *cols, const int *src, bool forward, int R, TaskPlan &tp);

@@ -70,8 +70,10 @@ int main(int argc, char **argv)
     for (int P = 0; P < N; ++P) { int l = 0; for (int e = lrow[P]; e < lrow[P + 1]; ++e) l = std::max(l, lf[lcol[e]] + 1); lf[P] = l; mf = std::max(mf, l); }
     for (int P = N - 1; P >= 0; --P) { int l = 0; for (int e = urow[P]; e < urow[P + 1]; ++e) l = std::max(l, lb[ucol[e]] + 1); lb[P] = l; mb = std::max(mb, l); }
     TaskPlan tf, tb;
-    if (!build_task_plan(N, lf.data(), mf + 1, lrow.data(), lcol.data(), nullptr, true, R, tf) ||
-        !build_task_plan(N, lb.data(), mb + 1, urow.data(), ucol.data(), upos.data(), false, R, tb)) {
+    if (!build_task_plan(N, lf.data(), mf + 1, lrow.data(), R, tf) ||
+        !build_task_plan(N, lb.data(), mb + 1, urow.data(), R, tb) ||
+        !task_plan_lanes(N, lrow.data(), lcol.data(), nullptr, true, R, tf) ||
+        !task_plan_lanes(N, urow.data(), ucol.data(), upos.data(), false, R, tb)) {
         printf("plan failed\n");
         return 1;
     }
