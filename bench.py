@@ -60,9 +60,9 @@ def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5):
         return 2 * (144 * st["lower_blocks"] + 48 * N) + (48 + 144 + 24 + 48 + 144 + 24 + 44 + 68) * N
     if cls == "ichol":       # one factorisation: A blocks in, L blocks out (lower and diagonal)
         return 144 * 2 * (st["lower_blocks"] + N)
-    if cls == "gmres_vec":   # CGS2 on 17N-float vectors with k basis vectors: (4k + 7) vector passes,
-        k = kbar             # k averaged over the solve's iterations
-        return 4 * 17 * N * (4 * k + 7)
+    if cls == "gmres_vec":   # CGS2 of a 17N-float vector on k basis vectors: two passes of h = V^T w
+        k = kbar             # (k + 1 reads) and w -= V h (k + 1 reads, w in and out), |w'|^2 in the
+        return 4 * 17 * N * (4 * k + 8)   # second (one read); k averaged over the solve's iterations
     return None
 
 

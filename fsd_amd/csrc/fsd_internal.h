@@ -124,6 +124,25 @@ __device__ inline int xcd_remap(int w, int n)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
+// Sum over aligned groups of G lanes (G = 2, 4, 8, 16) with DPP lane
+// exchanges inside a 16-lane row (quad_perm, row_half_mirror, row_mirror):
+// VALU only, no LDS round trip.  Every lane of a group gets the group's sum.
+template <int CTRL>
+__device__ __forceinline__ float dpp_get(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+template <int G>
+__device__ __forceinline__ float group_sum(float v)
+{
+    static_assert(G == 2 || G == 4 || G == 8 || G == 16, "group of 2, 4, 8 or 16 lanes");
+    v += dpp_get<0xB1>(v);                       // quad_perm(1, 0, 3, 2): lane ^ 1
+    if constexpr (G >= 4) v += dpp_get<0x4E>(v);  // quad_perm(2, 3, 0, 1): lane ^ 2
+    if constexpr (G >= 8) v += dpp_get<0x141>(v); // row_half_mirror: the other quad of the 8
+    if constexpr (G >= 16) v += dpp_get<0x140>(v); // row_mirror: the other 8 of the 16
+    return v;
+}
+
 struct LubTable {
     const float *dist;   // 1000
     const float *vals;   // 22000

@@ -353,19 +353,37 @@ __global__ void __launch_bounds__(512) spread_tile2_kernel(SpreadGeom g, const f
     const bool staged = total <= SP2_CAP;
     __syncthreads();
     if (staged) {
+        // staged slots sl = lt and lt + 512 (at most SP2_CAP of them): the halo cell holding the
+        // slot by binary search over the scan (the last cell whose base is <= sl is non-empty and
+        // holds it), then the particle's index and records; the loads of both slots overlap
+        static_assert(SP2_CAP <= 2 * 512, "two staged slots per thread");
+        int q2[2], sl2[2];
 #pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int q = lt * PER + k;
-            if (q < NQ) {
-                const int b = s_beg[q], o = s_base[q];
-                for (int m = 0; m < cnt[k]; ++m) {
-                    const int i = olist[b + m];
-                    s_o[o + m][0] = orig[3 * i]; s_o[o + m][1] = orig[3 * i + 1]; s_o[o + m][2] = orig[3 * i + 2];
-                    const float4 p = pos[i];
-                    s_p[o + m][0] = p.x; s_p[o + m][1] = p.y; s_p[o + m][2] = p.z;
-                    s_r[o + m][0] = rec[3 * (int64_t)i]; s_r[o + m][1] = rec[3 * (int64_t)i + 1];
-                    s_r[o + m][2] = rec[3 * (int64_t)i + 2];
-                }
+        for (int h = 0; h < 2; ++h) {
+            const int sl = lt + h * 512;
+            int lo = 0, hi = NQ - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_base[mid] <= sl) lo = mid; else hi = mid - 1;
+            }
+            q2[h] = lo;
+            sl2[h] = sl;
+        }
+        int i2[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const bool ok = sl2[h] < total;
+            i2[h] = olist[ok ? s_beg[q2[h]] + (sl2[h] - s_base[q2[h]]) : 0];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int sl = sl2[h], i = i2[h];
+            if (sl < total) {
+                s_o[sl][0] = orig[3 * i]; s_o[sl][1] = orig[3 * i + 1]; s_o[sl][2] = orig[3 * i + 2];
+                const float4 p = pos[i];
+                s_p[sl][0] = p.x; s_p[sl][1] = p.y; s_p[sl][2] = p.z;
+                s_r[sl][0] = rec[3 * (int64_t)i]; s_r[sl][1] = rec[3 * (int64_t)i + 1];
+                s_r[sl][2] = rec[3 * (int64_t)i + 2];
             }
         }
     }
@@ -640,6 +658,7 @@ __global__ void __launch_bounds__(512) contract_tile_kernel(SpreadGeom g, const 
 // halo load overlaps another's arithmetic.  Same arithmetic per output as
 // contract_tile_kernel.
 #define CT_NF 6
+#define CT_PCAP 128   // particles of a tile staged in LDS (about 45 in an 8^3 tile at volume fraction 0.45)
 template <int S, int PP>
 __global__ void __launch_bounds__(512) contract_tile2_kernel(SpreadGeom g, const float4 *pos, const int *orig,
                                                              const int *ostart, const int *olist, const float *grid,
@@ -650,6 +669,8 @@ __global__ void __launch_bounds__(512) contract_tile2_kernel(SpreadGeom g, const
     extern __shared__ float s_g[];           // [CT_NF][S][S][S]
     __shared__ int s_run_beg[TILE * TILE], s_run_off[TILE * TILE + 1];
     __shared__ int s_w[3][S];                // wrapped grid index of each halo offset
+    __shared__ int s_pi[CT_PCAP], s_po[CT_PCAP][3];   // staged particles: index, stencil origin
+    __shared__ float s_pp[CT_PCAP][3];                //   and position
     const int item = xcd_remap(blockIdx.x, gridDim.x);   // a slab of tiles per XCD: halos shared in its L2
     const int tile_id = item >> 1, grp = item & 1;
     const int f0 = grp ? CT_NF : 0, nf = grp ? 11 - CT_NF : CT_NF;
@@ -674,11 +695,29 @@ __global__ void __launch_bounds__(512) contract_tile2_kernel(SpreadGeom g, const
         s_w[ax][o] = v;
     }
     __syncthreads();
-    // grid block of this workgroup's fields, wrapped
-    for (int idx = lt; idx < nf * S3; idx += blockDim.x) {
-        const int c = idx / S3, r = idx - c * S3;
-        const int ax = r / (S * S), ay = (r / S) % S, az = r % S;
-        s_g[idx] = grid[(f0 + c) * M + ((int64_t)s_w[0][ax] * g.Ny + s_w[1][ay]) * g.Nz + s_w[2][az]];
+    // grid block of this workgroup's fields, wrapped: the loads of a thread are issued in
+    // batches (the launch has 512 threads), so their latencies overlap
+    {
+        constexpr int NT = 512, ITER = (CT_NF * S3 + NT - 1) / NT, BATCH = 13;
+#pragma unroll
+        for (int b0 = 0; b0 < ITER; b0 += BATCH) {
+            float v[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const int idx = lt + (b0 + u) * NT;
+                const int c = idx / S3, r = idx - c * S3;
+                const int ax = r / (S * S), ay = (r / S) % S, az = r % S;
+                const bool ok = b0 + u < ITER && idx < nf * S3;
+                const int64_t off = ok ? (f0 + c) * M + ((int64_t)s_w[0][ax] * g.Ny + s_w[1][ay]) * g.Nz + s_w[2][az]
+                                       : 0;
+                v[u] = grid[off];
+            }
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const int idx = lt + (b0 + u) * NT;
+                if (b0 + u < ITER && idx < nf * S3) s_g[idx] = v[u];
+            }
+        }
     }
     if (lt == 0) {
         s_run_off[0] = 0;
@@ -690,15 +729,39 @@ __global__ void __launch_bounds__(512) contract_tile2_kernel(SpreadGeom g, const
     const int wv = lt >> 6, lane = lt & 63, nw = blockDim.x >> 6;
     // fields XX XY XZ YZ YY YX ZX ZY (grid order 3..10) -> slots {XX, YX, ZX, ZY, YY, XY, XZ, YZ}
     constexpr int slot_of[11] = {0, 1, 2, 3, 8, 9, 10, 7, 4, 5, 6};
-    for (int q = wv; q < total; q += nw) {
+    // olist entry of the tile's q-th particle (runs are the tile's (x, y) cell columns)
+    auto entry = [&](int q) {
         int lo = 0, hi = nruns - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (s_run_off[mid] <= q) lo = mid; else hi = mid - 1;
         }
-        const int i = olist[s_run_beg[lo] + (q - s_run_off[lo])];
-        const int ox = orig[3 * i], oy = orig[3 * i + 1], oz = orig[3 * i + 2];
+        return s_run_beg[lo] + (q - s_run_off[lo]);
+    };
+    // the tile's particles (index, stencil origin, position) staged in LDS by one thread each,
+    // so the contraction loop below issues no global load (a tile with more than CT_PCAP
+    // particles reads the rest from global memory)
+    const int nst = min(total, CT_PCAP);
+    if (lt < nst) {
+        const int i = olist[entry(lt)];
+        s_pi[lt] = i;
+        s_po[lt][0] = orig[3 * i]; s_po[lt][1] = orig[3 * i + 1]; s_po[lt][2] = orig[3 * i + 2];
         const float4 p = pos[i];
+        s_pp[lt][0] = p.x; s_pp[lt][1] = p.y; s_pp[lt][2] = p.z;
+    }
+    __syncthreads();
+    for (int q = wv; q < total; q += nw) {
+        int i, ox, oy, oz;
+        float4 p;
+        if (q < CT_PCAP) {
+            i = s_pi[q];
+            ox = s_po[q][0]; oy = s_po[q][1]; oz = s_po[q][2];
+            p = make_float4(s_pp[q][0], s_pp[q][1], s_pp[q][2], 0.f);
+        } else {
+            i = olist[entry(q)];
+            ox = orig[3 * i]; oy = orig[3 * i + 1]; oz = orig[3 * i + 2];
+            p = pos[i];
+        }
         const int bx = (((ox % g.Nx) + g.Nx) % g.Nx) - x0;
         const int by = (((oy % g.Ny) + g.Ny) % g.Ny) - y0;
         const int bz = (((oz % g.Nz) + g.Nz) % g.Nz) - z0;
@@ -720,15 +783,22 @@ __global__ void __launch_bounds__(512) contract_tile2_kernel(SpreadGeom g, const
                     if (c < nf) acc[c] += w * s_g[c * S3 + r];
             }
         }
+        // wave sums: rows of 16 by DPP, then the three rows that hold the 36 lanes.  Lane c < nf
+        // writes field c and every other lane writes field 0 again (the same value to the same
+        // address): one store that every lane issues, with no branch around it
+        const int ce = lane < nf ? lane : 0;
+        float mine = 0.f;
+        int slot = 0;
 #pragma unroll
-        for (int c = 0; c < CT_NF; ++c)
-            for (int o = 32; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 64);
-        if (lane == 0) {
-            float *w = wave + 11 * (int64_t)i;
-#pragma unroll
-            for (int c = 0; c < CT_NF; ++c)
-                if (c < nf) w[slot_of[f0 + c]] = acc[c];
+        for (int c = 0; c < CT_NF; ++c) {
+            const float r = group_sum<16>(acc[c]);
+            const float t = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, r), 0)) +
+                            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, r), 16)) +
+                            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, r), 32));
+            mine = (ce == c) ? t : mine;
+            slot = (ce == c) ? (f0 ? slot_of[CT_NF + (c < 11 - CT_NF ? c : 0)] : slot_of[c]) : slot;
         }
+        wave[11 * (int64_t)i + slot] = mine;
     }
 }
 
@@ -838,11 +908,9 @@ __global__ void __launch_bounds__(256) realspace_kernel(RealParams rp, const flo
         }
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-        for (int o = G / 2; o > 0; o >>= 1) u[a] += __shfl_xor(u[a], o, G);
+    for (int a = 0; a < 3; ++a) u[a] = group_sum<G>(u[a]);
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
-        for (int o = G / 2; o > 0; o >>= 1) D[c] += __shfl_xor(D[c], o, G);
+    for (int c = 0; c < 8; ++c) D[c] = group_sum<G>(D[c]);
     if (!active || lane != 0) return;
     const float4 r0 = rec[3 * (int64_t)i], r1 = rec[3 * (int64_t)i + 1], r2 = rec[3 * (int64_t)i + 2];
     if (use_real) {

@@ -395,8 +395,7 @@ __global__ void __launch_bounds__(256) rfu_apply_kernel(int N, const float *U, c
         }
     }
 #pragma unroll
-    for (int r = 0; r < 6; ++r)
-        for (int o = G / 2; o > 0; o >>= 1) acc[r] += __shfl_xor(acc[r], o, G);
+    for (int r = 0; r < 6; ++r) acc[r] = group_sum<G>(acc[r]);
     if (active && lane < 6) {
         const float *S = lub_self + (int64_t)i * 36 + lane * 6;
         const float *ui = U + 6 * (int64_t)i;

@@ -507,7 +507,12 @@ static int gmres(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, floa
         if (tres <= ttol || total >= maxit) break;
         goal = goal * std::max(0.1f, 0.9f * ttol / tres);
     }
-    if (st) { st->iterations = total; st->restarts = restarts; st->true_residual = tres; st->initial_residual = r0; }
+    if (st) {
+        st->iterations = total;
+        st->restarts = restarts;
+        st->true_residual = tres;
+        st->initial_residual = r0;
+    }
     return (tres <= ttol) ? 0 : FSD_W_NOT_CONVERGED;
 }
 
@@ -558,30 +563,36 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
         do {
             ++i;
             ++iters;
+            // the new direction w is formed in place of the next basis vector
             float *vi = V + (int64_t)i * n, *vn = V + (int64_t)(i + 1) * n;
             if (right) {
                 float *zi = Z + (int64_t)i * n;
                 RC(k_saddle_precond(ctx, vi, zi));
-                RC(k_saddle_multiply(ctx, pos, zi, w));
+                RC(k_saddle_multiply(ctx, pos, zi, vn));
             } else {
                 RC(k_saddle_multiply(ctx, pos, vi, t));
-                RC(k_saddle_precond(ctx, t, w));
+                RC(k_saddle_precond(ctx, t, vn));
             }
             hipEvent_t ev;
             prof_begin(ctx, PROF_GMRES_VEC, &ev);
-            // CGS2: two passes of h = V^T w, w -= V h
-            RC(k_multidot(ctx, n, i + 1, V, n, w, d_h));
-            RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h, w));
-            RC(k_multidot(ctx, n, i + 1, V, n, w, d_h + KM));
-            RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h + KM, w));
-            RC(k_nrm2(ctx, n, w, d_h + 2 * KM));
+            // CGS2: two passes of h = V^T w, w -= V h.  The second pass also forms |w'|^2 (w' is
+            // the next basis slot itself); after it |w''|^2 = |w'|^2 - |h2|^2, with h2 tiny, so
+            // the norm needs no pass of its own
+            RC(k_multidot(ctx, n, i + 1, V, n, vn, d_h));
+            RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h, vn));
+            RC(k_multidot(ctx, n, i + 2, V, n, vn, d_h + KM));
+            RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h + KM, vn));
             prof_end(ctx, PROF_GMRES_VEC, ev);
-            CK(hipMemcpyAsync(ctx->h_part, d_h, sizeof(double) * (2 * KM + 1), hipMemcpyDeviceToHost, ctx->stream));
+            CK(hipMemcpyAsync(ctx->h_part, d_h, sizeof(double) * (KM + i + 2), hipMemcpyDeviceToHost, ctx->stream));
             CK(hipStreamSynchronize(ctx->stream));
-            const double hn = std::sqrt(ctx->h_part[2 * KM]);
-            for (int k = 0; k <= i; ++k) H[(size_t)k * R + i] = ctx->h_part[k] + ctx->h_part[KM + k];
+            double hh = 0.0;
+            for (int k = 0; k <= i; ++k) {
+                H[(size_t)k * R + i] = ctx->h_part[k] + ctx->h_part[KM + k];
+                hh += ctx->h_part[KM + k] * ctx->h_part[KM + k];
+            }
+            const double hn = std::sqrt(std::max(ctx->h_part[KM + i + 1] - hh, 0.0));
             H[(size_t)(i + 1) * R + i] = hn;
-            RC(k_axpby(ctx, n, hn > 0.0 ? (float)(1.0 / hn) : 0.0f, w, 0.0f, nullptr, vn));
+            RC(k_axpby(ctx, n, hn > 0.0 ? (float)(1.0 / hn) : 0.0f, vn, 0.0f, nullptr, vn));
             for (int k = 0; k < i; ++k) {
                 const double a = H[(size_t)k * R + i], c = H[(size_t)(k + 1) * R + i];
                 H[(size_t)k * R + i] = cs[k] * a + sn[k] * c;
