@@ -167,6 +167,9 @@ def main():
         infos.append(info)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    print("# timed steps: iterations %s, initial residuals %s"
+          % ([i["iterations"] for i in infos[-args.steps:]],
+             ["%.2e" % i["initial_residual"] for i in infos[-args.steps:]]), file=sys.stderr, flush=True)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     launches, kms = st.profile_get(dom)
     last = infos[-1]

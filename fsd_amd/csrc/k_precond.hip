@@ -769,52 +769,51 @@ __global__ void tp_post_kernel(int N, const int *seq, const int *perm, const flo
 __global__ void tp_pre_diff_kernel(int N, const int *seq, const int *perm, const float *Z, const float *bF,
                                    const float *bU, float4 *inp)
 {
-    const int P = blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= N) return;
-    const float *Zp = Z + (int64_t)P * 36;
+    // six threads per particle: thread r forms component r (Z_P is lower triangular)
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 6 * (int64_t)N) return;
+    const int P = (int)(q / 6), r = (int)(q - 6 * (int64_t)P);
+    const float *Zr = Z + (int64_t)P * 36 + 6 * r;
     const int64_t src = 6 * (int64_t)perm[P];
-    float d[6], a[6];
-    for (int c = 0; c < 6; ++c) d[c] = bF[src + c] - bU[src + c];
-    for (int r = 0; r < 6; ++r) {
-        float sa = 0.f;
-        for (int c = 0; c <= r; ++c) sa += Zp[6 * r + c] * d[c];
-        a[r] = sa;
-    }
+    float sa = 0.f;
+    for (int c = 0; c <= r; ++c) sa += Zr[c] * (bF[src + c] - bU[src + c]);
     const int k = seq[P];
-    inp[k] = make_float4(a[0], a[1], a[2], a[3]);
-    reinterpret_cast<float2 *>(inp + N + k)[0] = make_float2(a[4], a[5]);
+    float *dst = r < 4 ? reinterpret_cast<float *>(inp + k) + r : reinterpret_cast<float *>(inp + N + k) + (r - 4);
+    *dst = sa;
 }
 
-// backward inputs of right-hand side 0 from its forward results
+// backward inputs of right-hand side 0 from its forward results (two threads per particle:
+// the float4 and the float2 of the row)
 __global__ void tp_mid1_kernel(int N, const int *seq_f, const int *seq_b, const float4 *res_f, float4 *inp_b)
 {
-    const int P = blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= N) return;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 2 * (int64_t)N) return;
+    const int P = (int)(q >> 1);
     const int kf = seq_f[P], kb = seq_b[P];
-    inp_b[kb] = res_f[kf];
-    reinterpret_cast<float2 *>(inp_b + N + kb)[0] = reinterpret_cast<const float2 *>(res_f + N + kf)[0];
+    if ((q & 1) == 0) inp_b[kb] = res_f[kf];
+    else reinterpret_cast<float2 *>(inp_b + N + kb)[0] = reinterpret_cast<const float2 *>(res_f + N + kf)[0];
 }
 
 // g = Z_P^T u_P (u: right-hand side 0 of the backward results), then
-// x_F = b_F - g, x_S = b_S, x_U = g for particle perm[P]
+// x_F = b_F - g, x_S = b_S, x_U = g for particle perm[P]; six threads per
+// particle, thread a forms component a (and copies stresslet component a < 5)
 __global__ void tp_post_saddle_kernel(int N, const int *seq, const int *perm, const float *Z, const float4 *res,
                                       const float *b, float *x)
 {
-    const int P = blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= N) return;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 6 * (int64_t)N) return;
+    const int P = (int)(q / 6), a = (int)(q - 6 * (int64_t)P);
     const int k = seq[P];
     const float4 w0 = res[k];
     const float2 w1 = reinterpret_cast<const float2 *>(res + N + k)[0];
     const float u[6] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y};
     const float *Zp = Z + (int64_t)P * 36;
     const int64_t i = perm[P], n6 = 6 * (int64_t)N, n11 = 11 * (int64_t)N;
-    for (int a = 0; a < 6; ++a) {
-        float g = 0.f;
-        for (int c = a; c < 6; ++c) g += Zp[6 * c + a] * u[c];
-        x[6 * i + a] = b[6 * i + a] - g;
-        x[n11 + 6 * i + a] = g;
-    }
-    for (int j = 0; j < 5; ++j) x[n6 + 5 * i + j] = b[n6 + 5 * i + j];
+    float g = 0.f;
+    for (int c = a; c < 6; ++c) g += Zp[6 * c + a] * u[c];
+    x[6 * i + a] = b[6 * i + a] - g;
+    x[n11 + 6 * i + a] = g;
+    if (a < 5) x[n6 + 5 * i + a] = b[n6 + 5 * i + a];
 }
 
 // x = saddle preconditioner of b through the one-launch sweeps; returns 1 (nothing done)
@@ -827,17 +826,17 @@ int k_saddle_precond_tp(fsd_ctx *ctx, const float *b, float *x)
     prof_begin(ctx, PROF_TRSV, &ev);
     TaskDev F{ctx->pch.tp_f.nsteps, N, ctx->d_tp_finfo, ctx->d_tp_fblk, ctx->d_tp_finp, ctx->d_tp_fres};
     TaskDev B{ctx->pch.tp_b.nsteps, N, ctx->d_tp_binfo, ctx->d_tp_bblk, ctx->d_tp_binp, ctx->d_tp_bres};
-    hipLaunchKernelGGL(tp_pre_diff_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_fseq,
+    hipLaunchKernelGGL(tp_pre_diff_kernel, dim3(nblk(6 * (int64_t)N, 256)), dim3(256), 0, ctx->stream, N, ctx->d_tp_fseq,
                        ctx->d_perm, ctx->d_tp_Z, b, b + 11 * (int64_t)N, ctx->d_tp_finp);
     prof_begin(ctx, PROF_SWEEP, &es);
     hipLaunchKernelGGL(tp_sweep1_kernel, dim3(1), dim3(TP_LANES), TP1_LDS, ctx->stream, F);
     prof_end(ctx, PROF_SWEEP, es);
-    hipLaunchKernelGGL(tp_mid1_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->d_tp_fseq,
+    hipLaunchKernelGGL(tp_mid1_kernel, dim3(nblk(2 * (int64_t)N, 256)), dim3(256), 0, ctx->stream, N, ctx->d_tp_fseq,
                        ctx->d_tp_bseq, ctx->d_tp_fres, ctx->d_tp_binp);
     prof_begin(ctx, PROF_SWEEP, &es);
     hipLaunchKernelGGL(tp_sweep1_kernel, dim3(1), dim3(TP_LANES), TP1_LDS, ctx->stream, B);
     prof_end(ctx, PROF_SWEEP, es);
-    hipLaunchKernelGGL(tp_post_saddle_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_tp_bseq,
+    hipLaunchKernelGGL(tp_post_saddle_kernel, dim3(nblk(6 * (int64_t)N, 256)), dim3(256), 0, ctx->stream, N, ctx->d_tp_bseq,
                        ctx->d_perm, ctx->d_tp_Z, ctx->d_tp_bres, b, x);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_TRSV, ev);

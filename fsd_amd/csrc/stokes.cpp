@@ -483,7 +483,7 @@ static int build_precond(fsd_ctx *ctx, const float4 *pos)
 static int true_residual(fsd_ctx *ctx, const float4 *pos, const float *b, const float *x, float *out);
 
 static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
-                      bool right, fsd_solve_stats *st);
+                      bool right, bool r0_in_t, fsd_solve_stats *st);
 
 // Default: CUSP semantics (left preconditioning, converged when the
 // preconditioned residual estimate |s| <= tol ||b||, Solvers.cu:55-126).
@@ -492,14 +492,14 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
 // gmres_true_tol, and the true residual is then recomputed and the solve
 // continued (warm) if FP32 drift left it above the tolerance.
 static int gmres(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
-                 fsd_solve_stats *st)
+                 fsd_solve_stats *st, bool r0_in_t = false)
 {
     const float ttol = ctx->p.gmres_true_tol;
-    if (ttol <= 0.f) return gmres_core(ctx, pos, b, x, tol, maxit, R, false, st);
+    if (ttol <= 0.f) return gmres_core(ctx, pos, b, x, tol, maxit, R, false, false, st);
     int total = 0, restarts = 0, rc = 0;
     float tres = 0.f, goal = ttol, r0 = -1.f;
     for (int round = 0; round < 6; ++round) {
-        rc = gmres_core(ctx, pos, b, x, goal, maxit - total, R, true, st);
+        rc = gmres_core(ctx, pos, b, x, goal, maxit - total, R, true, r0_in_t && round == 0, st);
         if (rc < 0) return rc;
         if (st && round == 0) r0 = st->initial_residual;
         if (st) { total += st->iterations; restarts += st->restarts; }
@@ -521,7 +521,7 @@ static int gmres(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, floa
 // FP64 on the host.  left: Krylov space of M^-1 A, residual M^-1 (b - A x);
 // right: Krylov space of A M^-1 with Z = M^-1 V kept, residual b - A x.
 static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
-                      bool right, fsd_solve_stats *st)
+                      bool right, bool r0_in_t, fsd_solve_stats *st)
 {
     const int64_t n = 17 * (int64_t)ctx->N;
     float *V = ctx->d_V, *w = ctx->d_w, *t = ctx->d_t;
@@ -542,9 +542,13 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
     int iters = 0, restarts = 0;
     double resid = 0.0;
     for (;;) {
-        // r = A x - b (left: preconditioned); V_0 = -r / |r|
-        RC(k_saddle_multiply(ctx, pos, x, t));
-        RC(k_axpby(ctx, n, 1.0f, t, -1.0f, b, t));
+        // r = A x - b (left: preconditioned); V_0 = -r / |r|.  On the first cycle of a
+        // right-preconditioned solve the caller may have formed A x - b in t already (r0_in_t).
+        if (!(right && r0_in_t)) {
+            RC(k_saddle_multiply(ctx, pos, x, t));
+            RC(k_axpby(ctx, n, 1.0f, t, -1.0f, b, t));
+        }
+        r0_in_t = false;
         float *r = t;
         if (!right) {
             RC(k_saddle_precond(ctx, t, w));
@@ -1314,6 +1318,28 @@ int fsd_rk2_stage2(fsd_ctx *ctx, float coef1, const float *d_pos_in1_4, const fl
     return 0;
 }
 
+// initial guess of a step's solve in d_sol: the previous step's solution (Integrator.cu:777), or
+// its linear (guess 1) or quadratic (guess 2) extrapolation, of the highest order the solutions
+// kept allow.  d_sol = x_{n-1}, d_sol_prev = x_{n-2}, d_sol_prev2 = x_{n-3}
+static int initial_guess(fsd_ctx *ctx)
+{
+    const int64_t n17 = 17 * (int64_t)ctx->N;
+    if (ctx->p.guess < 1) return 0;
+    {
+        const int order = std::min((int)ctx->p.guess, (int)ctx->n_solved - 1);
+        if (order >= 1) RC(k_copy(ctx, n17, ctx->d_sol, ctx->d_t));
+        if (order == 1) {
+            RC(k_axpby(ctx, n17, 2.0f, ctx->d_sol, -1.0f, ctx->d_sol_prev, ctx->d_sol));
+        } else if (order == 2) {
+            RC(k_axpby(ctx, n17, 3.0f, ctx->d_sol, -3.0f, ctx->d_sol_prev, ctx->d_sol));
+            RC(k_axpby(ctx, n17, 1.0f, ctx->d_sol, 1.0f, ctx->d_sol_prev2, ctx->d_sol));
+        }
+        if (ctx->p.guess >= 2) std::swap(ctx->d_sol_prev, ctx->d_sol_prev2);   // x_{n-2} becomes the older one
+        RC(k_copy(ctx, n17, order >= 1 ? ctx->d_t : ctx->d_sol, ctx->d_sol_prev));
+    }
+    return 0;
+}
+
 // Stokes::integrateStepOne -> Stokes_StepOne (Stokes.cc:1197-1683, Stokes.cu:119-273)
 int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, float *d_ori3, float *d_pos_gb4,
              int32_t *d_image3, const float *d_force6, const float *d_noise_ang3, float *d_vel11,
@@ -1330,6 +1356,8 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     // kernels that do not depend on it are enqueued
     RC(precond_fetch(ctx, pos));
     std::thread planner(precond_plan, ctx);
+    const bool brownian = ctx->p.T > 0.f;
+    bool r0_in_t = false;   // d_t = A x0 - b formed below
     auto independent = [&]() -> int {
         // applied force: injected, or the reference force law with the external torque
         if (d_force6) {
@@ -1343,41 +1371,40 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
         RC(k_fill(ctx, n17, 0.f, ctx->d_rhs));
         RC(k_build_lub_pairs(ctx, pos));
         RC(k_setup_spread(ctx, pos));
+        if (!brownian) {
+            // without Brownian terms nothing else enters the right-hand side, and the first residual
+            // of a right-preconditioned solve needs no preconditioner: both are formed here
+            RC(k_strain_rhs(ctx, ctx->d_rhs, shear_rate, d_ori3, ctx->d_force));
+            RC(k_rfe(ctx, pos, shear_rate, d_ori3, ctx->d_rhs + 11 * (int64_t)N));
+            RC(initial_guess(ctx));
+            if (ctx->p.gmres_true_tol > 0.f) {
+                RC(k_saddle_multiply(ctx, pos, ctx->d_sol, ctx->d_t));
+                RC(k_axpby(ctx, n17, 1.0f, ctx->d_t, -1.0f, ctx->d_rhs, ctx->d_t));
+                r0_in_t = true;
+            }
+        }
         return 0;
     };
     const int rci = independent();
     planner.join();
     if (rci < 0) return rci;
     RC(precond_finish(ctx, pos));
-    if (ctx->p.T > 0.f) {
+    if (brownian) {
         // rhs[0:11N] = U_B^ff, rhs[11N:17N] = F_B^nf, Velocity[0:6N] = RFD drift
         RC(alloc_brownian(ctx));
         RC(bro_farfield(ctx, pos, timestep, ctx->d_rhs));
         RC(bro_nearfield(ctx, pos, timestep, ctx->d_rhs + 11 * (int64_t)N));
         RC(bro_rfd(ctx, pos, timestep, d_vel11));
+        RC(k_strain_rhs(ctx, ctx->d_rhs, shear_rate, d_ori3, ctx->d_force));
+        RC(k_rfe(ctx, pos, shear_rate, d_ori3, ctx->d_rhs + 11 * (int64_t)N));
+        RC(initial_guess(ctx));
     }
-    RC(k_strain_rhs(ctx, ctx->d_rhs, shear_rate, d_ori3, ctx->d_force));
-    RC(k_rfe(ctx, pos, shear_rate, d_ori3, ctx->d_rhs + 11 * (int64_t)N));
     // saddle point solve, warm-started from the previous step's solution
     fsd_solve_stats st;
     memset(&st, 0, sizeof st);
     st.true_residual = -1.f;
-    // initial guess: the previous step's solution (Integrator.cu:777), or the linear (guess 1) or
-    // quadratic (guess 2) extrapolation of the last solutions, of the highest order they allow.
-    // d_sol = x_{n-1}, d_sol_prev = x_{n-2}, d_sol_prev2 = x_{n-3}
-    if (ctx->p.guess >= 1) {
-        const int order = std::min((int)ctx->p.guess, (int)ctx->n_solved - 1);
-        if (order >= 1) RC(k_copy(ctx, n17, ctx->d_sol, ctx->d_t));
-        if (order == 1) {
-            RC(k_axpby(ctx, n17, 2.0f, ctx->d_sol, -1.0f, ctx->d_sol_prev, ctx->d_sol));
-        } else if (order == 2) {
-            RC(k_axpby(ctx, n17, 3.0f, ctx->d_sol, -3.0f, ctx->d_sol_prev, ctx->d_sol));
-            RC(k_axpby(ctx, n17, 1.0f, ctx->d_sol, 1.0f, ctx->d_sol_prev2, ctx->d_sol));
-        }
-        if (ctx->p.guess >= 2) std::swap(ctx->d_sol_prev, ctx->d_sol_prev2);   // x_{n-2} becomes the older one
-        RC(k_copy(ctx, n17, order >= 1 ? ctx->d_t : ctx->d_sol, ctx->d_sol_prev));
-    }
-    int rc = gmres(ctx, pos, ctx->d_rhs, ctx->d_sol, ctx->p.gmres_tol, ctx->p.gmres_maxiter, ctx->p.gmres_restart, &st);
+    int rc = gmres(ctx, pos, ctx->d_rhs, ctx->d_sol, ctx->p.gmres_tol, ctx->p.gmres_maxiter, ctx->p.gmres_restart, &st,
+                   r0_in_t);
     ctx->n_solved++;
     if (rc < 0) return rc;
     RC(k_axpby(ctx, 6 * (int64_t)N, 1.0f, ctx->d_sol + 11 * (int64_t)N, 1.0f, d_vel11, d_vel11));
