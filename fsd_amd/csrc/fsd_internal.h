@@ -275,6 +275,7 @@ struct fsd_ctx {
     float *d_V = nullptr, *d_w = nullptr, *d_t = nullptr, *d_t2 = nullptr;
     float *d_Z = nullptr;         // M^-1 V of the right-preconditioned solve (allocated on first use)
     double *d_part = nullptr; int part_blocks = 0; double *h_part = nullptr;
+    hipEvent_t gm_ev[2] = {nullptr, nullptr};   // readbacks of the Hessenberg columns (two slots)
     int kmax = 64;                // most vectors of one multidot (restart + 1, at least 64)
     float *d_rhs = nullptr, *d_sol = nullptr;
     float *d_sol_prev = nullptr;  // x_{n-2} when the initial guess extrapolates (guess >= 1)
@@ -363,6 +364,7 @@ int k_fill(fsd_ctx *ctx, int64_t n, float v, float *y);
 int k_multidot(fsd_ctx *ctx, int64_t n, int k, const float *V, int64_t ldv, const float *w, double *out);
 int k_multi_axpy(fsd_ctx *ctx, int64_t n, int k, const float *V, int64_t ldv, const double *h, float *w);
 int k_nrm2(fsd_ctx *ctx, int64_t n, const float *x, double *out);
+int k_scale_by_norm(fsd_ctx *ctx, int64_t n, float *v, const double *h, int K, double *hn_out);
 int k_saddle_precond(fsd_ctx *ctx, const float *b, float *x);
 int k_saddle_multiply(fsd_ctx *ctx, const float4 *pos, const float *x, float *y);
 

@@ -2,6 +2,8 @@
 // right-hand-side assembly and the explicit Euler step with shear.
 #include "fsd_internal.h"
 
+#include <algorithm>
+
 // ------------------------------------------------------------ vectors
 __global__ void axpby_kernel(int64_t n, float a, const float *x, float b, const float *y, float *z)
 {
@@ -100,6 +102,37 @@ int k_multi_axpy(fsd_ctx *ctx, int64_t n, int K, const float *V, int64_t ldv, co
 {
     if (K > 256) { ctx->err = "multi_axpy: more than 256 vectors"; return FSD_E_PARAM; }
     hipLaunchKernelGGL(multi_axpy_kernel, dim3(nblk(n, 256)), dim3(256), 0, ctx->stream, n, K, V, ldv, h, w);
+    FSD_HIP(hipGetLastError());
+    return 0;
+}
+
+// v *= 1 / hn, hn = sqrt(max(h[K] - sum_{k<K} h[k]^2, 0)): the norm of the second
+// Gram-Schmidt pass's result from its dot products (h[0..K)) and |w'|^2 (h[K]).
+// Every block forms hn the same way; block 0 stores it in *hn_out.
+__global__ void __launch_bounds__(256) scale_by_norm_kernel(int64_t n, float *v, const double *h, int K,
+                                                            double *hn_out)
+{
+    __shared__ float s_inv;
+    if (threadIdx.x < 64) {
+        double acc = 0.0;
+        for (int k = threadIdx.x; k < K; k += 64) acc += h[k] * h[k];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (threadIdx.x == 0) {
+            const double hn = sqrt(fmax(h[K] - acc, 0.0));
+            s_inv = hn > 0.0 ? (float)(1.0 / hn) : 0.f;
+            if (blockIdx.x == 0) *hn_out = hn;
+        }
+    }
+    __syncthreads();
+    const float inv = s_inv;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        v[q] *= inv;
+}
+
+int k_scale_by_norm(fsd_ctx *ctx, int64_t n, float *v, const double *h, int K, double *hn_out)
+{
+    hipLaunchKernelGGL(scale_by_norm_kernel, dim3(std::min<int64_t>(nblk(n, 256), 1024)), dim3(256), 0, ctx->stream,
+                       n, v, h, K, hn_out);
     FSD_HIP(hipGetLastError());
     return 0;
 }

@@ -288,7 +288,11 @@ static int alloc_all(fsd_ctx *ctx)
     ctx->part_blocks = (int)((n17 + 4095) / 4096);
     ctx->kmax = std::max(64, ctx->p.gmres_restart + 1);
     CK(hipMalloc(&ctx->d_part, sizeof(double) * ((size_t)ctx->part_blocks * ctx->kmax + 2 * ctx->kmax + 2)));
-    CK(hipHostMalloc((void **)&ctx->h_part, sizeof(double) * (2 * ctx->kmax + 2), hipHostMallocDefault));
+    // pinned: two slots of 2 kmax + 2 doubles (Hessenberg columns read back while the next
+    // iteration runs), then kmax solution coefficients
+    CK(hipHostMalloc((void **)&ctx->h_part, sizeof(double) * (2 * (2 * ctx->kmax + 2) + ctx->kmax),
+                     hipHostMallocDefault));
+    for (hipEvent_t *e : {&ctx->gm_ev[0], &ctx->gm_ev[1]}) CK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     CK(hipMalloc(&ctx->d_rhs, sizeof(float) * n17));
     CK(hipMalloc(&ctx->d_sol, sizeof(float) * n17));
     CK(hipMemset(ctx->d_sol, 0, sizeof(float) * n17));
@@ -335,6 +339,8 @@ static void free_all(fsd_ctx *ctx)
         if (p) hipFree(p);
     if (ctx->h_flags) hipHostFree(ctx->h_flags);
     if (ctx->h_part) hipHostFree(ctx->h_part);
+    for (hipEvent_t e : ctx->gm_ev)
+        if (e) hipEventDestroy(e);
     if (ctx->h_lan) hipHostFree(ctx->h_lan);
     if (ctx->plan_fwd) rocfft_plan_destroy(ctx->plan_fwd);
     if (ctx->plan_inv) rocfft_plan_destroy(ctx->plan_inv);
@@ -527,7 +533,7 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
     float *V = ctx->d_V, *w = ctx->d_w, *t = ctx->d_t;
     if (right && !ctx->d_Z) CK(hipMalloc(&ctx->d_Z, sizeof(float) * n * R));
     float *Z = ctx->d_Z;
-    const int KM = ctx->kmax;
+    const int KM = ctx->kmax, HS = 2 * KM + 2;
     double *d_h = ctx->d_part + (int64_t)ctx->part_blocks * KM;   // 2 kmax + 2 doubles after the partials
     std::vector<double> H((size_t)(R + 1) * R, 0.0), cs(R), sn(R), g(R + 1), yv(R + 1);
     auto hostsync = [&](int count) -> int {
@@ -563,18 +569,19 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
         RC(k_axpby(ctx, n, (float)(-1.0 / beta), r, 0.0f, nullptr, V));
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
-        int i = -1;
-        do {
-            ++i;
-            ++iters;
+        // Arnoldi step j runs entirely on the device (its new basis vector is normalised there), and
+        // its Hessenberg column is read back into slot j & 1 of the pinned buffer; so step j + 1 can
+        // be enqueued before the host has that column, and the host waits only for it.  Near
+        // convergence (estimate within 1.5 x the target) the next step is enqueued after the check.
+        auto enqueue = [&](int j) -> int {
             // the new direction w is formed in place of the next basis vector
-            float *vi = V + (int64_t)i * n, *vn = V + (int64_t)(i + 1) * n;
+            float *vj = V + (int64_t)j * n, *vn = V + (int64_t)(j + 1) * n;
             if (right) {
-                float *zi = Z + (int64_t)i * n;
-                RC(k_saddle_precond(ctx, vi, zi));
-                RC(k_saddle_multiply(ctx, pos, zi, vn));
+                float *zj = Z + (int64_t)j * n;
+                RC(k_saddle_precond(ctx, vj, zj));
+                RC(k_saddle_multiply(ctx, pos, zj, vn));
             } else {
-                RC(k_saddle_multiply(ctx, pos, vi, t));
+                RC(k_saddle_multiply(ctx, pos, vj, t));
                 RC(k_saddle_precond(ctx, t, vn));
             }
             hipEvent_t ev;
@@ -582,21 +589,30 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
             // CGS2: two passes of h = V^T w, w -= V h.  The second pass also forms |w'|^2 (w' is
             // the next basis slot itself); after it |w''|^2 = |w'|^2 - |h2|^2, with h2 tiny, so
             // the norm needs no pass of its own
-            RC(k_multidot(ctx, n, i + 1, V, n, vn, d_h));
-            RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h, vn));
-            RC(k_multidot(ctx, n, i + 2, V, n, vn, d_h + KM));
-            RC(k_multi_axpy(ctx, n, i + 1, V, n, d_h + KM, vn));
+            RC(k_multidot(ctx, n, j + 1, V, n, vn, d_h));
+            RC(k_multi_axpy(ctx, n, j + 1, V, n, d_h, vn));
+            RC(k_multidot(ctx, n, j + 2, V, n, vn, d_h + KM));
+            RC(k_multi_axpy(ctx, n, j + 1, V, n, d_h + KM, vn));
+            RC(k_scale_by_norm(ctx, n, vn, d_h + KM, j + 1, d_h + 2 * KM + 1));
             prof_end(ctx, PROF_GMRES_VEC, ev);
-            CK(hipMemcpyAsync(ctx->h_part, d_h, sizeof(double) * (KM + i + 2), hipMemcpyDeviceToHost, ctx->stream));
-            CK(hipStreamSynchronize(ctx->stream));
-            double hh = 0.0;
-            for (int k = 0; k <= i; ++k) {
-                H[(size_t)k * R + i] = ctx->h_part[k] + ctx->h_part[KM + k];
-                hh += ctx->h_part[KM + k] * ctx->h_part[KM + k];
+            CK(hipMemcpyAsync(ctx->h_part + (size_t)(j & 1) * HS, d_h, sizeof(double) * HS, hipMemcpyDeviceToHost,
+                              ctx->stream));
+            CK(hipEventRecord(ctx->gm_ev[j & 1], ctx->stream));
+            return 0;
+        };
+        RC(enqueue(0));
+        int i = -1, queued = 0;
+        do {
+            ++i;
+            ++iters;
+            if (queued == i && i + 1 < R && iters < maxit && resid > 1.5 * target) {
+                RC(enqueue(i + 1));
+                queued = i + 1;
             }
-            const double hn = std::sqrt(std::max(ctx->h_part[KM + i + 1] - hh, 0.0));
-            H[(size_t)(i + 1) * R + i] = hn;
-            RC(k_axpby(ctx, n, hn > 0.0 ? (float)(1.0 / hn) : 0.0f, vn, 0.0f, nullptr, vn));
+            CK(hipEventSynchronize(ctx->gm_ev[i & 1]));
+            const double *hs = ctx->h_part + (size_t)(i & 1) * HS;
+            for (int k = 0; k <= i; ++k) H[(size_t)k * R + i] = hs[k] + hs[KM + k];
+            H[(size_t)(i + 1) * R + i] = hs[2 * KM + 1];
             for (int k = 0; k < i; ++k) {
                 const double a = H[(size_t)k * R + i], c = H[(size_t)(k + 1) * R + i];
                 H[(size_t)k * R + i] = cs[k] * a + sn[k] * c;
@@ -611,6 +627,10 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
             g[i + 1] = -sn[i] * g[i];
             g[i] = cs[i] * g[i];
             resid = std::fabs(g[i + 1]);
+            if (!(resid <= target) && iters < maxit && i < R - 1 && queued == i) {
+                RC(enqueue(i + 1));
+                queued = i + 1;
+            }
         } while (!(resid <= target) && iters < maxit && i < R - 1);
         // y = H^-1 g ; x += V y (left) or Z y (right)
         for (int j = i; j >= 0; --j) {
@@ -618,8 +638,9 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
             for (int k = j + 1; k <= i; ++k) s -= H[(size_t)j * R + k] * yv[k];
             yv[j] = s / H[(size_t)j * R + j];
         }
-        for (int j = 0; j <= i; ++j) ctx->h_part[j] = -yv[j];
-        CK(hipMemcpyAsync(d_h, ctx->h_part, sizeof(double) * (i + 1), hipMemcpyHostToDevice, ctx->stream));
+        double *coef = ctx->h_part + 2 * (size_t)HS;
+        for (int j = 0; j <= i; ++j) coef[j] = -yv[j];
+        CK(hipMemcpyAsync(d_h, coef, sizeof(double) * (i + 1), hipMemcpyHostToDevice, ctx->stream));
         RC(k_multi_axpy(ctx, n, i + 1, right ? Z : V, n, d_h, x));
         CK(hipStreamSynchronize(ctx->stream));
         ++restarts;
