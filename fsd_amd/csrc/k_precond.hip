@@ -511,9 +511,10 @@ __global__ void __launch_bounds__(TP_LANES) tp_sweep_kernel(TaskDev S)
 // blocks.  Row values of right-hand side r are the 6 floats (a0..a5) of the
 // [3][nrows] float4 layout: r = 0 at (f4[0], f4[1].xy), r = 1 at
 // (f4[1].zw, f4[2]).
-__device__ __forceinline__ void row_ld(const float4 *A, uint32_t nr, uint32_t k, int r, float *v)
+template <int r>
+__device__ __forceinline__ void row_ld(const float4 *A, uint32_t nr, uint32_t k, float *v)
 {
-    if (r == 0) {
+    if constexpr (r == 0) {
         const float4 a = A[k];
         const float2 b = reinterpret_cast<const float2 *>(A + nr + k)[0];
         v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y;
@@ -524,9 +525,10 @@ __device__ __forceinline__ void row_ld(const float4 *A, uint32_t nr, uint32_t k,
     }
 }
 
-__device__ __forceinline__ void row_st(float4 *A, uint32_t nr, uint32_t k, int r, const float *v)
+template <int r>
+__device__ __forceinline__ void row_st(float4 *A, uint32_t nr, uint32_t k, const float *v)
 {
-    if (r == 0) {
+    if constexpr (r == 0) {
         A[k] = make_float4(v[0], v[1], v[2], v[3]);
         reinterpret_cast<float2 *>(A + nr + k)[0] = make_float2(v[4], v[5]);
     } else {
@@ -537,14 +539,15 @@ __device__ __forceinline__ void row_st(float4 *A, uint32_t nr, uint32_t k, int r
 
 // LDS: ring [TP_RING] float4 + [TP_RING] float2, partial sums [1024] float4 + [1024] float2
 #define TP1_LDS (TP_RING * 24 + TP_LANES * 24)
-__global__ void __launch_bounds__(TP_LANES) tp_sweep1_kernel(TaskDev S)
+// The sweep of right-hand side r (compile-time, so its row loads and stores are whole float4 /
+// float2 accesses); the kernel below calls it for r = blockIdx.x.
+template <int r>
+__device__ __forceinline__ void tp_sweep1_body(const TaskDev &S, float4 *tp1_lds)
 {
-    extern __shared__ float4 tp1_lds[];
     float4 *ring4 = tp1_lds;
     float2 *ring2 = reinterpret_cast<float2 *>(tp1_lds + TP_RING);
     float4 *red4 = reinterpret_cast<float4 *>(ring2 + TP_RING);
     float2 *red2 = reinterpret_cast<float2 *>(red4 + TP_LANES);
-    const int r = blockIdx.x;
     const uint32_t t = threadIdx.x;
     const uint32_t nr = (uint32_t)S.nrows;
     if (S.nsteps <= 0) return;
@@ -560,9 +563,9 @@ __global__ void __launch_bounds__(TP_LANES) tp_sweep1_kernel(TaskDev S)
                 const float4 v = b[i * TP_LANES];
                 m[4 * i] = v.x; m[4 * i + 1] = v.y; m[4 * i + 2] = v.z; m[4 * i + 3] = v.w;
             }
-            if (inf.x & TP_GLOBAL) row_ld(S.res, nr, (uint32_t)(inf.x & ~TP_GLOBAL), r, gx);
+            if (inf.x & TP_GLOBAL) row_ld<r>(S.res, nr, (uint32_t)(inf.x & ~TP_GLOBAL), gx);
         }
-        if (inf.y >= 0) row_ld(S.inp, nr, (uint32_t)(inf.y & ((1 << TP_ND_SHIFT) - 1)), r, in);
+        if (inf.y >= 0) row_ld<r>(S.inp, nr, (uint32_t)(inf.y & ((1 << TP_ND_SHIFT) - 1)), in);
     };
     fetch(0, cur);
     float pw[6];
@@ -597,7 +600,7 @@ __global__ void __launch_bounds__(TP_LANES) tp_sweep1_kernel(TaskDev S)
         }
         const int k = cur.y & ((1 << TP_ND_SHIFT) - 1), nd = cur.y >> TP_ND_SHIFT;
         // global store of the previous step, then the operands of step s + 1 and positions of s + 2
-        if (pk >= 0) row_st(S.res, nr, (uint32_t)pk, r, pw);
+        if (pk >= 0) row_st<r>(S.res, nr, (uint32_t)pk, pw);
         if (s + 1 < S.nsteps) fetch((uint32_t)s + 1, nxt);
         const int2 n2 = S.info[(size_t)min(s + 2, last) * TP_LANES + t];
         if (has && !own) {
@@ -636,7 +639,14 @@ __global__ void __launch_bounds__(TP_LANES) tp_sweep1_kernel(TaskDev S)
         cur = nxt;
         nxt = n2;
     }
-    if (pk >= 0) row_st(S.res, nr, (uint32_t)pk, r, pw);
+    if (pk >= 0) row_st<r>(S.res, nr, (uint32_t)pk, pw);
+}
+
+__global__ void __launch_bounds__(TP_LANES) tp_sweep1_kernel(TaskDev S)
+{
+    extern __shared__ float4 tp1_lds[];
+    if (blockIdx.x == 0) tp_sweep1_body<0>(S, tp1_lds);
+    else tp_sweep1_body<1>(S, tp1_lds);
 }
 
 // Z_P = L_PP^-1 of every lower-triangular diagonal block (row-major)
