@@ -571,6 +571,12 @@ __device__ __forceinline__ void tp_sweep1_body(const TaskDev &S, float4 *tp1_lds
     float pw[6];
     int pk = -1;
     for (int s = 0; s < S.nsteps; ++s) {
+        // the dependency's value from the LDS ring is read first (its slot was written before the
+        // barrier that ended the previous step; lanes whose dependency is global or absent read a
+        // valid slot whose value is not used), so its latency overlaps the wait below
+        const uint32_t rs = (uint32_t)cur.x & (TP_RING - 1);
+        const float4 ru = ring4[rs];
+        const float2 rv = ring2[rs];
         // everything in flight is needed now: this step's operands, the next step's
         // positions and (issued before them) the stores of two steps back
         __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
@@ -579,15 +585,9 @@ __device__ __forceinline__ void tp_sweep1_body(const TaskDev &S, float4 *tp1_lds
 #pragma unroll
         for (int a = 0; a < 6; ++a) p[a] = 0.f;
         if (has) {
-            float x[6];
-            if (cur.x & TP_GLOBAL) {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) x[c] = gx[c];
-            } else {
-                const float4 u = ring4[cur.x];
-                const float2 v = ring2[cur.x];
-                x[0] = u.x; x[1] = u.y; x[2] = u.z; x[3] = u.w; x[4] = v.x; x[5] = v.y;
-            }
+            const bool glob = (cur.x & TP_GLOBAL) != 0;
+            const float x[6] = {glob ? gx[0] : ru.x, glob ? gx[1] : ru.y, glob ? gx[2] : ru.z,
+                                glob ? gx[3] : ru.w, glob ? gx[4] : rv.x, glob ? gx[5] : rv.y};
 #pragma unroll
             for (int a = 0; a < 6; ++a)
 #pragma unroll
