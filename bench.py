@@ -223,6 +223,7 @@ def main():
             "preconditioner": "ichol",
             "gmres_restart": args.restart, "initial_guess": args.guess,
             "gmres_iterations_mean": float(np.mean(iters)),
+            "nlist_rebuilds": int(sum(i.get("nlist_rebuilt", 0) for i in infos[-args.steps:])),
             "true_residual_last": last.get("true_residual"),
             "nnz": last.get("nnz"), "lub_pairs": last.get("lub_pairs"), "levels": last.get("levels"),
             "parallelism": "single GPU",

@@ -80,6 +80,7 @@ class FsdSolveStats(C.Structure):
         ("lanczos_nf", C.c_int32),
         ("sweep_steps", C.c_int32),
         ("initial_residual", C.c_float),
+        ("nlist_rebuilt", C.c_int32),
     ]
 
     def as_dict(self):

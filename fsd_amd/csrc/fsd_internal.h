@@ -200,7 +200,10 @@ struct fsd_ctx {
     int *d_nneigh = nullptr, *d_nlist = nullptr;
     int ncell[3] = {0, 0, 0}; int ncells = 0; bool use_cells = false;
     int *d_cell_of = nullptr, *d_cell_count = nullptr, *d_cell_start = nullptr, *d_cell_list = nullptr;
-    int *d_flags = nullptr;     // [0] nlist overflow, [1] lub overflow, [2] ichol fail, [3] solve spin timeout
+    int *d_flags = nullptr;     // [0] nlist overflow, [1] lub overflow, [2] prune overflow, [4] nlist entries,
+                                // [5] lubrication pairs, [6] some particle moved r_buff / 2 since the list
+    float4 *d_pos_last = nullptr;   // positions the neighbour list was built at
+    int64_t nlist_builds = 0;       // neighbour lists built by this context
     int *h_flags = nullptr;     // pinned mirror
     bool nlist_valid = false;
     int64_t nlist_total = 0;
@@ -324,6 +327,7 @@ void prof_host(fsd_ctx *ctx, int slot, double ms);
 
 // kernel groups (launchers live with their kernels)
 int k_build_nlist(fsd_ctx *ctx, const float4 *pos);
+int k_update_nlist(fsd_ctx *ctx, const float4 *pos, bool *rebuilt);
 int k_build_lub_pairs(fsd_ctx *ctx, const float4 *pos);
 int k_rfu_apply(fsd_ctx *ctx, const float *U, float *F, const float *xF, float sign_out);
 int k_rfe(fsd_ctx *ctx, const float4 *pos, float shear_rate, const float *ori, float *F);

@@ -225,6 +225,37 @@ __global__ void nlist_allpairs_kernel(const float4 *pos, int N, BoxF b, float rl
     nneigh[i] = cnt < maxn ? cnt : maxn;
 }
 
+// NeighborList::distanceCheck of HOOMD: some particle has moved at least r_buff / 2 (minimum
+// image) since the list was built
+__global__ void max_disp_kernel(const float4 *pos, const float4 *last, int N, BoxF b, float lim2, int *flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float4 p = pos[i], q = last[i];
+    float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    box_min_image(b, dx, dy, dz);
+    if (dist2_nc(dx, dy, dz) >= lim2) *flag = 1;
+}
+
+// the list is kept while no particle has moved r_buff / 2 since it was built (the buffer then
+// still holds every pair within rcut), rebuilt otherwise or when the box changed
+int k_update_nlist(fsd_ctx *ctx, const float4 *pos, bool *rebuilt)
+{
+    *rebuilt = false;
+    if (ctx->nlist_valid) {
+        const float half = 0.5f * ctx->p.r_buff;
+        FSD_HIP(hipMemsetAsync(ctx->d_flags + 6, 0, sizeof(int), ctx->stream));
+        hipLaunchKernelGGL(max_disp_kernel, dim3(nblk(ctx->N, 256)), dim3(256), 0, ctx->stream, pos, ctx->d_pos_last,
+                           ctx->N, ctx->box, half * half, ctx->d_flags + 6);
+        FSD_HIP(hipGetLastError());
+        FSD_HIP(hipMemcpyAsync(ctx->h_flags + 6, ctx->d_flags + 6, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        FSD_HIP(hipStreamSynchronize(ctx->stream));
+        if (!ctx->h_flags[6]) return 0;
+    }
+    *rebuilt = true;
+    return k_build_nlist(ctx, pos);
+}
+
 int k_build_nlist(fsd_ctx *ctx, const float4 *pos)
 {
     const int N = ctx->N;
@@ -265,6 +296,8 @@ int k_build_nlist(fsd_ctx *ctx, const float4 *pos)
         FSD_HIP(hipMalloc(&ctx->d_nlist, sizeof(int) * (size_t)N * newmax));
         ctx->maxn = newmax;
     }
+    // positions the list was built at, for the rebuild check
+    FSD_HIP(hipMemcpyAsync(ctx->d_pos_last, pos, sizeof(float4) * (size_t)N, hipMemcpyDeviceToDevice, ctx->stream));
     prof_end(ctx, PROF_NLIST, ev);
     ctx->nlist_valid = true;
     ctx->lub_valid = false;

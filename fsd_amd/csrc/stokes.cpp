@@ -182,6 +182,7 @@ static int alloc_all(fsd_ctx *ctx)
         CK(hipMalloc(&ctx->d_cell_list, sizeof(int) * N));
     }
     CK(hipMalloc(&ctx->d_flags, sizeof(int) * 8));
+    CK(hipMalloc(&ctx->d_pos_last, sizeof(float4) * (size_t)N));
     CK(hipHostMalloc((void **)&ctx->h_flags, sizeof(int) * 8, hipHostMallocDefault));
     // lubrication records
     ctx->maxlub = 48;
@@ -322,7 +323,7 @@ static int alloc_all(fsd_ctx *ctx)
 static void free_all(fsd_ctx *ctx)
 {
     void *ptrs[] = {ctx->d_nneigh, ctx->d_nlist, ctx->d_cell_of, ctx->d_cell_count, ctx->d_cell_start,
-                    ctx->d_cell_list, ctx->d_flags, ctx->d_lub_n, ctx->d_lub_j, ctx->d_lub_a, ctx->d_lub_b,
+                    ctx->d_cell_list, ctx->d_flags, ctx->d_pos_last, ctx->d_lub_n, ctx->d_lub_j, ctx->d_lub_a, ctx->d_lub_b,
                     ctx->d_lub_self, ctx->d_grid, ctx->d_spec, ctx->d_gridk, ctx->d_fftwork, ctx->d_porig,
                     ctx->d_ocell, ctx->d_olist, ctx->d_ocount, ctx->d_ostart, ctx->d_scan_tmp, ctx->d_wave, ctx->d_rec,
                     ctx->d_ewald, ctx->d_res_dist, ctx->d_res_vals, ctx->d_pn, ctx->d_plist, ctx->d_pless,
@@ -1006,6 +1007,7 @@ int fsd_build_nlist(fsd_ctx *ctx, const float *d_pos4)
 {
     int rc = k_build_nlist(ctx, reinterpret_cast<const float4 *>(d_pos4));
     if (rc) return rc;
+    ctx->nlist_builds++;
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -1370,8 +1372,11 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     const int64_t n17 = 17 * (int64_t)N;
     float4 *pos = reinterpret_cast<float4 *>(d_pos4);
     if (!ctx->have_table) { ctx->err = "resistance table not set"; return FSD_E_STATE; }
-    // neighbour list (HOOMD NeighborList::compute) and sheared k vectors
-    RC(k_build_nlist(ctx, pos));
+    // neighbour list (HOOMD NeighborList::compute: rebuilt when some particle moved r_buff / 2)
+    // and sheared k vectors
+    bool rebuilt = false;
+    RC(k_update_nlist(ctx, pos, &rebuilt));
+    ctx->nlist_builds += rebuilt;
     RC(k_gridk(ctx));
     // preconditioner (Precondition_Wrap): its host phase runs on a second thread while the
     // kernels that do not depend on it are enqueued
@@ -1447,6 +1452,7 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     st.lower_blocks = ctx->pch.nlow;
     st.levels = ctx->pch.nlev_f;
     st.sweep_steps = ctx->pch.tp_ok ? ctx->pch.tp_f.nsteps + ctx->pch.tp_b.nsteps : 0;
+    st.nlist_rebuilt = rebuilt ? 1 : 0;
     st.lanczos_ff = (ctx->p.T > 0.f) ? ctx->last_lanczos_ff : 0;
     st.lanczos_nf = (ctx->p.T > 0.f) ? ctx->last_lanczos_nf : 0;
     // relaxer and Krylov-size bookkeeping (Stokes.cc:1658-1666)

@@ -98,6 +98,8 @@ typedef struct fsd_solve_stats {
     int32_t lanczos_nf;              /* Krylov size of the near-field Brownian sampler (T > 0) */
     int32_t sweep_steps;             /* steps of the one-launch triangular sweeps (0: one launch per level) */
     float initial_residual;          /* ||b - A x0|| / ||b|| (true residual) or its preconditioned form */
+    int32_t nlist_rebuilt;           /* fsd_step: 1 if the neighbour list was rebuilt this step (some particle
+                                        moved r_buff / 2 since the last build, or the box changed) */
 } fsd_solve_stats;
 
 /* ---- lifecycle: Stokes::Stokes + setParams + setSparseMath + AllocateWorkSpaces

@@ -203,6 +203,35 @@ def test_step_C1():
     assert np.abs(disp - expect).max() < 1e-5
 
 
+def test_nlist_kept_until_half_buffer_moved():
+    """the step keeps its neighbour list while no particle has moved r_buff / 2
+    since it was built (HOOMD's distance check) and rebuilds it after; a kept
+    list gives the velocities of a freshly built one (FP32 summation order
+    aside), over steps under shear"""
+    import torch
+    from fsd_amd.stokes import Stokes, ParticleState
+    N, phi, seed = CASES["C1"]
+    pos, L = packing(N, phi, seed)
+    F = _t(forces(N, 23))
+    # A keeps its list; B rebuilds it before every step (an explicit build)
+    a, b = Stokes(N, L, gmres_tol=1e-7, dt=0.01), Stokes(N, L, gmres_tol=1e-7, dt=0.01)
+    sa, sb = ParticleState(pos), ParticleState(pos)
+    rebuilt, worst = [], 0.0
+    for k in range(8):
+        b.build_nlist(sb.pos)
+        ia = a.integrateStepOne(k, sa, force6=F, shear_rate=0.5)
+        ib = b.integrateStepOne(k, sb, force6=F, shear_rate=0.5)
+        rebuilt.append(ia["nlist_rebuilt"])
+        assert ib["nlist_rebuilt"] == 0
+        va, vb = sa.velocity.cpu().numpy(), sb.velocity.cpu().numpy()
+        worst = max(worst, rel_err2(va, vb))
+        sb.pos.copy_(sa.pos)   # same positions for the next step
+        sb.image.copy_(sa.image)
+    print("rebuilt", rebuilt, "worst velocity rel diff", worst)
+    assert rebuilt[0] == 1 and 0 in rebuilt[1:] and 1 in rebuilt[1:]
+    assert worst < 1e-5
+
+
 def test_force_law_parity():
     st, orc, dpos, N = _setup("C1")
     F = st.force_law(dpos, T_ext=0.3).cpu().numpy()
