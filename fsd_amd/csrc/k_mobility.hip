@@ -300,8 +300,9 @@ __global__ void __launch_bounds__(512) spread_tile2_kernel(SpreadGeom g, const f
     __shared__ int s_beg[NQ];                    // first olist entry of each halo cell
     __shared__ int s_base[NQ + 1];               // exclusive scan of the cells' counts
     __shared__ int s_wsum[8];
-    __shared__ int s_o[SP2_CAP][3];
-    __shared__ float s_p[SP2_CAP][3];
+    // staged particle: the offsets (cx, cy, cy - py, cz) of its stencil in the tile's frame and its
+    // records scaled by the Gaussian's prefactor
+    __shared__ float4 s_c[SP2_CAP];
     __shared__ float4 s_r[SP2_CAP][3];
 
     const int tile_id = xcd_remap(blockIdx.x, gridDim.x);
@@ -379,11 +380,22 @@ __global__ void __launch_bounds__(512) spread_tile2_kernel(SpreadGeom g, const f
         for (int h = 0; h < 2; ++h) {
             const int sl = sl2[h], i = i2[h];
             if (sl < total) {
-                s_o[sl][0] = orig[3 * i]; s_o[sl][1] = orig[3 * i + 1]; s_o[sl][2] = orig[3 * i + 2];
+                // the particle's origin is the halo cell (hc) up to the period, so a point at local
+                // (lx, ly, lz) of its stencil is node o + (P - 1) - hc + l: with the node's position
+                // h (o + P - 1 - hc) - L/2 + h l, its offset from the particle is c + h l
+                const int q = q2[h];
+                const int hxc = q / (H * H), hyc = (q / H) % H, hzc = q % H;
+                const int ox = orig[3 * i], oy = orig[3 * i + 1], oz = orig[3 * i + 2];
                 const float4 p = pos[i];
-                s_p[sl][0] = p.x; s_p[sl][1] = p.y; s_p[sl][2] = p.z;
-                s_r[sl][0] = rec[3 * (int64_t)i]; s_r[sl][1] = rec[3 * (int64_t)i + 1];
-                s_r[sl][2] = rec[3 * (int64_t)i + 2];
+                const float cx = g.hx * (float)(ox + (PP - 1) - hxc) - g.Lx2 - p.x;
+                const float cy = g.hy * (float)(oy + (PP - 1) - hyc) - g.Ly2;
+                const float cz = g.hz * (float)(oz + (PP - 1) - hzc) - g.Lz2 - p.z;
+                s_c[sl] = make_float4(cx, cy, cy - p.y, cz);
+                const float4 r0 = rec[3 * (int64_t)i], r1 = rec[3 * (int64_t)i + 1], r2 = rec[3 * (int64_t)i + 2];
+                const float f = g.prefac;
+                s_r[sl][0] = make_float4(f * r0.x, f * r0.y, f * r0.z, f * r0.w);
+                s_r[sl][1] = make_float4(f * r1.x, f * r1.y, f * r1.z, f * r1.w);
+                s_r[sl][2] = make_float4(f * r2.x, f * r2.y, f * r2.z, 0.f);
             }
         }
     }
@@ -396,13 +408,23 @@ __global__ void __launch_bounds__(512) spread_tile2_kernel(SpreadGeom g, const f
     for (int c = 0; c < 11; ++c) acc[c] = 0.f;
     // one loop per source (separate code, so the staged loop reads LDS only)
     if (staged) {
+        // offsets of this point in the tile's frame; a staged particle's are added to them
+        const float ax = g.hx * (float)lx, ay = g.hy * (float)ly, az = g.hz * (float)lz;
+        const float nexp = -g.expfac;
         for (int a = 0; a < PP; ++a)
             for (int b = 0; b < PP; ++b) {
                 const int q0 = ((lx + a) * H + (ly + b)) * H + lz;
                 const int s1 = s_base[q0 + PP];
-                for (int sl = s_base[q0]; sl < s1; ++sl)
-                    spread_point_add(g, gx, gy, gz, s_o[sl][0], s_o[sl][1], s_o[sl][2], s_p[sl][0], s_p[sl][1],
-                                     s_p[sl][2], s_r[sl][0], s_r[sl][1], s_r[sl][2], acc);
+                for (int sl = s_base[q0]; sl < s1; ++sl) {
+                    const float4 c = s_c[sl];
+                    const float uy = c.y + ay;
+                    const float dx = c.x + ax + g.xy * uy, dy = c.z + ay, dz = c.w + az;
+                    const float e = __expf(nexp * (dx * dx + dy * dy + dz * dz));
+                    const float4 r0 = s_r[sl][0], r1 = s_r[sl][1], r2 = s_r[sl][2];
+                    acc[0] += e * r0.x; acc[1] += e * r0.y; acc[2] += e * r0.z;
+                    acc[3] += e * r0.w; acc[4] += e * r1.x; acc[5] += e * r1.y; acc[6] += e * r1.z;
+                    acc[7] += e * r1.w; acc[8] += e * r2.x; acc[9] += e * r2.y; acc[10] += e * r2.z;
+                }
             }
     } else {
         for (int a = 0; a < PP; ++a)
