@@ -32,7 +32,7 @@ GUESS_TEXT = {"previous": "previous solution", "extrapolate": "linear extrapolat
 # kernel-class name -> kernel name in rocprofv3 traces
 KERNEL_NAMES = {"spread": "spread_tile2_kernel<8, 6>", "green": "green_kernel", "contract": "contract_tile2_kernel<13, 6>",
                 "realspace": "realspace_kernel<16>", "rfu": "rfu_apply_kernel<8>", "sweep": "tp_sweep1_kernel",
-                "ichol": "ichol_level_kernel", "gmres_vec": "multidot_partial_kernel"}
+                "ichol": "ichol_level_kernel", "gmres_vec": "multidot_partial4_kernel"}
 
 
 def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5):
