@@ -854,6 +854,7 @@ struct RealParams {
     BoxF b;
     float rcut, dr, self_x, self_y;
     int ewald_n;
+    float x_scale, inv_dr;   // ewald_n / (rcut - dr), 1 / dr
 };
 
 // Mobility_RealSpace_kernel (Mobility.cu:1046-1226) + LinearCombination/Add4
@@ -886,11 +887,12 @@ __global__ void __launch_bounds__(256) realspace_kernel(RealParams rp, const flo
             const float d2 = dist2_nc(dx, dy, dz);
             if (!(d2 < maxd2 && d2 >= mind2)) continue;
             const float dist = sqrtf(d2);
-            float xr = (float)rp.ewald_n * (dist - rp.dr);
-            xr = xr / (rp.rcut - rp.dr);
+            // table position with the divisions by constants done once on the host (x_scale =
+            // ewald_n / (rcut - dr), inv_dr = 1 / dr)
+            const float xr = (dist - rp.dr) * rp.x_scale;
             int r_ind = (int)floorf(xr);
             r_ind = r_ind < 0 ? 0 : (r_ind > rp.ewald_n - 1 ? rp.ewald_n - 1 : r_ind);
-            const float fac = dist / rp.dr - (float)r_ind - 1.0f;
+            const float fac = dist * rp.inv_dr - (float)r_ind - 1.0f;
             const float4 m1 = tab[2 * r_ind], p1 = tab[2 * r_ind + 2];
             const float4 m2 = tab[2 * r_ind + 1], p2 = tab[2 * r_ind + 3];
             const float f1 = m1.x + (p1.x - m1.x) * fac;
@@ -900,7 +902,8 @@ __global__ void __launch_bounds__(256) realspace_kernel(RealParams rp, const flo
             const float h1 = m2.x + (p2.x - m2.x) * fac;
             const float h2 = m2.y + (p2.y - m2.y) * fac;
             const float h3 = m2.z + (p2.z - m2.z) * fac;
-            const float R[3] = {dx / dist, dy / dist, dz / dist};
+            const float idist = 1.0f / dist;
+            const float R[3] = {dx * idist, dy * idist, dz * idist};
             const float4 r0 = rec[3 * (int64_t)j], r1 = rec[3 * (int64_t)j + 1], r2 = rec[3 * (int64_t)j + 2];
             const float Fj[3] = {r0.x, r0.y, r0.z};
             const float C[8] = {r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z};
@@ -912,9 +915,19 @@ __global__ void __launch_bounds__(256) realspace_kernel(RealParams rp, const flo
                 rdC[a] = R[0] * Cm[0][a] + R[1] * Cm[1][a] + R[2] * Cm[2][a];
             }
             const float rCr = R[0] * Cdr[0] + R[1] * Cdr[1] + R[2] * Cdr[2];
+            // u_a += f1 F_a + (f2 - f1)(r.F) R_a + g1 (Cdr_a - rCr R_a) + g2 (rdC_a - 4 rCr R_a)
+            const float ku = (f2 - f1) * rdotf - (g1 + 4.f * g2) * rCr;
+            for (int a = 0; a < 3; ++a) u[a] += f1 * Fj[a] + g1 * Cdr[a] + g2 * rdC[a] + ku * R[a];
+            // D_ab += -g1 (R_a F_b - (r.F) R_a R_b) - g2 (d_ab (r.F) + F_a R_b - 4 (r.F) R_a R_b)
+            //       + h1 (C_ab - 4 C_ba) + h2 (R_a Cdr_b - rCr R_a R_b)
+            //       + h3 (d_ab rCr + Cdr_a R_b + R_a rdC_b + rdC_a R_b - 6 rCr R_a R_b - C_ba),
+            // gathered as X_a R_b + R_a Y_b + d_ab K2 + h1 C_ab + K3 C_ba
+            const float K1 = (g1 + 4.f * g2) * rdotf - (h2 + 6.f * h3) * rCr;
+            const float K2 = h3 * rCr - g2 * rdotf, K3 = -(4.f * h1 + h3);
+            float X[3], Y[3];
             for (int a = 0; a < 3; ++a) {
-                u[a] += f1 * Fj[a] + (f2 - f1) * rdotf * R[a];
-                u[a] += g1 * (Cdr[a] - rCr * R[a]) + g2 * (rdC[a] - 4.f * rCr * R[a]);
+                X[a] = K1 * R[a] - g2 * Fj[a] + h3 * (Cdr[a] + rdC[a]);
+                Y[a] = -g1 * Fj[a] + h2 * Cdr[a] + h3 * rdC[a];
             }
             const int A[8] = {0, 0, 0, 1, 1, 1, 2, 2};
             const int B[8] = {0, 1, 2, 2, 1, 0, 0, 1};
@@ -922,10 +935,9 @@ __global__ void __launch_bounds__(256) realspace_kernel(RealParams rp, const flo
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const int a = A[c], bb = B[c];
-                const float del = (a == bb) ? 1.f : 0.f;
-                D[c] += -g1 * (R[a] * Fj[bb] - rdotf * R[a] * R[bb]) - g2 * (del * rdotf + Fj[a] * R[bb] - 4.f * rdotf * R[a] * R[bb]);
-                D[c] += h1 * (C[c] - 4.f * C[Cs[c]]) + h2 * (R[a] * Cdr[bb] - rCr * R[a] * R[bb])
-                        + h3 * (del * rCr + Cdr[a] * R[bb] + R[a] * rdC[bb] + rdC[a] * R[bb] - 6.f * rCr * R[a] * R[bb] - C[Cs[c]]);
+                float d = X[a] * R[bb] + R[a] * Y[bb] + h1 * C[c] + K3 * C[Cs[c]];
+                if (a == bb) d += K2;
+                D[c] += d;
             }
         }
     }
@@ -1060,6 +1072,8 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
     RealParams rp;
     rp.b = ctx->box; rp.rcut = ps.rcut; rp.dr = 0.001f; rp.self_x = ps.self_x; rp.self_y = ps.self_y;
     rp.ewald_n = ps.ewald_n;
+    rp.x_scale = (float)((double)ps.ewald_n / ((double)ps.rcut - (double)rp.dr));
+    rp.inv_dr = 1.0f / rp.dr;
     // the processing order is the neighbour list's cell list (identity without a list, wave part
     // only, or for a box of fewer than three cells per side, whose list is built over all pairs)
     if (!ctx->nlist_valid && (parts & 1)) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
