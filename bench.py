@@ -35,7 +35,7 @@ KERNEL_NAMES = {"spread": "spread_tile2_kernel<8, 6>", "green": "green_kernel", 
                 "ichol": "ichol_level_kernel", "gmres_vec": "multidot_partial4_kernel"}
 
 
-def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5):
+def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5, gs_passes=2):
     """bytes a launch of kernel class `cls` must move at minimum (each input
     read once, each output written once), per the layouts in HBM"""
     M = pse["Nx"] * pse["Ny"] * pse["Nz"]
@@ -60,9 +60,11 @@ def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5):
         return 2 * (144 * st["lower_blocks"] + 48 * N) + (48 + 144 + 24 + 48 + 144 + 24 + 44 + 68) * N
     if cls == "ichol":       # one factorisation: A blocks in, L blocks out (lower and diagonal)
         return 144 * 2 * (st["lower_blocks"] + N)
-    if cls == "gmres_vec":   # CGS2 of a 17N-float vector on k basis vectors: two passes of h = V^T w
-        k = kbar             # (k + 1 reads) and w -= V h (k + 1 reads, w in and out), |w'|^2 in the
-        return 4 * 17 * N * (4 * k + 8)   # second (one read); k averaged over the solve's iterations
+    if cls == "gmres_vec":   # Gram-Schmidt of a 17N-float vector on k basis vectors, per pass h = V^T w
+        k = kbar             # (k + 1 reads) and w -= V h (k + 1 reads, w in and out), then the norm
+        # (one read, folded into the second pass of CGS2) and the scaling (in and out); k averaged over
+        # the solve's iterations
+        return 4 * 17 * N * (gs_passes * (2 * k + 4) + 3)
     return None
 
 
@@ -103,6 +105,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--restart", type=int, default=100, help="GMRES restart length")
+    ap.add_argument("--gram-schmidt", type=int, default=1, choices=[1, 2],
+                    help="classical Gram-Schmidt passes per Arnoldi step (the library's default is 2)")
     ap.add_argument("--guess", default="extrapolate", choices=["previous", "extrapolate", "quadratic"],
                     help="initial guess of each solve: the previous solution (the reference) or its extrapolation")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (diagnostics)")
@@ -138,7 +142,7 @@ def main():
           file=sys.stderr, flush=True)
     shear_rate = 1.0
     st = Stokes(N, L, xi=0.5, error=1e-3, gmres_tol=1e-4, gmres_true_tol=1e-4, dt=1e-3, gmres_restart=args.restart,
-                guess=args.guess)
+                guess=args.guess, gram_schmidt_passes=args.gram_schmidt)
     pse = st.pse_info()
     state = ParticleState(pos)
     dF = torch.from_numpy(F).cuda()
@@ -176,7 +180,7 @@ def main():
     avg_ms = kms / max(launches, 1)
     it_mean = float(np.mean(iters))
     kbar = 0.5 * (min(it_mean, args.restart) + 1.0)
-    bytes_per_launch = algorithmic_bytes(dom, N, None, pse, last, kbar)
+    bytes_per_launch = algorithmic_bytes(dom, N, None, pse, last, kbar, args.gram_schmidt)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_launch else None
     # HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
     # (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a plain copy)
@@ -217,11 +221,12 @@ def main():
         "data": "synthetic (seeded packing and forces)",
         "config": {
             "workload": "N=32768 phi=0.45 periodic suspension, shear rate 1, random forces, T=0, xi=0.5, "
-                        "PSE error 1e-3, IC(0)-preconditioned GMRES(%d) to true residual 1e-4, initial guess: %s"
-                        % (args.restart, GUESS_TEXT[args.guess]),
+                        "PSE error 1e-3, IC(0)-preconditioned GMRES(%d) with %s Gram-Schmidt to true residual 1e-4, "
+                        "initial guess: %s" % (args.restart, "classical" if args.gram_schmidt == 1 else "twice-classical",
+                                               GUESS_TEXT[args.guess]),
             "N": N, "phi": args.phi, "grid": [pse["Nx"], pse["Ny"], pse["Nz"]], "P": pse["P"],
             "preconditioner": "ichol",
-            "gmres_restart": args.restart, "initial_guess": args.guess,
+            "gmres_restart": args.restart, "initial_guess": args.guess, "gram_schmidt_passes": args.gram_schmidt,
             "gmres_iterations_mean": float(np.mean(iters)),
             "nlist_rebuilds": int(sum(i.get("nlist_rebuilt", 0) for i in infos[-args.steps:])),
             "true_residual_last": last.get("true_residual"),

@@ -49,6 +49,7 @@ class FsdParams(C.Structure):
         ("parity", C.c_int32),
         ("gmres_true_tol", C.c_float),
         ("guess", C.c_int32),
+        ("gs_passes", C.c_int32),
     ]
 
 

@@ -592,8 +592,13 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
             // the norm needs no pass of its own
             RC(k_multidot(ctx, n, j + 1, V, n, vn, d_h));
             RC(k_multi_axpy(ctx, n, j + 1, V, n, d_h, vn));
-            RC(k_multidot(ctx, n, j + 2, V, n, vn, d_h + KM));
-            RC(k_multi_axpy(ctx, n, j + 1, V, n, d_h + KM, vn));
+            if (ctx->p.gs_passes == 1) {   // one classical pass: no second projection, direct norm
+                CK(hipMemsetAsync(d_h + KM, 0, sizeof(double) * (j + 1), ctx->stream));
+                RC(k_nrm2(ctx, n, vn, d_h + KM + j + 1));
+            } else {
+                RC(k_multidot(ctx, n, j + 2, V, n, vn, d_h + KM));
+                RC(k_multi_axpy(ctx, n, j + 1, V, n, d_h + KM, vn));
+            }
             RC(k_scale_by_norm(ctx, n, vn, d_h + KM, j + 1, d_h + 2 * KM + 1));
             prof_end(ctx, PROF_GMRES_VEC, ev);
             CK(hipMemcpyAsync(ctx->h_part + (size_t)(j & 1) * HS, d_h, sizeof(double) * HS, hipMemcpyDeviceToHost,
@@ -885,6 +890,8 @@ int fsd_create(const fsd_params *p, fsd_ctx **out)
     if (q.rlub <= 0.f) q.rlub = 4.0f;
     if (q.rp <= 0.f) q.rp = 2.1f;
     if (q.r_buff < 0.f) q.r_buff = 0.4f;
+    if (q.gs_passes <= 0) q.gs_passes = 2;
+    if (q.gs_passes > 2) q.gs_passes = 2;
     if (q.N <= 0 || q.xi <= 0.f || q.error <= 0.f || q.error >= 1.f || q.Lx <= 0.f || q.Ly <= 0.f || q.Lz <= 0.f) {
         ctx->err = "invalid fsd_params";
         *out = ctx;

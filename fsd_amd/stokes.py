@@ -37,7 +37,7 @@ class Stokes:
                  ndsr=1e-1, kappa=1 / 0.05, k_n=1e4, beta_AF=0.0, epsq=1e-5, sqm_B1=0.0, sqm_B2=0.0,
                  N_mix=1, coef_B1_mask=1.0, coef_B2_mask=1.0, rot_diff=0.0, T_ext=0.0, omega_ext=0.0,
                  dt=1e-3, max_strain=0.5, gmres_tol=None, gmres_restart=50, gmres_maxiter=1000,
-                 parity=True, gmres_true_tol=0.0, guess="previous"):
+                 parity=True, gmres_true_tol=0.0, guess="previous", gram_schmidt_passes=2):
         self.lib = _lib.load()
         self.torch = _torch()
         L = box if not np.isscalar(box) else (box, box, box, 0.0, 0.0, 0.0)
@@ -62,6 +62,9 @@ class Stokes:
         if guess not in guesses:
             raise ValueError("guess must be 'previous' (the reference), 'extrapolate' (linear) or 'quadratic'")
         p.guess = guesses[guess]
+        if gram_schmidt_passes not in (1, 2):
+            raise ValueError("gram_schmidt_passes must be 1 or 2")
+        p.gs_passes = gram_schmidt_passes
         self.params = p
         self.N = int(N)
         self.fileprefix = fileprefix

@@ -71,6 +71,8 @@ typedef struct fsd_params {
                                         2 x_{n-1} - x_{n-2} of the last two steps' solutions; 2 = quadratic
                                         extrapolation 3 x_{n-1} - 3 x_{n-2} + x_{n-3} (lower orders while
                                         fewer solutions exist) */
+    int32_t gs_passes;               /* classical Gram-Schmidt passes per Arnoldi step: 2 (default, <= 0)
+                                        or 1 (half the vector traffic, less orthogonal basis) */
 } fsd_params;
 
 /* Derived PSE parameters (Stokes.cc:131-261). */

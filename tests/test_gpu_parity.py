@@ -179,6 +179,33 @@ def test_solve_true_residual_mode(case):
     assert e < 1e-2             # a 1e-4 residual bounds the velocity error only up to the conditioning
 
 
+@pytest.mark.parametrize("case", ["C1", "C2"])
+def test_solve_one_gram_schmidt_pass(case):
+    """one classical Gram-Schmidt pass per Arnoldi step (gs_passes = 1) reaches the
+    same true residual, checked with the oracle's FP64 operator, and the same
+    solution as two passes to within that residual"""
+    import torch
+    from fsd_amd.stokes import Stokes
+    st2, orc, dpos, N = _setup(case, gmres_true_tol=1e-5)
+    orc.build_precond()
+    b = orc.rhs(forces(N, 24), 1.0)
+    xs = []
+    for passes in (2, 1):
+        st = st2 if passes == 2 else Stokes(N, packing(*CASES[case])[1], gmres_true_tol=1e-5, gram_schmidt_passes=1)
+        if passes == 1:
+            st.build_nlist(dpos)
+        st.build_precond(dpos)
+        x, info = st.saddle_solve(dpos, _t(b))
+        xg = x.cpu().numpy().astype(np.float64)
+        r = np.linalg.norm(b - orc.saddle_multiply(xg)) / np.linalg.norm(b)
+        print("passes", passes, "iters", info["iterations"], "FP64 residual", r)
+        assert info["converged"] and r < 3e-5
+        xs.append(xg)
+    e = rel_err2(xs[1][11 * N:], xs[0][11 * N:])
+    print("velocity rel diff", e)
+    assert e < 1e-3
+
+
 def test_step_C1():
     """fsd_step with injected forces: velocities equal the oracle's solve and
     positions move by (U + shear) dt"""
