@@ -691,8 +691,8 @@ __global__ void __launch_bounds__(512) contract_tile2_kernel(SpreadGeom g, const
     extern __shared__ float s_g[];           // [CT_NF][S][S][S]
     __shared__ int s_run_beg[TILE * TILE], s_run_off[TILE * TILE + 1];
     __shared__ int s_w[3][S];                // wrapped grid index of each halo offset
-    __shared__ int s_pi[CT_PCAP], s_po[CT_PCAP][3];   // staged particles: index, stencil origin
-    __shared__ float s_pp[CT_PCAP][3];                //   and position
+    __shared__ int s_pi[CT_PCAP], s_pb[CT_PCAP];      // staged particles: index, local origin (bytes)
+    __shared__ float4 s_pc[CT_PCAP];                  //   and offsets from the particle
     const int item = xcd_remap(blockIdx.x, gridDim.x);   // a slab of tiles per XCD: halos shared in its L2
     const int tile_id = item >> 1, grp = item & 1;
     const int f0 = grp ? CT_NF : 0, nf = grp ? 11 - CT_NF : CT_NF;
@@ -760,49 +760,62 @@ __global__ void __launch_bounds__(512) contract_tile2_kernel(SpreadGeom g, const
         }
         return s_run_beg[lo] + (q - s_run_off[lo]);
     };
-    // the tile's particles (index, stencil origin, position) staged in LDS by one thread each,
-    // so the contraction loop below issues no global load (a tile with more than CT_PCAP
-    // particles reads the rest from global memory)
-    const int nst = min(total, CT_PCAP);
-    if (lt < nst) {
-        const int i = olist[entry(lt)];
-        s_pi[lt] = i;
-        s_po[lt][0] = orig[3 * i]; s_po[lt][1] = orig[3 * i + 1]; s_po[lt][2] = orig[3 * i + 2];
+    // the tile's particles staged in LDS by one thread each, so the contraction loop below issues no
+    // global load (a tile with more than CT_PCAP particles reads the rest from global memory): the
+    // index, the stencil origin in the block's local frame (packed bytes) and the offsets
+    // (h o - L/2 - p) of the stencil's first node from the particle, y also without p_y
+    auto stage = [&](int i, int &bpk, float4 &c) {
+        const int ox = orig[3 * i], oy = orig[3 * i + 1], oz = orig[3 * i + 2];
         const float4 p = pos[i];
-        s_pp[lt][0] = p.x; s_pp[lt][1] = p.y; s_pp[lt][2] = p.z;
-    }
-    __syncthreads();
-    for (int q = wv; q < total; q += nw) {
-        int i, ox, oy, oz;
-        float4 p;
-        if (q < CT_PCAP) {
-            i = s_pi[q];
-            ox = s_po[q][0]; oy = s_po[q][1]; oz = s_po[q][2];
-            p = make_float4(s_pp[q][0], s_pp[q][1], s_pp[q][2], 0.f);
-        } else {
-            i = olist[entry(q)];
-            ox = orig[3 * i]; oy = orig[3 * i + 1]; oz = orig[3 * i + 2];
-            p = pos[i];
-        }
         const int bx = (((ox % g.Nx) + g.Nx) % g.Nx) - x0;
         const int by = (((oy % g.Ny) + g.Ny) % g.Ny) - y0;
         const int bz = (((oz % g.Nz) + g.Nz) % g.Nz) - z0;
+        bpk = bx | (by << 8) | (bz << 16);
+        const float cy = g.hy * (float)oy - g.Ly2;
+        c = make_float4(g.hx * (float)ox - g.Lx2 - p.x, cy, cy - p.y, g.hz * (float)oz - g.Lz2 - p.z);
+    };
+    const int nst = min(total, CT_PCAP);
+    if (lt < nst) {
+        const int i = olist[entry(lt)];
+        int bpk;
+        float4 c;
+        stage(i, bpk, c);
+        s_pi[lt] = i;
+        s_pb[lt] = bpk;
+        s_pc[lt] = c;
+    }
+    __syncthreads();
+    // this lane's stencil column (ty, tz) and its offsets
+    const int ty = lane / PP, tz = lane - PP * (lane / PP);
+    const float aty = g.hy * (float)ty, atz = g.hz * (float)tz;
+    for (int q = wv; q < total; q += nw) {
+        int i, bpk;
+        float4 c;
+        if (q < CT_PCAP) {
+            i = s_pi[q];
+            bpk = s_pb[q];
+            c = s_pc[q];
+        } else {
+            i = olist[entry(q)];
+            stage(i, bpk, c);
+        }
+        const int bx = bpk & 0xff, by = (bpk >> 8) & 0xff, bz = bpk >> 16;
         float acc[CT_NF];
 #pragma unroll
-        for (int c = 0; c < CT_NF; ++c) acc[c] = 0.f;
+        for (int c2 = 0; c2 < CT_NF; ++c2) acc[c2] = 0.f;
         if (lane < PP * PP) {
-            const int ty = lane / PP, tz = lane - PP * ty;
-            const float uy = g.hy * (float)(oy + ty) - g.Ly2;
-            const float dz = g.hz * (float)(oz + tz) - g.Lz2 - p.z;
-            const float dy = uy - p.y;
+            const float uy = c.y + aty, dy = c.z + aty, dz = c.w + atz;
+            const float dyz = dy * dy + dz * dz;
+            const float dx0 = c.x + g.xy * uy;
+            const int rb = (bx * S + by + ty) * S + bz + tz;
 #pragma unroll
             for (int tx = 0; tx < PP; ++tx) {
-                const float dx = g.hx * (float)(ox + tx) - g.Lx2 + g.xy * uy - p.x;
-                const float w = quadw_prefac * __expf(-g.expfac * (dx * dx + dy * dy + dz * dz));
-                const int r = ((bx + tx) * S + (by + ty)) * S + (bz + tz);
+                const float dx = dx0 + g.hx * (float)tx;
+                const float w = __expf(-g.expfac * (dx * dx + dyz));
+                const int r = rb + tx * S * S;
 #pragma unroll
-                for (int c = 0; c < CT_NF; ++c)
-                    if (c < nf) acc[c] += w * s_g[c * S3 + r];
+                for (int c2 = 0; c2 < CT_NF; ++c2)
+                    if (c2 < nf) acc[c2] += w * s_g[c2 * S3 + r];
             }
         }
         // wave sums: rows of 16 by DPP, then the three rows that hold the 36 lanes.  Lane c < nf
@@ -812,13 +825,14 @@ __global__ void __launch_bounds__(512) contract_tile2_kernel(SpreadGeom g, const
         float mine = 0.f;
         int slot = 0;
 #pragma unroll
-        for (int c = 0; c < CT_NF; ++c) {
-            const float r = group_sum<16>(acc[c]);
-            const float t = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, r), 0)) +
-                            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, r), 16)) +
-                            __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, r), 32));
-            mine = (ce == c) ? t : mine;
-            slot = (ce == c) ? (f0 ? slot_of[CT_NF + (c < 11 - CT_NF ? c : 0)] : slot_of[c]) : slot;
+        for (int f = 0; f < CT_NF; ++f) {
+            const float r = group_sum<16>(acc[f]);
+            const float t = quadw_prefac *
+                            (__builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, r), 0)) +
+                             __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, r), 16)) +
+                             __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, r), 32)));
+            mine = (ce == f) ? t : mine;
+            slot = (ce == f) ? (f0 ? slot_of[CT_NF + (f < 11 - CT_NF ? f : 0)] : slot_of[f]) : slot;
         }
         wave[11 * (int64_t)i + slot] = mine;
     }
