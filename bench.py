@@ -160,8 +160,9 @@ def main():
     dom = max(kernel_classes, key=lambda c: totals[c][1])
     print("# warmup kernel-class times (launches, ms): %s" % json.dumps(totals), file=sys.stderr, flush=True)
 
-    # timed steps: HIP events only around the dominant class
-    st.profile(True, classes=None if args.profile_all else [dom])
+    # timed steps: HIP events only around the dominant class, one launch in eight (the events'
+    # own cost stays out of the step time; the timed launches give the kernel's average duration)
+    st.profile(True, classes=None if args.profile_all else [dom], every=1 if args.profile_all else 8)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     iters = []

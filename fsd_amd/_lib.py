@@ -127,6 +127,7 @@ SIGNATURES = [
     ("fsd_brownian_nearfield", _I, [_P, _P, C.c_uint32, _P]),
     ("fsd_rfd_drift", _I, [_P, _P, C.c_uint32, _P]),
     ("fsd_profile_enable", _I, [_P, _I]),
+    ("fsd_profile_every", _I, [_P, _I]),
     ("fsd_profile_get", _I, [_P, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)]),
     ("fsd_profile_reset", _I, [_P]),
 ]

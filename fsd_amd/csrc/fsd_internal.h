@@ -175,7 +175,8 @@ __device__ inline void lub_lookup(const LubTable &t, float dist, int c0, int nco
 // ------------------------------------------------------------------ context
 struct ProfSlot {
     const char *name;
-    int64_t launches;
+    int64_t launches;             // launches timed
+    int64_t seen = 0;             // launches of the class while enabled (one in prof_every is timed)
     double ms;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
 };
@@ -303,6 +304,7 @@ struct fsd_ctx {
     // profiling
     unsigned prof = 0;            // bit mask of timed kernel classes
     std::vector<ProfSlot> slots;
+    int prof_every = 1;
 };
 
 // kernel-launch helper with error capture

@@ -37,6 +37,7 @@ void prof_begin(fsd_ctx *ctx, int slot, hipEvent_t *ev)
 {
     *ev = nullptr;
     if (!(ctx->prof & (1u << slot))) return;
+    if (ctx->slots[slot].seen++ % ctx->prof_every != 0) return;
     hipEventCreate(ev);
     hipEventRecord(*ev, ctx->stream);
 }
@@ -1475,6 +1476,14 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
 int fsd_profile_enable(fsd_ctx *ctx, int32_t enable)
 {
     ctx->prof = (unsigned)enable;
+    return 0;
+}
+
+int fsd_profile_every(fsd_ctx *ctx, int32_t every)
+{
+    if (every < 1) { ctx->err = "fsd_profile_every: every must be at least 1"; return FSD_E_PARAM; }
+    ctx->prof_every = every;
+    for (auto &s : ctx->slots) s.seen = 0;
     return 0;
 }
 

@@ -310,8 +310,10 @@ class Stokes:
         return t
 
     # ---- profiling
-    def profile(self, enable=True, classes=None):
-        """time kernel classes with HIP events; classes: names from _lib.PROF_NAMES (None = all)"""
+    def profile(self, enable=True, classes=None, every=1):
+        """time kernel classes with HIP events; classes: names from _lib.PROF_NAMES (None = all);
+        every: time one launch in `every` of each class"""
+        self._check(self.lib.fsd_profile_every(self.ctx, int(every)))
         if not enable:
             mask = 0
         elif classes is None:

@@ -238,6 +238,9 @@ int fsd_rk2_stage2(fsd_ctx *ctx, float coef1, const float *d_pos_in1_4, const fl
 /* enable: bit mask of kernel classes to time (bit k = k-th name below, -1 = all, 0 = off);
  * an event pair is recorded around every launch of a selected class */
 int fsd_profile_enable(fsd_ctx *ctx, int32_t enable);
+/* time one launch in `every` of each selected class (default 1: every launch), so that the
+ * event records perturb a long run less; fsd_profile_get then reports the timed launches */
+int fsd_profile_every(fsd_ctx *ctx, int32_t every);
 /* name: "spread", "green", "contract", "fft", "realspace", "rfu", "trsv", "ichol",
  * "gmres_vec", "nlist", "precond_build", "sweep", and the host wall time of the preconditioner's
  * host plan ("plan_host") and of its upload/build phase ("finish_host"); returns launches (phases)
