@@ -594,13 +594,13 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
             RC(k_multidot(ctx, n, j + 1, V, n, vn, d_h));
             RC(k_multi_axpy(ctx, n, j + 1, V, n, d_h, vn));
             if (ctx->p.gs_passes == 1) {   // one classical pass: no second projection, direct norm
-                CK(hipMemsetAsync(d_h + KM, 0, sizeof(double) * (j + 1), ctx->stream));
                 RC(k_nrm2(ctx, n, vn, d_h + KM + j + 1));
+                RC(k_scale_by_norm(ctx, n, vn, d_h + KM + j + 1, 0, d_h + 2 * KM + 1));
             } else {
                 RC(k_multidot(ctx, n, j + 2, V, n, vn, d_h + KM));
                 RC(k_multi_axpy(ctx, n, j + 1, V, n, d_h + KM, vn));
+                RC(k_scale_by_norm(ctx, n, vn, d_h + KM, j + 1, d_h + 2 * KM + 1));
             }
-            RC(k_scale_by_norm(ctx, n, vn, d_h + KM, j + 1, d_h + 2 * KM + 1));
             prof_end(ctx, PROF_GMRES_VEC, ev);
             CK(hipMemcpyAsync(ctx->h_part + (size_t)(j & 1) * HS, d_h, sizeof(double) * HS, hipMemcpyDeviceToHost,
                               ctx->stream));
@@ -618,7 +618,8 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
             }
             CK(hipEventSynchronize(ctx->gm_ev[i & 1]));
             const double *hs = ctx->h_part + (size_t)(i & 1) * HS;
-            for (int k = 0; k <= i; ++k) H[(size_t)k * R + i] = hs[k] + hs[KM + k];
+            const bool two_pass = ctx->p.gs_passes != 1;
+            for (int k = 0; k <= i; ++k) H[(size_t)k * R + i] = two_pass ? hs[k] + hs[KM + k] : hs[k];
             H[(size_t)(i + 1) * R + i] = hs[2 * KM + 1];
             for (int k = 0; k < i; ++k) {
                 const double a = H[(size_t)k * R + i], c = H[(size_t)(k + 1) * R + i];
