@@ -3,7 +3,7 @@
 Integer and index results (neighbour sets, pruned lists, RCM permutation,
 sparsity patterns) must match bit-exactly; floating-point results within the
 tolerance written in each test (FP32 kernels vs an FP64 restatement of the
-same discretisation; the north-star bar is 1e-5 relative).
+same discretisation; the velocity bar is 1e-5 relative).
 """
 import numpy as np
 import pytest
