@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 GUESS_TEXT = {"previous": "previous solution", "extrapolate": "linear extrapolation of the last two solutions",
               "quadratic": "quadratic extrapolation of the last three solutions"}
 # kernel-class name -> kernel name in rocprofv3 traces
-KERNEL_NAMES = {"spread": "spread_tile2_kernel<8, 6>", "green": "green_kernel", "contract": "contract_tile2_kernel<13, 6>",
+KERNEL_NAMES = {"spread": "spread_tile2_kernel<8, 6>", "green": "fft_x_green_kernel<9>", "contract": "contract_tile2_kernel<13, 6>",
                 "realspace": "realspace_kernel<16>", "rfu": "rfu_apply_kernel<8>", "sweep": "tp_sweep1_kernel",
                 "ichol": "ichol_level_kernel", "gmres_vec": "multidot_partial4_kernel"}
 
@@ -44,7 +44,7 @@ def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5, gs_passes=2):
         return 76 * N + 44 * M
     if cls == "contract":    # 11 real grids in, pos + origin in, 11 floats per particle out
         return 44 * M + 28 * N + 44 * N
-    if cls == "green":       # 11 complex half spectra in and out + k vectors
+    if cls == "green":       # 11 complex half spectra in and out + k vectors (with the x transforms at Nx = 72)
         return 2 * 88 * Mh + 16 * Mh
     if cls == "realspace":   # nlist entries + per-particle pos/records/wave/addU in, 11N out
         return 4 * st["nlist_total"] + (16 + 48 + 44 + 24 + 44) * N

@@ -224,6 +224,10 @@ struct fsd_ctx {
     float2 *d_spec = nullptr;     // 11*Mh complex
     float4 *d_gridk = nullptr;    // Mh
     rocfft_plan plan_fwd = nullptr, plan_inv = nullptr;
+    // 2-D (y, z) transforms batched over the fields and x planes; the x direction and the Green's
+    // function then run in one kernel (fx_B: Nx / 8, 0 when the 3-D plans are used instead)
+    rocfft_plan plan_fwd2 = nullptr, plan_inv2 = nullptr;
+    int fx_B = 0;
     rocfft_execution_info info_fwd = nullptr, info_inv = nullptr;
     void *d_fftwork = nullptr; size_t fftwork_size = 0;
     int tiles[3] = {0, 0, 0}; int ntiles = 0; int tile = 8;
@@ -350,7 +354,7 @@ int k_normals(fsd_ctx *ctx, int64_t n, uint32_t step, uint32_t stream, float sd,
 int k_ff_noise(fsd_ctx *ctx, const float *noise, float fac);
 int k_wave_to_general(fsd_ctx *ctx, float *out);
 int k_shift_wrap(fsd_ctx *ctx, const float4 *pos, const float *psi6, float scale, float4 *out);
-int k_wave_contract(fsd_ctx *ctx, const float4 *pos);
+int k_wave_contract(fsd_ctx *ctx, const float4 *pos, bool inverse_yz = false);
 int k_precond_apply2(fsd_ctx *ctx, const float *b0, const float *b1, float *y0, float *y1, int n_rhs);
 int k_export_csr(fsd_ctx *ctx, const float4 *pos, int32_t *h_rowptr, int32_t *h_col, float *h_val,
                  int32_t *h_prow, int32_t *h_pcol, float *h_L, int64_t cap);

@@ -537,6 +537,168 @@ __global__ void green_kernel(float2 *spec, const float4 *gk, int64_t Mh, GreenGe
     for (int c = 0; c < 11; ++c) spec[c * Mh + id] = o[c];
 }
 
+// The x direction of the 3-D transform with the Green's function between its
+// forward and inverse halves, on the half spectrum after the (y, z) transforms:
+// element (c, x, ky, kz) at c Mh + (x Ny + ky) Nzh + kz.  One workgroup takes one
+// ky and FX_KC consecutive kz for all 11 fields, so every mode it holds after the
+// forward transform has all 11 fields in LDS and green_kernel's update applies
+// point by point; the inverse brings the columns back to x.  Nx = 8 B: forward,
+// an 8-point DFT over n1 for each n2 (x = B n1 + n2), the twiddle W^(n2 k1), then a
+// B-point DFT over n2 for each k1, leaving mode k1 + 8 k2 at B k1 + k2; the inverse
+// runs the same stages backwards with conjugate twiddles (unnormalised, as the
+// library's transforms).  W = exp(-2 pi i / Nx) from an LDS table built in FP64.
+#define FX_KC 4
+template <int B>
+__global__ void __launch_bounds__(256) fft_x_green_kernel(float2 *spec, const float4 *gk, int64_t Mh, GreenGeom g)
+{
+    constexpr int NX = 8 * B;
+    __shared__ float2 buf[11 * NX * FX_KC];   // [field][x][kz in the chunk]
+    __shared__ float2 tw[NX];
+    const int tid = threadIdx.x;
+    const int Nzh = g.Nz / 2 + 1;
+    const int nchunk = (Nzh + FX_KC - 1) / FX_KC;
+    const int ky = blockIdx.x / nchunk, kz0 = (blockIdx.x % nchunk) * FX_KC;
+    const int kc = min(FX_KC, Nzh - kz0);
+    if (tid < NX) {
+        double sn, cs;
+        sincospi(-2.0 * (double)tid / (double)NX, &sn, &cs);
+        tw[tid] = make_float2((float)cs, (float)sn);
+    }
+    const int nload = 11 * NX * kc;
+    for (int i = tid; i < nload; i += 256) {
+        const int c = i / (NX * kc), r = i - c * (NX * kc), x = r / kc, z = r - x * kc;
+        buf[(c * NX + x) * FX_KC + z] = spec[c * Mh + ((int64_t)x * g.Ny + ky) * Nzh + kz0 + z];
+    }
+    __syncthreads();
+    const int ncol = 11 * kc;
+    // W8^j and W_B^j = W^(8 j)
+    constexpr float R = 0.70710678118654752f;
+    const float2 w8[8] = {{1.f, 0.f}, {R, -R}, {0.f, -1.f}, {-R, -R}, {-1.f, 0.f}, {-R, R}, {0.f, 1.f}, {R, R}};
+    float2 wB[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j) wB[j] = tw[8 * j];
+    auto at = [&](int col, int x) -> float2 & { return buf[((col / kc) * NX + x) * FX_KC + col % kc]; };
+    auto cmul = [](float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+    auto cmulc = [](float2 a, float2 b) { return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); };
+    // forward stage A: 8-point DFTs, then the twiddles
+    for (int task = tid; task < ncol * B; task += 256) {
+        const int col = task / B, n2 = task - col * B;
+        float2 xs[8];
+#pragma unroll
+        for (int n1 = 0; n1 < 8; ++n1) xs[n1] = at(col, B * n1 + n2);
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) {
+            float2 y = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int n1 = 0; n1 < 8; ++n1) {
+                const float2 p = cmul(xs[n1], w8[(n1 * k1) & 7]);
+                y.x += p.x; y.y += p.y;
+            }
+            at(col, B * k1 + n2) = cmul(y, tw[n2 * k1]);
+        }
+    }
+    __syncthreads();
+    // forward stage B: B-point DFTs
+    for (int task = tid; task < ncol * 8; task += 256) {
+        const int col = task >> 3, k1 = task & 7;
+        float2 ys[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) ys[n2] = at(col, B * k1 + n2);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) {
+            float2 X = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int n2 = 0; n2 < B; ++n2) {
+                const float2 p = cmul(ys[n2], wB[(n2 * k2) % B]);
+                X.x += p.x; X.y += p.y;
+            }
+            at(col, B * k1 + k2) = X;
+        }
+    }
+    __syncthreads();
+    // Green's function (green_kernel's update) at every mode of the block
+    for (int pt = tid; pt < NX * kc; pt += 256) {
+        const int kx = pt / kc, z = pt - kx * kc;
+        const int pos = B * (kx & 7) + (kx >> 3);
+        const int k = kz0 + z;
+        const int64_t id = ((int64_t)kx * g.Ny + ky) * Nzh + k;
+        float2 v[11], o[11];
+#pragma unroll
+        for (int c = 0; c < 11; ++c) v[c] = buf[(c * NX + pos) * FX_KC + z];
+        green_apply(gk[id], id == 0, v, o);
+        if (has_nyquist(kx, ky, k, g.Nx, g.Ny, g.Nz)) {
+            const int ic = (g.Nx - kx) % g.Nx, jc = (g.Ny - ky) % g.Ny, kcj = (g.Nz - k) % g.Nz;
+            const float4 tkc = gridk_value(ic, jc, kcj, g.Nx, g.Ny, g.Nz, g.b, g.xi, g.eta, g.twopi_k);
+            float2 vc[11], oc[11];
+#pragma unroll
+            for (int c = 0; c < 11; ++c) vc[c] = make_float2(v[c].x, -v[c].y);
+            green_apply(tkc, false, vc, oc);
+#pragma unroll
+            for (int c = 0; c < 11; ++c) o[c] = make_float2(0.5f * (o[c].x + oc[c].x), 0.5f * (o[c].y - oc[c].y));
+        }
+#pragma unroll
+        for (int c = 0; c < 11; ++c) buf[(c * NX + pos) * FX_KC + z] = o[c];
+    }
+    __syncthreads();
+    // inverse stage B, then the conjugate twiddles
+    for (int task = tid; task < ncol * 8; task += 256) {
+        const int col = task >> 3, k1 = task & 7;
+        float2 Xs[B];
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) Xs[k2] = at(col, B * k1 + k2);
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) {
+            float2 y = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int k2 = 0; k2 < B; ++k2) {
+                const float2 p = cmulc(Xs[k2], wB[(n2 * k2) % B]);
+                y.x += p.x; y.y += p.y;
+            }
+            at(col, B * k1 + n2) = cmulc(y, tw[n2 * k1]);
+        }
+    }
+    __syncthreads();
+    // inverse stage A: 8-point DFTs back to x
+    for (int task = tid; task < ncol * B; task += 256) {
+        const int col = task / B, n2 = task - col * B;
+        float2 ys[8];
+#pragma unroll
+        for (int k1 = 0; k1 < 8; ++k1) ys[k1] = at(col, B * k1 + n2);
+#pragma unroll
+        for (int n1 = 0; n1 < 8; ++n1) {
+            float2 x = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int k1 = 0; k1 < 8; ++k1) {
+                const float2 p = cmulc(ys[k1], w8[(n1 * k1) & 7]);
+                x.x += p.x; x.y += p.y;
+            }
+            at(col, B * n1 + n2) = x;
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < nload; i += 256) {
+        const int c = i / (NX * kc), r = i - c * (NX * kc), x = r / kc, z = r - x * kc;
+        spec[c * Mh + ((int64_t)x * g.Ny + ky) * Nzh + kz0 + z] = buf[(c * NX + x) * FX_KC + z];
+    }
+}
+
+// launch of fft_x_green_kernel for Nx = 8 B (B in 5..11, so the block fits in 64 KiB of LDS); false
+// for other sizes
+static bool launch_fft_x_green(fsd_ctx *ctx, const GreenGeom &gg)
+{
+    const int Nzh = ctx->pse.Nz / 2 + 1;
+    const dim3 grid(ctx->pse.Ny * ((Nzh + FX_KC - 1) / FX_KC)), blk(256);
+    float2 *spec = ctx->d_spec;
+    const float4 *gk = ctx->d_gridk;
+    const int64_t Mh = ctx->Mh;
+    switch (ctx->fx_B) {
+#define FX_CASE(b) case b: hipLaunchKernelGGL(fft_x_green_kernel<b>, grid, blk, 0, ctx->stream, spec, gk, Mh, gg); return true;
+        FX_CASE(5) FX_CASE(6) FX_CASE(7) FX_CASE(8) FX_CASE(9) FX_CASE(10) FX_CASE(11)
+#undef FX_CASE
+        default: return false;
+    }
+}
+
 // ------------------------------------------------------------ contraction
 // Mobility_WaveSpace_ContractU/D (Mobility.cu:503-841): one wavefront per
 // particle, lanes over the (y, z) stencil pairs, loop over x.  Output per
@@ -988,8 +1150,9 @@ __global__ void __launch_bounds__(256) realspace_kernel(RealParams rp, const flo
 }
 
 // inverse transform of the 11 half spectra in d_spec and contraction of the
-// grids at the particles into d_wave (u and the velocity gradient)
-int k_wave_contract(fsd_ctx *ctx, const float4 *pos)
+// grids at the particles into d_wave (u and the velocity gradient); with
+// inverse_yz only the (y, z) part of the inverse remains (x has been done)
+int k_wave_contract(fsd_ctx *ctx, const float4 *pos, bool inverse_yz)
 {
     const int N = ctx->N;
     const fsd_pse_info &ps = ctx->pse;
@@ -1008,7 +1171,8 @@ int k_wave_contract(fsd_ctx *ctx, const float4 *pos)
     {
         void *in[1] = {ctx->d_spec};
         void *out[1] = {ctx->d_grid};
-        if (rocfft_execute(ctx->plan_inv, in, out, ctx->info_inv) != rocfft_status_success) {
+        if (rocfft_execute(inverse_yz ? ctx->plan_inv2 : ctx->plan_inv, in, out, ctx->info_inv) !=
+            rocfft_status_success) {
             ctx->err = "rocfft inverse failed";
             return FSD_E_HIP;
         }
@@ -1063,11 +1227,12 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
                                (int64_t)ctx->M);
         FSD_HIP(hipGetLastError());
         prof_end(ctx, PROF_SPREAD, ev);
+        const bool fx = ctx->fx_B > 0 && ctx->plan_fwd2 && ctx->plan_inv2;
         prof_begin(ctx, PROF_FFT, &ev);
         {
             void *in[1] = {ctx->d_grid};
             void *out[1] = {ctx->d_spec};
-            if (rocfft_execute(ctx->plan_fwd, in, out, ctx->info_fwd) != rocfft_status_success) {
+            if (rocfft_execute(fx ? ctx->plan_fwd2 : ctx->plan_fwd, in, out, ctx->info_fwd) != rocfft_status_success) {
                 ctx->err = "rocfft forward failed";
                 return FSD_E_HIP;
             }
@@ -1075,11 +1240,15 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
         prof_end(ctx, PROF_FFT, ev);
         prof_begin(ctx, PROF_GREEN, &ev);
         GreenGeom gg{ps.Nx, ps.Ny, ps.Nz, ctx->box, ctx->p.xi, ps.eta, k_twopi(ctx)};
-        hipLaunchKernelGGL(green_kernel, dim3(nblk(ctx->Mh, 256)), dim3(256), 0, ctx->stream, ctx->d_spec,
-                           ctx->d_gridk, (int64_t)ctx->Mh, gg);
+        if (fx) {   // x transforms and the Green's function in one pass
+            if (!launch_fft_x_green(ctx, gg)) { ctx->err = "no x-transform kernel for this grid"; return FSD_E_PARAM; }
+        } else {
+            hipLaunchKernelGGL(green_kernel, dim3(nblk(ctx->Mh, 256)), dim3(256), 0, ctx->stream, ctx->d_spec,
+                               ctx->d_gridk, (int64_t)ctx->Mh, gg);
+        }
         FSD_HIP(hipGetLastError());
         prof_end(ctx, PROF_GREEN, ev);
-        const int rcw = k_wave_contract(ctx, pos);
+        const int rcw = k_wave_contract(ctx, pos, fx);
         if (rcw) return rcw;
         wave = ctx->d_wave;
     }

@@ -234,6 +234,26 @@ static int alloc_all(fsd_ctx *ctx)
         rocfft_plan_get_work_buffer_size(ctx->plan_fwd, &w1);
         rocfft_plan_get_work_buffer_size(ctx->plan_inv, &w2);
         ctx->fftwork_size = std::max(w1, w2);
+        // the mobility's transforms: (y, z) by rocFFT, batched over the 11 fields and the x planes (the
+        // layout of the 3-D half spectrum), x and the Green's function in one kernel when Nx = 8 B,
+        // 5 <= B <= 11.  FSD_FFT_3D=1 keeps the 3-D plans and the separate Green's function.
+        const char *f3 = getenv("FSD_FFT_3D");
+        const int B = ps.Nx / 8;
+        if (!(f3 && f3[0] == '1') && ps.Nx % 8 == 0 && B >= 5 && B <= 11) {
+            size_t len2[2] = {(size_t)ps.Nz, (size_t)ps.Ny};
+            const size_t batch = 11 * (size_t)ps.Nx;
+            if (rocfft_plan_create(&ctx->plan_fwd2, rocfft_placement_notinplace, rocfft_transform_type_real_forward,
+                                   rocfft_precision_single, 2, len2, batch, nullptr) != rocfft_status_success ||
+                rocfft_plan_create(&ctx->plan_inv2, rocfft_placement_notinplace, rocfft_transform_type_real_inverse,
+                                   rocfft_precision_single, 2, len2, batch, nullptr) != rocfft_status_success) {
+                ctx->err = "rocfft_plan_create (2-D) failed";
+                return FSD_E_HIP;
+            }
+            rocfft_plan_get_work_buffer_size(ctx->plan_fwd2, &w1);
+            rocfft_plan_get_work_buffer_size(ctx->plan_inv2, &w2);
+            ctx->fftwork_size = std::max(ctx->fftwork_size, std::max(w1, w2));
+            ctx->fx_B = B;
+        }
         if (ctx->fftwork_size) CK(hipMalloc(&ctx->d_fftwork, ctx->fftwork_size));
         rocfft_execution_info_create(&ctx->info_fwd);
         rocfft_execution_info_create(&ctx->info_inv);
@@ -346,6 +366,8 @@ static void free_all(fsd_ctx *ctx)
     if (ctx->h_lan) hipHostFree(ctx->h_lan);
     if (ctx->plan_fwd) rocfft_plan_destroy(ctx->plan_fwd);
     if (ctx->plan_inv) rocfft_plan_destroy(ctx->plan_inv);
+    if (ctx->plan_fwd2) rocfft_plan_destroy(ctx->plan_fwd2);
+    if (ctx->plan_inv2) rocfft_plan_destroy(ctx->plan_inv2);
     if (ctx->info_fwd) rocfft_execution_info_destroy(ctx->info_fwd);
     if (ctx->info_inv) rocfft_execution_info_destroy(ctx->info_inv);
     for (auto &s : ctx->slots)

@@ -89,6 +89,32 @@ def test_mobility_parity(case, parts):
     assert e < 1e-5
 
 
+@pytest.mark.parametrize("N,phi,seed", [(4096, 0.30, 12), (32768, 0.45, 1)])
+def test_fused_x_transform_matches_3d_plans(N, phi, seed, monkeypatch):
+    """on grids with Nx = 8 B (40 and 72 here), the (y, z) transforms by rocFFT plus the
+    kernel doing x and the Green's function in one pass give the mobility of the 3-D
+    rocFFT plans and the separate Green's function kernel (FP32, different rounding)"""
+    import torch
+    from fsd_amd.stokes import Stokes
+    pos, L = packing(N, phi, seed)
+    print("N", N, "phi", phi, "seed", seed, flush=True)
+    p4 = np.zeros((N, 4), dtype=np.float32)
+    p4[:, :3] = pos
+    dpos = torch.from_numpy(p4.reshape(-1)).cuda()
+    gF = random_vec(11 * N, 3).astype(np.float32)
+    out = {}
+    for f3 in ("0", "1"):
+        monkeypatch.setenv("FSD_FFT_3D", f3)
+        st = Stokes(N, L)
+        st.build_nlist(dpos)
+        assert st.pse_info()["Nx"] % 8 == 0
+        out[f3] = [st.mobility(dpos, _t(gF), parts).cpu().numpy() for parts in (2, 3)]
+    for k, parts in enumerate((2, 3)):
+        e = rel_err(out["0"][k], out["1"][k])
+        print("parts", parts, "fused vs 3-D rel err", e)
+        assert e < 1e-5
+
+
 @pytest.mark.parametrize("case", ["C1", "C2"])
 def test_precond_pattern_bitexact(case):
     st, orc, dpos, N = _setup(case)
