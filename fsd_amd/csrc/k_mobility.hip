@@ -544,9 +544,89 @@ __global__ void green_kernel(float2 *spec, const float4 *gk, int64_t Mh, GreenGe
 // forward transform has all 11 fields in LDS and green_kernel's update applies
 // point by point; the inverse brings the columns back to x.  Nx = 8 B: forward,
 // an 8-point DFT over n1 for each n2 (x = B n1 + n2), the twiddle W^(n2 k1), then a
-// B-point DFT over n2 for each k1, leaving mode k1 + 8 k2 at B k1 + k2; the inverse
+// B-point DFT over n2 for each k1 (fx_dft8, fx_dftB), leaving mode k1 + 8 k2 at
+// B k1 + k2; the inverse
 // runs the same stages backwards with conjugate twiddles (unnormalised, as the
 // library's transforms).  W = exp(-2 pi i / Nx) from an LDS table built in FP64.
+// small DFTs of the x pass (INV: the conjugate kernel, unnormalised): 8 points by radix 2,
+// 9 points as 3 x 3, other lengths directly from the table wB[j] = W_B^j
+__device__ __forceinline__ float2 fx_mul_i(float2 v, bool inv)   // v * (inv ? i : -i)
+{
+    return inv ? make_float2(-v.y, v.x) : make_float2(v.y, -v.x);
+}
+template <bool INV>
+__device__ __forceinline__ void fx_dft4(const float2 c[4], float2 y[4])
+{
+    const float2 t0 = make_float2(c[0].x + c[2].x, c[0].y + c[2].y), t1 = make_float2(c[0].x - c[2].x, c[0].y - c[2].y);
+    const float2 t2 = make_float2(c[1].x + c[3].x, c[1].y + c[3].y);
+    const float2 t3 = fx_mul_i(make_float2(c[1].x - c[3].x, c[1].y - c[3].y), INV);
+    y[0] = make_float2(t0.x + t2.x, t0.y + t2.y); y[2] = make_float2(t0.x - t2.x, t0.y - t2.y);
+    y[1] = make_float2(t1.x + t3.x, t1.y + t3.y); y[3] = make_float2(t1.x - t3.x, t1.y - t3.y);
+}
+template <bool INV>
+__device__ __forceinline__ void fx_dft8(const float2 x[8], float2 y[8])
+{
+    constexpr float R = 0.70710678118654752f;
+    float2 a[4], b[4], e[4], o[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        a[n] = make_float2(x[n].x + x[n + 4].x, x[n].y + x[n + 4].y);
+        b[n] = make_float2(x[n].x - x[n + 4].x, x[n].y - x[n + 4].y);
+    }
+    // b_n W8^(-+n): W8 = R (1 -+ i), W8^2 = -+i, W8^3 = R (-1 -+ i)
+    b[1] = INV ? make_float2(R * (b[1].x - b[1].y), R * (b[1].x + b[1].y))
+               : make_float2(R * (b[1].x + b[1].y), R * (b[1].y - b[1].x));
+    b[2] = fx_mul_i(b[2], INV);
+    b[3] = INV ? make_float2(-R * (b[3].x + b[3].y), R * (b[3].x - b[3].y))
+               : make_float2(R * (b[3].y - b[3].x), -R * (b[3].x + b[3].y));
+    fx_dft4<INV>(a, e);
+    fx_dft4<INV>(b, o);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { y[2 * k] = e[k]; y[2 * k + 1] = o[k]; }
+}
+template <bool INV>
+__device__ __forceinline__ void fx_dft3(float2 a, float2 b, float2 c, float2 &X0, float2 &X1, float2 &X2)
+{
+    constexpr float H = 0.86602540378443865f;   // sqrt(3) / 2
+    const float2 sm = make_float2(b.x + c.x, b.y + c.y), d = make_float2(b.x - c.x, b.y - c.y);
+    const float2 t = fx_mul_i(make_float2(H * d.x, H * d.y), INV);
+    const float2 m = make_float2(a.x - 0.5f * sm.x, a.y - 0.5f * sm.y);
+    X0 = make_float2(a.x + sm.x, a.y + sm.y);
+    X1 = make_float2(m.x + t.x, m.y + t.y);
+    X2 = make_float2(m.x - t.x, m.y - t.y);
+}
+template <int B, bool INV>
+__device__ __forceinline__ void fx_dftB(const float2 x[B], float2 X[B], const float2 wB[B])
+{
+    auto mul = [](float2 a, float2 w) {   // a * w, or a * conj(w) for the inverse
+        return INV ? make_float2(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y)
+                   : make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+    };
+    if constexpr (B == 9) {   // n = 3 m1 + m2, k = k1 + 3 k2
+        float2 u[3][3];
+#pragma unroll
+        for (int m2 = 0; m2 < 3; ++m2) {
+            fx_dft3<INV>(x[m2], x[3 + m2], x[6 + m2], u[0][m2], u[1][m2], u[2][m2]);
+#pragma unroll
+            for (int k1 = 1; k1 < 3; ++k1)
+                if (m2 > 0) u[k1][m2] = mul(u[k1][m2], wB[m2 * k1]);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 3; ++k1) fx_dft3<INV>(u[k1][0], u[k1][1], u[k1][2], X[k1], X[k1 + 3], X[k1 + 6]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int n = 0; n < B; ++n) {
+                const float2 p = mul(x[n], wB[(n * k) % B]);
+                acc.x += p.x; acc.y += p.y;
+            }
+            X[k] = acc;
+        }
+    }
+}
+
 #define FX_KC 4
 template <int B>
 __global__ void __launch_bounds__(256) fft_x_green_kernel(float2 *spec, const float4 *gk, int64_t Mh, GreenGeom g)
@@ -571,9 +651,7 @@ __global__ void __launch_bounds__(256) fft_x_green_kernel(float2 *spec, const fl
     }
     __syncthreads();
     const int ncol = 11 * kc;
-    // W8^j and W_B^j = W^(8 j)
-    constexpr float R = 0.70710678118654752f;
-    const float2 w8[8] = {{1.f, 0.f}, {R, -R}, {0.f, -1.f}, {-R, -R}, {-1.f, 0.f}, {-R, R}, {0.f, 1.f}, {R, R}};
+    // W_B^j = W^(8 j)
     float2 wB[B];
 #pragma unroll
     for (int j = 0; j < B; ++j) wB[j] = tw[8 * j];
@@ -583,37 +661,23 @@ __global__ void __launch_bounds__(256) fft_x_green_kernel(float2 *spec, const fl
     // forward stage A: 8-point DFTs, then the twiddles
     for (int task = tid; task < ncol * B; task += 256) {
         const int col = task / B, n2 = task - col * B;
-        float2 xs[8];
+        float2 xs[8], y[8];
 #pragma unroll
         for (int n1 = 0; n1 < 8; ++n1) xs[n1] = at(col, B * n1 + n2);
+        fx_dft8<false>(xs, y);
 #pragma unroll
-        for (int k1 = 0; k1 < 8; ++k1) {
-            float2 y = make_float2(0.f, 0.f);
-#pragma unroll
-            for (int n1 = 0; n1 < 8; ++n1) {
-                const float2 p = cmul(xs[n1], w8[(n1 * k1) & 7]);
-                y.x += p.x; y.y += p.y;
-            }
-            at(col, B * k1 + n2) = cmul(y, tw[n2 * k1]);
-        }
+        for (int k1 = 0; k1 < 8; ++k1) at(col, B * k1 + n2) = cmul(y[k1], tw[n2 * k1]);
     }
     __syncthreads();
     // forward stage B: B-point DFTs
     for (int task = tid; task < ncol * 8; task += 256) {
         const int col = task >> 3, k1 = task & 7;
-        float2 ys[B];
+        float2 ys[B], X[B];
 #pragma unroll
         for (int n2 = 0; n2 < B; ++n2) ys[n2] = at(col, B * k1 + n2);
+        fx_dftB<B, false>(ys, X, wB);
 #pragma unroll
-        for (int k2 = 0; k2 < B; ++k2) {
-            float2 X = make_float2(0.f, 0.f);
-#pragma unroll
-            for (int n2 = 0; n2 < B; ++n2) {
-                const float2 p = cmul(ys[n2], wB[(n2 * k2) % B]);
-                X.x += p.x; X.y += p.y;
-            }
-            at(col, B * k1 + k2) = X;
-        }
+        for (int k2 = 0; k2 < B; ++k2) at(col, B * k1 + k2) = X[k2];
     }
     __syncthreads();
     // Green's function (green_kernel's update) at every mode of the block
@@ -643,37 +707,23 @@ __global__ void __launch_bounds__(256) fft_x_green_kernel(float2 *spec, const fl
     // inverse stage B, then the conjugate twiddles
     for (int task = tid; task < ncol * 8; task += 256) {
         const int col = task >> 3, k1 = task & 7;
-        float2 Xs[B];
+        float2 Xs[B], y[B];
 #pragma unroll
         for (int k2 = 0; k2 < B; ++k2) Xs[k2] = at(col, B * k1 + k2);
+        fx_dftB<B, true>(Xs, y, wB);
 #pragma unroll
-        for (int n2 = 0; n2 < B; ++n2) {
-            float2 y = make_float2(0.f, 0.f);
-#pragma unroll
-            for (int k2 = 0; k2 < B; ++k2) {
-                const float2 p = cmulc(Xs[k2], wB[(n2 * k2) % B]);
-                y.x += p.x; y.y += p.y;
-            }
-            at(col, B * k1 + n2) = cmulc(y, tw[n2 * k1]);
-        }
+        for (int n2 = 0; n2 < B; ++n2) at(col, B * k1 + n2) = cmulc(y[n2], tw[n2 * k1]);
     }
     __syncthreads();
     // inverse stage A: 8-point DFTs back to x
     for (int task = tid; task < ncol * B; task += 256) {
         const int col = task / B, n2 = task - col * B;
-        float2 ys[8];
+        float2 ys[8], x[8];
 #pragma unroll
         for (int k1 = 0; k1 < 8; ++k1) ys[k1] = at(col, B * k1 + n2);
+        fx_dft8<true>(ys, x);
 #pragma unroll
-        for (int n1 = 0; n1 < 8; ++n1) {
-            float2 x = make_float2(0.f, 0.f);
-#pragma unroll
-            for (int k1 = 0; k1 < 8; ++k1) {
-                const float2 p = cmulc(ys[k1], w8[(n1 * k1) & 7]);
-                x.x += p.x; x.y += p.y;
-            }
-            at(col, B * n1 + n2) = x;
-        }
+        for (int n1 = 0; n1 < 8; ++n1) at(col, B * n1 + n2) = x[n1];
     }
     __syncthreads();
     for (int i = tid; i < nload; i += 256) {
