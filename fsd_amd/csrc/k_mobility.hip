@@ -267,10 +267,11 @@ __global__ void __launch_bounds__(512) spread_kernel(SpreadGeom g, const float4 
 // lz) visits, for each of its 36 halo columns (lx + a, ly + b), the
 // consecutive staged particles of cells lz .. lz + P - 1: exactly the
 // particles whose stencil covers it.  A tile whose halo holds more than
-// SP2_CAP particles reads them from global memory instead.  Same weights and
-// accumulation as spread_kernel (each point sums its particles in the same
-// cell order, and writes once).
-#define SP2_CAP 640
+// SP2_CAP particles reads them from global memory instead (about 190 are staged
+// at volume fraction 0.45; the cap keeps the LDS at 50 KB, three workgroups per
+// CU).  Same weights and accumulation as spread_kernel (each point sums its
+// particles in the same cell order, and writes once).
+#define SP2_CAP 512
 // contribution of one particle (origin o, position p, records r) to grid point (gx, gy, gz)
 __device__ __forceinline__ void spread_point_add(const SpreadGeom &g, int gx, int gy, int gz, int o0, int o1, int o2,
                                                  float p0, float p1, float p2, float4 r0, float4 r1, float4 r2,
