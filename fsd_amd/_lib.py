@@ -50,6 +50,9 @@ class FsdParams(C.Structure):
         ("gmres_true_tol", C.c_float),
         ("guess", C.c_int32),
         ("gs_passes", C.c_int32),
+        ("precond", C.c_int32),
+        ("jacobi_iters", C.c_int32),
+        ("factor_sweeps", C.c_int32),
     ]
 
 

@@ -278,6 +278,13 @@ struct fsd_ctx {
     int *d_tp_fseq = nullptr, *d_tp_bseq = nullptr;      // N: sequence number of each permuted particle
     int *d_tp_fplace = nullptr, *d_tp_bplace = nullptr;  // 3 N: first lane and step of each row, step ends
     float *d_tp_Z = nullptr;                             // 36 N: inverse diagonal blocks
+    // application by Jacobi iterations (precond modes 1 and 2)
+    int precond_mode = 0;                                // 0: exact sweeps, 1: IC(0) + Jacobi, 2: learned + Jacobi
+    int jac_k = 0;                                       // Jacobi iterations per triangular solve
+    float *d_Fblk = nullptr;                             // 36 per lower block: Z_P L_PQ
+    int *d_lrowidx = nullptr;                            // row P of every lower block
+    float *d_jac = nullptr;                              // 4 x 6N iterates
+    float *d_Lblk2 = nullptr, *d_Ldiag2 = nullptr;       // second factor buffer of the parallel IC(0) sweeps
 
     // saddle / GMRES
     float *d_V = nullptr, *d_w = nullptr, *d_t = nullptr, *d_t2 = nullptr;
@@ -348,6 +355,7 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
 int k_prune(fsd_ctx *ctx, const float4 *pos);
 int k_build_blocks(fsd_ctx *ctx, const float4 *pos);
 int k_ichol(fsd_ctx *ctx, float lambda, int *failed);
+int k_ichol_sweeps(fsd_ctx *ctx, float lambda, int sweeps, int *failed);
 int k_sweep_setup(fsd_ctx *ctx);
 int k_sweep_pack(fsd_ctx *ctx);
 int k_normals(fsd_ctx *ctx, int64_t n, uint32_t step, uint32_t stream, float sd, float *out);
@@ -376,5 +384,10 @@ int k_multi_axpy(fsd_ctx *ctx, int64_t n, int k, const float *V, int64_t ldv, co
 int k_nrm2(fsd_ctx *ctx, int64_t n, const float *x, double *out);
 int k_scale_by_norm(fsd_ctx *ctx, int64_t n, float *v, const double *h, int K, double *hn_out);
 int k_saddle_precond(fsd_ctx *ctx, const float *b, float *x);
+// the factor's application by Jacobi iterations (k_jacobi.hip)
+int k_diag_inverse(fsd_ctx *ctx);
+int k_jacobi_pack(fsd_ctx *ctx);
+int k_saddle_precond_jacobi(fsd_ctx *ctx, const float *b, float *x);
+int k_precond_apply_jacobi(fsd_ctx *ctx, const float *b, float *y);
 int k_saddle_multiply(fsd_ctx *ctx, const float4 *pos, const float *x, float *y);
 

@@ -139,9 +139,16 @@ int k_build_blocks(fsd_ctx *ctx, const float4 *pos)
 // highest level among its lower neighbours); one launch per level, one
 // wavefront per particle, lane (a*6+b) owning entry (a,b).  Kernel
 // boundaries order the levels, so no particle ever waits on another.
+//
+// The same kernel runs the parallel fixed-point form (Chow & Patel, "Fine-grained
+// parallel incomplete LU factorization", SIAM J. Sci. Comput. 2015): items = null
+// puts every particle in one launch, and the rows Q < P are read from the previous
+// sweep's factor (Lblk_q, Ldiag_q) while row P's own new blocks are read back
+// from the output.  Its fixed point is the IC(0) factor of the level recurrence.
 __global__ void __launch_bounds__(64) ichol_level_kernel(const int *items, const int *lrow, const int *lcol,
                                                          const float *Ablk, const float *Adiag, float lambda,
-                                                         float *Lblk, float *Ldiag, int *fail)
+                                                         float *Lblk, float *Ldiag, const float *Lblk_q,
+                                                         const float *Ldiag_q, int *fail)
 {
     __shared__ float T[36];
     __shared__ float Lq[36];
@@ -149,7 +156,7 @@ __global__ void __launch_bounds__(64) ichol_level_kernel(const int *items, const
     const int lane = threadIdx.x;
     const int a = lane / 6, bcol = lane % 6;
     const bool ent = lane < 36;
-    const int P = items[blockIdx.x];
+    const int P = items ? items[blockIdx.x] : (int)blockIdx.x;
     const int beg = lrow[P], end = lrow[P + 1];
     float dacc = 0.f;  // sum_Q L_PQ L_PQ^T for entry (a, bcol)
     for (int e = beg; e < end; ++e) {
@@ -163,7 +170,7 @@ __global__ void __launch_bounds__(64) ichol_level_kernel(const int *items, const
                 const int c1 = lcol[p1], c2 = lcol[p2];
                 if (c1 == c2) {
                     const float *x = Lblk + (int64_t)p1 * 36 + a * 6;
-                    const float *y = Lblk + (int64_t)p2 * 36 + bcol * 6;
+                    const float *y = Lblk_q + (int64_t)p2 * 36 + bcol * 6;
                     float sacc = 0.f;
                     for (int c = 0; c < 6; ++c) sacc += x[c] * y[c];
                     tv -= sacc;
@@ -172,7 +179,7 @@ __global__ void __launch_bounds__(64) ichol_level_kernel(const int *items, const
                 else ++p2;
             }
             T[lane] = tv;
-            Lq[lane] = Ldiag[(int64_t)Q * 36 + lane];
+            Lq[lane] = Ldiag_q[(int64_t)Q * 36 + lane];
         }
         __syncthreads();
         // X L_QQ^T = T, column by column
@@ -228,8 +235,71 @@ int k_ichol(fsd_ctx *ctx, float lambda, int *failed)
     for (int l = 0; l < ctx->pch.nlev_f; ++l) {
         const int b = ctx->pch.lev_f_ptr[l], n = ctx->pch.lev_f_ptr[l + 1] - b;
         hipLaunchKernelGGL(ichol_level_kernel, dim3(n), dim3(64), 0, ctx->stream, ctx->d_lev_f + b, ctx->d_lrow,
-                           ctx->d_lcol, ctx->d_Ablk, ctx->d_Adiag, lambda, ctx->d_Lblk, ctx->d_Ldiag,
-                           ctx->d_flags + 2);
+                           ctx->d_lcol, ctx->d_Ablk, ctx->d_Adiag, lambda, ctx->d_Lblk, ctx->d_Ldiag, ctx->d_Lblk,
+                           ctx->d_Ldiag, ctx->d_flags + 2);
+    }
+    FSD_HIP(hipGetLastError());
+    FSD_HIP(hipMemcpyAsync(ctx->h_flags + 2, ctx->d_flags + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FSD_HIP(hipStreamSynchronize(ctx->stream));
+    prof_end(ctx, PROF_ICHOL, ev);
+    *failed = ctx->h_flags[2];
+    return 0;
+}
+
+// start of the parallel sweeps: L_PP = chol(A_PP + lambda shift), no blocks below the diagonal
+__global__ void ichol_start_kernel(int N, const float *Adiag, float lambda, float *Ldiag, int *fail)
+{
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= N) return;
+    float D[36];
+    for (int q = 0; q < 36; ++q) D[q] = Adiag[(int64_t)P * 36 + q];
+    for (int c = 0; c < 6; ++c) D[7 * c] += lambda * ((c < 3) ? 1.0f : 1.33333333f);
+    bool bad = false;
+    for (int c = 0; c < 6; ++c) {
+        float sd = D[c * 6 + c];
+        for (int q = 0; q < c; ++q) sd -= D[c * 6 + q] * D[c * 6 + q];
+        if (!(sd > 0.f)) { bad = true; sd = 1.f; }
+        const float d = sqrtf(sd);
+        D[c * 6 + c] = d;
+        for (int r = c + 1; r < 6; ++r) {
+            float v = D[r * 6 + c];
+            for (int q = 0; q < c; ++q) v -= D[r * 6 + q] * D[c * 6 + q];
+            D[r * 6 + c] = v / d;
+        }
+    }
+    for (int r = 0; r < 6; ++r)
+        for (int c = r + 1; c < 6; ++c) D[r * 6 + c] = 0.f;
+    for (int q = 0; q < 36; ++q) Ldiag[(int64_t)P * 36 + q] = D[q];
+    if (bad) atomicMax(fail, 1);
+}
+
+__global__ void zero_kernel(int64_t n, float *x)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.f;
+}
+
+// IC(0) by `sweeps` parallel fixed-point sweeps; the factor ends in (d_Lblk, d_Ldiag).
+// A non-positive pivot of the last sweep fails the attempt (the caller doubles lambda).
+int k_ichol_sweeps(fsd_ctx *ctx, float lambda, int sweeps, int *failed)
+{
+    const int N = ctx->N;
+    const int64_t nl = ctx->pch.nlow;
+    hipEvent_t ev;
+    prof_begin(ctx, PROF_ICHOL, &ev);
+    // buffers alternate so that the last sweep writes d_Lblk / d_Ldiag
+    float *bl[2] = {ctx->d_Lblk, ctx->d_Lblk2}, *bd[2] = {ctx->d_Ldiag, ctx->d_Ldiag2};
+    int cur = sweeps & 1;   // buffer holding the start
+    FSD_HIP(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(ichol_start_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_Adiag, lambda,
+                       bd[cur], ctx->d_flags + 2);
+    if (nl) hipLaunchKernelGGL(zero_kernel, dim3(nblk(36 * nl, 256)), dim3(256), 0, ctx->stream, 36 * nl, bl[cur]);
+    for (int s = 0; s < sweeps; ++s) {
+        FSD_HIP(hipMemsetAsync(ctx->d_flags + 2, 0, sizeof(int), ctx->stream));   // pivots of the last sweep decide
+        hipLaunchKernelGGL(ichol_level_kernel, dim3(N), dim3(64), 0, ctx->stream, (const int *)nullptr, ctx->d_lrow,
+                           ctx->d_lcol, ctx->d_Ablk, ctx->d_Adiag, lambda, bl[cur ^ 1], bd[cur ^ 1], bl[cur],
+                           bd[cur], ctx->d_flags + 2);
+        cur ^= 1;
     }
     FSD_HIP(hipGetLastError());
     FSD_HIP(hipMemcpyAsync(ctx->h_flags + 2, ctx->d_flags + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -665,6 +735,14 @@ __global__ void diag_inverse_kernel(int N, const float *Ldiag, float *Z)
         }
     }
     for (int q = 0; q < 36; ++q) Z[(int64_t)P * 36 + q] = Zm[q];
+}
+
+int k_diag_inverse(fsd_ctx *ctx)
+{
+    hipLaunchKernelGGL(diag_inverse_kernel, dim3(nblk(ctx->N, 128)), dim3(128), 0, ctx->stream, ctx->N, ctx->d_Ldiag,
+                       ctx->d_tp_Z);
+    FSD_HIP(hipGetLastError());
+    return 0;
 }
 
 // Lane arrays of a sweep from its row placement (the rules of task_plan_lanes):

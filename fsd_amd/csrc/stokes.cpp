@@ -210,10 +210,10 @@ static int alloc_all(fsd_ctx *ctx)
     CK(hipMalloc(&ctx->d_scan_tmp, sizeof(int) * 2 * (scan_excl_chunks(ctx->M) + 1)));
     if (ctx->tile + ps.P - 1 > 32) { ctx->err = "spreading support P too large for the tile halo"; return FSD_E_PARAM; }
     ctx->contract_lds = k_contract_lds(ctx);
-    // FSD_TRSV_LEVELS=1 runs the triangular sweeps one launch per level
+    // FSD_TRSV_LEVELS=1 runs the triangular sweeps one launch per level; the Jacobi modes need no sweep plans
     {
         const char *lv = getenv("FSD_TRSV_LEVELS");
-        if (lv && lv[0] == '1') ctx->tp_R = 0;
+        if ((lv && lv[0] == '1') || ctx->precond_mode != 0) ctx->tp_R = 0;
         else RC(k_sweep_setup(ctx));
         // FSD_SWEEP_JOINT=1 runs both right-hand sides of a sweep in one workgroup
         const char *jt = getenv("FSD_SWEEP_JOINT");
@@ -286,6 +286,13 @@ static int alloc_all(fsd_ctx *ctx)
     CK(hipMalloc(&ctx->d_upos, sizeof(int) * ctx->nlow_cap));
     CK(hipMalloc(&ctx->d_Ablk, sizeof(float) * 36 * ctx->nlow_cap));
     CK(hipMalloc(&ctx->d_Lblk, sizeof(float) * 36 * ctx->nlow_cap));
+    if (ctx->precond_mode != 0) {
+        CK(hipMalloc(&ctx->d_Fblk, sizeof(float) * 36 * ctx->nlow_cap));
+        CK(hipMalloc(&ctx->d_lrowidx, sizeof(int) * ctx->nlow_cap));
+        CK(hipMalloc(&ctx->d_jac, sizeof(float) * 24 * (size_t)N));
+    }
+    if (ctx->p.factor_sweeps > 0) CK(hipMalloc(&ctx->d_Lblk2, sizeof(float) * 36 * ctx->nlow_cap));
+    if (ctx->p.factor_sweeps > 0) CK(hipMalloc(&ctx->d_Ldiag2, sizeof(float) * 36 * (size_t)N));
     CK(hipMalloc(&ctx->d_Adiag, sizeof(float) * 36 * (size_t)N));
     CK(hipMalloc(&ctx->d_Ldiag, sizeof(float) * 36 * (size_t)N));
     CK(hipMalloc(&ctx->d_lev_f, sizeof(int) * N));
@@ -354,7 +361,7 @@ static void free_all(fsd_ctx *ctx)
                     ctx->d_force, ctx->d_mask1, ctx->d_mask2, ctx->d_tp_finfo, ctx->d_tp_binfo,
                     ctx->d_tp_fbsrc, ctx->d_tp_bbsrc, ctx->d_tp_fblk, ctx->d_tp_bblk,
                     ctx->d_tp_finp, ctx->d_tp_binp, ctx->d_tp_fres, ctx->d_tp_bres, ctx->d_tp_fseq, ctx->d_tp_bseq, ctx->d_tp_fplace, ctx->d_tp_bplace,
-                    ctx->d_tp_Z,
+                    ctx->d_tp_Z, ctx->d_Fblk, ctx->d_lrowidx, ctx->d_jac, ctx->d_Lblk2, ctx->d_Ldiag2,
                     ctx->d_Z, ctx->d_lanV, ctx->d_lanw, ctx->d_lany0, ctx->d_lan_h, ctx->d_noise, ctx->d_posp,
                     ctx->d_rfd_rhs, ctx->d_rfd_sol};
     for (void *p : ptrs)
@@ -384,6 +391,16 @@ static int grow_lower(fsd_ctx *ctx, int64_t need)
     for (float *q : {ctx->d_Ablk, ctx->d_Lblk}) hipFree(q);
     for (int **pp : {&ctx->d_lcol, &ctx->d_ucol, &ctx->d_upos}) CK(hipMalloc(pp, sizeof(int) * cap));
     for (float **pp : {&ctx->d_Ablk, &ctx->d_Lblk}) CK(hipMalloc(pp, sizeof(float) * 36 * cap));
+    if (ctx->d_Fblk) {
+        hipFree(ctx->d_Fblk);
+        hipFree(ctx->d_lrowidx);
+        CK(hipMalloc(&ctx->d_Fblk, sizeof(float) * 36 * cap));
+        CK(hipMalloc(&ctx->d_lrowidx, sizeof(int) * cap));
+    }
+    if (ctx->d_Lblk2) {
+        hipFree(ctx->d_Lblk2);
+        CK(hipMalloc(&ctx->d_Lblk2, sizeof(float) * 36 * cap));
+    }
     ctx->nlow_cap = cap;
     return 0;
 }
@@ -483,14 +500,16 @@ static int precond_finish(fsd_ctx *ctx, const float4 *pos)
     float lambda = ctx->ichol_relaxer;
     for (;;) {
         int failed = 0;
-        RC(k_ichol(ctx, lambda, &failed));
+        if (ctx->p.factor_sweeps > 0) RC(k_ichol_sweeps(ctx, lambda, ctx->p.factor_sweeps, &failed));
+        else RC(k_ichol(ctx, lambda, &failed));
         ctx->ichol_attempts++;
         if (!failed) break;
         lambda *= 2.0f;
         if (lambda > 1.0e7f) { ctx->err = "IC(0) did not succeed for any relaxer"; return FSD_E_ICHOL; }
     }
     ctx->ichol_relaxer = lambda;
-    if (ctx->pch.tp_ok) RC(k_sweep_pack(ctx));
+    if (ctx->precond_mode != 0) RC(k_jacobi_pack(ctx));
+    else if (ctx->pch.tp_ok) RC(k_sweep_pack(ctx));
     prof_host(ctx, PROF_FINISH_HOST, wall_ms() - t0);
     prof_end(ctx, PROF_PRECOND_BUILD, ctx->pc_ev);
     ctx->precond_valid = true;
@@ -916,6 +935,15 @@ int fsd_create(const fsd_params *p, fsd_ctx **out)
     if (q.r_buff < 0.f) q.r_buff = 0.4f;
     if (q.gs_passes <= 0) q.gs_passes = 2;
     if (q.gs_passes > 2) q.gs_passes = 2;
+    if (q.jacobi_iters <= 0) q.jacobi_iters = 3;
+    if (q.factor_sweeps < 0) q.factor_sweeps = 0;
+    if (q.precond < 0 || q.precond > 2) {
+        ctx->err = "precond must be 0 (IC(0), exact solves), 1 (IC(0), Jacobi) or 2 (learned, Jacobi)";
+        *out = ctx;
+        return FSD_E_PARAM;
+    }
+    ctx->precond_mode = q.precond;
+    ctx->jac_k = q.jacobi_iters;
     if (q.N <= 0 || q.xi <= 0.f || q.error <= 0.f || q.error >= 1.f || q.Lx <= 0.f || q.Ly <= 0.f || q.Lz <= 0.f) {
         ctx->err = "invalid fsd_params";
         *out = ctx;
@@ -1064,10 +1092,14 @@ int fsd_get_nlist(fsd_ctx *ctx, int32_t *h_nneigh, int32_t *h_nbrs, int64_t capa
     return tot > capacity && h_nbrs ? FSD_E_CAPACITY : 0;
 }
 
+// The lubrication pair records are a function of the positions handed in, and an
+// entry point cannot tell whether the caller moved the particles since the last
+// build (fsd_step moves them; callers update positions in place), so every entry
+// point that applies R_FU builds them for its own positions.
 static int ensure_lub(fsd_ctx *ctx, const float4 *pos)
 {
     if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
-    if (!ctx->lub_valid) RC(k_build_lub_pairs(ctx, pos));
+    RC(k_build_lub_pairs(ctx, pos));
     return 0;
 }
 
@@ -1097,7 +1129,6 @@ int fsd_brownian_farfield(fsd_ctx *ctx, const float *d_pos4, uint32_t timestep, 
 int fsd_brownian_nearfield(fsd_ctx *ctx, const float *d_pos4, uint32_t timestep, float *d_out6)
 {
     const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
-    ctx->lub_valid = false;
     RC(ensure_lub(ctx, pos));
     RC(alloc_brownian(ctx));
     RC(bro_nearfield(ctx, pos, timestep, d_out6));
@@ -1119,7 +1150,6 @@ int fsd_rfd_drift(fsd_ctx *ctx, const float *d_pos4, uint32_t timestep, float *d
 int fsd_rfu(fsd_ctx *ctx, const float *d_pos4, const float *d_U6, float *d_F6)
 {
     const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
-    ctx->lub_valid = false;
     RC(ensure_lub(ctx, pos));
     RC(k_rfu_apply(ctx, d_U6, d_F6, nullptr, 1.0f));
     CK(hipStreamSynchronize(ctx->stream));
@@ -1177,7 +1207,8 @@ int fsd_build_precond(fsd_ctx *ctx, const float *d_pos4, fsd_solve_stats *stats)
 int fsd_precond_apply(fsd_ctx *ctx, const float *d_x6, float *d_y6)
 {
     if (!ctx->precond_valid) { ctx->err = "preconditioner not built"; return FSD_E_STATE; }
-    RC(k_precond_apply2(ctx, d_x6, d_x6, d_y6, d_y6, 1));
+    if (ctx->precond_mode != 0) RC(k_precond_apply_jacobi(ctx, d_x6, d_y6));
+    else RC(k_precond_apply2(ctx, d_x6, d_x6, d_y6, d_y6, 1));
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -1474,6 +1505,8 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     // explicit Euler with shear (Integrator_ExplicitEuler_Shear_kernel)
     RC(k_euler_shear(ctx, pos, d_ori3, reinterpret_cast<float4 *>(d_pos_gb4), d_image3, d_vel11, ctx->d_mask1,
                      d_noise_ang3, ctx->p.dt, shear_rate, ctx->p.sqm_B1));
+    ctx->lub_valid = false;      // the pair records and the spreading set-up describe the old positions
+    ctx->spread_valid = false;
     CK(hipStreamSynchronize(ctx->stream));
     st.ichol_relaxer = ctx->ichol_relaxer;
     st.ichol_attempts = ctx->ichol_attempts;

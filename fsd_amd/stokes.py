@@ -37,7 +37,8 @@ class Stokes:
                  ndsr=1e-1, kappa=1 / 0.05, k_n=1e4, beta_AF=0.0, epsq=1e-5, sqm_B1=0.0, sqm_B2=0.0,
                  N_mix=1, coef_B1_mask=1.0, coef_B2_mask=1.0, rot_diff=0.0, T_ext=0.0, omega_ext=0.0,
                  dt=1e-3, max_strain=0.5, gmres_tol=None, gmres_restart=50, gmres_maxiter=1000,
-                 parity=True, gmres_true_tol=0.0, guess="previous", gram_schmidt_passes=2):
+                 parity=True, gmres_true_tol=0.0, guess="previous", gram_schmidt_passes=2,
+                 precond="ichol", jacobi_iters=3, factor_sweeps=0):
         self.lib = _lib.load()
         self.torch = _torch()
         L = box if not np.isscalar(box) else (box, box, box, 0.0, 0.0, 0.0)
@@ -65,6 +66,13 @@ class Stokes:
         if gram_schmidt_passes not in (1, 2):
             raise ValueError("gram_schmidt_passes must be 1 or 2")
         p.gs_passes = gram_schmidt_passes
+        modes = {"ichol": 0, "ichol_jacobi": 1, "learned": 2}
+        if precond not in modes:
+            raise ValueError("precond must be 'ichol' (exact triangular solves, the reference), 'ichol_jacobi' "
+                             "or 'learned'")
+        p.precond = modes[precond]
+        p.jacobi_iters = int(jacobi_iters)
+        p.factor_sweeps = int(factor_sweeps)
         self.params = p
         self.N = int(N)
         self.fileprefix = fileprefix
@@ -166,6 +174,13 @@ class Stokes:
 
     def _out(self, n):
         return self.torch.zeros(n, dtype=self.torch.float32, device="cuda")
+
+    def euler_shear(self, pos4, ori3, pos_gb4, image3, vel6, dt, shear_rate, mask1=None, noise3=None):
+        """Integrator_ExplicitEuler_Shear_kernel (Integrator.cu:126-228): advances the
+        positions, directors, unwrapped positions and images in place"""
+        self._sync()
+        self._check(self.lib.fsd_euler_shear(self.ctx, _ptr(pos4), _ptr(ori3), _ptr(pos_gb4), _ptr(image3),
+                                             _ptr(vel6), _ptr(mask1), _ptr(noise3), float(dt), float(shear_rate)))
 
     # ---- second-order Runge-Kutta stages (Integrator.cu:232-438; the reference's step never calls them)
     def rk2_stage1(self, pos4, ori3, pos_gb4, vel6, dt, shear_rate, noise3=None):

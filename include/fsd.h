@@ -73,6 +73,14 @@ typedef struct fsd_params {
                                         fewer solutions exist) */
     int32_t gs_passes;               /* classical Gram-Schmidt passes per Arnoldi step: 2 (default, <= 0)
                                         or 1 (half the vector traffic, less orthogonal basis) */
+    int32_t precond;                 /* factor of the saddle preconditioner G = P^T L^-T L^-1 P and how it is
+                                        applied: 0 = IC(0), exact triangular solves (the reference,
+                                        Precondition.cu:1802-1890); 1 = IC(0), jacobi_iters Jacobi iterations
+                                        per triangular solve; 2 = learned factor (fsd_learned_set_weights),
+                                        Jacobi iterations, IC(0) fallback */
+    int32_t jacobi_iters;            /* Jacobi iterations per triangular solve in modes 1 and 2 (<= 0: 3) */
+    int32_t factor_sweeps;           /* IC(0) by the level-scheduled recurrence (0) or by this many parallel
+                                        fixed-point sweeps (Chow & Patel 2015) from the block-diagonal start */
 } fsd_params;
 
 /* Derived PSE parameters (Stokes.cc:131-261). */

@@ -246,6 +246,54 @@ class Oracle:
         self.lib.orc_precond_apply(C.byref(self.L), _p(self.perm), self.N, _p(x), _p(y))
         return y
 
+    def factor_blocks(self):
+        """the IC(0) factor on the permuted particle graph as dense 6x6 blocks:
+        (Ld [N,6,6] diagonal blocks, rows, cols, Lo [nlow,6,6]) for the lower blocks
+        (row P > column Q), ordered by row then column"""
+        N = self.N
+        rp, col, val = self.B_rowptr, self.B_col, self.L_val
+        Ld = np.zeros((N, 6, 6))
+        blocks = {}
+        for r in range(6 * N):
+            P, a = divmod(r, 6)
+            for e in range(rp[r], rp[r + 1]):
+                Q, c = divmod(int(col[e]), 6)
+                if Q == P:
+                    Ld[P, a, c] = val[e]
+                elif Q < P:
+                    blocks.setdefault((P, Q), np.zeros((6, 6)))[a, c] = val[e]
+        keys = sorted(blocks)
+        rows = np.array([k[0] for k in keys], dtype=np.int64)
+        cols = np.array([k[1] for k in keys], dtype=np.int64)
+        Lo = np.array([blocks[k] for k in keys]).reshape(-1, 6, 6)
+        return Ld, rows, cols, Lo
+
+    def precond_apply_jacobi(self, x, k):
+        """G x with each triangular solve replaced by k Jacobi iterations on the
+        block factor: with Z_P = L_PP^-1 and F_PQ = Z_P L_PQ, y = c - F y from
+        y = c = Z t (t = x permuted), u = y - F^T u from u = y, result P^T Z^T u.
+        k >= the depth of the factor's DAG gives the exact solves."""
+        N = self.N
+        Ld, rows, cols, Lo = self.factor_blocks()
+        Z = np.linalg.inv(Ld)
+        F = Z[rows] @ Lo
+        t = np.asarray(x, dtype=np.float64).reshape(N, 6)[self.perm]
+        c = np.einsum('pij,pj->pi', Z, t)
+        y = c
+        for _ in range(k):
+            acc = np.zeros_like(c)
+            np.add.at(acc, rows, np.einsum('eij,ej->ei', F, y[cols]))
+            y = c - acc
+        u = y
+        for _ in range(k):
+            acc = np.zeros_like(c)
+            np.add.at(acc, cols, np.einsum('eji,ej->ei', F, u[rows]))
+            u = y - acc
+        g = np.einsum('pji,pj->pi', Z, u)
+        out = np.empty((N, 6))
+        out[self.perm] = g
+        return out.reshape(-1)
+
     # ---- saddle point
     def _system(self):
         s = System()

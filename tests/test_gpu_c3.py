@@ -1,0 +1,104 @@
+"""Parity at the benchmark's size: N = 32768 spheres at volume fraction 0.45
+(grid 72^3), against the FP64 oracle on the same seeded packing.
+
+Integer results (neighbour lists, pruned lists, RCM permutation, both CSR
+patterns) are bit-exact; operators are within 1e-5 relative; one velocity
+solve is within 1e-4 of the FP64 solution.  The FP64 solution is the GPU's
+answer x plus the FP64 correction d solving A d = b - A x (A the oracle's FP64
+operator, b - A x formed in FP64), converged until |b - A (x + d)| <= 1e-3 |b - A x|.
+"""
+import numpy as np
+import pytest
+
+from tests._cases import packing, rel_err, rel_err2, forces, random_vec, sorted_lists
+
+pytestmark = pytest.mark.gpu
+
+N, PHI, SEED = 32768, 0.45, 2024
+
+
+def _t(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32).reshape(-1)).cuda()
+
+
+@pytest.fixture(scope="module")
+def c3():
+    import torch
+    from fsd_amd.stokes import Stokes
+    from oracle.oracle import Oracle
+    print("C3 set-up: N", N, "phi", PHI, "seed", SEED, flush=True)
+    pos, L = packing(N, PHI, SEED)
+    p4 = np.zeros((N, 4), dtype=np.float32)
+    p4[:, :3] = pos
+    dpos = torch.from_numpy(p4.reshape(-1)).cuda()
+    st = Stokes(N, L, gmres_true_tol=1e-5, gmres_restart=100)
+    st.build_nlist(dpos)
+    orc = Oracle(pos, L)
+    return st, orc, dpos, pos, L
+
+
+def test_c3_nlist_bitexact(c3):
+    st, orc, dpos, pos, L = c3
+    nn, nb = st.get_nlist()
+    assert int(nn.sum()) == len(orc.nbrs)
+    assert sorted_lists(nn, nb) == sorted_lists(orc.nneigh, orc.nbrs)
+
+
+def test_c3_precond_pattern_bitexact(c3):
+    st, orc, dpos, pos, L = c3
+    info = st.build_precond(dpos)
+    g = st.get_precond()
+    orc.build_precond()
+    print("nnz", info["nnz"], "lower blocks", info["lower_blocks"], "levels", info["levels"], flush=True)
+    assert np.array_equal(g["pn"], orc.pn)
+    assert np.array_equal(g["plist"], orc.plist)
+    assert np.array_equal(g["perm"], orc.perm)
+    assert np.array_equal(g["rowptr"], orc.A_rowptr) and np.array_equal(g["col"], orc.A_col)
+    assert np.array_equal(g["prow"], orc.B_rowptr) and np.array_equal(g["pcol"], orc.B_col)
+    assert info["ichol_relaxer"] == orc.lam
+    ev, eL = rel_err(g["val"], orc.A_val), rel_err(g["L"], orc.L_val)
+    print("A values rel err", ev, "L values rel err", eL)
+    assert ev < 1e-5 and eL < 1e-4
+
+
+def test_c3_lubrication_operators(c3):
+    st, orc, dpos, pos, L = c3
+    U = random_vec(6 * N, 1).astype(np.float32)
+    e = [rel_err(st.rfu(dpos, _t(U)).cpu().numpy(), orc.rfu(U)),
+         rel_err(st.rfe(dpos, 1.0).cpu().numpy(), orc.rfe(1.0)),
+         rel_err(st.rsu(dpos, _t(U)).cpu().numpy(), orc.rsu(U)),
+         rel_err(st.rse(dpos, 1.0).cpu().numpy(), orc.rse(1.0))]
+    print("RFU, RFE, RSU, RSE rel err", e)
+    assert max(e) < 1e-5
+
+
+def test_c3_mobility(c3):
+    st, orc, dpos, pos, L = c3
+    gF = random_vec(11 * N, 3).astype(np.float32)
+    e = rel_err(st.mobility(dpos, _t(gF), 3).cpu().numpy(), orc.mobility(gF, 3))
+    print("mobility rel err", e)
+    assert e < 1e-5
+
+
+def test_c3_velocity_solve(c3):
+    """one saddle solve at shear rate 1 with random forces to a true residual of
+    1e-5; its velocities are within 1e-4 of the FP64 solution"""
+    st, orc, dpos, pos, L = c3
+    if orc.L is None:
+        orc.build_precond()
+    st.build_precond(dpos)
+    b = orc.rhs(forces(N, 22), 1.0)
+    x, info = st.saddle_solve(dpos, _t(b))
+    xg = x.cpu().numpy().astype(np.float64)
+    r = b - orc.saddle_multiply(xg)
+    r64 = np.linalg.norm(r) / np.linalg.norm(b)
+    print("GPU iterations", info["iterations"], "reported true residual", info["true_residual"],
+          "FP64 residual", r64, flush=True)
+    d, it, res = orc.gmres(r, tol=1e-3, restart=100, maxit=300)
+    xr = xg + d
+    r2 = np.linalg.norm(b - orc.saddle_multiply(xr)) / np.linalg.norm(b)
+    e = rel_err2(xg[11 * N:], xr[11 * N:])
+    print("FP64 correction iterations", it, "corrected residual", r2, "velocity rel err", e)
+    assert r64 < 2e-5 and r2 < 1e-7
+    assert e < 1e-4

@@ -1,0 +1,206 @@
+"""GPU parity of the sheared step against the FP64 oracle: a tilted (Lees-Edwards)
+box through the neighbour list, k vectors, spreading/contraction and one solve;
+the explicit Euler move under shear with squirmer slip and a wrap; a step at
+shear rate 1 with the stresslet output; the saddle operators after a step;
+the IC(0) relaxer doubling; and the dense-cluster fallbacks of spreading and
+contraction.  Tolerances are written in each test (FP32 kernels against an
+FP64 restatement of the same discretisation)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests._cases import packing, rel_err, rel_err2, forces, random_vec, sorted_lists
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32).reshape(-1)).cuda()
+
+
+def _wrap_into_box(pos, L, xy):
+    """HOOMD BoxDim::wrap of every position into the tilted box (oracle restatement)"""
+    from oracle.oracle import load, make_box
+    lib = load()
+    box = make_box(L, xy)
+    out = np.array(pos, dtype=np.float32).copy()
+    for i in range(len(out)):
+        w = (C.c_float * 3)(*[float(v) for v in out[i, :3]])
+        img = (C.c_int * 3)(0, 0, 0)
+        lib.orc_wrap(C.byref(box), w, img)
+        out[i, :3] = list(w)
+    return out
+
+
+def _p4(pos):
+    p4 = np.zeros((len(pos), 4), dtype=np.float32)
+    p4[:, :3] = pos[:, :3]
+    return p4
+
+
+@pytest.mark.parametrize("xy", [0.3, -0.3])
+def test_tilted_box_parity(xy):
+    """box tilt xy (Lees-Edwards strain): neighbour list bit-exact, sheared k vectors
+    and tilted spreading through the mobility (1e-5), R_FU (1e-5), and one solve's
+    velocities (1e-5 of the oracle's FP64 solve at 1e-11) with shear rate 1"""
+    from fsd_amd.stokes import Stokes
+    from oracle.oracle import Oracle
+    N, phi, seed = 343, 0.2, 11
+    pos, L = packing(N, phi, seed)
+    pos = _wrap_into_box(pos, L, xy)
+    print("tilted box xy", xy, "N", N, flush=True)
+    st = Stokes(N, (L, L, L, xy, 0.0, 0.0), gmres_tol=1e-6)
+    dpos = _t(_p4(pos))
+    st.build_nlist(dpos)
+    orc = Oracle(pos, L, xy=xy)
+    nn, nb = st.get_nlist()
+    assert sorted_lists(nn, nb) == sorted_lists(orc.nneigh, orc.nbrs)
+    gF = random_vec(11 * N, 3).astype(np.float32)
+    for parts in (1, 2, 3):
+        e = rel_err(st.mobility(dpos, _t(gF), parts).cpu().numpy(), orc.mobility(gF, parts))
+        print("mobility parts", parts, "rel err", e)
+        assert e < 1e-5
+    U = random_vec(6 * N, 1).astype(np.float32)
+    assert rel_err(st.rfu(dpos, _t(U)).cpu().numpy(), orc.rfu(U)) < 1e-5
+    st.build_precond(dpos)
+    orc.build_precond()
+    b = orc.rhs(forces(N, 21), 1.0)
+    x, info = st.saddle_solve(dpos, _t(b))
+    xo, it, res = orc.gmres(b, tol=1e-11)
+    e = rel_err2(x.cpu().numpy()[11 * N:], xo[11 * N:])
+    print("solve iterations", info["iterations"], "velocity rel err", e)
+    assert e < 1e-5
+
+
+def test_euler_shear_matches_oracle():
+    """Integrator_ExplicitEuler_Shear_kernel (Integrator.cu:126-228) through the C ABI:
+    shear rate 0.7, squirmer slip B1 with a mask, angular noise, and displacements
+    that wrap (images change), against the oracle to FP32 rounding"""
+    import torch
+    from fsd_amd.stokes import Stokes
+    from oracle.oracle import euler_shear
+    N, L, dt, g, B1 = 343, 19.3, 0.05, 0.7, 0.4
+    rng = np.random.default_rng(41)
+    pos = np.zeros((N, 4), np.float32)
+    pos[:, :3] = rng.uniform(-L / 2, L / 2, (N, 3))
+    ori = rng.standard_normal((N, 3)).astype(np.float32)
+    ori /= np.linalg.norm(ori, axis=1, keepdims=True)
+    gb = pos.copy()
+    img = rng.integers(-2, 3, (N, 3)).astype(np.int32)
+    u = rng.uniform(-20, 20, (N, 6)).astype(np.float32)
+    nz = rng.standard_normal((N, 3)).astype(np.float32)
+    mask = rng.uniform(0.0, 1.0, N).astype(np.float32)
+    st = Stokes(N, L, sqm_B1=B1)
+    dp, do, dg, di = _t(pos), _t(ori), _t(gb), torch.from_numpy(img.reshape(-1)).cuda()
+    st.euler_shear(dp, do, dg, di, _t(u), dt, g, mask1=_t(mask), noise3=_t(nz))
+    p1, o1, g1, i1 = euler_shear(pos, ori, gb, img, u.astype(np.float64), L, dt, g, B1=B1, mask1=mask, noise3=nz)
+    pg, og, gg = dp.cpu().numpy().reshape(N, 4), do.cpu().numpy().reshape(N, 3), dg.cpu().numpy().reshape(N, 4)
+    e = max(np.abs(pg[:, :3] - p1[:, :3]).max(), np.abs(og - o1).max(), np.abs(gg[:, :3] - g1[:, :3]).max())
+    print("Euler max abs err", e, "wrapped", int(np.sum(i1 != img)))
+    assert e < 2e-5
+    assert np.array_equal(di.cpu().numpy().reshape(N, 3), i1) and np.any(i1 != img)
+
+
+def test_step_shear_rate_1_with_stresslet():
+    """one fsd_step at shear rate 1 with stresslet output (period 1): the
+    right-hand side with E_inf and R_FE (Helper_Integrator.cu:114-158,
+    Integrator.cu:744-860) shows in the velocities, which equal the oracle's FP64
+    solve (1e-5); the stresslet sol[6N:11N] - R_SU U + R_SE:E (Integrator.cu:876-932)
+    equals the oracle's assembly (1e-5); the Euler move equals the oracle's for
+    these velocities"""
+    from fsd_amd.stokes import Stokes, ParticleState
+    from oracle.oracle import Oracle, euler_shear
+    N, phi, seed = 343, 0.2, 11
+    pos, L = packing(N, phi, seed)
+    dt = 1e-3
+    st = Stokes(N, L, gmres_tol=1e-7, dt=dt, period=1)
+    state = ParticleState(pos)
+    ori0 = state.ori.cpu().numpy().reshape(N, 3).copy()
+    F = forces(N, 22)
+    info = st.integrateStepOne(0, state, force6=_t(F), shear_rate=1.0)
+    vel = state.velocity.cpu().numpy().astype(np.float64)
+    orc = Oracle(pos, L)
+    orc.build_precond()
+    b = orc.rhs(F, 1.0)
+    xo, it, res = orc.gmres(b, tol=1e-12)
+    eU = rel_err2(vel[:6 * N], xo[11 * N:])
+    S = orc.rsu(xo[11 * N:], S=xo[6 * N:11 * N])
+    S = orc.rse(1.0, S=S)
+    eS = rel_err2(vel[6 * N:], S)
+    print("iterations", info["iterations"], "velocity rel err", eU, "stresslet rel err", eS)
+    assert eU < 1e-5
+    assert eS < 1e-5
+    p1, o1, g1, i1 = euler_shear(_p4(pos), ori0, _p4(pos), np.zeros((N, 3), np.int32), vel[:6 * N], L, dt, 1.0)
+    pg = state.pos.cpu().numpy().reshape(N, 4)
+    assert np.abs(pg[:, :3] - p1[:, :3]).max() < 1e-5
+
+
+def test_saddle_operators_after_a_step():
+    """after fsd_step has moved the particles, fsd_saddle_multiply at the new
+    positions builds its lubrication pairs for them: equal to the oracle at the
+    new positions (1e-5)"""
+    from fsd_amd.stokes import Stokes, ParticleState
+    from oracle.oracle import Oracle
+    N, phi, seed = 343, 0.2, 11
+    pos, L = packing(N, phi, seed)
+    st = Stokes(N, L, gmres_tol=1e-6, dt=0.05)
+    state = ParticleState(pos)
+    st.integrateStepOne(0, state, force6=_t(forces(N, 23)), shear_rate=1.0)
+    new = state.positions()
+    assert np.abs(new - pos).max() > 1e-3
+    orc = Oracle(new, L)
+    x = random_vec(17 * N, 5).astype(np.float32)
+    y = st.saddle_multiply(state.pos, _t(x)).cpu().numpy()
+    e = rel_err(y, orc.saddle_multiply(x))
+    print("saddle multiply after a step rel err", e)
+    assert e < 1e-5
+
+
+@pytest.mark.parametrize("n,half,seed", [(60, 3.0, 1), (40, 3.0, 1)])
+def test_ichol_relaxer_doubling(n, half, seed):
+    """an overlapping cluster whose IC(0) pivots are non-positive for lambda = 1:
+    the build doubles the relaxer (Precondition.cu:1274-1288) to the oracle's
+    lambda with the same number of attempts; pattern bit-exact, factor 1e-4"""
+    from fsd_amd.stokes import Stokes
+    from oracle.oracle import Oracle
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-half, half, (n, 3)).astype(np.float32)
+    L = 24.0
+    orc = Oracle(pos, L)
+    orc.build_precond()
+    st = Stokes(n, L)
+    dpos = _t(_p4(pos))
+    st.build_nlist(dpos)
+    info = st.build_precond(dpos)
+    g = st.get_precond()
+    print("oracle attempts", orc.attempts, "lambda", orc.lam, "GPU", info["ichol_attempts"], info["ichol_relaxer"])
+    assert orc.attempts > 1
+    assert info["ichol_attempts"] == orc.attempts and info["ichol_relaxer"] == orc.lam
+    assert np.array_equal(g["perm"], orc.perm) and np.array_equal(g["pcol"], orc.B_col)
+    assert rel_err(g["L"], orc.L_val) < 1e-4
+
+
+def test_dense_cluster_spreading_fallbacks():
+    """700 overlapping particles in a 10^3 region: spreading tiles whose halo holds
+    more than the LDS cap read particles from global memory, and contraction tiles
+    with more than the staged-particle cap take the overflow path; the wave-space
+    and full mobility still equal the oracle (1e-5)"""
+    from fsd_amd.stokes import Stokes
+    from oracle.oracle import Oracle
+    rng = np.random.default_rng(7)
+    N, L = 700, 24.0
+    pos = rng.uniform(-5.0, 5.0, (N, 3)).astype(np.float32)
+    st = Stokes(N, L)
+    dpos = _t(_p4(pos))
+    st.build_nlist(dpos)
+    orc = Oracle(pos, L)
+    info = st.pse_info()
+    h = L / info["Nx"]
+    print("grid", info["Nx"], "spacing", h, "particles per 13^3 halo ~", N * (13 * h / 10.0) ** 3, flush=True)
+    gF = random_vec(11 * N, 8).astype(np.float32)
+    for parts in (2, 3):
+        e = rel_err(st.mobility(dpos, _t(gF), parts).cpu().numpy(), orc.mobility(gF, parts))
+        print("dense cluster mobility parts", parts, "rel err", e)
+        assert e < 1e-5

@@ -236,3 +236,197 @@ def test_friction_type1_table():
         d = v[:k, cols] - v0[:k, cols]
         # sign pattern (+, -, -, +, +, +) with alpha > 0 and a positive polynomial below h0
         assert np.all(d[:, [0, 3, 4, 5]] > 0) and np.all(d[:, [1, 2]] < 0)
+
+
+# ------------------------------------------------- R_FE, R_SU, R_SE, force law
+# Table columns (Stokes_ResistanceTable.cc): the two-sphere functions of
+# Jeffrey (1992) and Kim & Karrila (1991, ch. 11).  The pins below place two
+# spheres on a coordinate axis at a table node and apply one velocity or one
+# strain mode, for which the force, torque or stresslet of sphere 0 is a single
+# table entry (or the sum of its "11" and "12" entries).
+COL = {n: k for k, n in enumerate("XA11 XA12 YA11 YA12 YB11 YB12 XC11 XC12 YC11 YC12 XG11 XG12 YG11 YG12 "
+                                   "YH11 YH12 XM11 XM12 YM11 YM12 ZM11 ZM12".split())}
+ROW = 500          # a node well inside the table (h ~ 0.14)
+
+
+def _two_spheres(axis):
+    from fsd_amd.tables import load_resistance_table
+    dist, vals = load_resistance_table()
+    v = vals.reshape(1000, 22).astype(np.float64)
+    pos = np.zeros((2, 3))
+    pos[1, axis] = float(dist[ROW])
+    return _oracle(pos, 30.0), (lambda name: v[ROW, COL[name]])
+
+
+def _S5_to_tensor(s):
+    """stresslet 5-vector [Sxx, Sxy, Sxz, Syz, Syy] -> traceless 3x3"""
+    xx, xy, xz, yz, yy = s
+    return np.array([[xx, xy, xz], [xy, yy, yz], [xz, yz, -xx - yy]])
+
+
+def _E_tensors(shear_rate, B2, mask2, ori):
+    """per-particle strain tensors of the oracle's R_FE / R_SE (Lubrication.cu:389-418):
+    E_inf with 2 Exy = shear rate, plus 0.6 B2 mask (p p^T - |p|^2 I/3), the
+    traceless part (the float director is unit only to rounding)"""
+    out = []
+    for m, p in zip(mask2, ori):
+        p = np.asarray(p, dtype=np.float64)
+        E = np.zeros((3, 3))
+        E[0, 1] = E[1, 0] = 0.5 * shear_rate
+        E += 0.6 * B2 * float(m) * (np.outer(p, p) - (p @ p) * np.eye(3) / 3.0)
+        out.append(E)
+    return out
+
+
+def test_rfe_two_sphere_table_columns():
+    """R_FE:E for two spheres on the x axis (r_hat from 0 to 1 = +x): simple shear
+    2Exy = 1 gives sphere 0 the force -(YG11 - YG12) y and torque -(YH11 + YH12) z
+    (the transverse and rotational coupling functions); uniaxial extension
+    E = e (x x - I/3) gives -(2e/3)(XG11 - XG12) x.  Sphere 1 gets the opposite force
+    and the same torque (Jeffrey 1992: G21 = -G12, H21 = H12)."""
+    o, t = _two_spheres(0)
+    F = o.rfe(1.0)
+    assert F[1] == pytest.approx(-(t("YG11") - t("YG12")), rel=1e-4)
+    assert F[5] == pytest.approx(-(t("YH11") + t("YH12")), rel=1e-4)
+    assert abs(F[0]) + abs(F[2]) + abs(F[3]) + abs(F[4]) < 1e-9 * abs(F[1])
+    assert F[7] == pytest.approx(-F[1], rel=1e-9) and F[11] == pytest.approx(F[5], rel=1e-9)
+    B2, m = 2.0, np.ones(2, np.float32)
+    ori = np.array([[1, 0, 0], [1, 0, 0]], np.float32)
+    e = 0.6 * B2
+    F = o.rfe(0.0, B2=B2, mask2=m, ori=ori)
+    assert F[0] == pytest.approx(-(2 * e / 3) * (t("XG11") - t("XG12")), rel=1e-4)
+    assert np.abs(F[1:6]).max() < 1e-9 * abs(F[0])
+    # strain transverse to the line of centres (r along z, shear in x-y) exerts no force
+    o, _ = _two_spheres(2)
+    assert np.abs(o.rfe(1.0)[:3]).max() < 1e-9
+
+
+def test_rsu_two_sphere_table_columns():
+    """S = -R_SU U for two spheres on the x axis with sphere 1 moving: U1 = x gives
+    sphere 0 Sxx = -(2/3) XG12 and Syy = +(1/3) XG12; U1 = y gives Sxy = -YG12;
+    W1 = z gives Sxy = -YH12; sphere 0's own motion U0 = x gives Sxx = -(2/3) XG11"""
+    o, t = _two_spheres(0)
+    for k, checks in [(6, {0: -(2 / 3) * t("XG12"), 4: (1 / 3) * t("XG12"), 1: 0.0}),
+                      (7, {1: -t("YG12"), 0: 0.0}),
+                      (11, {1: -t("YH12"), 0: 0.0}),
+                      (0, {0: -(2 / 3) * t("XG11"), 4: (1 / 3) * t("XG11")})]:
+        U = np.zeros(12)
+        U[k] = 1.0
+        S = o.rsu(U)
+        for c, want in checks.items():
+            assert S[c] == pytest.approx(want, rel=1e-4, abs=1e-9 * max(1.0, abs(t("XG11")))), (k, c)
+
+
+def test_rse_two_sphere_table_columns():
+    """R_SE:E for two spheres: the axisymmetric, mixed and transverse strain modes
+    select XM, YM and ZM (Jeffrey 1992's decomposition of M_ijkl): r along x with
+    E = e (x x - I/3) gives Sxx = (2e/3)(XM11 + XM12); r along x with 2Exy = 1 gives
+    Sxy = (YM11 + YM12)/2; r along z with 2Exy = 1 gives Sxy = (ZM11 + ZM12)/2"""
+    o, t = _two_spheres(0)
+    S = o.rse(1.0)
+    assert S[1] == pytest.approx(0.5 * (t("YM11") + t("YM12")), rel=1e-4)
+    B2, m = 2.0, np.ones(2, np.float32)
+    ori = np.array([[1, 0, 0], [1, 0, 0]], np.float32)
+    e = 0.6 * B2
+    S = o.rse(0.0, B2=B2, mask2=m, ori=ori)
+    assert S[0] == pytest.approx((2 * e / 3) * (t("XM11") + t("XM12")), rel=1e-4)
+    o, t = _two_spheres(2)
+    S = o.rse(1.0)
+    assert S[1] == pytest.approx(0.5 * (t("ZM11") + t("ZM12")), rel=1e-4)
+
+
+def test_rfe_rsu_reciprocal():
+    """Onsager reciprocity of the grand resistance tensor, R_SU = R_FE^T: for any
+    velocities U and per-particle strains E_k, U . (R_FE:E) = sum_k (-R_SU U)_k : E_k
+    on a packing (the RFE and RSU code paths share no arithmetic).  Equality holds
+    to the rounding of the unit vector, which both kernels form in float as the
+    reference does (|r_hat|^2 = 1 to ~1e-7, so S's implied Szz = -Sxx - Syy is
+    off by that much); a wrong sign or column is an O(1) error"""
+    pos, L = packing(343, 0.3, 17)
+    o = _oracle(pos, L)
+    rng = np.random.default_rng(5)
+    ori = rng.standard_normal((343, 3))
+    ori = (ori / np.linalg.norm(ori, axis=1, keepdims=True)).astype(np.float32)
+    mask = rng.uniform(0.2, 1.0, 343).astype(np.float32)
+    U = rng.standard_normal(6 * 343)
+    F = o.rfe(0.7, B2=1.3, mask2=mask, ori=ori)
+    S = o.rsu(U)
+    Es = _E_tensors(0.7, 1.3, mask, ori)
+    lhs = U @ F
+    rhs = sum(np.sum(_S5_to_tensor(S[5 * k:5 * k + 5]) * Es[k]) for k in range(343))
+    assert abs(lhs - rhs) <= 1e-6 * abs(lhs), (lhs, rhs)
+
+
+def test_rse_symmetric():
+    """R_SE is symmetric: E2 : (R_SE:E1) = E1 : (R_SE:E2) for two fields of
+    per-particle strains (shear plus random uniaxial modes), to the float rounding
+    of the unit vector as in test_rfe_rsu_reciprocal"""
+    pos, L = packing(343, 0.3, 17)
+    o = _oracle(pos, L)
+    rng = np.random.default_rng(6)
+    fields = []
+    for shear in (0.3, -0.7):
+        ori = rng.standard_normal((343, 3))
+        ori = (ori / np.linalg.norm(ori, axis=1, keepdims=True)).astype(np.float32)
+        mask = rng.uniform(0.2, 1.0, 343).astype(np.float32)
+        fields.append((shear, mask, ori))
+    S = [o.rse(sh, B2=0.9, mask2=m, ori=p) for sh, m, p in fields]
+    E = [_E_tensors(sh, 0.9, m, p) for sh, m, p in fields]
+    a = sum(np.sum(_S5_to_tensor(S[0][5 * k:5 * k + 5]) * E[1][k]) for k in range(343))
+    b = sum(np.sum(_S5_to_tensor(S[1][5 * k:5 * k + 5]) * E[0][k]) for k in range(343))
+    assert abs(a - b) <= 1e-6 * max(abs(a), 1.0), (a, b)
+
+
+@pytest.mark.parametrize("gap", [-0.02, -1e-3, 0.0, 0.05, 0.3, 0.49, 0.6])
+def test_force_law_closed_form(gap):
+    """Stokes_SetForce_manually_kernel (Helper_Stokes.cu:136-160) for a pair on the
+    x axis at surface gap g = r - 2 - sqrt(epsq): f = Ham/(12 (g^2 + epsq)) -
+    F0 exp(-kappa g) for 0 <= g and r <= 2 + 10/kappa, Ham/(12 epsq) - F0 - k_n |g|
+    for g < 0, zero beyond; F0 = 1/ndsr, Ham = F0 beta.  f > 0 pulls sphere 0
+    towards sphere 1; the pair force is antisymmetric (Newton's third law) and the
+    torque is T_ext along z"""
+    ndsr, k_n, kappa, beta, epsq, T_ext = 0.1, 1e4, 20.0, 0.3, 1e-5, 0.25
+    rcol = float(np.float32(2.0 + np.sqrt(np.float32(epsq))))
+    r = float(np.float32(rcol + gap))
+    pos = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]], np.float32)
+    o = _oracle(pos, 30.0)
+    F = o.force_law(ndsr=ndsr, k_n=k_n, kappa=kappa, beta=beta, epsq=epsq, T_ext=T_ext)
+    F0, Ham = 1.0 / ndsr, beta / ndsr
+    g = float(np.float32(np.float32(r) - np.float32(rcol)))
+    if g < 0:
+        f = Ham / (12 * epsq) - F0 - k_n * abs(g)
+    elif r <= 2 + 10 / kappa:
+        f = Ham / (12 * (g * g + epsq)) - F0 * np.exp(-kappa * g)
+    else:
+        f = 0.0
+    assert F[0] == pytest.approx(f, rel=1e-6, abs=1e-9)
+    assert F[6] == pytest.approx(-f, rel=1e-6, abs=1e-9)
+    assert np.abs(F[[1, 2, 7, 8]]).max() == 0.0
+    assert F[5] == T_ext and F[11] == T_ext and F[3] == 0.0
+    if gap == 0.6:
+        assert f == 0.0
+
+
+# ------------------------------------------ Jacobi application of the factor
+def test_jacobi_apply_limits():
+    """the oracle's k-step Jacobi application of the IC(0) factor: k at or beyond
+    the depth of the factor's DAG reproduces the exact triangular solves
+    (orc_precond_apply), and k = 0 is the block-diagonal preconditioner
+    P^T (L_d L_d^T)^-1 P"""
+    pos, L = packing(343, 0.2, 11)
+    o = _oracle(pos, L)
+    o.build_precond()
+    x = np.random.default_rng(3).standard_normal(6 * 343)
+    Ld, rows, cols, Lo = o.factor_blocks()
+    depth = np.zeros(343, dtype=np.int64)
+    for e in np.argsort(rows, kind="stable"):
+        depth[rows[e]] = max(depth[rows[e]], depth[cols[e]] + 1)
+    exact = o.precond_apply(x)
+    got = o.precond_apply_jacobi(x, int(depth.max()) + 1)
+    assert np.abs(got - exact).max() <= 1e-10 * np.abs(exact).max()
+    y0 = o.precond_apply_jacobi(x, 0)
+    D = np.linalg.inv(Ld @ np.transpose(Ld, (0, 2, 1)))
+    t = x.reshape(343, 6)[o.perm]
+    want = np.empty((343, 6))
+    want[o.perm] = np.einsum('pij,pj->pi', D, t)
+    assert np.abs(y0 - want.reshape(-1)).max() <= 1e-12 * np.abs(want).max()
