@@ -110,6 +110,7 @@ SIGNATURES = [
     ("fsd_build_nlist", _I, [_P, _P]),
     ("fsd_get_nlist", _I, [_P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     ("fsd_rfu", _I, [_P, _P, _P, _P]),
+    ("fsd_rfu_noalloc", _I, [_P, _P, _P, _P]),
     ("fsd_rfe", _I, [_P, _P, _F, _P, _P]),
     ("fsd_rsu", _I, [_P, _P, _P, _P]),
     ("fsd_rse", _I, [_P, _P, _F, _P, _P]),

@@ -342,6 +342,11 @@ void prof_host(fsd_ctx *ctx, int slot, double ms);
 int k_build_nlist(fsd_ctx *ctx, const float4 *pos);
 int k_update_nlist(fsd_ctx *ctx, const float4 *pos, bool *rebuilt);
 int k_build_lub_pairs(fsd_ctx *ctx, const float4 *pos);
+int k_lub_pairs_into(fsd_ctx *ctx, const float4 *pos, int maxlub, int *lub_n, int *lub_j, float4 *lub_a,
+                     float4 *lub_b, float *lub_self, int *overflow);
+int k_rfu_apply_with(fsd_ctx *ctx, const float *U, float *F, const int *lub_n, const int *lub_j, const float4 *lub_a,
+                     const float4 *lub_b, const float *lub_self, int maxlub);
+int noalloc_init_device();   // k_noalloc.hip: device address of the static scratch, once per device
 int k_rfu_apply(fsd_ctx *ctx, const float *U, float *F, const float *xF, float sign_out);
 int k_rfe(fsd_ctx *ctx, const float4 *pos, float shear_rate, const float *ori, float *F);
 int k_rsu(fsd_ctx *ctx, const float4 *pos, const float *U, float *S);

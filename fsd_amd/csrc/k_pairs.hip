@@ -388,6 +388,22 @@ int k_build_lub_pairs(fsd_ctx *ctx, const float4 *pos)
     return 0;
 }
 
+// Pair records into caller-provided buffers (capacity maxlub per particle); the
+// overflow flag must be zero on entry and is left > 0 when some particle has more
+// pairs than maxlub.  No allocation: used by the allocation-free entry point.
+int k_lub_pairs_into(fsd_ctx *ctx, const float4 *pos, int maxlub, int *lub_n, int *lub_j, float4 *lub_a,
+                     float4 *lub_b, float *lub_self, int *overflow)
+{
+    if (!ctx->have_table) { ctx->err = "resistance table not set"; return FSD_E_STATE; }
+    LubTable t{ctx->d_res_dist, ctx->d_res_vals, ctx->table_min, ctx->table_dr};
+    const float rl2 = ctx->p.rlub * ctx->p.rlub;
+    hipLaunchKernelGGL(lub_pairs_kernel, dim3(nblk(ctx->N, 128)), dim3(128), 0, ctx->stream, pos, ctx->N, ctx->box,
+                       ctx->d_nneigh, ctx->d_nlist, ctx->maxn, t, rl2, maxlub, lub_n, lub_j, lub_a, lub_b, lub_self,
+                       overflow);
+    FSD_HIP(hipGetLastError());
+    return 0;
+}
+
 // R_FU apply: G = 8 lanes per particle.  out[6i+r] = sign_out * (R U)_ir
 // (+ xF[6i+r] when xF != nullptr, i.e. the saddle row  x_F - R_FU x_U).
 template <int G>
@@ -458,6 +474,19 @@ int k_rfu_apply(fsd_ctx *ctx, const float *U, float *F, const float *xF, float s
                        sign_out, F, order);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_RFU, ev);
+    return 0;
+}
+
+// R_FU apply from caller-provided pair records (the allocation-free entry point)
+int k_rfu_apply_with(fsd_ctx *ctx, const float *U, float *F, const int *lub_n, const int *lub_j, const float4 *lub_a,
+                     const float4 *lub_b, const float *lub_self, int maxlub)
+{
+    if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    const int *order = ctx->use_cells ? ctx->d_cell_list : nullptr;
+    const int N = ctx->N;
+    hipLaunchKernelGGL(rfu_apply_kernel<8>, dim3(nblk((int64_t)N * 8, 256)), dim3(256), 0, ctx->stream, N, U, lub_n,
+                       lub_j, lub_a, lub_b, lub_self, maxlub, (const float *)nullptr, 1.0f, F, order);
+    FSD_HIP(hipGetLastError());
     return 0;
 }
 

@@ -961,6 +961,12 @@ int fsd_create(const fsd_params *p, fsd_ctx **out)
         *out = ctx;
         return FSD_E_HIP;
     }
+    // device address of the allocation-free entry point's scratch array (once per device)
+    if (noalloc_init_device() != 0) {
+        ctx->err = "hipGetSymbolAddress of the scratch array failed";
+        *out = ctx;
+        return FSD_E_HIP;
+    }
     ctx->slots.resize(PROF_COUNT);
     for (int s = 0; s < PROF_COUNT; ++s) { ctx->slots[s].name = kProfNames[s]; ctx->slots[s].launches = 0; ctx->slots[s].ms = 0; }
     rc = alloc_all(ctx);

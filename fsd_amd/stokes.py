@@ -208,6 +208,14 @@ class Stokes:
         self._check(self.lib.fsd_rfu(self.ctx, _ptr(pos), _ptr(U), _ptr(F)))
         return F
 
+    def rfu_noalloc(self, pos, U, F=None):
+        """R_FU U through the allocation-free entry point (pair records in the library's
+        fixed device array; N <= FSD_NOALLOC_MAX_N)"""
+        F = self._out(6 * self.N) if F is None else F
+        self._sync()
+        self._check(self.lib.fsd_rfu_noalloc(self.ctx, _ptr(pos), _ptr(U), _ptr(F)))
+        return F
+
     def rfe(self, pos, shear_rate, ori=None, F=None):
         F = self._out(6 * self.N) if F is None else F
         self._sync()

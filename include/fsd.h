@@ -158,6 +158,17 @@ int fsd_get_nlist(fsd_ctx *ctx, int32_t *h_nneigh, int32_t *h_nbrs, int64_t capa
 
 /* Lubrication_RFU_kernel (Lubrication.cu:69-309): d_F6 = R_FU d_U6 */
 int fsd_rfu(fsd_ctx *ctx, const float *d_pos4, const float *d_U6, float *d_F6);
+/* Allocation-free variant of fsd_rfu.  It has no counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The lubrication pair
+ * records go into a fixed device array of the library (one per GPU, held until
+ * the process ends) instead of context buffers that fsd_rfu regrows.  Limits:
+ * N <= FSD_NOALLOC_MAX_N (else FSD_E_PARAM) and at most FSD_NOALLOC_MAX_PAIRS
+ * lubrication pairs per particle (else FSD_E_CAPACITY).  Needs a neighbour list
+ * (fsd_build_nlist).  Calls that overlap on one GPU share the array and race. */
+#define FSD_NOALLOC_MAX_N 8192
+#define FSD_NOALLOC_MAX_PAIRS 48
+int fsd_rfu_noalloc(fsd_ctx *ctx, const float *d_pos4, const float *d_U6, float *d_F6);
 /* Lubrication_RFE_kernel (Lubrication.cu:350-608): d_F6 += R_FE : E */
 int fsd_rfe(fsd_ctx *ctx, const float *d_pos4, float shear_rate, const float *d_ori3, float *d_F6);
 /* Lubrication_RSU_kernel (Lubrication.cu:650-911): d_S5 -= R_SU d_U6 */
