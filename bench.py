@@ -3,16 +3,25 @@
 Workload (the configuration the headline metric is quoted on): N = 32768 unit
 spheres at volume fraction 0.45 in a periodic cube, simple shear rate 1,
 deterministic random forces U[-1,1]^3 (zero mean), T = 0, xi = 0.5, PSE error
-1e-3, IC(0)-preconditioned saddle solve run until the true relative residual
-||b - A x|| / ||b|| <= 1e-4.  A "step" is one full Stokes::integrateStepOne:
-neighbour list, preconditioner build (prune, RCM, IC(0)), force, right-hand
-side, GMRES solve, explicit Euler move.
+1e-3, saddle solve run until the true relative residual ||b - A x|| / ||b||
+<= 1e-4.  A "step" is one full Stokes::integrateStepOne: neighbour list,
+preconditioner build, force, right-hand side, GMRES solve, explicit Euler move.
 
-Prints one JSON line.  `value` is steps/s with all inputs resident in HBM.
-`roofline` is computed for the kernel class that takes the most time, from its
-algorithmic bytes and its launch durations measured with HIP events on the
-library's stream during the timed steps.  `cpu_baseline` times the FP64 CPU
-oracle on a bounded sample of the same step on this host.
+Two arms run on the same packing from the same start:
+  ichol   - the reference's preconditioner: IC(0) with exact triangular solves;
+  learned - the learned factor (a message-passing network on the lubrication
+            graph) applied by block Jacobi iterations, with the IC(0) fallbacks
+            (non-positive pivot; iteration count above IC(0)'s at the last
+            calibration step, one IC(0) step every --calibration steps) and
+            FP64-residual iterative refinement of the solve.
+`value` is the learned arm's steps/s (all inputs resident in HBM); the IC(0)
+arm and the ratio are in `config.arms`.
+
+Prints one JSON line.  `roofline` is computed for the kernel class that takes
+the most time in the learned arm, from its algorithmic bytes and its launch
+durations measured with HIP events on the library's stream during the timed
+steps.  `cpu_baseline` times the FP64 CPU oracle on a bounded sample of the
+same step on this host.
 """
 import argparse
 import json
@@ -32,7 +41,7 @@ GUESS_TEXT = {"previous": "previous solution", "extrapolate": "linear extrapolat
 # kernel-class name -> kernel name in rocprofv3 traces
 KERNEL_NAMES = {"spread": "spread_tile2_kernel<8, 6>", "green": "fft_x_green_kernel<9>", "contract": "contract_tile2_kernel<13, 6>",
                 "realspace": "realspace_kernel<16>", "rfu": "rfu_apply_kernel<8>", "sweep": "tp_sweep1_kernel",
-                "ichol": "ichol_level_kernel", "gmres_vec": "multidot_partial4_kernel"}
+                "ichol": "ichol_level_kernel", "gmres_vec": "multidot_partial4_kernel", "ljit": "lj_fwd_kernel"}
 
 
 def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5, gs_passes=2):
@@ -58,6 +67,10 @@ def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5, gs_passes=2):
         # Z d (b_F, b_U in 48 B, Z 144 B, out 24 B), reorder (48 B), Z^T u and the combination
         # (Z 144 B, u 24 B, b_F and b_S 44 B in, x_F, x_S, x_U 68 B out) per particle
         return 2 * (144 * st["lower_blocks"] + 48 * N) + (48 + 144 + 24 + 48 + 144 + 24 + 44 + 68) * N
+    if cls == "ljit":        # one block Jacobi iteration of the learned factor: every pair block W (144 B) and
+        # inverse pivot (144 B) once, the fixed right-hand side (t or v_k) and the previous iterate in, the
+        # new iterate out (24 B each per particle)
+        return 144 * (st["lower_blocks"] + N) + 3 * 24 * N
     if cls == "ichol":       # one factorisation: A blocks in, L blocks out (lower and diagonal)
         return 144 * 2 * (st["lower_blocks"] + N)
     if cls == "gmres_vec":   # Gram-Schmidt of a 17N-float vector on k basis vectors, per pass h = V^T w
@@ -68,12 +81,13 @@ def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5, gs_passes=2):
     return None
 
 
-def cpu_baseline(pos, L, F, shear_rate, gpu_iters, threads):
-    """FP64 oracle on this host: the step's fixed part (neighbour list, preconditioner
-    build, right-hand side) plus 3 preconditioned saddle applications, extrapolated to
-    the GPU's mean GMRES iterations per step"""
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    from oracle.oracle import Oracle
+def cpu_baseline(pos, L, F, shear_rate, gpu_iters):
+    """FP64 oracle on this host's cores (OpenMP): the step's fixed part (neighbour list,
+    preconditioner build, right-hand side) timed once plus 3 IC(0)-preconditioned saddle
+    applications timed and extrapolated to the IC(0) arm's mean GMRES iterations per step.
+    A full FP64 step to the true residual 1e-4 takes over a minute at N = 32768, so the
+    sample is bounded to about 20 s of CPU work."""
+    from oracle.oracle import Oracle, threads
     t0 = time.perf_counter()
     orc = Oracle(pos, L)
     orc.build_precond()
@@ -89,30 +103,87 @@ def cpu_baseline(pos, L, F, shear_rate, gpu_iters, threads):
     t_fixed = t1 - t0
     t_iter = (t2 - t1) / k
     step_s = t_fixed + gpu_iters * t_iter
-    return {"value": 1.0 / step_s, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": "FP64 oracle, one N=32768 step: neighbour list + preconditioner build + rhs timed "
-                      "(%.1f s), 3 preconditioned saddle applications timed (%.2f s each), extrapolated "
-                      "to %.1f GMRES iterations/step" % (t_fixed, t_iter, gpu_iters)}
+    return {"value": 1.0 / step_s, "unit": "steps/s", "cores": threads(), "kind": "port",
+            "sample": "FP64 oracle (IC(0) preconditioner), one N=32768 step: neighbour list + preconditioner "
+                      "build + rhs timed (%.1f s), 3 preconditioned saddle applications timed (%.2f s each), "
+                      "extrapolated to %.1f GMRES iterations/step (the IC(0) arm's mean)" % (t_fixed, t_iter, gpu_iters)}
+
+
+def run_arm(name, kw, pos, L, F, args, dom=None, log=None):
+    """warm-up and timed steps of one arm; returns (stats dict, Stokes, kernel-class totals)"""
+    import torch
+    from fsd_amd.stokes import Stokes, ParticleState
+    from fsd_amd import _lib
+    N = args.N
+    st = Stokes(N, L, xi=0.5, error=1e-3, gmres_tol=1e-4, gmres_true_tol=1e-4, dt=1e-3, gmres_restart=args.restart,
+                guess=args.guess, gram_schmidt_passes=args.gram_schmidt, **kw)
+    state = ParticleState(pos)
+    st.profile(True)
+    infos = []
+    for k in range(args.warmup):
+        infos.append(st.integrateStepOne(k, state, force6=F, shear_rate=1.0))
+        log("# %s warmup step %d: %s" % (name, k, infos[-1]))
+    totals = {c: st.profile_get(c) for c in _lib.PROF_NAMES}
+    st.profile_reset()
+    classes = ["spread", "green", "contract", "realspace", "rfu", "sweep", "ichol", "gmres_vec", "ljit"]
+    if dom is None:
+        dom = max(classes, key=lambda c: totals[c][1])
+    log("# %s warmup kernel-class times (launches, ms): %s" % (name, json.dumps(totals)))
+    # timed steps: HIP events only around the dominant class, one launch in eight
+    st.profile(True, classes=None if args.profile_all else [dom], every=1 if args.profile_all else 8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = []
+    for k in range(args.steps):
+        info = st.integrateStepOne(args.warmup + k, state, force6=F, shear_rate=1.0)
+        its.append(info["iterations"])
+        infos.append(info)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timed = infos[-args.steps:]
+    log("# %s timed steps: iterations %s, initial residuals %s, preconditioner %s"
+        % (name, its, ["%.2e" % i["initial_residual"] for i in timed], [i["precond_used"] for i in timed]))
+    out = {"ms_per_step": (t1 - t0) * 1e3 / args.steps, "steps_per_s": args.steps / (t1 - t0),
+           "gmres_iterations_mean": float(np.mean(its)),
+           "true_residual_max": float(max(i["true_residual"] for i in timed)),
+           "nlist_rebuilds": int(sum(i.get("nlist_rebuilt", 0) for i in timed)),
+           "nnz": timed[-1]["nnz"], "lub_pairs": timed[-1]["lub_pairs"], "lower_blocks": timed[-1]["lower_blocks"],
+           "levels": timed[-1]["levels"]}
+    if kw.get("precond") == "learned":
+        out["learned_steps"] = int(sum(i["precond_used"] == 1 for i in timed))
+        out["fallback_pivot"] = int(sum(i["fallback"] == 1 for i in timed))
+        out["fallback_iterations"] = int(sum(i["fallback"] == 2 for i in timed))
+        out["calibration_steps"] = int(sum(i["precond_used"] == 0 and i["fallback"] == 0 for i in timed))
+    if args.profile_all:
+        allt = {c: st.profile_get(c) for c in _lib.PROF_NAMES}
+        log("# %s timed kernel-class times (launches, ms): %s" % (name, json.dumps(allt)))
+    return out, st, dom, timed[-1]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--N", type=int, default=32768)
     ap.add_argument("--phi", type=float, default=0.45)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--restart", type=int, default=100, help="GMRES restart length")
     ap.add_argument("--gram-schmidt", type=int, default=1, choices=[1, 2],
-                    help="classical Gram-Schmidt passes per Arnoldi step (the library's default is 2)")
+                    help="classical Gram-Schmidt passes per Arnoldi step")
     ap.add_argument("--guess", default="extrapolate", choices=["previous", "extrapolate", "quadratic"],
-                    help="initial guess of each solve: the previous solution (the reference) or its extrapolation")
+                    help="initial iterate of each step's solve")
+    ap.add_argument("--jacobi", type=int, default=3, help="block Jacobi iterations per triangular solve (learned)")
+    ap.add_argument("--calibration", type=int, default=20,
+                    help="learned arm: one IC(0) step every this many steps (its iteration count caps the learned "
+                         "solves); 20 puts one such step in the 20 timed steps")
+    ap.add_argument("--arms", default="ichol,learned")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (diagnostics)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MB",
                     help="only run 5 device copies of MB megabytes (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
     if args.calib_copy:
         import torch
         n = args.calib_copy * (1 << 20) // 4
@@ -121,8 +192,7 @@ def main():
         for _ in range(5):
             b.copy_(a)
         torch.cuda.synchronize()
-        print("# copy calibration: 5 copies of %d bytes (read %d + write %d per copy)" % (4 * n, 4 * n, 4 * n),
-              file=sys.stderr)
+        log("# copy calibration: 5 copies of %d bytes (read %d + write %d per copy)" % (4 * n, 4 * n, 4 * n))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU: the FSD step of one suspension is not sharded across GPUs "
@@ -130,62 +200,36 @@ def main():
         sys.exit(2)
 
     import torch
-    from fsd_amd.stokes import Stokes, ParticleState
     from fsd_amd.workloads import random_packing, random_forces
-    from fsd_amd import _lib
 
     N = args.N
     t = time.time()
     pos, L = random_packing(N, args.phi, seed=args.seed)
     F = random_forces(N, seed=args.seed)
-    print("# workload generated in %.1f s: N=%d phi=%.2f L=%.3f" % (time.time() - t, N, args.phi, L),
-          file=sys.stderr, flush=True)
-    shear_rate = 1.0
-    st = Stokes(N, L, xi=0.5, error=1e-3, gmres_tol=1e-4, gmres_true_tol=1e-4, dt=1e-3, gmres_restart=args.restart,
-                guess=args.guess, gram_schmidt_passes=args.gram_schmidt)
-    pse = st.pse_info()
-    state = ParticleState(pos)
+    log("# workload generated in %.1f s: N=%d phi=%.2f L=%.3f" % (time.time() - t, N, args.phi, L))
     dF = torch.from_numpy(F).cuda()
-
-    # warm-up, timing every kernel class to find the dominant one
-    st.profile(True)
-    infos = []
-    for k in range(args.warmup):
-        infos.append(st.integrateStepOne(k, state, force6=dF, shear_rate=shear_rate))
-        print("# warmup step %d: %s" % (k, infos[-1]), file=sys.stderr, flush=True)
-    totals = {c: st.profile_get(c) for c in _lib.PROF_NAMES}
-    st.profile_reset()
-    # single kernels (trsv is the five-kernel preconditioner application, reported in the class times)
-    kernel_classes = ["spread", "green", "contract", "realspace", "rfu", "sweep", "ichol", "gmres_vec"]
-    dom = max(kernel_classes, key=lambda c: totals[c][1])
-    print("# warmup kernel-class times (launches, ms): %s" % json.dumps(totals), file=sys.stderr, flush=True)
-
-    # timed steps: HIP events only around the dominant class, one launch in eight (the events'
-    # own cost stays out of the step time; the timed launches give the kernel's average duration)
-    st.profile(True, classes=None if args.profile_all else [dom], every=1 if args.profile_all else 8)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    iters = []
-    for k in range(args.steps):
-        info = st.integrateStepOne(args.warmup + k, state, force6=dF, shear_rate=shear_rate)
-        iters.append(info["iterations"])
-        infos.append(info)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    print("# timed steps: iterations %s, initial residuals %s"
-          % ([i["iterations"] for i in infos[-args.steps:]],
-             ["%.2e" % i["initial_residual"] for i in infos[-args.steps:]]), file=sys.stderr, flush=True)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    launches, kms = st.profile_get(dom)
-    last = infos[-1]
+    arm_kw = {"ichol": dict(precond="ichol"),
+              "learned": dict(precond="learned", jacobi_iters=args.jacobi, learned_calibration=args.calibration,
+                              refine=True)}
+    names = [a for a in args.arms.split(",") if a]
+    arms, head = {}, names[-1]
+    for name in names:
+        res, st, dom, last = run_arm(name, arm_kw[name], pos, L, dF, args, log=log)
+        arms[name] = res
+        if name == head:
+            head_st, head_dom, head_last = st, dom, last
+        else:
+            del st
+    pse = head_st.pse_info()
+    launches, kms = head_st.profile_get(head_dom)
     avg_ms = kms / max(launches, 1)
-    it_mean = float(np.mean(iters))
+    it_mean = arms[head]["gmres_iterations_mean"]
     kbar = 0.5 * (min(it_mean, args.restart) + 1.0)
-    bytes_per_launch = algorithmic_bytes(dom, N, None, pse, last, kbar, args.gram_schmidt)
+    bytes_per_launch = algorithmic_bytes(head_dom, N, None, pse, head_last, kbar, args.gram_schmidt)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_launch else None
     # HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
     # (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a plain copy)
-    traffic, kname = None, KERNEL_NAMES.get(dom)
+    traffic, kname = None, KERNEL_NAMES.get(head_dom)
     tfile = os.path.join(ROOT, "profiles", "traffic_N32768.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
@@ -193,28 +237,27 @@ def main():
         k = tr.get("kernels", {}).get(kname)
         if k:
             traffic = 1024.0 * (k["fetch_kib"] * tr["fetch_factor"] + k["write_kib"] * tr["write_factor"])
-    if args.profile_all:
-        allt = {c: st.profile_get(c) for c in _lib.PROF_NAMES}
-        print("# timed kernel-class times (launches, ms): %s" % json.dumps(allt), file=sys.stderr, flush=True)
-
-    print("# timed: %.2f ms/step, dominant kernel class %s (%d launches, %.3f ms avg)"
-          % (ms_per_step, dom, launches, avg_ms), file=sys.stderr, flush=True)
+    log("# %s: %.2f ms/step, dominant kernel class %s (%d launches, %.3f ms avg)"
+        % (head, arms[head]["ms_per_step"], head_dom, launches, avg_ms))
     cpu = None
     if not args.no_cpu:
-        print("# CPU baseline (FP64 oracle) ...", file=sys.stderr, flush=True)
+        log("# CPU baseline (FP64 oracle) ...")
         try:
-            cpu = cpu_baseline(pos, L, F, shear_rate, float(np.mean(iters)), int(os.environ.get("OMP_NUM_THREADS", "16")))
+            ref_iters = arms.get("ichol", arms[head])["gmres_iterations_mean"]
+            cpu = cpu_baseline(pos, L, F, 1.0, ref_iters)
         except Exception as e:  # the baseline is reported, never required
-            print("# cpu baseline failed: %r" % (e,), file=sys.stderr)
-
+            log("# cpu baseline failed: %r" % (e,))
+    ratio = None
+    if "ichol" in arms and "learned" in arms:
+        ratio = arms["learned"]["steps_per_s"] / arms["ichol"]["steps_per_s"]
     out = {
         "metric": METRIC,
-        "value": 1e3 / ms_per_step,
+        "value": arms[head]["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": arms[head]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -222,20 +265,17 @@ def main():
         "data": "synthetic (seeded packing and forces)",
         "config": {
             "workload": "N=32768 phi=0.45 periodic suspension, shear rate 1, random forces, T=0, xi=0.5, "
-                        "PSE error 1e-3, IC(0)-preconditioned GMRES(%d) with %s Gram-Schmidt to true residual 1e-4, "
-                        "initial guess: %s" % (args.restart, "classical" if args.gram_schmidt == 1 else "twice-classical",
-                                               GUESS_TEXT[args.guess]),
+                        "PSE error 1e-3, GMRES(%d) with %s Gram-Schmidt to true residual 1e-4, initial guess: %s; "
+                        "value = %s arm" % (args.restart, "classical" if args.gram_schmidt == 1 else "twice-classical",
+                                            GUESS_TEXT[args.guess], head),
             "N": N, "phi": args.phi, "grid": [pse["Nx"], pse["Ny"], pse["Nz"]], "P": pse["P"],
-            "preconditioner": "ichol",
+            "preconditioner": head, "jacobi_iters": args.jacobi, "calibration_every": args.calibration,
             "gmres_restart": args.restart, "initial_guess": args.guess, "gram_schmidt_passes": args.gram_schmidt,
-            "gmres_iterations_mean": float(np.mean(iters)),
-            "nlist_rebuilds": int(sum(i.get("nlist_rebuilt", 0) for i in infos[-args.steps:])),
-            "true_residual_last": last.get("true_residual"),
-            "nnz": last.get("nnz"), "lub_pairs": last.get("lub_pairs"), "levels": last.get("levels"),
+            "arms": arms, "speedup_learned_vs_ichol": ratio,
             "parallelism": "single GPU",
         },
         "roofline": {
-            "kernel": kname or dom,
+            "kernel": kname or head_dom,
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

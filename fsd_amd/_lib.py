@@ -16,10 +16,11 @@ FSD_E_CAPACITY = -3
 FSD_E_ICHOL = -4
 FSD_E_STATE = -5
 FSD_W_NOT_CONVERGED = 1
+FSD_W_PIVOT = 2
 
 # kernel classes timed by fsd_profile_* (bit k of the enable mask)
 PROF_NAMES = ["spread", "green", "contract", "fft", "realspace", "rfu", "trsv", "ichol", "gmres_vec", "nlist",
-              "precond_build", "sweep", "plan_host", "finish_host"]
+              "precond_build", "sweep", "plan_host", "finish_host", "ljit", "learned_net"]
 
 
 class FsdParams(C.Structure):
@@ -53,6 +54,7 @@ class FsdParams(C.Structure):
         ("precond", C.c_int32),
         ("jacobi_iters", C.c_int32),
         ("factor_sweeps", C.c_int32),
+        ("refine", C.c_int32),
     ]
 
 
@@ -85,6 +87,8 @@ class FsdSolveStats(C.Structure):
         ("sweep_steps", C.c_int32),
         ("initial_residual", C.c_float),
         ("nlist_rebuilt", C.c_int32),
+        ("precond_used", C.c_int32),
+        ("fallback", C.c_int32),
     ]
 
     def as_dict(self):
@@ -111,6 +115,18 @@ SIGNATURES = [
     ("fsd_get_nlist", _I, [_P, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     ("fsd_rfu", _I, [_P, _P, _P, _P]),
     ("fsd_rfu_noalloc", _I, [_P, _P, _P, _P]),
+    ("fsd_learned_nparams", _I, [_P, C.POINTER(C.c_int64)]),
+    ("fsd_learned_set_weights", _I, [_P, _P, C.c_int64]),
+    ("fsd_learned_get_weights", _I, [_P, _P, C.c_int64]),
+    ("fsd_learned_get_grad", _I, [_P, _P, C.c_int64]),
+    ("fsd_learned_get_adam", _I, [_P, _P, _P, C.c_int64]),
+    ("fsd_learned_set_adam", _I, [_P, _P, _P, C.c_int64]),
+    ("fsd_learned_set_calibration", _I, [_P, _I]),
+    ("fsd_learned_inject_pivot", _I, [_P, _I]),
+    ("fsd_learned_stats", _I, [_P, _P]),
+    ("fsd_learned_export", _I, [_P, _P, _P, _P, _P, C.c_int64]),
+    ("fsd_learned_loss_grad", _I, [_P, _P, _I, _P, _P, _I, C.POINTER(C.c_double)]),
+    ("fsd_learned_adam", _I, [_P, _F, _I]),
     ("fsd_rfe", _I, [_P, _P, _F, _P, _P]),
     ("fsd_rsu", _I, [_P, _P, _P, _P]),
     ("fsd_rse", _I, [_P, _P, _F, _P, _P]),

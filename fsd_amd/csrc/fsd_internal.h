@@ -285,6 +285,14 @@ struct fsd_ctx {
     int *d_lrowidx = nullptr;                            // row P of every lower block
     float *d_jac = nullptr;                              // 4 x 6N iterates
     float *d_Lblk2 = nullptr, *d_Ldiag2 = nullptr;       // second factor buffer of the parallel IC(0) sweeps
+    // learned factor (precond mode 2, learned.h) and its fallbacks to IC(0)
+    struct LearnedState *lp = nullptr;
+    int lp_calib_every = 50;       // every this many steps one step runs IC(0) to measure its iteration count
+    int lp_iter_cap = 0;           // IC(0)'s iterations at the last such step (0: none yet)
+    int lp_inject = -1;            // particle whose learned pivot the next factor forces non-positive (tests)
+    bool lp_force_ichol = false;   // this step's preconditioner is IC(0) (calibration)
+    int lp_step_fallback = 0;      // 1: this step's learned factor had a non-positive pivot
+    int64_t lp_steps = 0, lp_pivot_fallbacks = 0, lp_iter_fallbacks = 0, lp_calibrations = 0;
 
     // saddle / GMRES
     float *d_V = nullptr, *d_w = nullptr, *d_t = nullptr, *d_t2 = nullptr;
@@ -293,6 +301,9 @@ struct fsd_ctx {
     hipEvent_t gm_ev[2] = {nullptr, nullptr};   // readbacks of the Hessenberg columns (two slots)
     int kmax = 64;                // most vectors of one multidot (restart + 1, at least 64)
     float *d_rhs = nullptr, *d_sol = nullptr;
+    // FP64-residual iterative refinement (allocated on first use)
+    double *d_x64 = nullptr, *d_r64 = nullptr, *d_ybot64 = nullptr, *d_norm64 = nullptr;
+    float *d_refx32 = nullptr, *d_refy32 = nullptr, *d_refr32 = nullptr, *d_refd = nullptr;
     float *d_sol_prev = nullptr;  // x_{n-2} when the initial guess extrapolates (guess >= 1)
     float *d_sol_prev2 = nullptr; // x_{n-3} for the quadratic extrapolation (guess = 2)
     int64_t n_solved = 0;         // steps solved by this context
@@ -334,7 +345,10 @@ static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 void prof_begin(fsd_ctx *ctx, int slot, hipEvent_t *ev);
 void prof_end(fsd_ctx *ctx, int slot, hipEvent_t ev);
 enum { PROF_SPREAD = 0, PROF_GREEN, PROF_CONTRACT, PROF_FFT, PROF_REAL, PROF_RFU, PROF_TRSV, PROF_ICHOL,
-       PROF_GMRES_VEC, PROF_NLIST, PROF_PRECOND_BUILD, PROF_SWEEP, PROF_PLAN_HOST, PROF_FINISH_HOST, PROF_COUNT };
+       PROF_GMRES_VEC, PROF_NLIST, PROF_PRECOND_BUILD, PROF_SWEEP, PROF_PLAN_HOST, PROF_FINISH_HOST,
+       PROF_LJIT,          // one block Jacobi iteration of the learned factor (forward or backward)
+       PROF_NET,           // features, network and factor assembly of the learned factor
+       PROF_COUNT };
 // host wall time of a phase, added to a slot when that slot is enabled
 void prof_host(fsd_ctx *ctx, int slot, double ms);
 
@@ -395,4 +409,25 @@ int k_jacobi_pack(fsd_ctx *ctx);
 int k_saddle_precond_jacobi(fsd_ctx *ctx, const float *b, float *x);
 int k_precond_apply_jacobi(fsd_ctx *ctx, const float *b, float *y);
 int k_saddle_multiply(fsd_ctx *ctx, const float4 *pos, const float *x, float *y);
+// FP64-residual refinement (k_refine.hip)
+int k_residual64(fsd_ctx *ctx, const float4 *pos, const float *b, const double *x64, float *x32, float *y32,
+                 double *ybot, double *r64, float *r32, double *norms);
+int k_to64(fsd_ctx *ctx, int64_t n, const float *x, double *y);
+int k_to32(fsd_ctx *ctx, int64_t n, const double *x, float *y);
+int k_add64(fsd_ctx *ctx, int64_t n, const float *d, double *x);
+// the learned factor (k_learned.hip)
+int lp_init(fsd_ctx *ctx);
+void lp_free(fsd_ctx *ctx);
+int64_t lp_nparam(fsd_ctx *ctx);
+int lp_set_weights(fsd_ctx *ctx, const float *h_w, int64_t n);
+int lp_get_weights(fsd_ctx *ctx, float *h_w, int64_t n, bool grad);
+int lp_adam_state(fsd_ctx *ctx, float *h_m, float *h_v, int64_t n, bool set);
+int lp_forward(fsd_ctx *ctx, const float4 *pos, float lambda, bool train, int inject, int *failed);
+int lp_apply(fsd_ctx *ctx, const float *b0, const float *b1, const float *bsaddle, float *x, int mode);
+int lp_loss_grad(fsd_ctx *ctx, int nsamp, const float *d_x, const float *d_b, double *loss, bool grad);
+int lp_adam(fsd_ctx *ctx, float lr, int step);
+bool lp_active(const fsd_ctx *ctx);
+bool lp_have_weights(const fsd_ctx *ctx);
+void lp_set_active(fsd_ctx *ctx, bool on);
+int lp_export(fsd_ctx *ctx, float *h_oute, float *h_outn, float *h_W, float *h_Dinv);
 

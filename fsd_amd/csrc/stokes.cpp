@@ -24,7 +24,7 @@ void fsd_ewald_self(float xi, float self_out[2]);
 // ------------------------------------------------------------ profiling
 static const char *kProfNames[PROF_COUNT] = {"spread", "green", "contract", "fft", "realspace", "rfu",
                                              "trsv", "ichol", "gmres_vec", "nlist", "precond_build", "sweep",
-                                             "plan_host", "finish_host"};
+                                             "plan_host", "finish_host", "ljit", "learned_net"};
 
 void prof_host(fsd_ctx *ctx, int slot, double ms)
 {
@@ -210,10 +210,11 @@ static int alloc_all(fsd_ctx *ctx)
     CK(hipMalloc(&ctx->d_scan_tmp, sizeof(int) * 2 * (scan_excl_chunks(ctx->M) + 1)));
     if (ctx->tile + ps.P - 1 > 32) { ctx->err = "spreading support P too large for the tile halo"; return FSD_E_PARAM; }
     ctx->contract_lds = k_contract_lds(ctx);
-    // FSD_TRSV_LEVELS=1 runs the triangular sweeps one launch per level; the Jacobi modes need no sweep plans
+    // FSD_TRSV_LEVELS=1 runs the triangular sweeps one launch per level; IC(0) applied by Jacobi
+    // iterations needs no sweep plans (the learned mode keeps them for its IC(0) fallback)
     {
         const char *lv = getenv("FSD_TRSV_LEVELS");
-        if ((lv && lv[0] == '1') || ctx->precond_mode != 0) ctx->tp_R = 0;
+        if ((lv && lv[0] == '1') || ctx->precond_mode == 1) ctx->tp_R = 0;
         else RC(k_sweep_setup(ctx));
         // FSD_SWEEP_JOINT=1 runs both right-hand sides of a sweep in one workgroup
         const char *jt = getenv("FSD_SWEEP_JOINT");
@@ -350,6 +351,7 @@ static int alloc_all(fsd_ctx *ctx)
 
 static void free_all(fsd_ctx *ctx)
 {
+    lp_free(ctx);
     void *ptrs[] = {ctx->d_nneigh, ctx->d_nlist, ctx->d_cell_of, ctx->d_cell_count, ctx->d_cell_start,
                     ctx->d_cell_list, ctx->d_flags, ctx->d_pos_last, ctx->d_lub_n, ctx->d_lub_j, ctx->d_lub_a, ctx->d_lub_b,
                     ctx->d_lub_self, ctx->d_grid, ctx->d_spec, ctx->d_gridk, ctx->d_fftwork, ctx->d_porig,
@@ -362,6 +364,8 @@ static void free_all(fsd_ctx *ctx)
                     ctx->d_tp_fbsrc, ctx->d_tp_bbsrc, ctx->d_tp_fblk, ctx->d_tp_bblk,
                     ctx->d_tp_finp, ctx->d_tp_binp, ctx->d_tp_fres, ctx->d_tp_bres, ctx->d_tp_fseq, ctx->d_tp_bseq, ctx->d_tp_fplace, ctx->d_tp_bplace,
                     ctx->d_tp_Z, ctx->d_Fblk, ctx->d_lrowidx, ctx->d_jac, ctx->d_Lblk2, ctx->d_Ldiag2,
+                    ctx->d_x64, ctx->d_r64, ctx->d_ybot64, ctx->d_norm64, ctx->d_refx32, ctx->d_refy32,
+                    ctx->d_refr32, ctx->d_refd,
                     ctx->d_Z, ctx->d_lanV, ctx->d_lanw, ctx->d_lany0, ctx->d_lan_h, ctx->d_noise, ctx->d_posp,
                     ctx->d_rfd_rhs, ctx->d_rfd_sol};
     for (void *p : ptrs)
@@ -461,9 +465,9 @@ static void precond_plan(fsd_ctx *ctx)
     prof_host(ctx, PROF_PLAN_HOST, wall_ms() - t0);
 }
 
-static int precond_finish(fsd_ctx *ctx, const float4 *pos)
+// uploads of the host plan and the pair blocks A (everything a factor needs)
+static int precond_upload_blocks(fsd_ctx *ctx, const float4 *pos)
 {
-    const double t0 = wall_ms();
     const int N = ctx->N;
     PrecondHost &h = ctx->pch;
     auto up = [ctx](void *d, const void *src, size_t bytes) {
@@ -495,7 +499,13 @@ static int precond_finish(fsd_ctx *ctx, const float4 *pos)
         CK(up(ctx->d_upos, h.upos.data(), sizeof(int) * ctx->pch.nlow));
     }
     RC(k_build_blocks(ctx, pos));
-    // IC(0) with the relaxer doubling on a non-positive pivot (Precondition.cu:1274-1288, 1455-1489)
+    return 0;
+}
+
+// IC(0) with the relaxer doubling on a non-positive pivot (Precondition.cu:1274-1288, 1455-1489)
+// and the operands of its application
+static int precond_ichol(fsd_ctx *ctx)
+{
     ctx->ichol_attempts = 0;
     float lambda = ctx->ichol_relaxer;
     for (;;) {
@@ -508,8 +518,35 @@ static int precond_finish(fsd_ctx *ctx, const float4 *pos)
         if (lambda > 1.0e7f) { ctx->err = "IC(0) did not succeed for any relaxer"; return FSD_E_ICHOL; }
     }
     ctx->ichol_relaxer = lambda;
-    if (ctx->precond_mode != 0) RC(k_jacobi_pack(ctx));
+    lp_set_active(ctx, false);
+    if (ctx->precond_mode == 1) RC(k_jacobi_pack(ctx));
     else if (ctx->pch.tp_ok) RC(k_sweep_pack(ctx));
+    return 0;
+}
+
+static int precond_finish(fsd_ctx *ctx, const float4 *pos)
+{
+    const double t0 = wall_ms();
+    RC(precond_upload_blocks(ctx, pos));
+    ctx->lp_step_fallback = 0;
+    // learned factor (mode 2), unless this step calibrates IC(0) or the factor has a non-positive pivot
+    bool learned = false;
+    if (ctx->precond_mode == 2) {
+        learned = lp_have_weights(ctx) && !ctx->lp_force_ichol;
+        if (learned) {
+            int failed = 0;
+            RC(lp_forward(ctx, pos, ctx->ichol_relaxer, false, ctx->lp_inject, &failed));
+            ctx->lp_inject = -1;
+            if (failed) {
+                ctx->lp_pivot_fallbacks++;
+                ctx->lp_step_fallback = 1;
+                learned = false;
+            }
+        }
+        lp_set_active(ctx, learned);
+        if (learned) ctx->ichol_attempts = 0;
+    }
+    if (!learned) RC(precond_ichol(ctx));
     prof_host(ctx, PROF_FINISH_HOST, wall_ms() - t0);
     prof_end(ctx, PROF_PRECOND_BUILD, ctx->pc_ev);
     ctx->precond_valid = true;
@@ -540,9 +577,13 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
 // residual estimate is the true residual ||b - A x||: it stops at
 // gmres_true_tol, and the true residual is then recomputed and the solve
 // continued (warm) if FP32 drift left it above the tolerance.
+static int gmres_refined(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, int maxit, int R,
+                         fsd_solve_stats *st);
+
 static int gmres(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
                  fsd_solve_stats *st, bool r0_in_t = false)
 {
+    if (ctx->p.refine && ctx->p.gmres_true_tol > 0.f) return gmres_refined(ctx, pos, b, x, maxit, R, st);
     const float ttol = ctx->p.gmres_true_tol;
     if (ttol <= 0.f) return gmres_core(ctx, pos, b, x, tol, maxit, R, false, false, st);
     int total = 0, restarts = 0, rc = 0;
@@ -563,6 +604,57 @@ static int gmres(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, floa
         st->initial_residual = r0;
     }
     return (tres <= ttol) ? 0 : FSD_W_NOT_CONVERGED;
+}
+
+// FP64-residual iterative refinement (fsd_params.refine): x is accumulated in FP64;
+// each round forms r = b - A x with the lubrication product and the subtraction in
+// FP64 (k_residual64), stops at |r| <= gmres_true_tol |b|, and otherwise solves
+// A d = r with the FP32 right-preconditioned GMRES to a relative tolerance aimed at
+// the target (at most 0.5, at least 1e-4, the FP32 floor) and adds d in FP64.
+static int gmres_refined(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, int maxit, int R,
+                         fsd_solve_stats *st)
+{
+    const int64_t n = 17 * (int64_t)ctx->N;
+    if (!ctx->d_x64) {
+        CK(hipMalloc(&ctx->d_x64, sizeof(double) * n));
+        CK(hipMalloc(&ctx->d_r64, sizeof(double) * n));
+        CK(hipMalloc(&ctx->d_ybot64, sizeof(double) * 6 * (size_t)ctx->N));
+        CK(hipMalloc(&ctx->d_norm64, sizeof(double) * 2));
+        for (float **pp : {&ctx->d_refx32, &ctx->d_refy32, &ctx->d_refr32, &ctx->d_refd})
+            CK(hipMalloc(pp, sizeof(float) * n));
+    }
+    const double ttol = ctx->p.gmres_true_tol;
+    RC(k_to64(ctx, n, x, ctx->d_x64));
+    int total = 0, restarts = 0, rounds = 0;
+    double rel = 1.0, r0 = -1.0;
+    for (;;) {
+        RC(k_residual64(ctx, pos, b, ctx->d_x64, ctx->d_refx32, ctx->d_refy32, ctx->d_ybot64, ctx->d_r64,
+                        ctx->d_refr32, ctx->d_norm64));
+        CK(hipMemcpyAsync(ctx->h_part, ctx->d_norm64, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        rel = std::sqrt(ctx->h_part[0]) / std::max(std::sqrt(ctx->h_part[1]), 1e-300);
+        if (r0 < 0.0) r0 = rel;
+        if (rel <= ttol || total >= maxit || rounds >= 12) break;
+        RC(k_fill(ctx, n, 0.f, ctx->d_refd));
+        const float tau = (float)std::min(0.5, std::max(1e-4, 0.5 * ttol / rel));
+        fsd_solve_stats si;
+        memset(&si, 0, sizeof si);
+        const int rc = gmres_core(ctx, pos, ctx->d_refr32, ctx->d_refd, tau, maxit - total, R, true, false, &si);
+        if (rc < 0) return rc;
+        total += si.iterations;
+        restarts += si.restarts;
+        RC(k_add64(ctx, n, ctx->d_refd, ctx->d_x64));
+        ++rounds;
+    }
+    RC(k_to32(ctx, n, ctx->d_x64, x));
+    if (st) {
+        st->iterations = total;
+        st->restarts = restarts;
+        st->residual = (float)rel;
+        st->true_residual = (float)rel;
+        st->initial_residual = (float)r0;
+    }
+    return rel <= ttol ? 0 : FSD_W_NOT_CONVERGED;
 }
 
 // Restarted GMRES(R), classical Gram-Schmidt with one reorthogonalisation and
@@ -701,6 +793,38 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
         st->residual = (float)(resid / (bn > 0 ? bn : 1.0));
     }
     return (resid <= target || bn == 0.0) ? 0 : FSD_W_NOT_CONVERGED;
+}
+
+// The saddle solve with this step's preconditioner.  With the learned factor
+// the solve may take at most IC(0)'s iteration count of the last calibration
+// step; a solve that reaches it without converging continues from its iterate
+// with IC(0), built then (the iteration fallback).
+static int solve_saddle(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, int maxit, fsd_solve_stats *st,
+                        bool r0_in_t)
+{
+    const bool learned = ctx->precond_mode == 2 && lp_active(ctx);
+    int cap = maxit;
+    if (learned && ctx->lp_iter_cap > 0) cap = std::min(maxit, ctx->lp_iter_cap);
+    int rc = gmres(ctx, pos, b, x, ctx->p.gmres_tol, cap, ctx->p.gmres_restart, st, r0_in_t);
+    if (rc < 0) return rc;
+    st->precond_used = learned ? 1 : 0;
+    st->fallback = ctx->lp_step_fallback;
+    if (learned && rc == FSD_W_NOT_CONVERGED && cap < maxit) {
+        ctx->lp_iter_fallbacks++;
+        RC(precond_ichol(ctx));
+        fsd_solve_stats s2;
+        memset(&s2, 0, sizeof s2);
+        s2.true_residual = -1.f;
+        rc = gmres(ctx, pos, b, x, ctx->p.gmres_tol, maxit - st->iterations, ctx->p.gmres_restart, &s2, false);
+        if (rc < 0) return rc;
+        st->iterations += s2.iterations;
+        st->restarts += s2.restarts;
+        st->residual = s2.residual;
+        st->true_residual = s2.true_residual;
+        st->precond_used = 0;
+        st->fallback = 2;
+    }
+    return rc;
 }
 
 // ----------------------------------------------------------- Brownian motion
@@ -1205,6 +1329,8 @@ int fsd_build_precond(fsd_ctx *ctx, const float *d_pos4, fsd_solve_stats *stats)
         stats->lower_blocks = ctx->pch.nlow;
         stats->levels = ctx->pch.nlev_f;
         stats->sweep_steps = ctx->pch.tp_ok ? ctx->pch.tp_f.nsteps + ctx->pch.tp_b.nsteps : 0;
+        stats->precond_used = lp_active(ctx) ? 1 : 0;
+        stats->fallback = ctx->lp_step_fallback;
     }
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1213,7 +1339,8 @@ int fsd_build_precond(fsd_ctx *ctx, const float *d_pos4, fsd_solve_stats *stats)
 int fsd_precond_apply(fsd_ctx *ctx, const float *d_x6, float *d_y6)
 {
     if (!ctx->precond_valid) { ctx->err = "preconditioner not built"; return FSD_E_STATE; }
-    if (ctx->precond_mode != 0) RC(k_precond_apply_jacobi(ctx, d_x6, d_y6));
+    if (ctx->precond_mode == 2 && lp_active(ctx)) RC(lp_apply(ctx, d_x6, nullptr, nullptr, d_y6, 0));
+    else if (ctx->precond_mode == 1) RC(k_precond_apply_jacobi(ctx, d_x6, d_y6));
     else RC(k_precond_apply2(ctx, d_x6, d_x6, d_y6, d_y6, 1));
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1351,7 +1478,7 @@ int fsd_saddle_solve(fsd_ctx *ctx, const float *d_pos4, const float *d_rhs17, fl
     fsd_solve_stats st;
     memset(&st, 0, sizeof st);
     st.true_residual = -1.f;
-    int rc = gmres(ctx, pos, d_rhs17, d_sol17, ctx->p.gmres_tol, ctx->p.gmres_maxiter, ctx->p.gmres_restart, &st);
+    int rc = solve_saddle(ctx, pos, d_rhs17, d_sol17, ctx->p.gmres_maxiter, &st, false);
     if (rc < 0) return rc;
     if (want_true_residual) RC(true_residual(ctx, pos, d_rhs17, d_sol17, &st.true_residual));
     st.ichol_relaxer = ctx->ichol_relaxer;
@@ -1471,7 +1598,7 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
             RC(k_strain_rhs(ctx, ctx->d_rhs, shear_rate, d_ori3, ctx->d_force));
             RC(k_rfe(ctx, pos, shear_rate, d_ori3, ctx->d_rhs + 11 * (int64_t)N));
             RC(initial_guess(ctx));
-            if (ctx->p.gmres_true_tol > 0.f) {
+            if (ctx->p.gmres_true_tol > 0.f && !ctx->p.refine) {
                 RC(k_saddle_multiply(ctx, pos, ctx->d_sol, ctx->d_t));
                 RC(k_axpby(ctx, n17, 1.0f, ctx->d_t, -1.0f, ctx->d_rhs, ctx->d_t));
                 r0_in_t = true;
@@ -1482,6 +1609,8 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     const int rci = independent();
     planner.join();
     if (rci < 0) return rci;
+    ctx->lp_force_ichol = ctx->precond_mode == 2 && lp_have_weights(ctx) && ctx->lp_calib_every > 0 &&
+                          ctx->lp_steps % ctx->lp_calib_every == 0;
     RC(precond_finish(ctx, pos));
     if (brownian) {
         // rhs[0:11N] = U_B^ff, rhs[11N:17N] = F_B^nf, Velocity[0:6N] = RFD drift
@@ -1497,10 +1626,17 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     fsd_solve_stats st;
     memset(&st, 0, sizeof st);
     st.true_residual = -1.f;
-    int rc = gmres(ctx, pos, ctx->d_rhs, ctx->d_sol, ctx->p.gmres_tol, ctx->p.gmres_maxiter, ctx->p.gmres_restart, &st,
-                   r0_in_t);
+    int rc = solve_saddle(ctx, pos, ctx->d_rhs, ctx->d_sol, ctx->p.gmres_maxiter, &st, r0_in_t);
     ctx->n_solved++;
     if (rc < 0) return rc;
+    if (ctx->precond_mode == 2) {
+        if (ctx->lp_force_ichol) {   // calibration step: IC(0)'s iteration count caps the learned solves
+            ctx->lp_iter_cap = st.iterations;
+            ctx->lp_calibrations++;
+        }
+        ctx->lp_force_ichol = false;
+        ctx->lp_steps++;
+    }
     RC(k_axpby(ctx, 6 * (int64_t)N, 1.0f, ctx->d_sol + 11 * (int64_t)N, 1.0f, d_vel11, d_vel11));
     if (ctx->p.period > 0 && (int)(timestep + 1) % ctx->p.period == 0) {
         float *S = d_vel11 + 6 * (int64_t)N;
@@ -1569,3 +1705,86 @@ int fsd_profile_reset(fsd_ctx *ctx)
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------ learned factor (mode 2)
+int fsd_learned_nparams(fsd_ctx *ctx, int64_t *n)
+{
+    RC(lp_init(ctx));
+    *n = lp_nparam(ctx);
+    return 0;
+}
+
+int fsd_learned_set_weights(fsd_ctx *ctx, const float *h_w, int64_t n) { return lp_set_weights(ctx, h_w, n); }
+
+int fsd_learned_get_weights(fsd_ctx *ctx, float *h_w, int64_t n) { return lp_get_weights(ctx, h_w, n, false); }
+
+int fsd_learned_get_grad(fsd_ctx *ctx, float *h_g, int64_t n) { return lp_get_weights(ctx, h_g, n, true); }
+
+int fsd_learned_get_adam(fsd_ctx *ctx, float *h_m, float *h_v, int64_t n)
+{
+    return lp_adam_state(ctx, h_m, h_v, n, false);
+}
+
+int fsd_learned_set_adam(fsd_ctx *ctx, const float *h_m, const float *h_v, int64_t n)
+{
+    return lp_adam_state(ctx, const_cast<float *>(h_m), const_cast<float *>(h_v), n, true);
+}
+
+int fsd_learned_set_calibration(fsd_ctx *ctx, int32_t every)
+{
+    if (every < 0) { ctx->err = "calibration period must be >= 0"; return FSD_E_PARAM; }
+    ctx->lp_calib_every = every;
+    ctx->lp_steps = 0;
+    return 0;
+}
+
+int fsd_learned_inject_pivot(fsd_ctx *ctx, int32_t particle)
+{
+    if (particle < 0 || particle >= ctx->N) { ctx->err = "particle out of range"; return FSD_E_PARAM; }
+    ctx->lp_inject = particle;
+    return 0;
+}
+
+int fsd_learned_stats(fsd_ctx *ctx, int64_t *h_out4)
+{
+    h_out4[0] = ctx->lp_steps;
+    h_out4[1] = ctx->lp_pivot_fallbacks;
+    h_out4[2] = ctx->lp_iter_fallbacks;
+    h_out4[3] = ctx->lp_calibrations;
+    return 0;
+}
+
+int fsd_learned_loss_grad(fsd_ctx *ctx, const float *d_pos4, int32_t nsamp, const float *d_x17, const float *d_b17,
+                          int32_t want_grad, double *loss)
+{
+    if (ctx->precond_mode != 2) { ctx->err = "the learned factor needs precond mode 2"; return FSD_E_STATE; }
+    const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
+    RC(precond_fetch(ctx, pos));
+    precond_plan(ctx);
+    RC(precond_upload_blocks(ctx, pos));
+    prof_end(ctx, PROF_PRECOND_BUILD, ctx->pc_ev);
+    int failed = 0;
+    RC(lp_forward(ctx, pos, ctx->ichol_relaxer, true, -1, &failed));
+    lp_set_active(ctx, !failed);
+    ctx->precond_valid = !failed;
+    if (failed) {
+        ctx->err = "learned factor has a non-positive pivot";
+        return FSD_W_PIVOT;
+    }
+    RC(lp_loss_grad(ctx, nsamp, d_x17, d_b17, loss, want_grad != 0));
+    return 0;
+}
+
+int fsd_learned_export(fsd_ctx *ctx, float *h_oute, float *h_outn, float *h_W, float *h_Dinv, int64_t edge_cap)
+{
+    if (!lp_active(ctx)) { ctx->err = "no learned factor is active"; return FSD_E_STATE; }
+    if (ctx->pch.nlow > edge_cap) return FSD_E_CAPACITY;
+    return lp_export(ctx, h_oute, h_outn, h_W, h_Dinv);
+}
+
+int fsd_learned_adam(fsd_ctx *ctx, float lr, int32_t step)
+{
+    if (step < 1) { ctx->err = "Adam step counts from 1"; return FSD_E_PARAM; }
+    RC(lp_init(ctx));
+    return lp_adam(ctx, lr, step);
+}

@@ -38,7 +38,8 @@ class Stokes:
                  N_mix=1, coef_B1_mask=1.0, coef_B2_mask=1.0, rot_diff=0.0, T_ext=0.0, omega_ext=0.0,
                  dt=1e-3, max_strain=0.5, gmres_tol=None, gmres_restart=50, gmres_maxiter=1000,
                  parity=True, gmres_true_tol=0.0, guess="previous", gram_schmidt_passes=2,
-                 precond="ichol", jacobi_iters=3, factor_sweeps=0):
+                 precond="ichol", jacobi_iters=3, factor_sweeps=0, learned_weights="default",
+                 learned_calibration=50, refine=False):
         self.lib = _lib.load()
         self.torch = _torch()
         L = box if not np.isscalar(box) else (box, box, box, 0.0, 0.0, 0.0)
@@ -73,6 +74,9 @@ class Stokes:
         p.precond = modes[precond]
         p.jacobi_iters = int(jacobi_iters)
         p.factor_sweeps = int(factor_sweeps)
+        if refine and not gmres_true_tol:
+            raise ValueError("refine needs gmres_true_tol (the FP64 true residual it refines to)")
+        p.refine = 1 if refine else 0
         self.params = p
         self.N = int(N)
         self.fileprefix = fileprefix
@@ -82,6 +86,15 @@ class Stokes:
         rc = self.lib.fsd_create(C.byref(p), C.byref(self.ctx))
         self._check(rc)
         self.setResistanceTable()
+        if p.precond == 2:
+            # the learned factor: the committed trained weights unless the caller passes its own
+            # (None: no weights, every step then runs IC(0))
+            if learned_weights is not None:
+                from . import learned
+                w = learned.load_weights() if isinstance(learned_weights, str) and learned_weights == "default" \
+                    else np.asarray(learned_weights, dtype=np.float32)
+                self.learned_set_weights(w)
+            self._check(self.lib.fsd_learned_set_calibration(self.ctx, int(learned_calibration)))
 
     # ---- error handling
     def _check(self, rc):
@@ -295,6 +308,66 @@ class Stokes:
         d = st.as_dict()
         d["converged"] = rc == 0
         return x, d
+
+    # ---- learned preconditioner factor (precond="learned")
+    def learned_nparams(self):
+        n = C.c_int64(0)
+        self._check(self.lib.fsd_learned_nparams(self.ctx, C.byref(n)))
+        return n.value
+
+    def learned_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(self.lib.fsd_learned_set_weights(self.ctx, w.ctypes.data_as(C.c_void_p), w.size))
+
+    def learned_get_weights(self, grad=False):
+        w = np.zeros(self.learned_nparams(), dtype=np.float32)
+        f = self.lib.fsd_learned_get_grad if grad else self.lib.fsd_learned_get_weights
+        self._check(f(self.ctx, w.ctypes.data_as(C.c_void_p), w.size))
+        return w
+
+    def learned_loss_grad(self, pos, X, B, want_grad=True):
+        """loss (and gradient, learned_get_weights(grad=True)) of nsamp samples: X, B are
+        [nsamp, 17N] device tensors with B = saddle_multiply(X); None on a non-positive pivot"""
+        loss = C.c_double(0.0)
+        self._sync()
+        rc = self.lib.fsd_learned_loss_grad(self.ctx, _ptr(pos), int(X.shape[0]), _ptr(X), _ptr(B),
+                                            1 if want_grad else 0, C.byref(loss))
+        self._check(rc)
+        return None if rc == _lib.FSD_W_PIVOT else loss.value
+
+    def learned_adam_state(self, m=None, v=None):
+        """get (no arguments) or set the Adam moments"""
+        n = self.learned_nparams()
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        if m is None and v is None:
+            m, v = np.zeros(n, dtype=np.float32), np.zeros(n, dtype=np.float32)
+            self._check(self.lib.fsd_learned_get_adam(self.ctx, vp(m), vp(v), n))
+            return m, v
+        m = np.ascontiguousarray(m, dtype=np.float32)
+        v = np.ascontiguousarray(v, dtype=np.float32)
+        self._check(self.lib.fsd_learned_set_adam(self.ctx, vp(m), vp(v), n))
+
+    def learned_adam(self, lr, step):
+        self._check(self.lib.fsd_learned_adam(self.ctx, float(lr), int(step)))
+
+    def learned_inject_pivot(self, particle):
+        self._check(self.lib.fsd_learned_inject_pivot(self.ctx, int(particle)))
+
+    def learned_stats(self):
+        out = np.zeros(4, dtype=np.int64)
+        self._check(self.lib.fsd_learned_stats(self.ctx, out.ctypes.data_as(C.c_void_p)))
+        return dict(steps=int(out[0]), pivot_fallbacks=int(out[1]), iteration_fallbacks=int(out[2]),
+                    calibrations=int(out[3]))
+
+    def learned_export(self, n_lower):
+        N = self.N
+        oute = np.zeros((n_lower, 32), dtype=np.float32)
+        outn = np.zeros((N, 32), dtype=np.float32)
+        W = np.zeros((n_lower, 6, 6), dtype=np.float32)
+        Dinv = np.zeros((N, 6, 6), dtype=np.float32)
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+        self._check(self.lib.fsd_learned_export(self.ctx, vp(oute), vp(outn), vp(W), vp(Dinv), int(n_lower)))
+        return dict(oute=oute, outn=outn, W=W, Dinv=Dinv)
 
     def force_law(self, pos, T_ext=0.0):
         F = self._out(6 * self.N)

@@ -37,6 +37,7 @@ extern "C" {
 #define FSD_E_ICHOL (-4)          /* IC(0) failed after the relaxer limit */
 #define FSD_E_STATE (-5)          /* call order violated (e.g. no nlist yet) */
 #define FSD_W_NOT_CONVERGED 1     /* GMRES hit its iteration limit (Solvers.cu:121-125) */
+#define FSD_W_PIVOT 2             /* the learned factor has a non-positive pivot (no loss computed) */
 
 typedef struct fsd_ctx fsd_ctx;
 
@@ -77,10 +78,15 @@ typedef struct fsd_params {
                                         applied: 0 = IC(0), exact triangular solves (the reference,
                                         Precondition.cu:1802-1890); 1 = IC(0), jacobi_iters Jacobi iterations
                                         per triangular solve; 2 = learned factor (fsd_learned_set_weights),
-                                        Jacobi iterations, IC(0) fallback */
+                                        jacobi_iters block Jacobi iterations per triangular solve, IC(0)
+                                        with exact solves as its fallback */
     int32_t jacobi_iters;            /* Jacobi iterations per triangular solve in modes 1 and 2 (<= 0: 3) */
     int32_t factor_sweeps;           /* IC(0) by the level-scheduled recurrence (0) or by this many parallel
                                         fixed-point sweeps (Chow & Patel 2015) from the block-diagonal start */
+    int32_t refine;                  /* 1: FP64-residual iterative refinement of the saddle solve (the solution
+                                        accumulated in FP64, r = b - A x with the lubrication product and the
+                                        subtraction in FP64, FP32 GMRES for each correction) to the true
+                                        residual gmres_true_tol; 0: plain FP32 GMRES (Solvers.cu:55-126) */
 } fsd_params;
 
 /* Derived PSE parameters (Stokes.cc:131-261). */
@@ -110,6 +116,10 @@ typedef struct fsd_solve_stats {
     float initial_residual;          /* ||b - A x0|| / ||b|| (true residual) or its preconditioned form */
     int32_t nlist_rebuilt;           /* fsd_step: 1 if the neighbour list was rebuilt this step (some particle
                                         moved r_buff / 2 since the last build, or the box changed) */
+    int32_t precond_used;            /* the solve's factor: 0 IC(0), 1 the learned factor (precond mode 2) */
+    int32_t fallback;                /* mode 2: 0 none, 1 the learned factor had a non-positive pivot (IC(0)
+                                        used), 2 the learned solve reached IC(0)'s iteration count and was
+                                        finished with IC(0) */
 } fsd_solve_stats;
 
 /* ---- lifecycle: Stokes::Stokes + setParams + setSparseMath + AllocateWorkSpaces
@@ -266,6 +276,45 @@ int fsd_profile_every(fsd_ctx *ctx, int32_t every);
  * and total milliseconds */
 int fsd_profile_get(fsd_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
 int fsd_profile_reset(fsd_ctx *ctx);
+
+/* ---- learned preconditioner factor (precond mode 2) ----
+ * No counterpart in the reference: these replace the IC(0) factorisation of the
+ * saddle preconditioner (Precondition.cu:1116-1307, 1448-1489) with a factor that a
+ * message-passing network predicts on IC(0)'s own pattern (Precondition.cu:530-836).
+ * The factor is block LDL^T, (D + W) D^-1 (D + W)^T, W on the pattern's lower pair
+ * blocks and D one 6x6 pivot per particle; a step falls back to IC(0) when a pivot
+ * of D is non-positive, and a solve finishes with IC(0) when it reaches the
+ * iteration count IC(0) needed at the last calibration step (every
+ * fsd_learned_set_calibration steps, default 50, one step runs IC(0)).
+ * Parameters are FP32 (fsd_learned_nparams of them); the network runs them in BF16. */
+int fsd_learned_nparams(fsd_ctx *ctx, int64_t *n);
+int fsd_learned_set_weights(fsd_ctx *ctx, const float *h_w, int64_t n);
+int fsd_learned_get_weights(fsd_ctx *ctx, float *h_w, int64_t n);
+/* gradient of the last fsd_learned_loss_grad with want_grad != 0 */
+int fsd_learned_get_grad(fsd_ctx *ctx, float *h_g, int64_t n);
+/* Adam moments (first, second), nparams each, so training can move between contexts */
+int fsd_learned_get_adam(fsd_ctx *ctx, float *h_m, float *h_v, int64_t n);
+int fsd_learned_set_adam(fsd_ctx *ctx, const float *h_m, const float *h_v, int64_t n);
+int fsd_learned_set_calibration(fsd_ctx *ctx, int32_t every);
+/* the next learned factor gets a non-positive pivot at this particle (failure-path tests) */
+int fsd_learned_inject_pivot(fsd_ctx *ctx, int32_t particle);
+/* h_out4: steps, pivot fallbacks, iteration fallbacks, calibration steps */
+int fsd_learned_stats(fsd_ctx *ctx, int64_t *h_out4);
+/* network outputs (32 per lower block, 32 per particle), pair blocks W (36 per lower
+ * block) and inverse pivots D^-1 (36 per particle) of the active learned factor, in
+ * the pattern's order (fsd_get_precond's permuted lower blocks); any may be null */
+int fsd_learned_export(fsd_ctx *ctx, float *h_oute, float *h_outn, float *h_W, float *h_Dinv, int64_t edge_cap);
+/* Training loss for nsamp samples: d_x17 holds x_s, d_b17 holds b_s = A x_s (A the saddle
+ * operator, fsd_saddle_multiply), 17N each, consecutive.  With g_s = G (b_F - b_U)_s the
+ * learned preconditioner's G (jacobi_iters iterations each way), the loss is
+ * sum_s |g_s - (b_F - x_F)_s|^2 + |g_s - x_U,s|^2 over sum_s |x_s|^2: the part of
+ * |P(A x) - x|^2 / |x|^2 that depends on the factor, P the saddle preconditioner.
+ * Builds the pattern at d_pos4 (needs the neighbour list); want_grad != 0 also forms
+ * the gradient (fsd_learned_get_grad).  Returns FSD_W_PIVOT on a non-positive pivot. */
+int fsd_learned_loss_grad(fsd_ctx *ctx, const float *d_pos4, int32_t nsamp, const float *d_x17, const float *d_b17,
+                          int32_t want_grad, double *loss);
+/* one Adam step (Kingma & Ba 2015; beta 0.9, 0.999) on the last gradient */
+int fsd_learned_adam(fsd_ctx *ctx, float lr, int32_t step);
 
 #ifdef __cplusplus
 }

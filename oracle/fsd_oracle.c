@@ -1336,3 +1336,5 @@ void orc_rk2_stage2(int N, double c1, const float *pos1, const float *ori1, doub
         for (int a = 0; a < 3; ++a) ori_out3[3 * i + a] = (float)pv[a];
     }
 }
+
+int orc_threads(void) { return omp_get_max_threads(); }

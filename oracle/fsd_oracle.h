@@ -51,6 +51,8 @@ void orc_wrap(const orc_box *b, float p[3], int img[3]);
 
 /* ---- parameters (Stokes.cc:118-261) ---- */
 int orc_pse_params(const orc_box *b, float xi, float error, float max_strain, orc_pse *out);
+/* OpenMP threads the oracle runs on */
+int orc_threads(void);
 /* gridk: M float4 (kx, ky, kz, w), parity!=0 reproduces the 3.1416926536 factor */
 void orc_gridk(const orc_box *b, const orc_pse *p, int parity, double *gridk4);
 

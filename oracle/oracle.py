@@ -89,8 +89,14 @@ def load():
         lib.fsd_ewald_table_fill.argtypes = [C.c_float, C.c_int, _P]
         lib.fsd_ewald_functions.argtypes = [C.c_double, C.c_double, _P]
         lib.fsd_ewald_self.argtypes = [C.c_float, _P]
+        lib.orc_threads.restype = C.c_int
         _lib = lib
     return _lib
+
+
+def threads():
+    """OpenMP threads the oracle's parallel loops use"""
+    return int(load().orc_threads())
 
 
 def _p(a):

@@ -1,0 +1,133 @@
+"""GPU parity of the learned preconditioner factor against its FP64 restatement
+(oracle/learned.py), and its fallbacks to IC(0).
+
+The oracle rounds to BF16 where the GPU does (weights, matrix-product inputs,
+pre-activation gradients into input-gradient products), so the remaining
+differences are FP32 accumulation order and the rare BF16 rounding flips it
+causes; the tolerances below are written for that.  The factor's application
+(block Jacobi iterations on W and D^-1) is compared on the GPU's own W and
+D^-1, where only FP32 rounding remains (1e-5).
+"""
+import numpy as np
+import pytest
+
+from tests._cases import packing, rel_err, rel_err2, random_vec
+
+pytestmark = pytest.mark.gpu
+
+CASES = {"C1": (343, 0.20, 11), "D1": (512, 0.45, 21)}
+
+
+def _setup(case, w, k=3, **kw):
+    import torch
+    from fsd_amd.stokes import Stokes
+    from oracle.oracle import Oracle
+    from oracle.learned import LearnedOracle
+    N, phi, seed = CASES[case]
+    pos, L = packing(N, phi, seed)
+    print("learned case", case, "N", N, "phi", phi, "seed", seed, flush=True)
+    st = Stokes(N, L, precond="learned", jacobi_iters=k, learned_weights=w, **kw)
+    p4 = np.zeros((N, 4), dtype=np.float32)
+    p4[:, :3] = pos
+    dpos = torch.from_numpy(p4.reshape(-1)).cuda()
+    st.build_nlist(dpos)
+    orc = Oracle(pos, L)
+    orc.build_precond()
+    return st, orc, LearnedOracle(orc, k=k), dpos, N
+
+
+def _weights(seed=1, head=0.3):
+    from oracle.learned import init_weights
+    return init_weights(seed, scale_out=head)
+
+
+@pytest.mark.parametrize("case", ["C1", "D1"])
+def test_learned_factor_parity(case):
+    w = _weights()
+    st, orc, lo, dpos, N = _setup(case, w)
+    info = st.build_precond(dpos)
+    assert info["precond_used"] == 1 and info["fallback"] == 0
+    g = st.learned_export(lo.E)
+    c = lo.forward(w, emulate=True)
+    W, D, Dinv, failed = lo.assemble(c["oute"], c["outn"])
+    assert not failed
+    e_oe = rel_err(g["oute"][:, :7], c["oute"][:, :7]) if lo.E else 0.0
+    e_on = rel_err(g["outn"][:, :2], c["outn"][:, :2])
+    e_W = rel_err(g["W"], W) if lo.E else 0.0
+    e_D = rel_err(g["Dinv"], Dinv)
+    print("E", lo.E, "outputs edge/particle", e_oe, e_on, "W", e_W, "Dinv", e_D)
+    assert e_oe < 2e-2 and e_on < 2e-2 and e_W < 2e-2 and e_D < 2e-2
+
+
+@pytest.mark.parametrize("case", ["C1", "D1"])
+def test_learned_apply_parity(case):
+    """G by k Jacobi iterations each way on the GPU's own factor (1e-5), and, with k at
+    least the depth of the pattern, equal to the exact ((D + W) D^-1 (D + W)^T)^-1"""
+    import torch
+    w = _weights()
+    for k in (3, 64):
+        st, orc, lo, dpos, N = _setup(case, w, k=k)
+        st.build_precond(dpos)
+        g = st.learned_export(lo.E)
+        x = random_vec(6 * N, 4)
+        y = st.precond_apply(torch.from_numpy(x.astype(np.float32)).cuda()).cpu().numpy()
+        W, Dinv = g["W"].astype(np.float64), g["Dinv"].astype(np.float64)
+        ref = lo.apply_G(W, Dinv, x, k=k)
+        e = rel_err(y, ref)
+        print("k", k, "apply rel err", e)
+        assert e < 1e-5
+        if k == 64:
+            D = np.linalg.inv(Dinv)
+            assert rel_err(ref, lo.G_exact(W, D, x)) < 1e-8
+
+
+@pytest.mark.parametrize("case", ["C1", "D1"])
+def test_learned_gradient_parity(case):
+    """loss and gradient over all 74304 parameters against the oracle's reverse sweep;
+    the oracle's own gradient is pinned by finite differences (tests/test_learned_cpu.py)"""
+    import torch
+    w = _weights()
+    st, orc, lo, dpos, N = _setup(case, w)
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((3, 17 * N)).astype(np.float32)
+    B = np.stack([orc.saddle_multiply(x.astype(np.float64)) for x in X]).astype(np.float32)
+    dX, dB = torch.from_numpy(X).cuda(), torch.from_numpy(B).cuda()
+    loss = st.learned_loss_grad(dpos, dX, dB)
+    gg = st.learned_get_weights(grad=True)
+    lref, gref = lo.grad(w, X.astype(np.float64), B.astype(np.float64), emulate=True)
+    el, eg = abs(loss - lref) / abs(lref), rel_err2(gg, gref)
+    print("loss", loss, lref, "rel", el, "grad rel L2", eg, "|g|", np.linalg.norm(gref))
+    assert el < 1e-3 and eg < 3e-2
+
+
+def test_learned_pivot_fallback():
+    """a non-positive pivot of the learned factor falls back to IC(0) for that step;
+    the solve is then the IC(0) solve"""
+    import torch
+    w = _weights()
+    st, orc, lo, dpos, N = _setup("D1", w, gmres_true_tol=1e-5, gmres_restart=100)
+    st.learned_inject_pivot(5)
+    info = st.build_precond(dpos)
+    assert info["precond_used"] == 0 and info["fallback"] == 1
+    assert st.learned_stats()["pivot_fallbacks"] == 1
+    b = torch.from_numpy(random_vec(17 * N, 9).astype(np.float32)).cuda()
+    x, si = st.saddle_solve(dpos, b)
+    assert si["precond_used"] == 0 and si["converged"]
+    # the next build is learned again
+    info = st.build_precond(dpos)
+    assert info["precond_used"] == 1 and info["fallback"] == 0
+
+
+def test_trained_factor_gain():
+    """the committed trained network against the method without it (zero network
+    outputs: the pair blocks with uncorrected pivots, block symmetric Gauss-Seidel)
+    and against IC(0): mean GMRES iterations per step to the true residual 1e-4 on a
+    held-out N = 4096, phi = 0.45 sheared suspension (training used N = 1024)"""
+    import os
+    from fsd_amd import learned
+    if not os.path.exists(learned.DEFAULT):
+        pytest.skip("no trained weights committed")
+    res = learned.evaluate([("untrained", learned.zero_head_weights()), ("trained", learned.load_weights())],
+                           N=4096, phi=0.45, seed=91, steps=4, warmup=2, log=lambda s: print(s, flush=True))
+    print("mean iterations", res)
+    assert res["trained"] < res["untrained"]

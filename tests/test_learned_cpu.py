@@ -1,0 +1,73 @@
+"""CPU checks of the learned factor's restatement (oracle/learned.py) and of the
+product-side parameter layout (fsd_amd/learned.py).
+
+The analytic gradient of the oracle's loss is pinned by central finite
+differences of its FP64 loss (a directional derivative and single
+parameters of every layer); its G by the exact block LDL^T inverse."""
+import numpy as np
+import pytest
+
+from tests._cases import packing
+
+
+@pytest.fixture(scope="module")
+def dense():
+    from oracle.oracle import Oracle
+    from oracle.learned import LearnedOracle
+    pos, L = packing(128, 0.45, 7)
+    orc = Oracle(pos, L)
+    orc.build_precond()
+    lo = LearnedOracle(orc, k=3)
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((2, 17 * 128))
+    B = np.array([orc.saddle_multiply(x) for x in X])
+    return orc, lo, X, B
+
+
+def test_layouts_agree():
+    from fsd_amd import learned as PL
+    from oracle import learned as OL
+    assert PL.LAYERS == OL.LAYERS and PL.nparams() == OL.nparams() == 74304
+
+
+def test_zero_heads_give_symmetric_gauss_seidel(dense):
+    """with zero network outputs W is the pair blocks A and D = A_PP + lambda S"""
+    from oracle.learned import init_weights
+    orc, lo, X, B = dense
+    c = lo.forward(init_weights(1, scale_out=0.0), emulate=False)
+    assert np.abs(c["oute"]).max() == 0 and np.abs(c["outn"]).max() == 0
+    W, D, Dinv, failed = lo.assemble(c["oute"], c["outn"])
+    assert lo.E > 100 and not failed
+    assert np.abs(W - lo.Ablk).max() <= 1e-5 * np.abs(lo.Ablk).max()
+    assert np.allclose(D, lo.Dt)
+
+
+def test_deep_jacobi_is_exact(dense):
+    from oracle.learned import init_weights
+    orc, lo, X, B = dense
+    c = lo.forward(init_weights(2, scale_out=0.3))
+    W, D, Dinv, failed = lo.assemble(c["oute"], c["outn"])
+    d = np.random.default_rng(1).standard_normal(6 * lo.N)
+    ex = lo.G_exact(W, D, d)
+    assert np.abs(lo.apply_G(W, Dinv, d, k=200) - ex).max() < 1e-10 * np.abs(ex).max()
+    # a truncated application differs
+    assert np.abs(lo.apply_G(W, Dinv, d, k=1) - ex).max() > 1e-6 * np.abs(ex).max()
+
+
+def test_gradient_finite_differences(dense):
+    from oracle.learned import init_weights, layer_offsets
+    orc, lo, X, B = dense
+    w = init_weights(1, scale_out=0.3).astype(np.float64)
+    loss, g = lo.grad(w, X, B, emulate=False)
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(w.size)
+    eps = 1e-6
+    fd = (lo.loss(w + eps * v, X, B, emulate=False) - lo.loss(w - eps * v, X, B, emulate=False)) / (2 * eps)
+    assert abs(fd - g @ v) < 1e-6 * abs(fd)
+    offs, _ = layer_offsets()
+    for wo, bo in offs:           # one weight and one bias of every layer
+        for i in (wo + 7, bo + 1):
+            e = np.zeros(w.size)
+            e[i] = 1e-5
+            fdi = (lo.loss(w + e, X, B, emulate=False) - lo.loss(w - e, X, B, emulate=False)) / 2e-5
+            assert abs(fdi - g[i]) <= 1e-6 * max(abs(fdi), 1e-3), (i, fdi, g[i])
