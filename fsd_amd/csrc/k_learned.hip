@@ -58,6 +58,11 @@ __device__ __forceinline__ float silu_d(float z)
     return s * (1.0f + z * (1.0f - s));
 }
 __device__ __forceinline__ float sgnlog(float v) { return copysignf(log1pf(fabsf(v)), v); }
+// the network's raw outputs enter the factor bounded: pair-coefficient multipliers
+// 1 + tanh(c) in (0, 2), fill weights sigmoid(c + 4) in (0, 1) (0.982 at c = 0: the
+// zero-output factor subtracts nearly the whole fill term), pivot shift lambda e^a S
+constexpr float LP_GAMMA0 = 4.0f;
+__device__ __forceinline__ float lp_gamma(float c) { return sigm(c + LP_GAMMA0); }
 
 // 6x6 symmetric positive definite inverse through Cholesky (lower triangle of D
 // read); false on a non-positive pivot
@@ -343,7 +348,7 @@ __global__ void lp_to_bf_kernel(int64_t n, const float *x, uint16_t *y)
 }
 
 // ------------------------------------------------------------ factor assembly
-// W_e = sum_k (1 + c_k) basis_k (the pair block A when every c_k = 0)
+// W_e = sum_k (1 + tanh c_k) basis_k (the pair block A when every c_k = 0)
 __global__ void lp_edge_W_kernel(int64_t E, const float *geo, const float *oute, float *W)
 {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -358,27 +363,28 @@ __global__ void lp_edge_W_kernel(int64_t E, const float *geo, const float *oute,
     const float *o = oute + e * lp::OE;
     float w = 0.f;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) w += (1.0f + o[k]) * basis_entry(k, a, c, r, E3, ph);
+    for (int k = 0; k < 6; ++k) w += (1.0f + tanhf(o[k])) * basis_entry(k, a, c, r, E3, ph);
     W[t] = w;
 }
 
-// D_P = A_PP + lambda S + diag(alpha_t, alpha_r) - sum_{Q < P} gamma_e W_e Dt_Q^-1 W_e^T;
-// Dinv_P = D_P^-1, a non-positive pivot sets fail bit 1
+// D_P = A_PP + lambda diag(e^a_t, e^a_r) S - sum_{Q < P} gamma_e A_e Dt_Q^-1 A_e^T with
+// gamma_e = sigmoid(c_e + 4) and A_e the pair block; Dinv_P = D_P^-1, a non-positive pivot
+// sets fail bit 1.  The fill term uses the pair block, not W: the pivots then do not
+// depend on W, and (D + W) D^-1 (D + W)^T is positive definite for any W once D is.
 __global__ void lp_node_D_kernel(int N, const int *lrow, const int *lcol, const float *Adiag, float lambda,
-                                 const float *outn, const float *oute, const float *W, const float *Dtinv, float *Dinv,
-                                 int *fail)
+                                 const float *outn, const float *oute, const float *Ablk, const float *Dtinv,
+                                 float *Dinv, int *fail)
 {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= N) return;
     float D[36];
     for (int q = 0; q < 36; ++q) D[q] = Adiag[(int64_t)P * 36 + q];
-    const float at = outn[(int64_t)P * lp::ON], ar = outn[(int64_t)P * lp::ON + 1];
-    for (int c = 0; c < 6; ++c) D[7 * c] += lambda * (c < 3 ? 1.0f : 4.0f / 3.0f) + (c < 3 ? at : ar);
+    const float at = expf(outn[(int64_t)P * lp::ON]), ar = expf(outn[(int64_t)P * lp::ON + 1]);
+    for (int c = 0; c < 6; ++c) D[7 * c] += lambda * (c < 3 ? at : ar * (4.0f / 3.0f));
     for (int e = lrow[P]; e < lrow[P + 1]; ++e) {
-        const float gam = oute[(int64_t)e * lp::OE + 6];
-        if (gam == 0.f) continue;
-        const float *We = W + (int64_t)e * 36, *Ti = Dtinv + (int64_t)lcol[e] * 36;
-        float T[36];   // W Dt^-1
+        const float gam = lp_gamma(oute[(int64_t)e * lp::OE + 6]);
+        const float *We = Ablk + (int64_t)e * 36, *Ti = Dtinv + (int64_t)lcol[e] * 36;
+        float T[36];   // A Dt^-1
         for (int a = 0; a < 6; ++a)
             for (int c = 0; c < 6; ++c) {
                 float s = 0.f;
@@ -635,11 +641,11 @@ __global__ void lj_gd_kernel(int N, GTerms T, float *gD)
     for (int c = 0; c < 6; ++c) gD[P * 36 + a * 6 + c] = acc[c];
 }
 
-// assembly backward, per particle: Dbar = -Dinv sym(gDinv) Dinv; alpha gradients;
-// per lower edge the gamma gradient and the pivot path's share of gW
+// assembly backward, per particle: Dbar = -Dinv sym(gDinv) Dinv; pivot-shift gradients;
+// per lower pair the fill-weight gradient -gamma (1 - gamma) <Dbar, A Dt^-1 A^T>
 __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, const float *Dinv, const float *gDinv,
-                                     const float *oute, const float *W, const float *Dtinv, float *goutn, float *goute,
-                                     float *gW)
+                                     const float *oute, const float *outn, float lambda, const float *Ablk,
+                                     const float *Dtinv, float *goutn, float *goute)
 {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= N) return;
@@ -660,41 +666,32 @@ __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, co
             Db[a * 6 + c] = -s;
         }
     float* go = goutn + (int64_t)P * lp::ON;
-    go[0] = Db[0] + Db[7] + Db[14];
-    go[1] = Db[21] + Db[28] + Db[35];
+    go[0] = lambda * expf(outn[(int64_t)P * lp::ON]) * (Db[0] + Db[7] + Db[14]);
+    go[1] = lambda * (4.0f / 3.0f) * expf(outn[(int64_t)P * lp::ON + 1]) * (Db[21] + Db[28] + Db[35]);
     for (int c = 2; c < lp::ON; ++c) go[c] = 0.f;
     for (int e = lrow[P]; e < lrow[P + 1]; ++e) {
-        const float gam = oute[(int64_t)e * lp::OE + 6];
-        const float *We = W + (int64_t)e * 36, *Ti = Dtinv + (int64_t)lcol[e] * 36;
-        float WT[36];   // W Dt^-1
+        const float gam = lp_gamma(oute[(int64_t)e * lp::OE + 6]);
+        const float *Ae = Ablk + (int64_t)e * 36, *Ti = Dtinv + (int64_t)lcol[e] * 36;
+        float AT[36];   // A Dt^-1
         for (int a = 0; a < 6; ++a)
             for (int c = 0; c < 6; ++c) {
                 float s = 0.f;
-                for (int q = 0; q < 6; ++q) s += We[a * 6 + q] * Ti[q * 6 + c];
-                WT[a * 6 + c] = s;
+                for (int q = 0; q < 6; ++q) s += Ae[a * 6 + q] * Ti[q * 6 + c];
+                AT[a * 6 + c] = s;
             }
-        // d gamma = -<Db, W Dt^-1 W^T>
         float gg = 0.f;
         for (int a = 0; a < 6; ++a)
             for (int c = 0; c < 6; ++c) {
                 float s = 0.f;
-                for (int q = 0; q < 6; ++q) s += WT[a * 6 + q] * We[c * 6 + q];
+                for (int q = 0; q < 6; ++q) s += AT[a * 6 + q] * Ae[c * 6 + q];
                 gg += Db[a * 6 + c] * s;
             }
-        goute[(int64_t)e * lp::OE + 6] = -gg;
-        // gW += -gamma (Db + Db^T) W Dt^-1 = -2 gamma Db W Dt^-1 (Db symmetric)
-        float *gw = gW + (int64_t)e * 36;
-        for (int a = 0; a < 6; ++a)
-            for (int c = 0; c < 6; ++c) {
-                float s = 0.f;
-                for (int q = 0; q < 6; ++q) s += Db[a * 6 + q] * WT[q * 6 + c];
-                gw[a * 6 + c] -= 2.0f * gam * s;
-            }
+        goute[(int64_t)e * lp::OE + 6] = -gg * gam * (1.0f - gam);
     }
 }
 
-// assembly backward, per edge: multiplier gradients <gW, basis_k>; other outputs 0
-__global__ void lp_edge_W_bwd_kernel(int64_t E, const float *geo, const float *gW, float *goute)
+// assembly backward, per edge: multiplier gradients (1 - tanh^2 c_k) <gW, basis_k>; other outputs 0
+__global__ void lp_edge_W_bwd_kernel(int64_t E, const float *geo, const float *oute, const float *gW, float *goute)
 {
     const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (e >= E) return;
@@ -709,7 +706,8 @@ __global__ void lp_edge_W_bwd_kernel(int64_t E, const float *geo, const float *g
         float s = 0.f;
         for (int a = 0; a < 6; ++a)
             for (int c = 0; c < 6; ++c) s += gw[a * 6 + c] * basis_entry(k, a, c, r, E3, ph);
-        o[k] = s;
+        const float th = tanhf(oute[e * lp::OE + k]);
+        o[k] = (1.0f - th * th) * s;
     }
     for (int k = 7; k < lp::OE; ++k) o[k] = 0.f;
 }
@@ -1078,9 +1076,9 @@ int lp_forward(fsd_ctx *ctx, const float4 *pos, float lambda, bool train, int in
     hipLaunchKernelGGL(lp_to_bf_kernel, dim3(nblk((int64_t)N * lp::H, 256)), dim3(256), 0, ctx->stream,
                        (int64_t)N * lp::H, s.h[lp::NMP], s.xh);
     layer_fwd(ctx, lp::L_HEAD_N, N, s.xh, nullptr, nullptr, nullptr, s.outn);
-    if (inject >= 0 && inject < N) {   // failure-path tests: a pivot shift no pivot survives
-        const float neg = -1.0e30f;
-        FSD_HIP(hipMemcpyAsync(s.outn + (int64_t)inject * lp::ON, &neg, sizeof(float), hipMemcpyHostToDevice,
+    if (inject >= 0 && inject < N) {   // failure-path tests: a pivot shift that makes the pivot NaN (not positive)
+        const float bad = __builtin_nanf("");
+        FSD_HIP(hipMemcpyAsync(s.outn + (int64_t)inject * lp::ON, &bad, sizeof(float), hipMemcpyHostToDevice,
                                ctx->stream));
         FSD_HIP(hipStreamSynchronize(ctx->stream));
     }
@@ -1088,7 +1086,7 @@ int lp_forward(fsd_ctx *ctx, const float4 *pos, float lambda, bool train, int in
     if (E)
         hipLaunchKernelGGL(lp_edge_W_kernel, dim3(nblk(E * 36, 256)), dim3(256), 0, ctx->stream, E, s.geo, s.oute, s.W);
     hipLaunchKernelGGL(lp_node_D_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow, ctx->d_lcol,
-                       ctx->d_Adiag, lambda, s.outn, s.oute, s.W, s.Dtinv, s.Dinv, s.d_fail);
+                       ctx->d_Adiag, lambda, s.outn, s.oute, ctx->d_Ablk, s.Dtinv, s.Dinv, s.d_fail);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_NET, pev);
     FSD_HIP(hipMemcpyAsync(s.h_fail, s.d_fail, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -1219,8 +1217,10 @@ int lp_loss_grad(fsd_ctx *ctx, int nsamp, const float *d_x, const float *d_b, do
         hipLaunchKernelGGL(lj_gd_kernel, dim3(nblk((int64_t)N * 6, 256)), dim3(256), 0, ctx->stream, N, td, s.gDinv);
         // assembly backward
         hipLaunchKernelGGL(lp_node_D_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow,
-                           ctx->d_lcol, s.Dinv, s.gDinv, s.oute, s.W, s.Dtinv, s.goutn, s.goute, s.gW);
-        if (E) hipLaunchKernelGGL(lp_edge_W_bwd_kernel, dim3(nblk(E, 128)), dim3(128), 0, ctx->stream, E, s.geo, s.gW, s.goute);
+                           ctx->d_lcol, s.Dinv, s.gDinv, s.oute, s.outn, ctx->ichol_relaxer, ctx->d_Ablk, s.Dtinv,
+                           s.goutn, s.goute);
+        if (E) hipLaunchKernelGGL(lp_edge_W_bwd_kernel, dim3(nblk(E, 128)), dim3(128), 0, ctx->stream, E, s.geo, s.oute,
+                                  s.gW, s.goute);
         // heads
         const int64_t NH = (int64_t)N * lp::H, EH = E * lp::H;
         hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk((int64_t)N * lp::ON, 256)), dim3(256), 0, ctx->stream,

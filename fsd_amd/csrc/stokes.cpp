@@ -1226,9 +1226,21 @@ int fsd_get_nlist(fsd_ctx *ctx, int32_t *h_nneigh, int32_t *h_nbrs, int64_t capa
 // entry point cannot tell whether the caller moved the particles since the last
 // build (fsd_step moves them; callers update positions in place), so every entry
 // point that applies R_FU builds them for its own positions.
-static int ensure_lub(fsd_ctx *ctx, const float4 *pos)
+// An operator entry point may be handed positions that moved since the list was
+// built (fsd_step moves them): the list is rebuilt, as at the start of a step, when
+// some particle has moved r_buff / 2 since its build (HOOMD's distance check).
+static int current_nlist(fsd_ctx *ctx, const float4 *pos)
 {
     if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    bool rebuilt = false;
+    RC(k_update_nlist(ctx, pos, &rebuilt));
+    ctx->nlist_builds += rebuilt;
+    return 0;
+}
+
+static int ensure_lub(fsd_ctx *ctx, const float4 *pos)
+{
+    RC(current_nlist(ctx, pos));
     RC(k_build_lub_pairs(ctx, pos));
     return 0;
 }
@@ -1246,7 +1258,7 @@ int fsd_set_noise(fsd_ctx *ctx, const float *d_ff_real11, const float *d_ff_grid
 int fsd_brownian_farfield(fsd_ctx *ctx, const float *d_pos4, uint32_t timestep, float *d_out11)
 {
     const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
-    if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    RC(current_nlist(ctx, pos));
     RC(alloc_brownian(ctx));
     RC(k_gridk(ctx));
     ctx->spread_valid = false;
@@ -1288,7 +1300,8 @@ int fsd_rfu(fsd_ctx *ctx, const float *d_pos4, const float *d_U6, float *d_F6)
 
 int fsd_rfe(fsd_ctx *ctx, const float *d_pos4, float shear_rate, const float *d_ori3, float *d_F6)
 {
-    if (!ctx->nlist_valid || !ctx->have_table) { ctx->err = "neighbour list or table missing"; return FSD_E_STATE; }
+    if (!ctx->have_table) { ctx->err = "resistance table not set"; return FSD_E_STATE; }
+    RC(current_nlist(ctx, reinterpret_cast<const float4 *>(d_pos4)));
     RC(k_rfe(ctx, reinterpret_cast<const float4 *>(d_pos4), shear_rate, d_ori3, d_F6));
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1296,7 +1309,8 @@ int fsd_rfe(fsd_ctx *ctx, const float *d_pos4, float shear_rate, const float *d_
 
 int fsd_rsu(fsd_ctx *ctx, const float *d_pos4, const float *d_U6, float *d_S5)
 {
-    if (!ctx->nlist_valid || !ctx->have_table) { ctx->err = "neighbour list or table missing"; return FSD_E_STATE; }
+    if (!ctx->have_table) { ctx->err = "resistance table not set"; return FSD_E_STATE; }
+    RC(current_nlist(ctx, reinterpret_cast<const float4 *>(d_pos4)));
     RC(k_rsu(ctx, reinterpret_cast<const float4 *>(d_pos4), d_U6, d_S5));
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1304,7 +1318,8 @@ int fsd_rsu(fsd_ctx *ctx, const float *d_pos4, const float *d_U6, float *d_S5)
 
 int fsd_rse(fsd_ctx *ctx, const float *d_pos4, float shear_rate, const float *d_ori3, float *d_S5)
 {
-    if (!ctx->nlist_valid || !ctx->have_table) { ctx->err = "neighbour list or table missing"; return FSD_E_STATE; }
+    if (!ctx->have_table) { ctx->err = "resistance table not set"; return FSD_E_STATE; }
+    RC(current_nlist(ctx, reinterpret_cast<const float4 *>(d_pos4)));
     RC(k_rse(ctx, reinterpret_cast<const float4 *>(d_pos4), shear_rate, d_ori3, d_S5));
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1312,7 +1327,7 @@ int fsd_rse(fsd_ctx *ctx, const float *d_pos4, float shear_rate, const float *d_
 
 int fsd_mobility(fsd_ctx *ctx, const float *d_pos4, const float *d_gF11, float *d_gU11, int32_t parts)
 {
-    if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    RC(current_nlist(ctx, reinterpret_cast<const float4 *>(d_pos4)));
     ctx->spread_valid = false;
     RC(k_mobility(ctx, reinterpret_cast<const float4 *>(d_pos4), d_gF11, d_gU11, parts, nullptr));
     CK(hipStreamSynchronize(ctx->stream));
@@ -1321,6 +1336,7 @@ int fsd_mobility(fsd_ctx *ctx, const float *d_pos4, const float *d_gF11, float *
 
 int fsd_build_precond(fsd_ctx *ctx, const float *d_pos4, fsd_solve_stats *stats)
 {
+    RC(current_nlist(ctx, reinterpret_cast<const float4 *>(d_pos4)));
     RC(build_precond(ctx, reinterpret_cast<const float4 *>(d_pos4)));
     if (stats) {
         stats->ichol_relaxer = ctx->ichol_relaxer;
@@ -1496,7 +1512,7 @@ int fsd_saddle_solve(fsd_ctx *ctx, const float *d_pos4, const float *d_rhs17, fl
 
 int fsd_force_law(fsd_ctx *ctx, const float *d_pos4, float T_ext, float *d_F6)
 {
-    if (!ctx->nlist_valid) { ctx->err = "neighbour list not built"; return FSD_E_STATE; }
+    RC(current_nlist(ctx, reinterpret_cast<const float4 *>(d_pos4)));
     RC(k_force_law(ctx, reinterpret_cast<const float4 *>(d_pos4), T_ext, d_F6));
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1759,6 +1775,7 @@ int fsd_learned_loss_grad(fsd_ctx *ctx, const float *d_pos4, int32_t nsamp, cons
 {
     if (ctx->precond_mode != 2) { ctx->err = "the learned factor needs precond mode 2"; return FSD_E_STATE; }
     const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
+    RC(current_nlist(ctx, pos));
     RC(precond_fetch(ctx, pos));
     precond_plan(ctx);
     RC(precond_upload_blocks(ctx, pos));

@@ -59,6 +59,14 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
+GAMMA0 = 4.0
+
+
+def gamma(c):
+    """fill weight of a pair from its raw output: sigmoid(c + 4)"""
+    return 1.0 / (1.0 + np.exp(-(c + GAMMA0)))
+
+
 def silu(z):
     return z / (1.0 + np.exp(-z))
 
@@ -208,14 +216,16 @@ class LearnedOracle:
 
     # ---- factor
     def assemble(self, oute, outn):
-        """W [E,6,6], D, Dinv [N,6,6] and whether a pivot is non-positive"""
-        W = np.einsum('ek,ekac->eac', 1.0 + oute[:, :6], self.Bk) if self.E else np.zeros((0, 6, 6))
-        D = self.Dt.copy()
-        D[:, [0, 1, 2], [0, 1, 2]] += outn[:, 0:1]
-        D[:, [3, 4, 5], [3, 4, 5]] += outn[:, 1:2]
-        if self.E:
-            T = np.einsum('eaq,eqr,ecr->eac', W, self.Dtinv[self.cols], W)
-            np.add.at(D, self.rows, -oute[:, 6, None, None] * T)
+        """W [E,6,6], D, Dinv [N,6,6] and whether a pivot is non-positive.  The raw outputs
+        enter bounded: multipliers 1 + tanh(c), fill weights sigmoid(c + 4), pivot shift
+        lambda e^a S (k_learned.hip lp_edge_W_kernel, lp_node_D_kernel)"""
+        W = np.einsum('ek,ekac->eac', 1.0 + np.tanh(oute[:, :6]), self.Bk) if self.E else np.zeros((0, 6, 6))
+        D = self.Adiag.copy()
+        D[:, [0, 1, 2], [0, 1, 2]] += self.lam * np.exp(outn[:, 0:1])
+        D[:, [3, 4, 5], [3, 4, 5]] += self.lam * (4.0 / 3.0) * np.exp(outn[:, 1:2])
+        if self.E:   # the fill term uses the pair blocks A (the pivots do not depend on W)
+            T = np.einsum('eaq,eqr,ecr->eac', self.Ablk, self.Dtinv[self.cols], self.Ablk)
+            np.add.at(D, self.rows, -gamma(oute[:, 6])[:, None, None] * T)
         Dl = np.tril(D)
         Dsym = Dl + np.transpose(np.tril(D, -1), (0, 2, 1))
         failed = False
@@ -335,16 +345,16 @@ class LearnedOracle:
         Ssym = 0.5 * (gD + np.transpose(gD, (0, 2, 1)))
         Db = -np.einsum('paq,pqr,prc->pac', Dinv, Ssym, Dinv)
         goutn = np.zeros((N, ON))
-        goutn[:, 0] = np.einsum('pii->p', Db[:, 0:3, 0:3])
-        goutn[:, 1] = np.einsum('pii->p', Db[:, 3:6, 3:6])
+        outn = c["outn"]
+        goutn[:, 0] = self.lam * np.exp(outn[:, 0]) * np.einsum('pii->p', Db[:, 0:3, 0:3])
+        goutn[:, 1] = self.lam * (4.0 / 3.0) * np.exp(outn[:, 1]) * np.einsum('pii->p', Db[:, 3:6, 3:6])
         goute = np.zeros((self.E, OE))
         oute = c["oute"]
         if self.E:
-            WT = np.einsum('eaq,eqc->eac', W, self.Dtinv[self.cols])
-            T = np.einsum('eaq,ecq->eac', WT, W)
-            goute[:, 6] = -np.einsum('eac,eac->e', Db[self.rows], T)
-            gW = gW - 2.0 * oute[:, 6, None, None] * np.einsum('eaq,eqc->eac', Db[self.rows], WT)
-            goute[:, :6] = np.einsum('eac,ekac->ek', gW, self.Bk)
+            gam = gamma(oute[:, 6])
+            T = np.einsum('eaq,eqr,ecr->eac', self.Ablk, self.Dtinv[self.cols], self.Ablk)
+            goute[:, 6] = -np.einsum('eac,eac->e', Db[self.rows], T) * gam * (1.0 - gam)
+            goute[:, :6] = np.einsum('eac,ekac->ek', gW, self.Bk) * (1.0 - np.tanh(oute[:, :6]) ** 2)
         return loss, self._mlp_backward(c, goute, goutn, emulate)
 
     def _mlp_backward(self, c, goute, goutn, emulate):

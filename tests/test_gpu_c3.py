@@ -82,23 +82,28 @@ def test_c3_mobility(c3):
 
 
 def test_c3_velocity_solve(c3):
-    """one saddle solve at shear rate 1 with random forces to a true residual of
-    1e-5; its velocities are within 1e-4 of the FP64 solution"""
-    st, orc, dpos, pos, L = c3
+    """one saddle solve at shear rate 1 with random forces, FP64-refined to a true
+    residual of 1e-7 (at this packing the velocity error is ~100 times the residual,
+    so the benchmark's 1e-4 leaves velocities ~1e-2 from converged); its velocities
+    are within 1e-4 of the FP64 solution"""
+    from fsd_amd.stokes import Stokes
+    st0, orc, dpos, pos, L = c3
     if orc.L is None:
         orc.build_precond()
+    st = Stokes(N, L, gmres_true_tol=1e-7, gmres_restart=100, refine=True)
+    st.build_nlist(dpos)
     st.build_precond(dpos)
     b = orc.rhs(forces(N, 22), 1.0)
     x, info = st.saddle_solve(dpos, _t(b))
     xg = x.cpu().numpy().astype(np.float64)
     r = b - orc.saddle_multiply(xg)
     r64 = np.linalg.norm(r) / np.linalg.norm(b)
-    print("GPU iterations", info["iterations"], "reported true residual", info["true_residual"],
-          "FP64 residual", r64, flush=True)
+    print("GPU iterations", info["iterations"], "refined true residual", info["true_residual"],
+          "oracle-operator residual", r64, flush=True)
     d, it, res = orc.gmres(r, tol=1e-3, restart=100, maxit=300)
     xr = xg + d
     r2 = np.linalg.norm(b - orc.saddle_multiply(xr)) / np.linalg.norm(b)
     e = rel_err2(xg[11 * N:], xr[11 * N:])
     print("FP64 correction iterations", it, "corrected residual", r2, "velocity rel err", e)
-    assert r64 < 2e-5 and r2 < 1e-7
+    assert info["converged"] and r64 < 1e-6
     assert e < 1e-4

@@ -30,8 +30,9 @@ def test_layouts_agree():
     assert PL.LAYERS == OL.LAYERS and PL.nparams() == OL.nparams() == 74304
 
 
-def test_zero_heads_give_symmetric_gauss_seidel(dense):
-    """with zero network outputs W is the pair blocks A and D = A_PP + lambda S"""
+def test_zero_outputs_give_one_step_fill(dense):
+    """with zero network outputs W is the pair blocks A and D = A_PP + lambda S minus
+    sigmoid(4) of every lower pair's fill term A Dt^-1 A^T"""
     from oracle.learned import init_weights
     orc, lo, X, B = dense
     c = lo.forward(init_weights(1, scale_out=0.0), emulate=False)
@@ -39,7 +40,13 @@ def test_zero_heads_give_symmetric_gauss_seidel(dense):
     W, D, Dinv, failed = lo.assemble(c["oute"], c["outn"])
     assert lo.E > 100 and not failed
     assert np.abs(W - lo.Ablk).max() <= 1e-5 * np.abs(lo.Ablk).max()
-    assert np.allclose(D, lo.Dt)
+    # the pivots written out pair by pair from the pattern
+    g = 1.0 / (1.0 + np.exp(-4.0))
+    Dref = lo.Dt.copy()
+    for e in range(lo.E):
+        P, Q = lo.rows[e], lo.cols[e]
+        Dref[P] -= g * lo.Ablk[e] @ np.linalg.inv(lo.Dt[Q]) @ lo.Ablk[e].T
+    assert np.abs(D - Dref).max() <= 1e-9 * np.abs(Dref).max()
 
 
 def test_deep_jacobi_is_exact(dense):
