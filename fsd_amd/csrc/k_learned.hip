@@ -59,10 +59,16 @@ __device__ __forceinline__ float silu_d(float z)
 }
 __device__ __forceinline__ float sgnlog(float v) { return copysignf(log1pf(fabsf(v)), v); }
 // the network's raw outputs enter the factor bounded: pair-coefficient multipliers
-// 1 + tanh(c) in (0, 2), fill weights sigmoid(c + 4) in (0, 1) (0.982 at c = 0: the
-// zero-output factor subtracts nearly the whole fill term), pivot shift lambda e^a S
-constexpr float LP_GAMMA0 = 4.0f;
-__device__ __forceinline__ float lp_gamma(float c) { return sigm(c + LP_GAMMA0); }
+// 1 + tanh(c) in (0, 2), fill weights 2 sigmoid(c + g0) in (0, 2) with g0 = ln(0.49 / 0.51)
+// (0.98 at c = 0: the zero-output factor subtracts nearly the whole one-step fill term;
+// IC(0)'s smaller corrected pivots subtract more), pivot shift lambda e^a S
+constexpr float LP_GAMMA0 = -0.040005334f;
+__device__ __forceinline__ float lp_gamma(float c) { return 2.0f * sigm(c + LP_GAMMA0); }
+__device__ __forceinline__ float lp_gamma_d(float c)
+{
+    const float sg = sigm(c + LP_GAMMA0);
+    return 2.0f * sg * (1.0f - sg);
+}
 
 // 6x6 symmetric positive definite inverse through Cholesky (lower triangle of D
 // read); false on a non-positive pivot
@@ -368,12 +374,12 @@ __global__ void lp_edge_W_kernel(int64_t E, const float *geo, const float *oute,
 }
 
 // D_P = A_PP + lambda diag(e^a_t, e^a_r) S - sum_{Q < P} gamma_e A_e Dt_Q^-1 A_e^T with
-// gamma_e = sigmoid(c_e + 4) and A_e the pair block; Dinv_P = D_P^-1, a non-positive pivot
+// gamma_e = 2 sigmoid(c_e + g0) and A_e the pair block; Dinv_P = D_P^-1, a non-positive pivot
 // sets fail bit 1.  The fill term uses the pair block, not W: the pivots then do not
 // depend on W, and (D + W) D^-1 (D + W)^T is positive definite for any W once D is.
 __global__ void lp_node_D_kernel(int N, const int *lrow, const int *lcol, const float *Adiag, float lambda,
                                  const float *outn, const float *oute, const float *Ablk, const float *Dtinv,
-                                 float *Dinv, int *fail)
+                                 float *Dinv, float *Dout, int *fail)
 {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= N) return;
@@ -404,6 +410,9 @@ __global__ void lp_node_D_kernel(int N, const int *lrow, const int *lcol, const 
         for (int q = 0; q < 36; ++q) Di[q] = 0.f;
     }
     for (int q = 0; q < 36; ++q) Dinv[(int64_t)P * 36 + q] = Di[q];
+    if (Dout)   // the pivot as used (lower triangle mirrored)
+        for (int a = 0; a < 6; ++a)
+            for (int c = 0; c < 6; ++c) Dout[(int64_t)P * 36 + a * 6 + c] = (c <= a) ? D[a * 6 + c] : D[c * 6 + a];
 }
 
 // --------------------------------------------------- block Jacobi application
@@ -643,9 +652,10 @@ __global__ void lj_gd_kernel(int N, GTerms T, float *gD)
 
 // assembly backward, per particle: Dbar = -Dinv sym(gDinv) Dinv; pivot-shift gradients;
 // per lower pair the fill-weight gradient -gamma (1 - gamma) <Dbar, A Dt^-1 A^T>
+// (dbar_direct: gDinv already holds the gradient with respect to D itself, symmetric)
 __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, const float *Dinv, const float *gDinv,
-                                     const float *oute, const float *outn, float lambda, const float *Ablk,
-                                     const float *Dtinv, float *goutn, float *goute)
+                                     int dbar_direct, const float *oute, const float *outn, float lambda,
+                                     const float *Ablk, const float *Dtinv, float *goutn, float *goute)
 {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= N) return;
@@ -653,24 +663,27 @@ __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, co
     float S[36], T[36], Db[36];
     for (int a = 0; a < 6; ++a)
         for (int c = 0; c < 6; ++c) S[a * 6 + c] = 0.5f * (G[a * 6 + c] + G[c * 6 + a]);
-    for (int a = 0; a < 6; ++a)
-        for (int c = 0; c < 6; ++c) {
-            float s = 0.f;
-            for (int q = 0; q < 6; ++q) s += Di[a * 6 + q] * S[q * 6 + c];
-            T[a * 6 + c] = s;
-        }
-    for (int a = 0; a < 6; ++a)
-        for (int c = 0; c < 6; ++c) {
-            float s = 0.f;
-            for (int q = 0; q < 6; ++q) s += T[a * 6 + q] * Di[q * 6 + c];
-            Db[a * 6 + c] = -s;
-        }
+    if (dbar_direct) {
+        for (int q = 0; q < 36; ++q) Db[q] = S[q];
+    } else {
+        for (int a = 0; a < 6; ++a)
+            for (int c = 0; c < 6; ++c) {
+                float s = 0.f;
+                for (int q = 0; q < 6; ++q) s += Di[a * 6 + q] * S[q * 6 + c];
+                T[a * 6 + c] = s;
+            }
+        for (int a = 0; a < 6; ++a)
+            for (int c = 0; c < 6; ++c) {
+                float s = 0.f;
+                for (int q = 0; q < 6; ++q) s += T[a * 6 + q] * Di[q * 6 + c];
+                Db[a * 6 + c] = -s;
+            }
+    }
     float* go = goutn + (int64_t)P * lp::ON;
     go[0] = lambda * expf(outn[(int64_t)P * lp::ON]) * (Db[0] + Db[7] + Db[14]);
     go[1] = lambda * (4.0f / 3.0f) * expf(outn[(int64_t)P * lp::ON + 1]) * (Db[21] + Db[28] + Db[35]);
     for (int c = 2; c < lp::ON; ++c) go[c] = 0.f;
     for (int e = lrow[P]; e < lrow[P + 1]; ++e) {
-        const float gam = lp_gamma(oute[(int64_t)e * lp::OE + 6]);
         const float *Ae = Ablk + (int64_t)e * 36, *Ti = Dtinv + (int64_t)lcol[e] * 36;
         float AT[36];   // A Dt^-1
         for (int a = 0; a < 6; ++a)
@@ -686,7 +699,7 @@ __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, co
                 for (int q = 0; q < 6; ++q) s += AT[a * 6 + q] * Ae[c * 6 + q];
                 gg += Db[a * 6 + c] * s;
             }
-        goute[(int64_t)e * lp::OE + 6] = -gg * gam * (1.0f - gam);
+        goute[(int64_t)e * lp::OE + 6] = -gg * lp_gamma_d(oute[(int64_t)e * lp::OE + 6]);
     }
 }
 
@@ -787,6 +800,71 @@ __global__ void lp_pack_kernel(int ki, int no, const float *w, uint16_t *wt, uin
     const uint16_t b = f2bf(w[q]);
     wn[q] = b;
     wt[n * ki + k] = b;
+}
+
+// ------------------------------------------------- supervised factor targets
+// IC(0)'s own block LDL^T: pivots D_P = L_PP L_PP^T, pair blocks W_PQ = L_PQ L_QQ^T.
+// loss += |D_P - D_P^IC|^2 / (N |D_P^IC|^2) and gDbar = its gradient with respect to D_P
+__global__ void lp_pivot_target_kernel(int N, const float *Ldiag, const float *D, double *loss, float *gD)
+{
+    __shared__ double red[128];
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (P < N) {
+        const float *Lp = Ldiag + (int64_t)P * 36, *Dp = D + (int64_t)P * 36;
+        float T[36];
+        float nt = 0.f;
+        for (int a = 0; a < 6; ++a)
+            for (int c = 0; c < 6; ++c) {
+                float v = 0.f;
+                for (int q = 0; q <= (a < c ? a : c); ++q) v += Lp[a * 6 + q] * Lp[c * 6 + q];
+                T[a * 6 + c] = v;
+                nt += v * v;
+            }
+        const float w = 1.0f / ((float)N * fmaxf(nt, 1e-30f));
+        for (int q = 0; q < 36; ++q) {
+            const float d = Dp[q] - T[q];
+            l += (double)w * d * d;
+            gD[(int64_t)P * 36 + q] = 2.0f * w * d;
+        }
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 64; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
+// loss += |W_e - W_e^IC|^2 / (E |A_e|^2) and gW = its gradient, e = (P, Q)
+__global__ void lp_pair_target_kernel(int64_t E, const int *lcol, const float *Lblk, const float *Ldiag,
+                                      const float *Ablk, const float *W, double *loss, float *gW)
+{
+    __shared__ double red[128];
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (e < E) {
+        const float *Le = Lblk + e * 36, *Lq = Ldiag + (int64_t)lcol[e] * 36, *Ae = Ablk + e * 36, *We = W + e * 36;
+        float na = 0.f;
+        for (int q = 0; q < 36; ++q) na += Ae[q] * Ae[q];
+        const float w = 1.0f / ((float)E * fmaxf(na, 1e-30f));
+        for (int a = 0; a < 6; ++a)
+            for (int c = 0; c < 6; ++c) {
+                float t = 0.f;   // (L_PQ L_QQ^T)_ac, L_QQ lower triangular
+                for (int q = 0; q <= c; ++q) t += Le[a * 6 + q] * Lq[c * 6 + q];
+                const float d = We[a * 6 + c] - t;
+                l += (double)w * d * d;
+                gW[e * 36 + a * 6 + c] = 2.0f * w * d;
+            }
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 64; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
 }
 
 // =================================================================== host side
@@ -937,6 +1015,7 @@ int lp_init(fsd_ctx *ctx)
     LRC(lalloc(ctx, s.xh, N * lp::H));
     LRC(lalloc(ctx, s.outn, N * lp::ON));
     LRC(lalloc(ctx, s.Dinv, N * 36));
+    LRC(lalloc(ctx, s.D, N * 36));
     LRC(lalloc(ctx, s.Dtinv, N * 36));
     LRC(lalloc(ctx, s.gh, N * lp::H));
     LRC(lalloc(ctx, s.gh2, N * lp::H));
@@ -958,7 +1037,7 @@ void lp_free(fsd_ctx *ctx)
 {
     if (!ctx->lp) return;
     LearnedState &s = *ctx->lp;
-    void *ptrs[] = {s.d_w, s.d_grad, s.d_m, s.d_v, s.d_wn, s.xn0, s.zn0, s.agg, s.xh, s.outn, s.Dinv, s.Dtinv,
+    void *ptrs[] = {s.d_w, s.d_grad, s.d_m, s.d_v, s.d_wn, s.xn0, s.zn0, s.agg, s.xh, s.outn, s.Dinv, s.Dtinv, s.D,
                     s.xe0, s.ze0, s.oute, s.geo, s.W, s.erow, s.d_fail, s.gh, s.gh2, s.ge, s.gm, s.gzn, s.gze, s.gxe,
                     s.gxn, s.goute, s.goutn, s.gW, s.gDinv, s.gbf, s.part, s.jv, s.d_loss};
     for (void *p : ptrs)
@@ -1086,7 +1165,7 @@ int lp_forward(fsd_ctx *ctx, const float4 *pos, float lambda, bool train, int in
     if (E)
         hipLaunchKernelGGL(lp_edge_W_kernel, dim3(nblk(E * 36, 256)), dim3(256), 0, ctx->stream, E, s.geo, s.oute, s.W);
     hipLaunchKernelGGL(lp_node_D_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow, ctx->d_lcol,
-                       ctx->d_Adiag, lambda, s.outn, s.oute, ctx->d_Ablk, s.Dtinv, s.Dinv, s.d_fail);
+                       ctx->d_Adiag, lambda, s.outn, s.oute, ctx->d_Ablk, s.Dtinv, s.Dinv, s.D, s.d_fail);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_NET, pev);
     FSD_HIP(hipMemcpyAsync(s.h_fail, s.d_fail, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -1132,6 +1211,69 @@ int lp_apply(fsd_ctx *ctx, const float *b0, const float *b1, const float *bsaddl
     hipLaunchKernelGGL(lj_post_kernel, g, b, 0, ctx->stream, N, ctx->d_perm, xc, bsaddle, x, mode);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_TRSV, ev);
+    return 0;
+}
+
+// Backward of the factor assembly and the network: from gW (pair blocks) and gDinv
+// (inverse pivots, or the pivots themselves when dbar_direct) to every parameter's gradient
+static int lp_backward(fsd_ctx *ctx, bool dbar_direct)
+{
+    LearnedState &s = *ctx->lp;
+    const int N = ctx->N;
+    const int64_t E = ctx->pch.nlow;
+    // assembly backward
+    hipLaunchKernelGGL(lp_node_D_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow,
+                       ctx->d_lcol, s.Dinv, s.gDinv, dbar_direct ? 1 : 0, s.oute, s.outn, ctx->ichol_relaxer,
+                       ctx->d_Ablk, s.Dtinv, s.goutn, s.goute);
+    if (E) hipLaunchKernelGGL(lp_edge_W_bwd_kernel, dim3(nblk(E, 128)), dim3(128), 0, ctx->stream, E, s.geo, s.oute,
+                              s.gW, s.goute);
+    // heads
+    const int64_t NH = (int64_t)N * lp::H, EH = E * lp::H;
+    hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk((int64_t)N * lp::ON, 256)), dim3(256), 0, ctx->stream,
+                       (int64_t)N * lp::ON, s.goutn, (const float *)nullptr, s.gzn, s.gbf);
+    LRC(layer_bwd(ctx, lp::L_HEAD_N, N, s.xh, s.gzn, s.gbf, s.gh));
+    if (E) {
+        hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk(E * lp::OE, 256)), dim3(256), 0, ctx->stream, E * lp::OE,
+                           s.goute, (const float *)nullptr, s.gze, s.gbf);
+    }
+    LRC(layer_bwd(ctx, lp::L_HEAD_E, E, s.xm[lp::NMP], s.gze, s.gbf, s.gxe));
+    if (E) {
+        hipLaunchKernelGGL(lp_scatter_h_kernel, dim3(nblk(NH, 256)), dim3(256), 0, ctx->stream, N, ctx->d_lrow,
+                           ctx->d_urow, ctx->d_upos, s.gxe, s.gh, s.gh2);
+        std::swap(s.gh, s.gh2);
+        hipLaunchKernelGGL(lp_edge_resid_kernel, dim3(nblk(EH, 256)), dim3(256), 0, ctx->stream, E, s.gxe,
+                           (const float *)nullptr, s.ge);
+    }
+    for (int l = lp::NMP - 1; l >= 0; --l) {
+        // update: h_l+1 = h_l + silu([h_l | agg_l] Wu + b)
+        hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk(NH, 256)), dim3(256), 0, ctx->stream, NH, s.gh, s.zu[l],
+                           s.gzn, s.gbf);
+        LRC(layer_bwd(ctx, lp::L_UPD + 2 * l, N, s.xu[l], s.gzn, s.gbf, s.gxn));
+        hipLaunchKernelGGL(lp_gh_upd_kernel, dim3(nblk(NH, 256)), dim3(256), 0, ctx->stream, N, s.gh, s.gxn, s.gh2);
+        std::swap(s.gh, s.gh2);
+        if (!E) continue;
+        // messages: m_l at both end particles' aggregates, and e_l+1 = e_l + m_l
+        hipLaunchKernelGGL(lp_gm_kernel, dim3(nblk(EH, 256)), dim3(256), 0, ctx->stream, E, s.erow, ctx->d_lcol,
+                           s.ge, s.gxn, s.gm);
+        hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk(EH, 256)), dim3(256), 0, ctx->stream, EH, s.gm, s.zm[l],
+                           s.gze, s.gbf);
+        LRC(layer_bwd(ctx, lp::L_MSG + 2 * l, E, s.xm[l], s.gze, s.gbf, s.gxe));
+        hipLaunchKernelGGL(lp_scatter_h_kernel, dim3(nblk(NH, 256)), dim3(256), 0, ctx->stream, N, ctx->d_lrow,
+                           ctx->d_urow, ctx->d_upos, s.gxe, s.gh, s.gh2);
+        std::swap(s.gh, s.gh2);
+        hipLaunchKernelGGL(lp_edge_resid_kernel, dim3(nblk(EH, 256)), dim3(256), 0, ctx->stream, E, s.gxe, s.ge,
+                           s.gm);
+        std::swap(s.ge, s.gm);
+    }
+    // encoders
+    hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk(NH, 256)), dim3(256), 0, ctx->stream, NH, s.gh, s.zn0, s.gzn,
+                       s.gbf);
+    LRC(layer_bwd(ctx, lp::L_ENC_N, N, s.xn0, s.gzn, s.gbf, nullptr));
+    if (E)
+        hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk(EH, 256)), dim3(256), 0, ctx->stream, EH, s.ge, s.ze0,
+                           s.gze, s.gbf);
+    LRC(layer_bwd(ctx, lp::L_ENC_E, E, s.xe0, s.gze, s.gbf, nullptr));
+        FSD_HIP(hipGetLastError());
     return 0;
 }
 
@@ -1215,60 +1357,32 @@ int lp_loss_grad(fsd_ctx *ctx, int nsamp, const float *d_x, const float *d_b, do
     if (grad) {
         if (E) hipLaunchKernelGGL(lj_gw_kernel, dim3(nblk(E * 6, 256)), dim3(256), 0, ctx->stream, E, s.erow, ctx->d_lcol, tw, s.gW);
         hipLaunchKernelGGL(lj_gd_kernel, dim3(nblk((int64_t)N * 6, 256)), dim3(256), 0, ctx->stream, N, td, s.gDinv);
-        // assembly backward
-        hipLaunchKernelGGL(lp_node_D_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow,
-                           ctx->d_lcol, s.Dinv, s.gDinv, s.oute, s.outn, ctx->ichol_relaxer, ctx->d_Ablk, s.Dtinv,
-                           s.goutn, s.goute);
-        if (E) hipLaunchKernelGGL(lp_edge_W_bwd_kernel, dim3(nblk(E, 128)), dim3(128), 0, ctx->stream, E, s.geo, s.oute,
-                                  s.gW, s.goute);
-        // heads
-        const int64_t NH = (int64_t)N * lp::H, EH = E * lp::H;
-        hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk((int64_t)N * lp::ON, 256)), dim3(256), 0, ctx->stream,
-                           (int64_t)N * lp::ON, s.goutn, (const float *)nullptr, s.gzn, s.gbf);
-        LRC(layer_bwd(ctx, lp::L_HEAD_N, N, s.xh, s.gzn, s.gbf, s.gh));
-        if (E) {
-            hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk(E * lp::OE, 256)), dim3(256), 0, ctx->stream, E * lp::OE,
-                               s.goute, (const float *)nullptr, s.gze, s.gbf);
-        }
-        LRC(layer_bwd(ctx, lp::L_HEAD_E, E, s.xm[lp::NMP], s.gze, s.gbf, s.gxe));
-        if (E) {
-            hipLaunchKernelGGL(lp_scatter_h_kernel, dim3(nblk(NH, 256)), dim3(256), 0, ctx->stream, N, ctx->d_lrow,
-                               ctx->d_urow, ctx->d_upos, s.gxe, s.gh, s.gh2);
-            std::swap(s.gh, s.gh2);
-            hipLaunchKernelGGL(lp_edge_resid_kernel, dim3(nblk(EH, 256)), dim3(256), 0, ctx->stream, E, s.gxe,
-                               (const float *)nullptr, s.ge);
-        }
-        for (int l = lp::NMP - 1; l >= 0; --l) {
-            // update: h_l+1 = h_l + silu([h_l | agg_l] Wu + b)
-            hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk(NH, 256)), dim3(256), 0, ctx->stream, NH, s.gh, s.zu[l],
-                               s.gzn, s.gbf);
-            LRC(layer_bwd(ctx, lp::L_UPD + 2 * l, N, s.xu[l], s.gzn, s.gbf, s.gxn));
-            hipLaunchKernelGGL(lp_gh_upd_kernel, dim3(nblk(NH, 256)), dim3(256), 0, ctx->stream, N, s.gh, s.gxn, s.gh2);
-            std::swap(s.gh, s.gh2);
-            if (!E) continue;
-            // messages: m_l at both end particles' aggregates, and e_l+1 = e_l + m_l
-            hipLaunchKernelGGL(lp_gm_kernel, dim3(nblk(EH, 256)), dim3(256), 0, ctx->stream, E, s.erow, ctx->d_lcol,
-                               s.ge, s.gxn, s.gm);
-            hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk(EH, 256)), dim3(256), 0, ctx->stream, EH, s.gm, s.zm[l],
-                               s.gze, s.gbf);
-            LRC(layer_bwd(ctx, lp::L_MSG + 2 * l, E, s.xm[l], s.gze, s.gbf, s.gxe));
-            hipLaunchKernelGGL(lp_scatter_h_kernel, dim3(nblk(NH, 256)), dim3(256), 0, ctx->stream, N, ctx->d_lrow,
-                               ctx->d_urow, ctx->d_upos, s.gxe, s.gh, s.gh2);
-            std::swap(s.gh, s.gh2);
-            hipLaunchKernelGGL(lp_edge_resid_kernel, dim3(nblk(EH, 256)), dim3(256), 0, ctx->stream, E, s.gxe, s.ge,
-                               s.gm);
-            std::swap(s.ge, s.gm);
-        }
-        // encoders
-        hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk(NH, 256)), dim3(256), 0, ctx->stream, NH, s.gh, s.zn0, s.gzn,
-                           s.gbf);
-        LRC(layer_bwd(ctx, lp::L_ENC_N, N, s.xn0, s.gzn, s.gbf, nullptr));
-        if (E)
-            hipLaunchKernelGGL(lp_act_bwd_kernel, dim3(nblk(EH, 256)), dim3(256), 0, ctx->stream, EH, s.ge, s.ze0,
-                               s.gze, s.gbf);
-        LRC(layer_bwd(ctx, lp::L_ENC_E, E, s.xe0, s.gze, s.gbf, nullptr));
+        LRC(lp_backward(ctx, false));
     }
     FSD_HIP(hipGetLastError());
+    FSD_HIP(hipMemcpyAsync(s.h_loss, s.d_loss, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FSD_HIP(hipStreamSynchronize(ctx->stream));
+    *loss = s.h_loss[0];
+    return 0;
+}
+
+// Supervised factor loss: the learned factor's pivots and pair blocks against IC(0)'s
+// block LDL^T on the same pattern (d_Lblk, d_Ldiag must hold IC(0)'s factor), relative
+// errors averaged over particles and over pairs; with grad, every parameter's gradient.
+// lp_forward(train = true) must have run.
+int lp_factor_loss_grad(fsd_ctx *ctx, double *loss, bool grad)
+{
+    LearnedState &s = *ctx->lp;
+    const int N = ctx->N;
+    const int64_t E = ctx->pch.nlow;
+    FSD_HIP(hipMemsetAsync(s.d_loss, 0, 2 * sizeof(double), ctx->stream));
+    hipLaunchKernelGGL(lp_pivot_target_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_Ldiag, s.D,
+                       s.d_loss, s.gDinv);
+    if (E)
+        hipLaunchKernelGGL(lp_pair_target_kernel, dim3(nblk(E, 128)), dim3(128), 0, ctx->stream, E, ctx->d_lcol,
+                           ctx->d_Lblk, ctx->d_Ldiag, ctx->d_Ablk, s.W, s.d_loss, s.gW);
+    FSD_HIP(hipGetLastError());
+    if (grad) LRC(lp_backward(ctx, true));
     FSD_HIP(hipMemcpyAsync(s.h_loss, s.d_loss, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     FSD_HIP(hipStreamSynchronize(ctx->stream));
     *loss = s.h_loss[0];

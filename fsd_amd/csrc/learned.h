@@ -44,7 +44,7 @@ struct LearnedState {
     // particle buffers
     uint16_t *xn0 = nullptr, *xu[lp::NMP] = {}, *xh = nullptr;
     float *zn0 = nullptr, *h[lp::NMP + 1] = {}, *zu[lp::NMP] = {}, *agg = nullptr, *outn = nullptr;
-    float *Dinv = nullptr, *Dtinv = nullptr;
+    float *Dinv = nullptr, *Dtinv = nullptr, *D = nullptr;
     // edge buffers
     uint16_t *xe0 = nullptr, *xm[lp::NMP + 1] = {};
     float *ze0 = nullptr, *e[lp::NMP + 1] = {}, *zm[lp::NMP] = {}, *m[lp::NMP] = {}, *oute = nullptr;

@@ -1799,6 +1799,38 @@ int fsd_learned_export(fsd_ctx *ctx, float *h_oute, float *h_outn, float *h_W, f
     return lp_export(ctx, h_oute, h_outn, h_W, h_Dinv);
 }
 
+int fsd_learned_factor_loss(fsd_ctx *ctx, const float *d_pos4, int32_t want_grad, double *loss)
+{
+    if (ctx->precond_mode != 2) { ctx->err = "the learned factor needs precond mode 2"; return FSD_E_STATE; }
+    const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
+    RC(current_nlist(ctx, pos));
+    RC(precond_fetch(ctx, pos));
+    precond_plan(ctx);
+    RC(precond_upload_blocks(ctx, pos));
+    prof_end(ctx, PROF_PRECOND_BUILD, ctx->pc_ev);
+    // the targets: IC(0) with the relaxer doubling, on this pattern
+    ctx->ichol_attempts = 0;
+    float lambda = ctx->ichol_relaxer;
+    for (;;) {
+        int failed = 0;
+        RC(k_ichol(ctx, lambda, &failed));
+        ctx->ichol_attempts++;
+        if (!failed) break;
+        lambda *= 2.0f;
+        if (lambda > 1.0e7f) { ctx->err = "IC(0) did not succeed for any relaxer"; return FSD_E_ICHOL; }
+    }
+    int failed = 0;
+    RC(lp_forward(ctx, pos, lambda, true, -1, &failed));
+    lp_set_active(ctx, false);
+    ctx->precond_valid = false;
+    if (failed) {
+        ctx->err = "learned factor has a non-positive pivot";
+        return FSD_W_PIVOT;
+    }
+    RC(lp_factor_loss_grad(ctx, loss, want_grad != 0));
+    return 0;
+}
+
 int fsd_learned_adam(fsd_ctx *ctx, float lr, int32_t step)
 {
     if (step < 1) { ctx->err = "Adam step counts from 1"; return FSD_E_PARAM; }

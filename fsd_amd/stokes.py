@@ -335,6 +335,15 @@ class Stokes:
         self._check(rc)
         return None if rc == _lib.FSD_W_PIVOT else loss.value
 
+    def learned_factor_loss(self, pos, want_grad=True):
+        """supervised loss of the learned factor against IC(0)'s at these positions (and its
+        gradient, learned_get_weights(grad=True)); None on a non-positive pivot"""
+        loss = C.c_double(0.0)
+        self._sync()
+        rc = self.lib.fsd_learned_factor_loss(self.ctx, _ptr(pos), 1 if want_grad else 0, C.byref(loss))
+        self._check(rc)
+        return None if rc == _lib.FSD_W_PIVOT else loss.value
+
     def learned_adam_state(self, m=None, v=None):
         """get (no arguments) or set the Adam moments"""
         n = self.learned_nparams()

@@ -313,6 +313,12 @@ int fsd_learned_export(fsd_ctx *ctx, float *h_oute, float *h_outn, float *h_W, f
  * the gradient (fsd_learned_get_grad).  Returns FSD_W_PIVOT on a non-positive pivot. */
 int fsd_learned_loss_grad(fsd_ctx *ctx, const float *d_pos4, int32_t nsamp, const float *d_x17, const float *d_b17,
                           int32_t want_grad, double *loss);
+/* Supervised factor loss: the learned factor's pivots D_P and pair blocks W_PQ against
+ * IC(0)'s own block LDL^T on the same pattern (L_PP L_PP^T and L_PQ L_QQ^T), as the
+ * relative squared errors averaged over particles plus those averaged over pairs
+ * (pair errors relative to the pair block A).  Builds the pattern and IC(0) at d_pos4;
+ * want_grad != 0 also forms the gradient.  Returns FSD_W_PIVOT on a non-positive pivot. */
+int fsd_learned_factor_loss(fsd_ctx *ctx, const float *d_pos4, int32_t want_grad, double *loss);
 /* one Adam step (Kingma & Ba 2015; beta 0.9, 0.999) on the last gradient */
 int fsd_learned_adam(fsd_ctx *ctx, float lr, int32_t step);
 

@@ -20,6 +20,7 @@
 #pragma STDC FP_CONTRACT OFF
 
 void fsd_ewald_self(float xi, float self_out[2]);  /* shared table code */
+void fsd_ewald_functions(double r, double xi, double out[7]);
 
 /* ------------------------------------------------------------------ box */
 /* HOOMD BoxDim::minImage (device form, rint rounding). */
@@ -1338,3 +1339,174 @@ void orc_rk2_stage2(int N, double c1, const float *pos1, const float *ori1, doub
 }
 
 int orc_threads(void) { return omp_get_max_threads(); }
+
+/* ------------------------------------------- direct (spectrally converged) Ewald sum */
+/* The FTS grand mobility of the periodic RPY suspension without the PSE
+ * discretisation: the same Ewald splitting as Mobility.cu:1345-1542 (real-space
+ * functions of Stokes.cc:412-666, wave-space operator of Mobility.cu:315-479), but
+ * the real-space functions evaluated in FP64 at every pair and periodic image within
+ * r_cut, the wave-space part summed directly over every reciprocal vector with
+ * |k| <= k_cut (structure factors, no Gaussian gridding, no FFT), the exact 2 pi,
+ * and self terms in double.  With erfc(xi r_cut) and exp(-k_cut^2 / 4 xi^2) below
+ * ~1e-14 the result is the Ewald sum itself, independent of xi: the reference
+ * against which the PSE's method error is measured. */
+static void ewald_self_double(double xi, double s[2])
+{
+    const double pi = M_PI, sp = sqrt(pi);
+    s[0] = (1. + 4. * sp * xi * erfc(2. * xi) - exp(-4. * xi * xi)) / (4. * sp * xi);
+    s[1] = -3. * erfc(2. * xi) / 10. - 3. / (80. * sp * xi * xi * xi) - 9. / (40. * sp * xi)
+           + 3. * exp(-4. * xi * xi) * (1. + 10. * xi * xi) / (80. * sp * xi * xi * xi);
+}
+
+void orc_mobility_direct(int N, const float *pos4, const orc_box *b, double xi, double r_cut, double k_cut,
+                         const double *gF, double *gU, int parts)
+{
+    double *F3 = malloc(sizeof(double) * 3 * N), *C8 = malloc(sizeof(double) * 8 * N);
+    double *u3 = calloc(3 * (size_t)N, sizeof(double)), *D8 = calloc(8 * (size_t)N, sizeof(double));
+    for (int i = 0; i < N; ++i) {
+        for (int a = 0; a < 3; ++a) F3[3 * i + a] = gF[6 * i + a];
+        ts2c(gF + 6 * i + 3, gF + 6 * (size_t)N + 5 * i, C8 + 8 * i);
+    }
+    const double Lx = b->Lx, Ly = b->Ly, Lz = b->Lz, xy = b->xy;
+    const double V = Lx * Ly * Lz;
+    if (parts & 1) {
+        double self[2];
+        ewald_self_double(xi, self);
+        /* periodic images n1 a1 + n2 a2 + n3 a3, a1 = (Lx,0,0), a2 = (xy Ly, Ly, 0), a3 = (0,0,Lz) */
+        const int n3m = (int)ceil(r_cut / Lz) + 1, n2m = (int)ceil(r_cut / Ly) + 1;
+        const int n1m = (int)ceil((r_cut + fabs(xy) * Ly * n2m) / Lx) + 1;
+#pragma omp parallel for schedule(dynamic, 8)
+        for (int i = 0; i < N; ++i) {
+            const double *Fi = F3 + 3 * i, *Ci = C8 + 8 * i;
+            double u[3], D[8];
+            u[0] = self[0] * Fi[0]; u[1] = self[0] * Fi[1]; u[2] = self[0] * Fi[2];
+            D[0] = self[1] * (Ci[0] - 4. * Ci[0]); D[1] = self[1] * (Ci[1] - 4. * Ci[5]);
+            D[2] = self[1] * (Ci[2] - 4. * Ci[6]); D[3] = self[1] * (Ci[3] - 4. * Ci[7]);
+            D[4] = self[1] * (Ci[4] - 4. * Ci[4]); D[5] = self[1] * (Ci[5] - 4. * Ci[1]);
+            D[6] = self[1] * (Ci[6] - 4. * Ci[2]); D[7] = self[1] * (Ci[7] - 4. * Ci[3]);
+            for (int j = 0; j < N; ++j) {
+                const double d0[3] = {(double)pos4[4 * i] - pos4[4 * j], (double)pos4[4 * i + 1] - pos4[4 * j + 1],
+                                      (double)pos4[4 * i + 2] - pos4[4 * j + 2]};
+                for (int n3 = -n3m; n3 <= n3m; ++n3)
+                    for (int n2 = -n2m; n2 <= n2m; ++n2)
+                        for (int n1 = -n1m; n1 <= n1m; ++n1) {
+                            if (i == j && n1 == 0 && n2 == 0 && n3 == 0) continue;
+                            const double R0[3] = {d0[0] + n1 * Lx + n2 * xy * Ly, d0[1] + n2 * Ly, d0[2] + n3 * Lz};
+                            const double r2 = R0[0] * R0[0] + R0[1] * R0[1] + R0[2] * R0[2];
+                            if (r2 >= r_cut * r_cut || r2 == 0.0) continue;
+                            const double dist = sqrt(r2);
+                            double fv[7];
+                            fsd_ewald_functions(dist, xi, fv);
+                            /* the table's stored UC entries are g1 / 2 and -g2 / 2, used sign-flipped */
+                            const double f1 = fv[0], f2 = fv[1], g1 = -fv[2] / 2, g2 = fv[3] / 2;
+                            const double h1 = fv[4], h2 = fv[5], h3 = fv[6];
+                            const double R[3] = {R0[0] / dist, R0[1] / dist, R0[2] / dist};
+                            const double *Fj = F3 + 3 * j, *Cj = C8 + 8 * j;
+                            const double rdotf = dot3(R, Fj);
+                            const double Cm[3][3] = {{Cj[0], Cj[1], Cj[2]}, {Cj[5], Cj[4], Cj[3]},
+                                                     {Cj[6], Cj[7], -(Cj[0] + Cj[4])}};
+                            double Cdr[3], rdC[3];
+                            for (int a = 0; a < 3; ++a) {
+                                Cdr[a] = Cm[a][0] * R[0] + Cm[a][1] * R[1] + Cm[a][2] * R[2];
+                                rdC[a] = R[0] * Cm[0][a] + R[1] * Cm[1][a] + R[2] * Cm[2][a];
+                            }
+                            const double rCr = dot3(R, Cdr);
+                            for (int a = 0; a < 3; ++a) {
+                                u[a] += f1 * Fj[a] + (f2 - f1) * rdotf * R[a];
+                                u[a] += g1 * (Cdr[a] - rCr * R[a]) + g2 * (rdC[a] - 4. * rCr * R[a]);
+                            }
+                            static const int A[8] = {0, 0, 0, 1, 1, 1, 2, 2};
+                            static const int B[8] = {0, 1, 2, 2, 1, 0, 0, 1};
+                            static const int Cs[8] = {0, 5, 6, 7, 4, 1, 2, 3};
+                            for (int c = 0; c < 8; ++c) {
+                                const int a = A[c], bb = B[c];
+                                const double del = (a == bb) ? 1.0 : 0.0;
+                                D[c] += -g1 * (R[a] * Fj[bb] - rdotf * R[a] * R[bb])
+                                        - g2 * (del * rdotf + Fj[a] * R[bb] - 4. * rdotf * R[a] * R[bb]);
+                                D[c] += h1 * (Cj[c] - 4. * Cj[Cs[c]]) + h2 * (R[a] * Cdr[bb] - rCr * R[a] * R[bb])
+                                        + h3 * (del * rCr + Cdr[a] * R[bb] + R[a] * rdC[bb] + rdC[a] * R[bb]
+                                                - 6. * rCr * R[a] * R[bb] - Cj[Cs[c]]);
+                            }
+                        }
+            }
+            for (int a = 0; a < 3; ++a) u3[3 * i + a] += u[a];
+            for (int c = 0; c < 8; ++c) D8[8 * i + c] += D[c];
+        }
+    }
+    if (parts & 2) {
+        /* reciprocal vectors of the (sheared) box, the construction of gridk_value with the exact 2 pi */
+        const int nxm = (int)ceil(k_cut * Lx / (2 * M_PI)) + 1, nym = (int)ceil(k_cut * Ly / (2 * M_PI) + fabs(xy) * nxm * Ly / Lx) + 1;
+        const int nzm = (int)ceil(k_cut * Lz / (2 * M_PI)) + 1;
+        const int nth = omp_get_max_threads();
+        double *acc = calloc((size_t)nth * 11 * N, sizeof(double));
+        const double xisq = xi * xi;
+#pragma omp parallel
+        {
+            double *my = acc + (size_t)omp_get_thread_num() * 11 * N;
+            double complex *ph = malloc(sizeof(double complex) * N);
+#pragma omp for schedule(dynamic, 1) collapse(2)
+            for (int nx = -nxm; nx <= nxm; ++nx)
+                for (int ny = -nym; ny <= nym; ++ny)
+                    for (int nz = -nzm; nz <= nzm; ++nz) {
+                        const double kx = 2 * M_PI * nx / Lx, ky = 2 * M_PI * (ny - xy * nx * Ly / Lx) / Ly;
+                        const double kz = 2 * M_PI * nz / Lz;
+                        const double k2 = kx * kx + ky * ky + kz * kz;
+                        if (k2 == 0.0 || k2 > k_cut * k_cut) continue;
+                        const double kv[3] = {kx, ky, kz};
+                        const double w = 6.0 * M_PI * (1.0 + k2 / 4.0 / xisq) * exp(-k2 / 4.0 / xisq) / k2 / V;
+                        /* structure factors F(k) = sum_j F_j e^{-i k.x_j}, C(k) likewise */
+                        double complex f[3] = {0, 0, 0}, cc[8] = {0};
+                        for (int j = 0; j < N; ++j) {
+                            const double kr = kx * pos4[4 * j] + ky * pos4[4 * j + 1] + kz * pos4[4 * j + 2];
+                            ph[j] = cos(kr) + I * sin(kr);
+                            const double complex e = conj(ph[j]);
+                            for (int a = 0; a < 3; ++a) f[a] += F3[3 * j + a] * e;
+                            for (int c = 0; c < 8; ++c) cc[c] += C8[8 * j + c] * e;
+                        }
+                        /* Green's function of Mobility.cu:315-479 on (f, C) */
+                        const double complex Cm[3][3] = {{cc[0], cc[1], cc[2]}, {cc[5], cc[4], cc[3]},
+                                                         {cc[6], cc[7], -(cc[0] + cc[4])}};
+                        const double kk = sqrt(k2);
+                        const double complex kdF = (kv[0] * f[0] + kv[1] * f[1] + kv[2] * f[2]) / k2;
+                        double complex Cdk[3];
+                        for (int a = 0; a < 3; ++a) Cdk[a] = Cm[a][0] * kv[0] + Cm[a][1] * kv[1] + Cm[a][2] * kv[2];
+                        const double complex kCk = (Cdk[0] * kv[0] + Cdk[1] * kv[1] + Cdk[2] * kv[2]) / k2;
+                        const double sk = sin(kk), ck = cos(kk);
+                        const double j0 = sk / kk, j1t = 3.0 * (sk - kk * ck) / (k2 * kk);
+                        const double Buf = w * j0 * j0, Buc = -w * j0 * j1t, Bdf = w * j0 * j1t;
+                        const double Bdc = w * 9.0 * ((sk - kk * ck) / (k2 * kk)) * ((sk - kk * ck) / (k2 * kk));
+                        double complex ou[3], od[3][3];
+                        for (int a = 0; a < 3; ++a) ou[a] = (f[a] - kv[a] * kdF) * Buf + I * (Cdk[a] - kv[a] * kCk) * Buc;
+                        for (int a = 0; a < 3; ++a)
+                            for (int c = 0; c < 3; ++c)
+                                od[a][c] = I * (f[a] * kv[c] - kv[a] * kv[c] * kdF) * Bdf
+                                           + (Cdk[a] * kv[c] - kv[a] * kv[c] * kCk) * Bdc;
+                        /* gradient components in the stored order XX XY XZ YZ YY YX ZX ZY, then the
+                         * contraction's transposed slots (Mobility.cu:832-833) as wave_space writes them */
+                        const double complex g8[8] = {od[0][0], od[0][1], od[0][2], od[1][2], od[1][1], od[1][0],
+                                                      od[2][0], od[2][1]};
+                        const int slot[8] = {0, 5, 6, 7, 4, 1, 2, 3};
+                        for (int i = 0; i < N; ++i) {
+                            const double complex p = ph[i];
+                            for (int a = 0; a < 3; ++a) my[11 * (size_t)i + a] += creal(ou[a] * p);
+                            for (int c = 0; c < 8; ++c) my[11 * (size_t)i + 3 + slot[c]] += creal(g8[c] * p);
+                        }
+                    }
+            free(ph);
+        }
+        for (int t = 0; t < nth; ++t)
+            for (int i = 0; i < N; ++i) {
+                const double *a = acc + (size_t)t * 11 * N + 11 * (size_t)i;
+                for (int c = 0; c < 3; ++c) u3[3 * i + c] += a[c];
+                for (int c = 0; c < 8; ++c) D8[8 * i + c] += a[3 + c];
+            }
+        free(acc);
+    }
+    for (int i = 0; i < N; ++i) {
+        double W[3], E5[5];
+        d2we(D8 + 8 * i, W, E5);
+        for (int a = 0; a < 3; ++a) { gU[6 * i + a] = u3[3 * i + a]; gU[6 * i + 3 + a] = W[a]; }
+        for (int c = 0; c < 5; ++c) gU[6 * (size_t)N + 5 * i + c] = E5[c];
+    }
+    free(F3); free(C8); free(u3); free(D8);
+}

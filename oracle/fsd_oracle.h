@@ -51,6 +51,10 @@ void orc_wrap(const orc_box *b, float p[3], int img[3]);
 
 /* ---- parameters (Stokes.cc:118-261) ---- */
 int orc_pse_params(const orc_box *b, float xi, float error, float max_strain, orc_pse *out);
+/* Direct Ewald sum of the FTS grand mobility (no PSE): real space over all images
+ * within r_cut with the functions in FP64, wave space over all |k| <= k_cut */
+void orc_mobility_direct(int N, const float *pos4, const orc_box *b, double xi, double r_cut, double k_cut,
+                         const double *gF, double *gU, int parts);
 /* OpenMP threads the oracle runs on */
 int orc_threads(void);
 /* gridk: M float4 (kx, ky, kz, w), parity!=0 reproduces the 3.1416926536 factor */

@@ -59,12 +59,17 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-GAMMA0 = 4.0
+GAMMA0 = float(np.float32(np.log(0.49 / 0.51)))
 
 
 def gamma(c):
-    """fill weight of a pair from its raw output: sigmoid(c + 4)"""
-    return 1.0 / (1.0 + np.exp(-(c + GAMMA0)))
+    """fill weight of a pair from its raw output: 2 sigmoid(c + g0), g0 = ln(0.49 / 0.51)"""
+    return 2.0 / (1.0 + np.exp(-(c + GAMMA0)))
+
+
+def gamma_d(c):
+    sg = 1.0 / (1.0 + np.exp(-(c + GAMMA0)))
+    return 2.0 * sg * (1.0 - sg)
 
 
 def silu(z):
@@ -344,6 +349,36 @@ class LearnedOracle:
         # factor assembly backward
         Ssym = 0.5 * (gD + np.transpose(gD, (0, 2, 1)))
         Db = -np.einsum('paq,pqr,prc->pac', Dinv, Ssym, Dinv)
+        return loss, self._assembly_backward(c, gW, Db, emulate)
+
+    def factor_targets(self, orc):
+        """IC(0)'s block LDL^T on the pattern: pivots L_PP L_PP^T and pair blocks L_PQ L_QQ^T"""
+        Ld, rows, cols, Lo = orc.factor_blocks()
+        assert np.array_equal(rows, self.rows) and np.array_equal(cols, self.cols)
+        Dic = np.einsum('pac,pbc->pab', Ld, Ld)
+        Wic = np.einsum('eac,ebc->eab', Lo, Ld[cols]) if self.E else np.zeros((0, 6, 6))
+        return Dic, Wic
+
+    def factor_loss(self, w, Dic, Wic, emulate=True, grad=False):
+        """mean over particles of |D_P - D_P^IC|^2 / |D_P^IC|^2 plus mean over pairs of
+        |W_e - W_e^IC|^2 / |A_e|^2 (k_learned.hip lp_pivot_target_kernel, lp_pair_target_kernel)"""
+        c = self.forward(w, emulate)
+        W, D, Dinv, failed = self.assemble(c["oute"], c["outn"])
+        nD = (Dic ** 2).sum(axis=(1, 2))
+        loss = float((((D - Dic) ** 2).sum(axis=(1, 2)) / nD).mean())
+        gD = 2.0 * (D - Dic) / (self.N * nD[:, None, None])
+        gW = np.zeros_like(W)
+        if self.E:
+            nA = (self.Ablk ** 2).sum(axis=(1, 2))
+            loss += float((((W - Wic) ** 2).sum(axis=(1, 2)) / nA).mean())
+            gW = 2.0 * (W - Wic) / (self.E * nA[:, None, None])
+        if not grad:
+            return loss
+        Db = 0.5 * (gD + np.transpose(gD, (0, 2, 1)))
+        return loss, self._assembly_backward(c, gW, Db, emulate)
+
+    def _assembly_backward(self, c, gW, Db, emulate):
+        N = self.N
         goutn = np.zeros((N, ON))
         outn = c["outn"]
         goutn[:, 0] = self.lam * np.exp(outn[:, 0]) * np.einsum('pii->p', Db[:, 0:3, 0:3])
@@ -351,11 +386,10 @@ class LearnedOracle:
         goute = np.zeros((self.E, OE))
         oute = c["oute"]
         if self.E:
-            gam = gamma(oute[:, 6])
             T = np.einsum('eaq,eqr,ecr->eac', self.Ablk, self.Dtinv[self.cols], self.Ablk)
-            goute[:, 6] = -np.einsum('eac,eac->e', Db[self.rows], T) * gam * (1.0 - gam)
+            goute[:, 6] = -np.einsum('eac,eac->e', Db[self.rows], T) * gamma_d(oute[:, 6])
             goute[:, :6] = np.einsum('eac,ekac->ek', gW, self.Bk) * (1.0 - np.tanh(oute[:, :6]) ** 2)
-        return loss, self._mlp_backward(c, goute, goutn, emulate)
+        return self._mlp_backward(c, goute, goutn, emulate)
 
     def _mlp_backward(self, c, goute, goutn, emulate):
         q = lambda x: self._q(x, emulate)

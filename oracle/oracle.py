@@ -90,6 +90,8 @@ def load():
         lib.fsd_ewald_functions.argtypes = [C.c_double, C.c_double, _P]
         lib.fsd_ewald_self.argtypes = [C.c_float, _P]
         lib.orc_threads.restype = C.c_int
+        lib.orc_mobility_direct.argtypes = [C.c_int, _P, C.POINTER(Box), C.c_double, C.c_double, C.c_double,
+                                            _P, _P, C.c_int]
         _lib = lib
     return _lib
 
@@ -202,6 +204,19 @@ class Oracle:
         return gU
 
     # ---- preconditioner
+    def mobility_direct(self, gF, parts=3, xi=None, tol=1e-14):
+        """O2: the FTS grand mobility by the direct Ewald sum (no PSE discretisation),
+        converged to ~tol: real space to xi r_cut = sqrt(-ln tol) + 0.5, wave space to
+        k_cut = 2 xi sqrt(-ln tol); independent of xi"""
+        xi = self.pse.xi if xi is None else xi
+        s = np.sqrt(-np.log(tol))
+        r_cut, k_cut = (s + 0.5) / xi, 2.0 * xi * s
+        gF = np.ascontiguousarray(gF, dtype=np.float64)
+        gU = np.zeros(11 * self.N)
+        self.lib.orc_mobility_direct(self.N, _p(self.pos4), C.byref(self.box), float(xi), float(r_cut),
+                                     float(k_cut), _p(gF), _p(gU), int(parts))
+        return gU
+
     def prune(self):
         N = self.N
         pn = np.zeros(N, dtype=np.int32)

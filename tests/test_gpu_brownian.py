@@ -135,7 +135,7 @@ def test_c2_brownian_100_steps():
     N, phi, seed = 4096, 0.3, 12
     pos, L = packing(N, phi, seed)
     print("C2 Brownian: N", N, "phi", phi, "100 steps", flush=True)
-    st = Stokes(N, L, T=T, dt=DT, gmres_tol=1e-6)
+    st = Stokes(N, L, T=T, dt=DT, gmres_tol=1e-5, gmres_restart=100, gmres_maxiter=3000)
     state = ParticleState(pos)
     zero = torch.zeros(6 * N, dtype=torch.float32, device="cuda")
     pse = st.pse_info()
@@ -148,7 +148,7 @@ def test_c2_brownian_100_steps():
         st.set_noise(*[_t(a) for a in noise])
         start = state.positions().copy()
         info = st.integrateStepOne(step, state, force6=zero, shear_rate=0.0)
-        assert info["converged"]
+        assert info["converged"], (step, info)
         lan.append((info["lanczos_ff"], info["lanczos_nf"]))
         if step in (0, 99):
             ffr, ffg, nf, rfd = noise

@@ -131,3 +131,19 @@ def test_trained_factor_gain():
                            N=4096, phi=0.45, seed=91, steps=4, warmup=2, log=lambda s: print(s, flush=True))
     print("mean iterations", res)
     assert res["trained"] < res["untrained"]
+
+
+@pytest.mark.parametrize("case", ["C1", "D1"])
+def test_learned_factor_loss_parity(case):
+    """the supervised objective (the learned factor's pivots and pair blocks against
+    IC(0)'s block LDL^T): loss and gradient against the oracle's (IC(0) in FP64 there,
+    FP32 on the GPU: the targets agree to 1e-4)"""
+    w = _weights(seed=4)
+    st, orc, lo, dpos, N = _setup(case, w)
+    loss = st.learned_factor_loss(dpos)
+    gg = st.learned_get_weights(grad=True)
+    Dic, Wic = lo.factor_targets(orc)
+    lref, gref = lo.factor_loss(w, Dic, Wic, emulate=True, grad=True)
+    el, eg = abs(loss - lref) / abs(lref), rel_err2(gg, gref)
+    print("factor loss", loss, lref, "rel", el, "grad rel L2", eg)
+    assert el < 1e-3 and eg < 1e-2

@@ -32,7 +32,7 @@ def test_layouts_agree():
 
 def test_zero_outputs_give_one_step_fill(dense):
     """with zero network outputs W is the pair blocks A and D = A_PP + lambda S minus
-    sigmoid(4) of every lower pair's fill term A Dt^-1 A^T"""
+    0.98 of every lower pair's fill term A Dt^-1 A^T"""
     from oracle.learned import init_weights
     orc, lo, X, B = dense
     c = lo.forward(init_weights(1, scale_out=0.0), emulate=False)
@@ -41,7 +41,8 @@ def test_zero_outputs_give_one_step_fill(dense):
     assert lo.E > 100 and not failed
     assert np.abs(W - lo.Ablk).max() <= 1e-5 * np.abs(lo.Ablk).max()
     # the pivots written out pair by pair from the pattern
-    g = 1.0 / (1.0 + np.exp(-4.0))
+    from oracle.learned import gamma
+    g = gamma(0.0)
     Dref = lo.Dt.copy()
     for e in range(lo.E):
         P, Q = lo.rows[e], lo.cols[e]
@@ -78,3 +79,21 @@ def test_gradient_finite_differences(dense):
             e[i] = 1e-5
             fdi = (lo.loss(w + e, X, B, emulate=False) - lo.loss(w - e, X, B, emulate=False)) / 2e-5
             assert abs(fdi - g[i]) <= 1e-6 * max(abs(fdi), 1e-3), (i, fdi, g[i])
+
+
+def test_factor_loss_gradient_finite_differences(dense):
+    """the supervised objective (learned factor against IC(0)'s LDL^T blocks): analytic
+    gradient against central differences of the FP64 loss, and zero loss when the targets
+    are the factor itself"""
+    from oracle.learned import init_weights
+    orc, lo, X, B = dense
+    Dic, Wic = lo.factor_targets(orc)
+    w = init_weights(3, scale_out=0.3).astype(np.float64)
+    loss, g = lo.factor_loss(w, Dic, Wic, emulate=False, grad=True)
+    v = np.random.default_rng(6).standard_normal(w.size)
+    eps = 1e-6
+    fd = (lo.factor_loss(w + eps * v, Dic, Wic, emulate=False) - lo.factor_loss(w - eps * v, Dic, Wic, emulate=False)) / (2 * eps)
+    assert loss > 0 and abs(fd - g @ v) < 1e-6 * abs(fd)
+    c = lo.forward(w, emulate=False)
+    W, D, Dinv, failed = lo.assemble(c["oute"], c["outn"])
+    assert lo.factor_loss(w, D, W, emulate=False) < 1e-20

@@ -430,3 +430,38 @@ def test_jacobi_apply_limits():
     want = np.empty((343, 6))
     want[o.perm] = np.einsum('pij,pj->pi', D, t)
     assert np.abs(y0 - want.reshape(-1)).max() <= 1e-12 * np.abs(want).max()
+
+
+# ------------------------------------------------- direct Ewald sum (O2)
+def test_direct_ewald_xi_independent_and_hasimoto():
+    """the direct Ewald sum (no PSE grid) is independent of the splitting xi to 1e-10
+    on a C1 packing (every term converged), and a lone sphere's translational
+    self-mobility is Hasimoto's 1 - 2.837297/L + 4.18879/L^3 and its rotational one
+    0.75 (1 - (4 pi / 3)/L^3)"""
+    from tests._cases import rel_err
+    pos, L = packing(343, 0.2, 11)
+    o = _oracle(pos, L)
+    x = np.random.default_rng(1).standard_normal(11 * 343)
+    a, b = o.mobility_direct(x, xi=0.5), o.mobility_direct(x, xi=0.7)
+    assert rel_err(a, b) < 1e-10
+    L1 = 20.0
+    o1 = _oracle(np.zeros((1, 3)), L1, error=1e-6, parity=False)
+    e = np.zeros(11)
+    e[0] = 1.0
+    u = o1.mobility_direct(e)
+    assert u[0] == pytest.approx(1.0 - 2.837297 / L1 + 4.18879 / L1 ** 3, rel=1e-6)
+    e[0], e[3] = 0.0, 1.0
+    assert o1.mobility_direct(e)[3] == pytest.approx(0.75 * (1.0 - 4.18879 / L1 ** 3), rel=2e-6)
+
+
+def test_pse_converges_to_direct_ewald():
+    """the PSE grand mobility (O1, the reference's discretisation) approaches the direct
+    Ewald sum as its error tolerance tightens: relative max-norm differences below
+    8 x the tolerance and falling with it"""
+    from tests._cases import rel_err
+    pos, L = packing(343, 0.2, 11)
+    x = np.random.default_rng(2).standard_normal(11 * 343)
+    ref = _oracle(pos, L).mobility_direct(x)
+    errs = [rel_err(_oracle(pos, L, error=tol, parity=False).mobility(x, 3), ref) for tol in (1e-3, 3e-4, 1e-4)]
+    assert errs[0] < 8e-3 and errs[1] < 8 * 3e-4 and errs[2] < 8e-4
+    assert errs[0] > errs[1] > errs[2]
