@@ -9,6 +9,7 @@ preconditioner build, force, right-hand side, GMRES solve, explicit Euler move.
 
 Two arms run on the same packing from the same start:
   ichol   - the reference's preconditioner: IC(0) with exact triangular solves;
+  ichol_jacobi (optional) - IC(0) applied by block Jacobi iterations;
   learned - the learned factor (a message-passing network on the lubrication
             graph) applied by block Jacobi iterations, with the IC(0) fallbacks
             (non-positive pivot; iteration count above IC(0)'s at the last
@@ -178,7 +179,10 @@ def main():
     ap.add_argument("--calibration", type=int, default=20,
                     help="learned arm: one IC(0) step every this many steps (its iteration count caps the learned "
                          "solves); 20 puts one such step in the 20 timed steps")
-    ap.add_argument("--arms", default="ichol,learned")
+    ap.add_argument("--ichol-jacobi", type=int, default=8,
+                    help="block Jacobi iterations per triangular solve of the ichol_jacobi arm")
+    ap.add_argument("--arms", default="ichol,learned",
+                    help="comma-separated arms to run (ichol, ichol_jacobi, learned); the last is `value`")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (diagnostics)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MB",
                     help="only run 5 device copies of MB megabytes (calibrates the HBM byte counters) and exit")
@@ -209,6 +213,7 @@ def main():
     log("# workload generated in %.1f s: N=%d phi=%.2f L=%.3f" % (time.time() - t, N, args.phi, L))
     dF = torch.from_numpy(F).cuda()
     arm_kw = {"ichol": dict(precond="ichol"),
+              "ichol_jacobi": dict(precond="ichol_jacobi", jacobi_iters=args.ichol_jacobi),
               "learned": dict(precond="learned", jacobi_iters=args.jacobi, learned_calibration=args.calibration,
                               refine=True)}
     names = [a for a in args.arms.split(",") if a]

@@ -128,6 +128,7 @@ SIGNATURES = [
     ("fsd_learned_loss_grad", _I, [_P, _P, _I, _P, _P, _I, C.POINTER(C.c_double)]),
     ("fsd_learned_factor_loss", _I, [_P, _P, _I, C.POINTER(C.c_double)]),
     ("fsd_learned_adam", _I, [_P, _F, _I]),
+    ("fsd_learned_revert", _I, [_P]),
     ("fsd_rfe", _I, [_P, _P, _F, _P, _P]),
     ("fsd_rsu", _I, [_P, _P, _P, _P]),
     ("fsd_rse", _I, [_P, _P, _F, _P, _P]),

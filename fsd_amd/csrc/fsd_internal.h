@@ -426,6 +426,7 @@ int lp_forward(fsd_ctx *ctx, const float4 *pos, float lambda, bool train, int in
 int lp_apply(fsd_ctx *ctx, const float *b0, const float *b1, const float *bsaddle, float *x, int mode);
 int lp_loss_grad(fsd_ctx *ctx, int nsamp, const float *d_x, const float *d_b, double *loss, bool grad);
 int lp_adam(fsd_ctx *ctx, float lr, int step);
+int lp_revert(fsd_ctx *ctx);
 int lp_factor_loss_grad(fsd_ctx *ctx, double *loss, bool grad);
 bool lp_active(const fsd_ctx *ctx);
 bool lp_have_weights(const fsd_ctx *ctx);

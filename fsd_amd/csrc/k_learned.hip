@@ -59,15 +59,16 @@ __device__ __forceinline__ float silu_d(float z)
 }
 __device__ __forceinline__ float sgnlog(float v) { return copysignf(log1pf(fabsf(v)), v); }
 // the network's raw outputs enter the factor bounded: pair-coefficient multipliers
-// 1 + tanh(c) in (0, 2), fill weights 2 sigmoid(c + g0) in (0, 2) with g0 = ln(0.49 / 0.51)
-// (0.98 at c = 0: the zero-output factor subtracts nearly the whole one-step fill term;
-// IC(0)'s smaller corrected pivots subtract more), pivot shift lambda e^a S
-constexpr float LP_GAMMA0 = -0.040005334f;
-__device__ __forceinline__ float lp_gamma(float c) { return 2.0f * sigm(c + LP_GAMMA0); }
+// 1 + tanh(c) in (0, 2), fill weights sigmoid(c + 4) in (0, 1) (0.98 at c = 0: the
+// zero-output factor subtracts nearly the whole one-step fill term; at most the whole of
+// it, which keeps the pivots positive), pivot shift lambda e^a S (a < 0 shrinks the
+// pivots towards IC(0)'s)
+constexpr float LP_GAMMA0 = 4.0f;
+__device__ __forceinline__ float lp_gamma(float c) { return sigm(c + LP_GAMMA0); }
 __device__ __forceinline__ float lp_gamma_d(float c)
 {
     const float sg = sigm(c + LP_GAMMA0);
-    return 2.0f * sg * (1.0f - sg);
+    return sg * (1.0f - sg);
 }
 
 // 6x6 symmetric positive definite inverse through Cholesky (lower triangle of D
@@ -374,7 +375,7 @@ __global__ void lp_edge_W_kernel(int64_t E, const float *geo, const float *oute,
 }
 
 // D_P = A_PP + lambda diag(e^a_t, e^a_r) S - sum_{Q < P} gamma_e A_e Dt_Q^-1 A_e^T with
-// gamma_e = 2 sigmoid(c_e + g0) and A_e the pair block; Dinv_P = D_P^-1, a non-positive pivot
+// gamma_e = sigmoid(c_e + 4) and A_e the pair block; Dinv_P = D_P^-1, a non-positive pivot
 // sets fail bit 1.  The fill term uses the pair block, not W: the pivots then do not
 // depend on W, and (D + W) D^-1 (D + W)^T is positive definite for any W once D is.
 __global__ void lp_node_D_kernel(int N, const int *lrow, const int *lcol, const float *Adiag, float lambda,
@@ -1039,7 +1040,7 @@ void lp_free(fsd_ctx *ctx)
     LearnedState &s = *ctx->lp;
     void *ptrs[] = {s.d_w, s.d_grad, s.d_m, s.d_v, s.d_wn, s.xn0, s.zn0, s.agg, s.xh, s.outn, s.Dinv, s.Dtinv, s.D,
                     s.xe0, s.ze0, s.oute, s.geo, s.W, s.erow, s.d_fail, s.gh, s.gh2, s.ge, s.gm, s.gzn, s.gze, s.gxe,
-                    s.gxn, s.goute, s.goutn, s.gW, s.gDinv, s.gbf, s.part, s.jv, s.d_loss};
+                    s.gxn, s.goute, s.goutn, s.gW, s.gDinv, s.gbf, s.part, s.jv, s.d_loss, s.d_wbak};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (int l = 0; l <= lp::NMP; ++l) {
@@ -1389,9 +1390,30 @@ int lp_factor_loss_grad(fsd_ctx *ctx, double *loss, bool grad)
     return 0;
 }
 
+// undo the last Adam step (the parameters and moments it started from)
+int lp_revert(fsd_ctx *ctx)
+{
+    LearnedState &s = *ctx->lp;
+    if (!s.d_wbak) { ctx->err = "no Adam step to revert"; return FSD_E_STATE; }
+    const size_t b = sizeof(float) * s.nparam;
+    FSD_HIP(hipMemcpyAsync(s.d_w, s.d_wbak, b, hipMemcpyDeviceToDevice, ctx->stream));
+    FSD_HIP(hipMemcpyAsync(s.d_m, s.d_wbak + s.nparam, b, hipMemcpyDeviceToDevice, ctx->stream));
+    FSD_HIP(hipMemcpyAsync(s.d_v, s.d_wbak + 2 * s.nparam, b, hipMemcpyDeviceToDevice, ctx->stream));
+    LRC(pack_weights(ctx));
+    FSD_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
 int lp_adam(fsd_ctx *ctx, float lr, int step)
 {
     LearnedState &s = *ctx->lp;
+    if (!s.d_wbak) LRC(lalloc(ctx, s.d_wbak, 3 * s.nparam));
+    {
+        const size_t b = sizeof(float) * s.nparam;
+        FSD_HIP(hipMemcpyAsync(s.d_wbak, s.d_w, b, hipMemcpyDeviceToDevice, ctx->stream));
+        FSD_HIP(hipMemcpyAsync(s.d_wbak + s.nparam, s.d_m, b, hipMemcpyDeviceToDevice, ctx->stream));
+        FSD_HIP(hipMemcpyAsync(s.d_wbak + 2 * s.nparam, s.d_v, b, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
     hipLaunchKernelGGL(lp_adam_kernel, dim3(nblk(s.nparam, 256)), dim3(256), 0, ctx->stream, s.nparam, s.d_w,

@@ -37,6 +37,7 @@ struct LearnedState {
     lp::Layer L[lp::NLAYER];
     int64_t nparam = 0, nbf = 0;
     float *d_w = nullptr, *d_grad = nullptr, *d_m = nullptr, *d_v = nullptr;   // FP32 master, gradient, Adam
+    float *d_wbak = nullptr;                     // parameters and moments before the last Adam step
     uint16_t *d_wt = nullptr, *d_wn = nullptr;   // BF16 copies: [no][ki] (forward), [ki][no] (input gradients)
     bool have_w = false;
     int64_t ecap = 0;                            // edges (lower blocks) the edge buffers hold

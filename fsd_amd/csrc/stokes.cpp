@@ -1831,6 +1831,12 @@ int fsd_learned_factor_loss(fsd_ctx *ctx, const float *d_pos4, int32_t want_grad
     return 0;
 }
 
+int fsd_learned_revert(fsd_ctx *ctx)
+{
+    if (!ctx->lp) { ctx->err = "no learned factor"; return FSD_E_STATE; }
+    return lp_revert(ctx);
+}
+
 int fsd_learned_adam(fsd_ctx *ctx, float lr, int32_t step)
 {
     if (step < 1) { ctx->err = "Adam step counts from 1"; return FSD_E_PARAM; }

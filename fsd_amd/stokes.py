@@ -356,6 +356,10 @@ class Stokes:
         v = np.ascontiguousarray(v, dtype=np.float32)
         self._check(self.lib.fsd_learned_set_adam(self.ctx, vp(m), vp(v), n))
 
+    def learned_revert(self):
+        """undo the last Adam step"""
+        self._check(self.lib.fsd_learned_revert(self.ctx))
+
     def learned_adam(self, lr, step):
         self._check(self.lib.fsd_learned_adam(self.ctx, float(lr), int(step)))
 

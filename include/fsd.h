@@ -321,6 +321,8 @@ int fsd_learned_loss_grad(fsd_ctx *ctx, const float *d_pos4, int32_t nsamp, cons
 int fsd_learned_factor_loss(fsd_ctx *ctx, const float *d_pos4, int32_t want_grad, double *loss);
 /* one Adam step (Kingma & Ba 2015; beta 0.9, 0.999) on the last gradient */
 int fsd_learned_adam(fsd_ctx *ctx, float lr, int32_t step);
+/* undo the last Adam step (parameters and moments), e.g. after it produced a non-positive pivot */
+int fsd_learned_revert(fsd_ctx *ctx);
 
 #ifdef __cplusplus
 }

@@ -59,17 +59,17 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-GAMMA0 = float(np.float32(np.log(0.49 / 0.51)))
+GAMMA0 = 4.0
 
 
 def gamma(c):
-    """fill weight of a pair from its raw output: 2 sigmoid(c + g0), g0 = ln(0.49 / 0.51)"""
-    return 2.0 / (1.0 + np.exp(-(c + GAMMA0)))
+    """fill weight of a pair from its raw output: sigmoid(c + 4)"""
+    return 1.0 / (1.0 + np.exp(-(c + GAMMA0)))
 
 
 def gamma_d(c):
-    sg = 1.0 / (1.0 + np.exp(-(c + GAMMA0)))
-    return 2.0 * sg * (1.0 - sg)
+    sg = gamma(c)
+    return sg * (1.0 - sg)
 
 
 def silu(z):

@@ -124,20 +124,21 @@ def test_brownian_step():
 def test_c2_brownian_100_steps():
     """C2 as configured: N = 4096 at phi = 0.3, kT = 1, dt = 1e-3, 100 Brownian steps
     (far field: wave-space noise and Lanczos to 1e-3 with at most 100 vectors; near
-    field: Lanczos on R_FU; RFD drift), zero applied force so each step's displacement
-    is the Brownian one.  Noise is injected step by step; at the first and the last
-    step the displacement U dt is compared with the oracle's composition of the same
-    pieces at the GPU's positions of that step (per-step error, 5e-3: the Lanczos
-    samplers agree to their tolerance 1e-3)."""
+    field: Lanczos on R_FU; RFD drift; solves FP64-refined to a true residual of 1e-6),
+    with the reference's interparticle force law (which keeps the Brownian particles
+    from overlapping).  Noise is injected step by step; at the first and the last step
+    the displacement U dt is compared with the oracle's composition of the same pieces
+    (and the same force law) at the GPU's positions of that step (per-step error, 5e-3:
+    the Lanczos samplers agree to their tolerance 1e-3)."""
     import torch
     from fsd_amd.stokes import Stokes, ParticleState
     from oracle.oracle import Oracle
     N, phi, seed = 4096, 0.3, 12
     pos, L = packing(N, phi, seed)
     print("C2 Brownian: N", N, "phi", phi, "100 steps", flush=True)
-    st = Stokes(N, L, T=T, dt=DT, gmres_tol=1e-5, gmres_restart=100, gmres_maxiter=3000)
+    st = Stokes(N, L, T=T, dt=DT, gmres_tol=1e-5, gmres_true_tol=1e-6, gmres_restart=100, gmres_maxiter=3000,
+                refine=True)
     state = ParticleState(pos)
-    zero = torch.zeros(6 * N, dtype=torch.float32, device="cuda")
     pse = st.pse_info()
     M = pse["Nx"] * pse["Ny"] * pse["Nz"]
     errs, lan = {}, []
@@ -147,9 +148,11 @@ def test_c2_brownian_100_steps():
                  rng.standard_normal(6 * N).astype(np.float32), rng.standard_normal(6 * N).astype(np.float32))
         st.set_noise(*[_t(a) for a in noise])
         start = state.positions().copy()
-        info = st.integrateStepOne(step, state, force6=zero, shear_rate=0.0)
+        info = st.integrateStepOne(step, state, force6=None, shear_rate=0.0)
         assert info["converged"], (step, info)
         lan.append((info["lanczos_ff"], info["lanczos_nf"]))
+        if step % 10 == 0:
+            print("step", step, "iterations", info["iterations"], "Lanczos sizes", lan[-1], flush=True)
         if step in (0, 99):
             ffr, ffg, nf, rfd = noise
             orc = Oracle(start, L)
@@ -157,6 +160,7 @@ def test_c2_brownian_100_steps():
             b = np.zeros(17 * N)
             b[:11 * N] = orc.bro_ff_wave(ffg, T, DT, parity=True) + orc.bro_ff_real(ffr, T, DT, tol=1e-3, m0=2)
             b[11 * N:] = orc.bro_nf(np.sqrt(2 * T / DT) * nf.astype(np.float64), tol=1e-3, m0=2)
+            b[11 * N:] -= orc.force_law()
             b[11 * N:] = orc.rfe(0.0, F=b[11 * N:])
             x, it, res = orc.gmres(b, tol=1e-8)
             ref = (x[11 * N:] + orc.rfd_drift(rfd.astype(np.float64), T, 1e-3, tol=1e-8)) * DT

@@ -138,9 +138,10 @@ def test_learned_factor_loss_parity(case):
     """the supervised objective (the learned factor's pivots and pair blocks against
     IC(0)'s block LDL^T): loss and gradient against the oracle's (IC(0) in FP64 there,
     FP32 on the GPU: the targets agree to 1e-4)"""
-    w = _weights(seed=4)
+    w = _weights(seed=4, head=0.1)
     st, orc, lo, dpos, N = _setup(case, w)
     loss = st.learned_factor_loss(dpos)
+    assert loss is not None, "non-positive pivot"
     gg = st.learned_get_weights(grad=True)
     Dic, Wic = lo.factor_targets(orc)
     lref, gref = lo.factor_loss(w, Dic, Wic, emulate=True, grad=True)
