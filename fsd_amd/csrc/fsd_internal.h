@@ -422,7 +422,8 @@ int64_t lp_nparam(fsd_ctx *ctx);
 int lp_set_weights(fsd_ctx *ctx, const float *h_w, int64_t n);
 int lp_get_weights(fsd_ctx *ctx, float *h_w, int64_t n, bool grad);
 int lp_adam_state(fsd_ctx *ctx, float *h_m, float *h_v, int64_t n, bool set);
-int lp_forward(fsd_ctx *ctx, const float4 *pos, float lambda, bool train, int inject, int *failed);
+int lp_forward(fsd_ctx *ctx, const float4 *pos, bool train, int inject, int *failed);
+void lp_last_factor(const fsd_ctx *ctx, float *lambda, int *shrunk, int *plain);
 int lp_apply(fsd_ctx *ctx, const float *b0, const float *b1, const float *bsaddle, float *x, int mode);
 int lp_loss_grad(fsd_ctx *ctx, int nsamp, const float *d_x, const float *d_b, double *loss, bool grad);
 int lp_adam(fsd_ctx *ctx, float lr, int step);

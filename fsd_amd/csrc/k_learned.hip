@@ -59,16 +59,22 @@ __device__ __forceinline__ float silu_d(float z)
 }
 __device__ __forceinline__ float sgnlog(float v) { return copysignf(log1pf(fabsf(v)), v); }
 // the network's raw outputs enter the factor bounded: pair-coefficient multipliers
-// 1 + tanh(c) in (0, 2), fill weights sigmoid(c + 4) in (0, 1) (0.98 at c = 0: the
-// zero-output factor subtracts nearly the whole one-step fill term; at most the whole of
-// it, which keeps the pivots positive), pivot shift lambda e^a S (a < 0 shrinks the
-// pivots towards IC(0)'s)
-constexpr float LP_GAMMA0 = 4.0f;
-__device__ __forceinline__ float lp_gamma(float c) { return sigm(c + LP_GAMMA0); }
+// 1 + tanh(c) in (0, 2), fill weights 2 sigmoid(c + c0) in (0, 2) (0.98 at c = 0: the
+// zero-output factor subtracts nearly the whole one-step fill term; above 1 the fill
+// can reach IC(0)'s, whose pivots D_Q are below A_QQ + lambda S), pivot shift
+// lambda e^a S (a < 0 shrinks the pivots towards IC(0)'s)
+constexpr float LP_GAMMA0 = -0.036004f;   // logit(0.491)
+__device__ __forceinline__ float lp_gamma(float c) { return 2.0f * sigm(c + LP_GAMMA0); }
 __device__ __forceinline__ float lp_gamma_d(float c)
 {
     const float sg = sigm(c + LP_GAMMA0);
-    return sg * (1.0f - sg);
+    return 2.0f * sg * (1.0f - sg);
+}
+
+__global__ void lp_fill_int_kernel(int n, int v, int *p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // 6x6 symmetric positive definite inverse through Cholesky (lower triangle of D
@@ -374,22 +380,34 @@ __global__ void lp_edge_W_kernel(int64_t E, const float *geo, const float *oute,
     W[t] = w;
 }
 
-// D_P = A_PP + lambda diag(e^a_t, e^a_r) S - sum_{Q < P} gamma_e A_e Dt_Q^-1 A_e^T with
-// gamma_e = sigmoid(c_e + 4) and A_e the pair block; Dinv_P = D_P^-1, a non-positive pivot
-// sets fail bit 1.  The fill term uses the pair block, not W: the pivots then do not
-// depend on W, and (D + W) D^-1 (D + W)^T is positive definite for any W once D is.
+// D_P = A_PP + lambda diag(e^a_t, e^a_r) S - s_P sum_{Q < P} gamma_e A_e Dt_Q^-1 A_e^T with
+// gamma_e = 2 sigmoid(c_e + c0), A_e the pair block and Dt_Q = A_QQ + lambda S.  The pivot is
+// positive definite by construction: s_P is the first of 1, 1/2, 1/4, 1/8 for which the
+// Cholesky factorisation of D_P succeeds (level 0 .. 3); when none does, D_P = Dt_P
+// (level 4), positive definite by the check of lp_node_feat_kernel.  lvl[P] records the
+// level for the backward pass; fail[1] counts particles above level 0, fail[2] those at
+// level 4.  Non-finite network outputs set fail bit 1 (the factor is then not used).
+// The fill term uses the pair block, not W: the pivots then do not depend on W, and
+// (D + W) D^-1 (D + W)^T is positive definite for any W once D is.
+constexpr int LP_LEVELS = 4;
 __global__ void lp_node_D_kernel(int N, const int *lrow, const int *lcol, const float *Adiag, float lambda,
                                  const float *outn, const float *oute, const float *Ablk, const float *Dtinv,
-                                 float *Dinv, float *Dout, int *fail)
+                                 float *Dinv, float *Dout, int *lvl, int *fail)
 {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= N) return;
-    float D[36];
-    for (int q = 0; q < 36; ++q) D[q] = Adiag[(int64_t)P * 36 + q];
+    float B[36], F[36];
+    for (int q = 0; q < 36; ++q) {
+        B[q] = Adiag[(int64_t)P * 36 + q];
+        F[q] = 0.f;
+    }
     const float at = expf(outn[(int64_t)P * lp::ON]), ar = expf(outn[(int64_t)P * lp::ON + 1]);
-    for (int c = 0; c < 6; ++c) D[7 * c] += lambda * (c < 3 ? at : ar * (4.0f / 3.0f));
+    bool finite = isfinite(at) && isfinite(ar);
+    for (int c = 0; c < 6; ++c) B[7 * c] += lambda * (c < 3 ? at : ar * (4.0f / 3.0f));
     for (int e = lrow[P]; e < lrow[P + 1]; ++e) {
-        const float gam = lp_gamma(oute[(int64_t)e * lp::OE + 6]);
+        const float *o = oute + (int64_t)e * lp::OE;
+        const float gam = lp_gamma(o[6]);
+        for (int k = 0; k < 7; ++k) finite = finite && isfinite(o[k]);
         const float *We = Ablk + (int64_t)e * 36, *Ti = Dtinv + (int64_t)lcol[e] * 36;
         float T[36];   // A Dt^-1
         for (int a = 0; a < 6; ++a)
@@ -402,14 +420,30 @@ __global__ void lp_node_D_kernel(int N, const int *lrow, const int *lcol, const 
             for (int c = 0; c < 6; ++c) {
                 float s = 0.f;
                 for (int q = 0; q < 6; ++q) s += T[a * 6 + q] * We[c * 6 + q];
-                D[a * 6 + c] -= gam * s;
+                F[a * 6 + c] += gam * s;
             }
     }
-    float Di[36];
-    if (!chol6_inv(D, Di)) {
+    float D[36], Di[36];
+    int level = LP_LEVELS;
+    float sc = 1.0f;
+    if (finite)
+        for (int j = 0; j < LP_LEVELS; ++j, sc *= 0.5f) {
+            for (int q = 0; q < 36; ++q) D[q] = B[q] - sc * F[q];
+            if (chol6_inv(D, Di)) {
+                level = j;
+                break;
+            }
+        }
+    else
         atomicOr(fail, 1);
-        for (int q = 0; q < 36; ++q) Di[q] = 0.f;
+    if (level == LP_LEVELS) {   // the uncorrected pivot A_PP + lambda S
+        for (int q = 0; q < 36; ++q) D[q] = Adiag[(int64_t)P * 36 + q];
+        for (int c = 0; c < 6; ++c) D[7 * c] += lambda * (c < 3 ? 1.0f : 4.0f / 3.0f);
+        for (int q = 0; q < 36; ++q) Di[q] = Dtinv[(int64_t)P * 36 + q];
+        atomicAdd(fail + 2, 1);
     }
+    if (level > 0) atomicAdd(fail + 1, 1);
+    lvl[P] = level;
     for (int q = 0; q < 36; ++q) Dinv[(int64_t)P * 36 + q] = Di[q];
     if (Dout)   // the pivot as used (lower triangle mirrored)
         for (int a = 0; a < 6; ++a)
@@ -652,14 +686,24 @@ __global__ void lj_gd_kernel(int N, GTerms T, float *gD)
 }
 
 // assembly backward, per particle: Dbar = -Dinv sym(gDinv) Dinv; pivot-shift gradients;
-// per lower pair the fill-weight gradient -gamma (1 - gamma) <Dbar, A Dt^-1 A^T>
+// per lower pair the fill-weight gradient -s_P gamma (1 - gamma) <Dbar, A Dt^-1 A^T> with
+// the pivot's level s_P = 2^-level (no output gradient at level 4)
 // (dbar_direct: gDinv already holds the gradient with respect to D itself, symmetric)
 __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, const float *Dinv, const float *gDinv,
                                      int dbar_direct, const float *oute, const float *outn, float lambda,
-                                     const float *Ablk, const float *Dtinv, float *goutn, float *goute)
+                                     const float *Ablk, const float *Dtinv, const int *lvl, float *goutn,
+                                     float *goute)
 {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= N) return;
+    float* go = goutn + (int64_t)P * lp::ON;
+    for (int c = 0; c < lp::ON; ++c) go[c] = 0.f;
+    const int level = lvl[P];
+    if (level >= LP_LEVELS) {   // D_P = A_PP + lambda S does not depend on the outputs
+        for (int e = lrow[P]; e < lrow[P + 1]; ++e) goute[(int64_t)e * lp::OE + 6] = 0.f;
+        return;
+    }
+    const float sc = ldexpf(1.0f, -level);
     const float *Di = Dinv + (int64_t)P * 36, *G = gDinv + (int64_t)P * 36;
     float S[36], T[36], Db[36];
     for (int a = 0; a < 6; ++a)
@@ -680,10 +724,8 @@ __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, co
                 Db[a * 6 + c] = -s;
             }
     }
-    float* go = goutn + (int64_t)P * lp::ON;
     go[0] = lambda * expf(outn[(int64_t)P * lp::ON]) * (Db[0] + Db[7] + Db[14]);
     go[1] = lambda * (4.0f / 3.0f) * expf(outn[(int64_t)P * lp::ON + 1]) * (Db[21] + Db[28] + Db[35]);
-    for (int c = 2; c < lp::ON; ++c) go[c] = 0.f;
     for (int e = lrow[P]; e < lrow[P + 1]; ++e) {
         const float *Ae = Ablk + (int64_t)e * 36, *Ti = Dtinv + (int64_t)lcol[e] * 36;
         float AT[36];   // A Dt^-1
@@ -700,7 +742,7 @@ __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, co
                 for (int q = 0; q < 6; ++q) s += AT[a * 6 + q] * Ae[c * 6 + q];
                 gg += Db[a * 6 + c] * s;
             }
-        goute[(int64_t)e * lp::OE + 6] = -gg * lp_gamma_d(oute[(int64_t)e * lp::OE + 6]);
+        goute[(int64_t)e * lp::OE + 6] = -sc * gg * lp_gamma_d(oute[(int64_t)e * lp::OE + 6]);
     }
 }
 
@@ -1025,6 +1067,7 @@ int lp_init(fsd_ctx *ctx)
     LRC(lalloc(ctx, s.goutn, N * lp::ON));
     LRC(lalloc(ctx, s.gDinv, N * 36));
     LRC(lalloc(ctx, s.d_fail, 4));
+    LRC(lalloc(ctx, s.lvl, N));
     LRC(lalloc(ctx, s.d_loss, 4));
     FSD_HIP(hipHostMalloc((void **)&s.h_fail, sizeof(int) * 4, hipHostMallocDefault));
     FSD_HIP(hipHostMalloc((void **)&s.h_loss, sizeof(double) * 4, hipHostMallocDefault));
@@ -1040,7 +1083,7 @@ void lp_free(fsd_ctx *ctx)
     LearnedState &s = *ctx->lp;
     void *ptrs[] = {s.d_w, s.d_grad, s.d_m, s.d_v, s.d_wn, s.xn0, s.zn0, s.agg, s.xh, s.outn, s.Dinv, s.Dtinv, s.D,
                     s.xe0, s.ze0, s.oute, s.geo, s.W, s.erow, s.d_fail, s.gh, s.gh2, s.ge, s.gm, s.gzn, s.gze, s.gxe,
-                    s.gxn, s.goute, s.goutn, s.gW, s.gDinv, s.gbf, s.part, s.jv, s.d_loss, s.d_wbak};
+                    s.gxn, s.goute, s.goutn, s.gW, s.gDinv, s.gbf, s.part, s.jv, s.d_loss, s.d_wbak, s.lvl};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (int l = 0; l <= lp::NMP; ++l) {
@@ -1063,6 +1106,13 @@ void lp_free(fsd_ctx *ctx)
 int64_t lp_nparam(fsd_ctx *ctx) { return ctx->lp ? ctx->lp->nparam : 0; }
 bool lp_active(const fsd_ctx *ctx) { return ctx->lp && ctx->lp->active; }
 bool lp_have_weights(const fsd_ctx *ctx) { return ctx->lp && ctx->lp->have_w; }
+void lp_last_factor(const fsd_ctx *ctx, float *lambda, int *shrunk, int *plain)
+{
+    if (!ctx->lp) return;
+    *lambda = ctx->lp->lambda;
+    *shrunk = ctx->lp->shrunk;
+    *plain = ctx->lp->plain;
+}
 void lp_set_active(fsd_ctx *ctx, bool on)
 {
     if (ctx->lp) ctx->lp->active = on;
@@ -1117,10 +1167,12 @@ int lp_get_weights(fsd_ctx *ctx, float *h_w, int64_t n, bool grad)
 }
 
 // Features, network forward and factor assembly on the current pattern (the
-// IC(0) path's prune, RCM, lower pattern and pair blocks must be built).
-// *failed: bit 0 a non-positive pivot of the learned factor, bit 1 of an
-// uncorrected pivot A_PP + lambda S.
-int lp_forward(fsd_ctx *ctx, const float4 *pos, float lambda, bool train, int inject, int *failed)
+// IC(0) path's prune, RCM, lower pattern and pair blocks must be built).  The
+// pivot shift lambda starts at 1 and doubles while an uncorrected pivot
+// A_PP + lambda S is not positive definite (the relaxer rule of IC(0),
+// Precondition.cu:1274-1288, applied to the factor's own pivots).
+// *failed: bit 0 non-finite network outputs, bit 1 no lambda up to 1e7.
+int lp_forward(fsd_ctx *ctx, const float4 *pos, bool train, int inject, int *failed)
 {
     LRC(lp_init(ctx));
     LearnedState &s = *ctx->lp;
@@ -1130,13 +1182,26 @@ int lp_forward(fsd_ctx *ctx, const float4 *pos, float lambda, bool train, int in
     LubTable t{ctx->d_res_dist, ctx->d_res_vals, ctx->table_min, ctx->table_dr};
     hipEvent_t pev;
     prof_begin(ctx, PROF_NET, &pev);
-    FSD_HIP(hipMemsetAsync(s.d_fail, 0, sizeof(int), ctx->stream));
     hipLaunchKernelGGL(lp_erow_kernel, dim3(nblk(N, 256)), dim3(256), 0, ctx->stream, N, ctx->d_lrow, s.erow);
     if (E)
         hipLaunchKernelGGL(lp_edge_feat_kernel, dim3(nblk(E, 256)), dim3(256), 0, ctx->stream, E, s.erow, ctx->d_lcol,
                            ctx->d_perm, pos, ctx->box, t, s.geo, s.xe0);
-    hipLaunchKernelGGL(lp_node_feat_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_lrow,
-                       ctx->d_urow, ctx->d_Adiag, lambda, s.xn0, s.Dtinv, s.d_fail);
+    float lambda = 1.0f;
+    for (;;) {
+        hipLaunchKernelGGL(lp_fill_int_kernel, dim3(1), dim3(64), 0, ctx->stream, 4, 0, s.d_fail);
+        hipLaunchKernelGGL(lp_node_feat_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N, ctx->d_lrow,
+                           ctx->d_urow, ctx->d_Adiag, lambda, s.xn0, s.Dtinv, s.d_fail);
+        FSD_HIP(hipMemcpyAsync(s.h_fail, s.d_fail, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        FSD_HIP(hipStreamSynchronize(ctx->stream));
+        if (!(s.h_fail[0] & 2)) break;
+        lambda *= 2.0f;
+        if (lambda > 1.0e7f) {
+            prof_end(ctx, PROF_NET, pev);
+            *failed = 2;
+            return 0;
+        }
+    }
+    s.lambda = lambda;
     // encoders
     layer_fwd(ctx, lp::L_ENC_N, N, s.xn0, nullptr, train ? s.zn0 : nullptr, nullptr, s.h[0]);
     layer_fwd(ctx, lp::L_ENC_E, E, s.xe0, nullptr, train ? s.ze0 : nullptr, nullptr, s.e[0]);
@@ -1166,12 +1231,14 @@ int lp_forward(fsd_ctx *ctx, const float4 *pos, float lambda, bool train, int in
     if (E)
         hipLaunchKernelGGL(lp_edge_W_kernel, dim3(nblk(E * 36, 256)), dim3(256), 0, ctx->stream, E, s.geo, s.oute, s.W);
     hipLaunchKernelGGL(lp_node_D_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow, ctx->d_lcol,
-                       ctx->d_Adiag, lambda, s.outn, s.oute, ctx->d_Ablk, s.Dtinv, s.Dinv, s.D, s.d_fail);
+                       ctx->d_Adiag, lambda, s.outn, s.oute, ctx->d_Ablk, s.Dtinv, s.Dinv, s.D, s.lvl, s.d_fail);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_NET, pev);
-    FSD_HIP(hipMemcpyAsync(s.h_fail, s.d_fail, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    FSD_HIP(hipMemcpyAsync(s.h_fail, s.d_fail, sizeof(int) * 3, hipMemcpyDeviceToHost, ctx->stream));
     FSD_HIP(hipStreamSynchronize(ctx->stream));
     *failed = s.h_fail[0];
+    s.shrunk = s.h_fail[1];
+    s.plain = s.h_fail[2];
     return 0;
 }
 
@@ -1224,8 +1291,8 @@ static int lp_backward(fsd_ctx *ctx, bool dbar_direct)
     const int64_t E = ctx->pch.nlow;
     // assembly backward
     hipLaunchKernelGGL(lp_node_D_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow,
-                       ctx->d_lcol, s.Dinv, s.gDinv, dbar_direct ? 1 : 0, s.oute, s.outn, ctx->ichol_relaxer,
-                       ctx->d_Ablk, s.Dtinv, s.goutn, s.goute);
+                       ctx->d_lcol, s.Dinv, s.gDinv, dbar_direct ? 1 : 0, s.oute, s.outn, s.lambda,
+                       ctx->d_Ablk, s.Dtinv, s.lvl, s.goutn, s.goute);
     if (E) hipLaunchKernelGGL(lp_edge_W_bwd_kernel, dim3(nblk(E, 128)), dim3(128), 0, ctx->stream, E, s.geo, s.oute,
                               s.gW, s.goute);
     // heads

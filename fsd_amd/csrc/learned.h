@@ -51,7 +51,10 @@ struct LearnedState {
     float *ze0 = nullptr, *e[lp::NMP + 1] = {}, *zm[lp::NMP] = {}, *m[lp::NMP] = {}, *oute = nullptr;
     float *geo = nullptr, *W = nullptr;
     int *erow = nullptr;                         // row particle of every lower block
-    int *d_fail = nullptr, *h_fail = nullptr;
+    int *d_fail = nullptr, *h_fail = nullptr;   // [0] failure bits, [1] pivots above level 0, [2] at level 4
+    int *lvl = nullptr;                          // per particle: the level of its pivot safeguard
+    float lambda = 1.0f;                         // pivot shift of the last factor (doubled until A_PP + lambda S > 0)
+    int shrunk = 0, plain = 0;                   // pivots of the last factor above level 0 / at level 4
     // backward buffers
     float *gh = nullptr, *gh2 = nullptr, *ge = nullptr, *gm = nullptr, *gzn = nullptr, *gze = nullptr;
     float *gxe = nullptr, *gxn = nullptr, *goute = nullptr, *goutn = nullptr, *gW = nullptr, *gDinv = nullptr;

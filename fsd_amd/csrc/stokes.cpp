@@ -535,7 +535,7 @@ static int precond_finish(fsd_ctx *ctx, const float4 *pos)
         learned = lp_have_weights(ctx) && !ctx->lp_force_ichol;
         if (learned) {
             int failed = 0;
-            RC(lp_forward(ctx, pos, ctx->ichol_relaxer, false, ctx->lp_inject, &failed));
+            RC(lp_forward(ctx, pos, false, ctx->lp_inject, &failed));
             ctx->lp_inject = -1;
             if (failed) {
                 ctx->lp_pivot_fallbacks++;
@@ -1761,12 +1761,17 @@ int fsd_learned_inject_pivot(fsd_ctx *ctx, int32_t particle)
     return 0;
 }
 
-int fsd_learned_stats(fsd_ctx *ctx, int64_t *h_out4)
+int fsd_learned_stats(fsd_ctx *ctx, int64_t *h_out6)
 {
-    h_out4[0] = ctx->lp_steps;
-    h_out4[1] = ctx->lp_pivot_fallbacks;
-    h_out4[2] = ctx->lp_iter_fallbacks;
-    h_out4[3] = ctx->lp_calibrations;
+    h_out6[0] = ctx->lp_steps;
+    h_out6[1] = ctx->lp_pivot_fallbacks;
+    h_out6[2] = ctx->lp_iter_fallbacks;
+    h_out6[3] = ctx->lp_calibrations;
+    float lambda = 1.0f;
+    int shrunk = 0, plain = 0;
+    lp_last_factor(ctx, &lambda, &shrunk, &plain);
+    h_out6[4] = shrunk;
+    h_out6[5] = plain;
     return 0;
 }
 
@@ -1781,7 +1786,7 @@ int fsd_learned_loss_grad(fsd_ctx *ctx, const float *d_pos4, int32_t nsamp, cons
     RC(precond_upload_blocks(ctx, pos));
     prof_end(ctx, PROF_PRECOND_BUILD, ctx->pc_ev);
     int failed = 0;
-    RC(lp_forward(ctx, pos, ctx->ichol_relaxer, true, -1, &failed));
+    RC(lp_forward(ctx, pos, true, -1, &failed));
     lp_set_active(ctx, !failed);
     ctx->precond_valid = !failed;
     if (failed) {
@@ -1808,9 +1813,9 @@ int fsd_learned_factor_loss(fsd_ctx *ctx, const float *d_pos4, int32_t want_grad
     precond_plan(ctx);
     RC(precond_upload_blocks(ctx, pos));
     prof_end(ctx, PROF_PRECOND_BUILD, ctx->pc_ev);
-    // the targets: IC(0) with the relaxer doubling, on this pattern
+    // the targets: IC(0) with the relaxer doubling from 1, on this pattern
     ctx->ichol_attempts = 0;
-    float lambda = ctx->ichol_relaxer;
+    float lambda = 1.0f;
     for (;;) {
         int failed = 0;
         RC(k_ichol(ctx, lambda, &failed));
@@ -1820,7 +1825,7 @@ int fsd_learned_factor_loss(fsd_ctx *ctx, const float *d_pos4, int32_t want_grad
         if (lambda > 1.0e7f) { ctx->err = "IC(0) did not succeed for any relaxer"; return FSD_E_ICHOL; }
     }
     int failed = 0;
-    RC(lp_forward(ctx, pos, lambda, true, -1, &failed));
+    RC(lp_forward(ctx, pos, true, -1, &failed));
     lp_set_active(ctx, false);
     ctx->precond_valid = false;
     if (failed) {

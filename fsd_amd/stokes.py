@@ -367,10 +367,14 @@ class Stokes:
         self._check(self.lib.fsd_learned_inject_pivot(self.ctx, int(particle)))
 
     def learned_stats(self):
-        out = np.zeros(4, dtype=np.int64)
+        """counters of the learned preconditioner: steps, fallbacks to IC(0) (non-finite
+        network outputs, more iterations than IC(0)), calibration steps, and of the last
+        factor the pivots built with a reduced fill term and those replaced by the
+        uncorrected pivot A_PP + lambda S"""
+        out = np.zeros(6, dtype=np.int64)
         self._check(self.lib.fsd_learned_stats(self.ctx, out.ctypes.data_as(C.c_void_p)))
         return dict(steps=int(out[0]), pivot_fallbacks=int(out[1]), iteration_fallbacks=int(out[2]),
-                    calibrations=int(out[3]))
+                    calibrations=int(out[3]), reduced_fill_pivots=int(out[4]), plain_pivots=int(out[5]))
 
     def learned_export(self, n_lower):
         N = self.N

@@ -296,10 +296,13 @@ int fsd_learned_get_grad(fsd_ctx *ctx, float *h_g, int64_t n);
 int fsd_learned_get_adam(fsd_ctx *ctx, float *h_m, float *h_v, int64_t n);
 int fsd_learned_set_adam(fsd_ctx *ctx, const float *h_m, const float *h_v, int64_t n);
 int fsd_learned_set_calibration(fsd_ctx *ctx, int32_t every);
-/* the next learned factor gets a non-positive pivot at this particle (failure-path tests) */
+/* the next learned factor gets a non-finite pivot shift at this particle, which makes
+ * the build fall back to IC(0) (failure-path tests) */
 int fsd_learned_inject_pivot(fsd_ctx *ctx, int32_t particle);
-/* h_out4: steps, pivot fallbacks, iteration fallbacks, calibration steps */
-int fsd_learned_stats(fsd_ctx *ctx, int64_t *h_out4);
+/* h_out6: steps, pivot fallbacks, iteration fallbacks, calibration steps, and of the
+ * last learned factor the pivots built with a reduced fill term (safeguard level 1-3)
+ * and those replaced by A_PP + lambda S (level 4) */
+int fsd_learned_stats(fsd_ctx *ctx, int64_t *h_out6);
 /* network outputs (32 per lower block, 32 per particle), pair blocks W (36 per lower
  * block) and inverse pivots D^-1 (36 per particle) of the active learned factor, in
  * the pattern's order (fsd_get_precond's permuted lower blocks); any may be null */

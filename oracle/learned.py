@@ -59,17 +59,26 @@ def bf16(x):
     return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
 
 
-GAMMA0 = 4.0
+GAMMA0 = -0.036004   # logit(0.491): gamma(0) = 0.982
+LEVELS = 4   # pivot safeguard: fill scales 1, 1/2, 1/4, 1/8, then the uncorrected pivot
+
+
+def _is_pd(M):
+    try:
+        np.linalg.cholesky(M)
+        return True
+    except np.linalg.LinAlgError:
+        return False
 
 
 def gamma(c):
-    """fill weight of a pair from its raw output: sigmoid(c + 4)"""
-    return 1.0 / (1.0 + np.exp(-(c + GAMMA0)))
+    """fill weight of a pair from its raw output: 2 sigmoid(c + c0), in (0, 2)"""
+    return 2.0 / (1.0 + np.exp(-(c + GAMMA0)))
 
 
 def gamma_d(c):
-    sg = gamma(c)
-    return sg * (1.0 - sg)
+    sg = 0.5 * gamma(c)
+    return 2.0 * sg * (1.0 - sg)
 
 
 def silu(z):
@@ -134,8 +143,8 @@ def table_lookup(dist, dist_tab, vals, table_min=1e-4, table_dr=0.004305):
 class LearnedOracle:
     """The learned factor on the pattern of an oracle whose build_precond has run."""
 
-    def __init__(self, orc, lam=1.0, k=3):
-        self.N, self.k, self.lam = orc.N, k, lam
+    def __init__(self, orc, lam=None, k=3):
+        self.N, self.k = orc.N, k
         N = self.N
         self.perm = orc.perm.astype(np.int64)
         rp, col, val = orc.B_rowptr, orc.B_col, orc.B_val
@@ -181,8 +190,17 @@ class LearnedOracle:
         self.fn[:, 1] = 0.25 * nup
         self.fn[:, 2] = 0.25 * np.log1p(np.maximum(trU, 0) / 3.0)
         self.fn[:, 3] = 0.25 * np.log1p(np.maximum(trW, 0) / 3.0)
+        # the pivot shift: lambda from 1, doubled while some A_PP + lambda S is not positive
+        # definite (k_learned.hip lp_forward)
+        if lam is None:
+            lam = 1.0
+            while not all(_is_pd(Ad[P] + lam * np.diag(S6)) for P in range(N)):
+                lam *= 2.0
+                assert lam <= 1e7
+        self.lam = lam
         self.Dt = Ad + lam * np.einsum('c,cd->cd', S6, np.eye(6))[None]
         self.Dtinv = np.linalg.inv(self.Dt)
+        self.levels = np.zeros(N, dtype=np.int64)
         self.Bk = basis(self.geo)
 
     # ---- network
@@ -221,25 +239,34 @@ class LearnedOracle:
 
     # ---- factor
     def assemble(self, oute, outn):
-        """W [E,6,6], D, Dinv [N,6,6] and whether a pivot is non-positive.  The raw outputs
-        enter bounded: multipliers 1 + tanh(c), fill weights sigmoid(c + 4), pivot shift
-        lambda e^a S (k_learned.hip lp_edge_W_kernel, lp_node_D_kernel)"""
+        """W [E,6,6], D, Dinv [N,6,6] and whether a network output is not finite.  The raw
+        outputs enter bounded: multipliers 1 + tanh(c), fill weights 2 sigmoid(c + c0), pivot
+        shift lambda e^a S (k_learned.hip lp_edge_W_kernel, lp_node_D_kernel).  Pivot
+        safeguard: D_P = A_PP + lambda e^a S - s F_P with s the first of 1, 1/2, 1/4, 1/8
+        for which D_P is positive definite (level 0 .. 3), else A_PP + lambda S (level 4);
+        the levels are kept in self.levels for the backward pass."""
         W = np.einsum('ek,ekac->eac', 1.0 + np.tanh(oute[:, :6]), self.Bk) if self.E else np.zeros((0, 6, 6))
-        D = self.Adiag.copy()
-        D[:, [0, 1, 2], [0, 1, 2]] += self.lam * np.exp(outn[:, 0:1])
-        D[:, [3, 4, 5], [3, 4, 5]] += self.lam * (4.0 / 3.0) * np.exp(outn[:, 1:2])
+        B = self.Adiag.copy()
+        B[:, [0, 1, 2], [0, 1, 2]] += self.lam * np.exp(outn[:, 0:1])
+        B[:, [3, 4, 5], [3, 4, 5]] += self.lam * (4.0 / 3.0) * np.exp(outn[:, 1:2])
+        F = np.zeros_like(B)
         if self.E:   # the fill term uses the pair blocks A (the pivots do not depend on W)
             T = np.einsum('eaq,eqr,ecr->eac', self.Ablk, self.Dtinv[self.cols], self.Ablk)
-            np.add.at(D, self.rows, -gamma(oute[:, 6])[:, None, None] * T)
-        Dl = np.tril(D)
-        Dsym = Dl + np.transpose(np.tril(D, -1), (0, 2, 1))
-        failed = False
-        try:
-            np.linalg.cholesky(Dsym)
-        except np.linalg.LinAlgError:
-            failed = True
-        Dinv = np.linalg.inv(Dsym)
-        return W, Dsym, Dinv, failed
+            np.add.at(F, self.rows, gamma(oute[:, 6])[:, None, None] * T)
+        failed = not (np.all(np.isfinite(outn[:, :2])) and np.all(np.isfinite(oute[:, :7])))
+        D = np.empty_like(B)
+        for P in range(self.N):
+            self.levels[P] = LEVELS
+            for j in range(LEVELS):
+                Dj = B[P] - 0.5 ** j * F[P]
+                Dj = np.tril(Dj) + np.tril(Dj, -1).T
+                if _is_pd(Dj):
+                    self.levels[P], D[P] = j, Dj
+                    break
+            if self.levels[P] == LEVELS:
+                D[P] = self.Dt[P]
+        Dinv = np.linalg.inv(D)
+        return W, D, Dinv, failed
 
     def lower(self, W, v):
         out = np.zeros((self.N, 6))
@@ -381,13 +408,15 @@ class LearnedOracle:
         N = self.N
         goutn = np.zeros((N, ON))
         outn = c["outn"]
-        goutn[:, 0] = self.lam * np.exp(outn[:, 0]) * np.einsum('pii->p', Db[:, 0:3, 0:3])
-        goutn[:, 1] = self.lam * (4.0 / 3.0) * np.exp(outn[:, 1]) * np.einsum('pii->p', Db[:, 3:6, 3:6])
+        live = self.levels < LEVELS          # level 4 pivots do not depend on the outputs
+        sc = np.where(live, 0.5 ** np.minimum(self.levels, LEVELS - 1), 0.0)
+        goutn[:, 0] = live * self.lam * np.exp(outn[:, 0]) * np.einsum('pii->p', Db[:, 0:3, 0:3])
+        goutn[:, 1] = live * self.lam * (4.0 / 3.0) * np.exp(outn[:, 1]) * np.einsum('pii->p', Db[:, 3:6, 3:6])
         goute = np.zeros((self.E, OE))
         oute = c["oute"]
         if self.E:
             T = np.einsum('eaq,eqr,ecr->eac', self.Ablk, self.Dtinv[self.cols], self.Ablk)
-            goute[:, 6] = -np.einsum('eac,eac->e', Db[self.rows], T) * gamma_d(oute[:, 6])
+            goute[:, 6] = -sc[self.rows] * np.einsum('eac,eac->e', Db[self.rows], T) * gamma_d(oute[:, 6])
             goute[:, :6] = np.einsum('eac,ekac->ek', gW, self.Bk) * (1.0 - np.tanh(oute[:, :6]) ** 2)
         return self._mlp_backward(c, goute, goutn, emulate)
 

@@ -97,3 +97,74 @@ def test_factor_loss_gradient_finite_differences(dense):
     c = lo.forward(w, emulate=False)
     W, D, Dinv, failed = lo.assemble(c["oute"], c["outn"])
     assert lo.factor_loss(w, D, W, emulate=False) < 1e-20
+
+
+def _shrunk_weights(seed, a_bias, c_bias):
+    """weights whose particle head biases a_bias (pivot shift lambda e^a) and edge fill
+    bias c_bias push pivots out of the positive definite range"""
+    from oracle.learned import init_weights, layer_offsets, L_HEAD_E, L_HEAD_N
+    w = init_weights(seed, scale_out=0.3).astype(np.float64)
+    offs, _ = layer_offsets()
+    w[offs[L_HEAD_N][1] + 0] = a_bias
+    w[offs[L_HEAD_N][1] + 1] = a_bias
+    w[offs[L_HEAD_E][1] + 6] = c_bias
+    return w
+
+
+@pytest.fixture(scope="module")
+def overfilled(dense):
+    """the dense case with every fill term A_PQ Dt_Q^-1 A_PQ^T made 12x larger (Dt^-1
+    scaled), so that with a small pivot shift the whole fill makes pivots indefinite (the
+    near-field blocks of these packings keep the unscaled pivots positive definite)"""
+    import copy
+    orc, lo, X, B = dense
+    lo4 = copy.deepcopy(lo)
+    lo4.Dtinv = 12.0 * lo.Dtinv
+    return orc, lo4
+
+
+def test_pivot_safeguard(overfilled):
+    """pivots that a small shift and the whole fill term make indefinite: the assembly
+    halves the fill term up to three times and then takes the uncorrected pivot
+    A_PP + lambda S; every pivot is positive definite and each is the one its level says"""
+    from oracle.learned import gamma, LEVELS, _is_pd
+    orc, lo = overfilled
+    w = _shrunk_weights(1, -8.0, 8.0)
+    c = lo.forward(w, emulate=False)
+    W, D, Dinv, failed = lo.assemble(c["oute"], c["outn"])
+    lv = lo.levels.copy()
+    print("lambda", lo.lam, "levels", np.bincount(lv, minlength=LEVELS + 1))
+    assert not failed and np.all(np.bincount(lv, minlength=LEVELS + 1) > 0)
+    S = np.diag([1.0, 1.0, 1.0, 4.0 / 3.0, 4.0 / 3.0, 4.0 / 3.0])
+    F = np.zeros((lo.N, 6, 6))
+    for e in range(lo.E):
+        F[lo.rows[e]] += gamma(c["oute"][e, 6]) * lo.Ablk[e] @ lo.Dtinv[lo.cols[e]] @ lo.Ablk[e].T
+    for P in range(lo.N):
+        assert _is_pd(D[P])
+        Bp = lo.Adiag[P] + lo.lam * np.diag(np.exp(np.repeat(c["outn"][P, :2], 3))) @ S
+        if lv[P] < LEVELS:
+            ref = Bp - 0.5 ** lv[P] * F[P]
+            assert np.abs(D[P] - ref).max() <= 1e-9 * np.abs(ref).max()
+            for j in range(lv[P]):      # the levels before it were not positive definite
+                assert not _is_pd(Bp - 0.5 ** j * F[P])
+        else:
+            assert np.allclose(D[P], lo.Dt[P])
+
+
+def test_safeguard_gradient_finite_differences(overfilled):
+    """the supervised objective's gradient where pivots sit at levels 1 .. 4 (scaled
+    fill gradient, none at level 4) against central differences"""
+    orc, lo = overfilled
+    Dic, Wic = lo.factor_targets(orc)
+    w = _shrunk_weights(2, -8.0, 8.0)
+    loss, g = lo.factor_loss(w, Dic, Wic, emulate=False, grad=True)
+    lv = lo.levels.copy()
+    assert lv.max() > 0
+    v = np.random.default_rng(8).standard_normal(w.size)
+    eps = 1e-7
+    fp = lo.factor_loss(w + eps * v, Dic, Wic, emulate=False)
+    assert np.array_equal(lo.levels, lv)
+    fm = lo.factor_loss(w - eps * v, Dic, Wic, emulate=False)
+    assert np.array_equal(lo.levels, lv)
+    fd = (fp - fm) / (2 * eps)
+    assert abs(fd - g @ v) < 1e-5 * abs(fd)
