@@ -42,7 +42,8 @@ GUESS_TEXT = {"previous": "previous solution", "extrapolate": "linear extrapolat
 # kernel-class name -> kernel name in rocprofv3 traces
 KERNEL_NAMES = {"spread": "spread_tile2_kernel<8, 6>", "green": "fft_x_green_kernel<9>", "contract": "contract_tile2_kernel<13, 6>",
                 "realspace": "realspace_kernel<16>", "rfu": "rfu_apply_kernel<8>", "sweep": "tp_sweep1_kernel",
-                "ichol": "ichol_level_kernel", "gmres_vec": "multidot_partial4_kernel", "ljit": "lj_fwd_kernel"}
+                "ichol": "ichol_level_kernel", "gmres_vec": "multidot_partial4_kernel", "ljit": "lj_fwd_kernel",
+                "jacobi": "jac_fwd_kernel"}
 
 
 def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5, gs_passes=2):
@@ -72,6 +73,10 @@ def algorithmic_bytes(cls, N, info, pse, st, kbar=25.5, gs_passes=2):
         # inverse pivot (144 B) once, the fixed right-hand side (t or v_k) and the previous iterate in, the
         # new iterate out (24 B each per particle)
         return 144 * (st["lower_blocks"] + N) + 3 * 24 * N
+    if cls == "jacobi":      # one block Jacobi iteration on the IC(0) factor: every folded block Z_P L_PQ
+        # (144 B) and its column index (4 B) once, the fixed right-hand side and the previous iterate in,
+        # the new iterate out (24 B each per particle), the row offsets (4 B)
+        return 148 * st["lower_blocks"] + (3 * 24 + 4) * N
     if cls == "ichol":       # one factorisation: A blocks in, L blocks out (lower and diagonal)
         return 144 * 2 * (st["lower_blocks"] + N)
     if cls == "gmres_vec":   # Gram-Schmidt of a 17N-float vector on k basis vectors, per pass h = V^T w
@@ -117,7 +122,7 @@ def run_arm(name, kw, pos, L, F, args, dom=None, log=None):
     from fsd_amd import _lib
     N = args.N
     st = Stokes(N, L, xi=0.5, error=1e-3, gmres_tol=1e-4, gmres_true_tol=1e-4, dt=1e-3, gmres_restart=args.restart,
-                guess=args.guess, gram_schmidt_passes=args.gram_schmidt, **kw)
+                guess=args.guess, gram_schmidt_passes=args.gram_schmidt, autotune=not args.no_autotune, **kw)
     state = ParticleState(pos)
     st.profile(True)
     infos = []
@@ -126,12 +131,12 @@ def run_arm(name, kw, pos, L, F, args, dom=None, log=None):
         log("# %s warmup step %d: %s" % (name, k, infos[-1]))
     totals = {c: st.profile_get(c) for c in _lib.PROF_NAMES}
     st.profile_reset()
-    classes = ["spread", "green", "contract", "realspace", "rfu", "sweep", "ichol", "gmres_vec", "ljit"]
-    if dom is None:
-        dom = max(classes, key=lambda c: totals[c][1])
+    log("# %s launch shapes (autotuner): %s" % (name, json.dumps(st.tuning())))
+    classes = ["spread", "green", "contract", "realspace", "rfu", "sweep", "ichol", "gmres_vec", "ljit", "jacobi"]
     log("# %s warmup kernel-class times (launches, ms): %s" % (name, json.dumps(totals)))
-    # timed steps: HIP events only around the dominant class, one launch in eight
-    st.profile(True, classes=None if args.profile_all else [dom], every=1 if args.profile_all else 8)
+    # timed steps: HIP events around one launch in eight of every kernel class (every launch with
+    # --profile-all); the dominant class is the one with the most sampled time in these steps
+    st.profile(True, classes=classes, every=1 if args.profile_all else 8)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     its = []
@@ -142,9 +147,12 @@ def run_arm(name, kw, pos, L, F, args, dom=None, log=None):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     timed = infos[-args.steps:]
+    if dom is None:
+        dom = max(classes, key=lambda c: st.profile_get(c)[1])
     log("# %s timed steps: iterations %s, initial residuals %s, preconditioner %s"
         % (name, its, ["%.2e" % i["initial_residual"] for i in timed], [i["precond_used"] for i in timed]))
     out = {"ms_per_step": (t1 - t0) * 1e3 / args.steps, "steps_per_s": args.steps / (t1 - t0),
+           "launch_shapes": st.tuning()[0],
            "gmres_iterations_mean": float(np.mean(its)),
            "true_residual_max": float(max(i["true_residual"] for i in timed)),
            "nlist_rebuilds": int(sum(i.get("nlist_rebuilt", 0) for i in timed)),
@@ -155,9 +163,8 @@ def run_arm(name, kw, pos, L, F, args, dom=None, log=None):
         out["fallback_pivot"] = int(sum(i["fallback"] == 1 for i in timed))
         out["fallback_iterations"] = int(sum(i["fallback"] == 2 for i in timed))
         out["calibration_steps"] = int(sum(i["precond_used"] == 0 and i["fallback"] == 0 for i in timed))
-    if args.profile_all:
-        allt = {c: st.profile_get(c) for c in _lib.PROF_NAMES}
-        log("# %s timed kernel-class times (launches, ms): %s" % (name, json.dumps(allt)))
+    allt = {c: st.profile_get(c) for c in classes}
+    log("# %s timed kernel-class times (sampled launches, ms): %s" % (name, json.dumps(allt)))
     return out, st, dom, timed[-1]
 
 
@@ -180,10 +187,12 @@ def main():
                     help="learned arm: one IC(0) step every this many steps (its iteration count caps the learned "
                          "solves); 20 puts one such step in the 20 timed steps")
     ap.add_argument("--ichol-jacobi", type=int, default=8,
-                    help="block Jacobi iterations per triangular solve of the ichol_jacobi arm")
-    ap.add_argument("--arms", default="ichol,learned",
+                    help="block Jacobi iterations per triangular solve of the ichol_jacobi arm and of the "
+                         "learned arm's IC(0) steps (calibration and fallback)")
+    ap.add_argument("--arms", default="ichol,ichol_jacobi,learned",
                     help="comma-separated arms to run (ichol, ichol_jacobi, learned); the last is `value`")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (diagnostics)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MB",
                     help="only run 5 device copies of MB megabytes (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -215,7 +224,7 @@ def main():
     arm_kw = {"ichol": dict(precond="ichol"),
               "ichol_jacobi": dict(precond="ichol_jacobi", jacobi_iters=args.ichol_jacobi),
               "learned": dict(precond="learned", jacobi_iters=args.jacobi, learned_calibration=args.calibration,
-                              refine=True)}
+                              refine=True, fallback_jacobi=args.ichol_jacobi)}
     names = [a for a in args.arms.split(",") if a]
     arms, head = {}, names[-1]
     for name in names:
@@ -235,6 +244,11 @@ def main():
     # HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary
     # (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by its reading for a plain copy)
     traffic, kname = None, KERNEL_NAMES.get(head_dom)
+    shapes = arms[head]["launch_shapes"]
+    if head_dom == "realspace":      # the autotuner's lanes per particle
+        kname = "realspace_kernel<%d>" % shapes["real_lanes"]
+    elif head_dom == "rfu":
+        kname = "rfu_apply_kernel<%d>" % shapes["rfu_lanes"]
     tfile = os.path.join(ROOT, "profiles", "traffic_N32768.json")
     if os.path.exists(tfile):
         with open(tfile) as f:

@@ -20,7 +20,7 @@ FSD_W_PIVOT = 2
 
 # kernel classes timed by fsd_profile_* (bit k of the enable mask)
 PROF_NAMES = ["spread", "green", "contract", "fft", "realspace", "rfu", "trsv", "ichol", "gmres_vec", "nlist",
-              "precond_build", "sweep", "plan_host", "finish_host", "ljit", "learned_net"]
+              "precond_build", "sweep", "plan_host", "finish_host", "ljit", "learned_net", "jacobi"]
 
 
 class FsdParams(C.Structure):
@@ -55,6 +55,8 @@ class FsdParams(C.Structure):
         ("jacobi_iters", C.c_int32),
         ("factor_sweeps", C.c_int32),
         ("refine", C.c_int32),
+        ("fallback_jacobi", C.c_int32),
+        ("autotune", C.c_int32),
     ]
 
 
@@ -134,6 +136,9 @@ SIGNATURES = [
     ("fsd_rse", _I, [_P, _P, _F, _P, _P]),
     ("fsd_mobility", _I, [_P, _P, _P, _P, _I]),
     ("fsd_build_precond", _I, [_P, _P, C.POINTER(FsdSolveStats)]),
+    ("fsd_autotune", _I, [_P, _P, _P, _P]),
+    ("fsd_get_tuning", _I, [_P, _P, _P]),
+    ("fsd_set_tuning", _I, [_P, _P]),
     ("fsd_get_precond", _I, [_P, _P, _P, C.c_int64, _P, _P, _P, _P, _P, _P, _P, C.c_int64,
                              C.POINTER(C.c_int64)]),
     ("fsd_precond_apply", _I, [_P, _P, _P]),

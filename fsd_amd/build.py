@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libfsd.so")
 ARCH = os.environ.get("FSD_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["k_pairs.hip", "k_mobility.hip", "k_precond.hip", "k_solver.hip", "k_brownian.hip", "k_jacobi.hip", "k_noalloc.hip", "k_learned.hip", "k_refine.hip"]
-HOST_HIP_SOURCES = ["stokes.cpp"]      # host code using the HIP runtime, compiled by hipcc
+HOST_HIP_SOURCES = ["stokes.cpp", "autotune.cpp"]      # host code using the HIP runtime, compiled by hipcc
 CXX_SOURCES = ["rcm.cpp", "sweep_plan.cpp", "precond_plan.cpp", "lanczos_host.cpp"]
 C_SOURCES = ["ewald_table.c"]          # binary128 table (gcc + libquadmath)
 

@@ -178,12 +178,27 @@ struct ProfSlot {
     int64_t launches;             // launches timed
     int64_t seen = 0;             // launches of the class while enabled (one in prof_every is timed)
     double ms;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    struct Pending { hipEvent_t b, e; int n; };   // event pair around n launches of the class
+    std::vector<Pending> pending;
+};
+
+// Launch shapes chosen by the autotuner (fsd_autotune): each knob selects one of the
+// kernels or launch shapes that compute the same operator.
+struct fsd_tuning {
+    int spread = 0;          // 0: spread_tile2_kernel (tile 8, P 6), 1: spread_kernel (one workgroup per tile)
+    int contract = 0;        // 0: tile contraction (contract_tile2/contract_tile, when it fits in LDS),
+                             // 1: one wave per particle (contract_kernel)
+    int xpass = 0;           // 0: x transforms fused with the Green's function (when built), 1: 3-D rocFFT
+    int real_g = 16, real_blk = 256;   // real-space sum: lanes per particle (4, 8, 16), workgroup size
+    int rfu_g = 8, rfu_blk = 256;      // R_FU product: lanes per particle (8, 16), workgroup size
+    int tuned = 0;
+    float us[3] = {0.f, 0.f, 0.f};     // best time per apply (us): wave part, real part, R_FU
 };
 
 struct fsd_ctx {
     fsd_params p;
     fsd_pse_info pse;
+    fsd_tuning tune;
     std::string err;
     hipStream_t stream = nullptr;
     BoxF box;
@@ -281,6 +296,7 @@ struct fsd_ctx {
     // application by Jacobi iterations (precond modes 1 and 2)
     int precond_mode = 0;                                // 0: exact sweeps, 1: IC(0) + Jacobi, 2: learned + Jacobi
     int jac_k = 0;                                       // Jacobi iterations per triangular solve
+    int ic_jac_k = 0;                                    // the same for IC(0) (modes 1 and 2); 0: exact solves
     float *d_Fblk = nullptr;                             // 36 per lower block: Z_P L_PQ
     int *d_lrowidx = nullptr;                            // row P of every lower block
     float *d_jac = nullptr;                              // 4 x 6N iterates
@@ -344,10 +360,14 @@ static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
 // profiling scopes
 void prof_begin(fsd_ctx *ctx, int slot, hipEvent_t *ev);
 void prof_end(fsd_ctx *ctx, int slot, hipEvent_t ev);
+// end of a scope that enclosed n back-to-back launches of the class (one event pair for the
+// batch, so that the event overhead is shared by n launches of a short kernel)
+void prof_end_n(fsd_ctx *ctx, int slot, hipEvent_t ev, int n);
 enum { PROF_SPREAD = 0, PROF_GREEN, PROF_CONTRACT, PROF_FFT, PROF_REAL, PROF_RFU, PROF_TRSV, PROF_ICHOL,
        PROF_GMRES_VEC, PROF_NLIST, PROF_PRECOND_BUILD, PROF_SWEEP, PROF_PLAN_HOST, PROF_FINISH_HOST,
-       PROF_LJIT,          // one block Jacobi iteration of the learned factor (forward or backward)
+       PROF_LJIT,          // the forward block Jacobi iterations of the learned factor (k per application)
        PROF_NET,           // features, network and factor assembly of the learned factor
+       PROF_JAC,           // the forward block Jacobi iterations on the IC(0) factor (k per application)
        PROF_COUNT };
 // host wall time of a phase, added to a slot when that slot is enabled
 void prof_host(fsd_ctx *ctx, int slot, double ms);
@@ -398,6 +418,7 @@ int k_euler_shear(fsd_ctx *ctx, float4 *pos, float *ori, float4 *pos_gb, int *im
 int k_axpby(fsd_ctx *ctx, int64_t n, float a, const float *x, float b, const float *y, float *z);
 int k_copy(fsd_ctx *ctx, int64_t n, const float *x, float *y);
 int k_fill(fsd_ctx *ctx, int64_t n, float v, float *y);
+int autotune_run(fsd_ctx *ctx, const float4 *pos);
 int k_multidot(fsd_ctx *ctx, int64_t n, int k, const float *V, int64_t ldv, const float *w, double *out);
 int k_multi_axpy(fsd_ctx *ctx, int64_t n, int k, const float *V, int64_t ldv, const double *h, float *w);
 int k_nrm2(fsd_ctx *ctx, int64_t n, const float *x, double *out);

@@ -157,16 +157,19 @@ int k_jacobi_pack(fsd_ctx *ctx)
 // buffer that holds u
 static const float *jac_sweeps(fsd_ctx *ctx, float *const buf[4])
 {
-    const int N = ctx->N, k = ctx->jac_k;
+    const int N = ctx->N, k = ctx->ic_jac_k;
     const dim3 g(jac_blocks(N)), b(256);
     const float *c = buf[0];
     const float *y = c;
     float *fo[2] = {buf[1], buf[2]};
+    hipEvent_t e1;
+    prof_begin(ctx, PROF_JAC, &e1);   // the k forward iterations, one event pair
     for (int it = 0; it < k; ++it) {
         float *out = fo[it & 1];
         hipLaunchKernelGGL(jac_fwd_kernel, g, b, 0, ctx->stream, N, ctx->d_lrow, ctx->d_lcol, ctx->d_Fblk, c, y, out);
         y = out;
     }
+    prof_end_n(ctx, PROF_JAC, e1, k);
     // the backward iterates never overwrite y: they alternate between buf[3] and the forward buffer y does not use
     float *bo[2] = {buf[3], (y == buf[1]) ? buf[2] : buf[1]};
     const float *u = y;

@@ -1255,25 +1255,22 @@ int lp_apply(fsd_ctx *ctx, const float *b0, const float *b1, const float *bsaddl
     float *xb[2] = {ctx->d_jac + 18 * (int64_t)N, ctx->d_jac + 12 * (int64_t)N};
     hipLaunchKernelGGL(lj_pre_kernel, g, b, 0, ctx->stream, N, ctx->d_perm, s.Dinv, b0, b1, t, v[0]);
     int cur = 0;
+    hipEvent_t e1;
+    prof_begin(ctx, PROF_LJIT, &e1);   // the k forward iterations, one event pair
     for (int it = 0; it < k; ++it) {
-        hipEvent_t e1;
-        prof_begin(ctx, PROF_LJIT, &e1);
         hipLaunchKernelGGL(lj_fwd_kernel, g, b, 0, ctx->stream, N, ctx->d_lrow, ctx->d_lcol, s.W, s.Dinv, t, v[cur],
                            (float *)nullptr, v[cur ^ 1]);
-        prof_end(ctx, PROF_LJIT, e1);
         cur ^= 1;
     }
+    prof_end_n(ctx, PROF_LJIT, e1, k);
     const float *vk = v[cur];
     // backward iterates alternate between the two buffers vk does not occupy
     float *bo[2] = {xb[0], (vk == v[0]) ? v[1] : v[0]};
     const float *xc = vk;
     for (int it = 0; it < k; ++it) {
         float *out = bo[it & 1];
-        hipEvent_t e1;
-        prof_begin(ctx, PROF_LJIT, &e1);
         hipLaunchKernelGGL(lj_bwd_kernel, g, b, 0, ctx->stream, N, ctx->d_urow, ctx->d_ucol, ctx->d_upos, s.W, s.Dinv,
                            vk, xc, (float *)nullptr, out);
-        prof_end(ctx, PROF_LJIT, e1);
         xc = out;
     }
     hipLaunchKernelGGL(lj_post_kernel, g, b, 0, ctx->stream, N, ctx->d_perm, xc, bsaddle, x, mode);

@@ -1086,7 +1086,7 @@ struct RealParams {
 
 // Mobility_RealSpace_kernel (Mobility.cu:1046-1226) + LinearCombination/Add4
 // + Mobility_D2WE_kernel + Saddle_MakeGeneralizedU_kernel (+ the saddle's
-// x_U term), G = 16 lanes per particle.
+// x_U term), G lanes per particle (4, 8 or 16, the autotuner's choice; 16 by default).
 // Particles are taken in cell order (`order`, the neighbour list's cell-sorted
 // particles) with each XCD given one contiguous range of workgroups, so the
 // neighbours gathered by concurrent particles share that XCD's L2.
@@ -1230,11 +1230,12 @@ int k_wave_contract(fsd_ctx *ctx, const float4 *pos, bool inverse_yz)
     }
     prof_end(ctx, PROF_FFT, ev);
     prof_begin(ctx, PROF_CONTRACT, &ev);
-    if (ctx->contract_lds && ctx->contract_split)
+    const bool tiled = ctx->tune.contract == 0 && ctx->contract_lds;
+    if (tiled && ctx->contract_split)
         hipLaunchKernelGGL((contract_tile2_kernel<13, 6>), dim3(2 * ctx->ntiles), dim3(512), ctx->contract_lds,
                            ctx->stream, g, pos, ctx->d_porig, ctx->d_ostart, ctx->d_olist, ctx->d_grid, (int64_t)ctx->M,
                            ps.quadW * ps.prefac, ctx->d_wave);
-    else if (ctx->contract_lds)
+    else if (tiled)
         hipLaunchKernelGGL(contract_tile_kernel, dim3(ctx->ntiles), dim3(512), ctx->contract_lds, ctx->stream, g, pos,
                            ctx->d_porig, ctx->d_ostart, ctx->d_olist, ctx->d_grid, (int64_t)ctx->M,
                            ps.quadW * ps.prefac, ctx->d_wave);
@@ -1269,7 +1270,7 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
         }
         hipEvent_t ev;
         prof_begin(ctx, PROF_SPREAD, &ev);
-        if (ctx->tile == 8 && ctx->pse.P == 6)
+        if (ctx->tile == 8 && ctx->pse.P == 6 && ctx->tune.spread == 0)
             hipLaunchKernelGGL((spread_tile2_kernel<8, 6>), dim3(ctx->ntiles), dim3(512), 0, ctx->stream, g, pos,
                                ctx->d_porig, rec, ctx->d_ostart, ctx->d_olist, ctx->d_grid, (int64_t)ctx->M);
         else
@@ -1278,7 +1279,7 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
                                (int64_t)ctx->M);
         FSD_HIP(hipGetLastError());
         prof_end(ctx, PROF_SPREAD, ev);
-        const bool fx = ctx->fx_B > 0 && ctx->plan_fwd2 && ctx->plan_inv2;
+        const bool fx = ctx->tune.xpass == 0 && ctx->fx_B > 0 && ctx->plan_fwd2 && ctx->plan_inv2;
         prof_begin(ctx, PROF_FFT, &ev);
         {
             void *in[1] = {ctx->d_grid};
@@ -1314,9 +1315,19 @@ int k_mobility(fsd_ctx *ctx, const float4 *pos, const float *gF, float *gU, int 
     const int *order = (ctx->nlist_valid && ctx->use_cells) ? ctx->d_cell_list : nullptr;
     hipEvent_t ev;
     prof_begin(ctx, PROF_REAL, &ev);
-    hipLaunchKernelGGL(realspace_kernel<16>, dim3(nblk((int64_t)N * 16, 256)), dim3(256), 0, ctx->stream, rp, pos,
-                       rec, reinterpret_cast<const float4 *>(ctx->d_ewald), ctx->d_nneigh, ctx->d_nlist, ctx->maxn, N,
-                       wave, addU, (parts & 1) ? 1 : 0, gU, order);
+    const int G = ctx->tune.real_g, blk = ctx->tune.real_blk;
+    const dim3 rg(nblk((int64_t)N * G, blk)), rb(blk);
+    const float4 *tab = reinterpret_cast<const float4 *>(ctx->d_ewald);
+    const int ur = (parts & 1) ? 1 : 0;
+    if (G == 4)
+        hipLaunchKernelGGL(realspace_kernel<4>, rg, rb, 0, ctx->stream, rp, pos, rec, tab, ctx->d_nneigh, ctx->d_nlist,
+                           ctx->maxn, N, wave, addU, ur, gU, order);
+    else if (G == 8)
+        hipLaunchKernelGGL(realspace_kernel<8>, rg, rb, 0, ctx->stream, rp, pos, rec, tab, ctx->d_nneigh, ctx->d_nlist,
+                           ctx->maxn, N, wave, addU, ur, gU, order);
+    else
+        hipLaunchKernelGGL(realspace_kernel<16>, rg, rb, 0, ctx->stream, rp, pos, rec, tab, ctx->d_nneigh,
+                           ctx->d_nlist, ctx->maxn, N, wave, addU, ur, gU, order);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_REAL, ev);
     return 0;

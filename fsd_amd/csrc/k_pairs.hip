@@ -404,7 +404,7 @@ int k_lub_pairs_into(fsd_ctx *ctx, const float4 *pos, int maxlub, int *lub_n, in
     return 0;
 }
 
-// R_FU apply: G = 8 lanes per particle.  out[6i+r] = sign_out * (R U)_ir
+// R_FU apply: G lanes per particle (8 or 16: lanes 0-5 write the rows).  out[6i+r] = sign_out * (R U)_ir
 // (+ xF[6i+r] when xF != nullptr, i.e. the saddle row  x_F - R_FU x_U).
 template <int G>
 __global__ void __launch_bounds__(256) rfu_apply_kernel(int N, const float *U, const int *lub_n, const int *lub_j,
@@ -469,9 +469,14 @@ int k_rfu_apply(fsd_ctx *ctx, const float *U, float *F, const float *xF, float s
     const int N = ctx->N;
     hipEvent_t ev;
     prof_begin(ctx, PROF_RFU, &ev);
-    hipLaunchKernelGGL(rfu_apply_kernel<8>, dim3(nblk((int64_t)N * 8, 256)), dim3(256), 0, ctx->stream, N, U,
-                       ctx->d_lub_n, ctx->d_lub_j, ctx->d_lub_a, ctx->d_lub_b, ctx->d_lub_self, ctx->maxlub, xF,
-                       sign_out, F, order);
+    const int G = ctx->tune.rfu_g, blk = ctx->tune.rfu_blk;
+    const dim3 rg(nblk((int64_t)N * G, blk)), rb(blk);
+    if (G == 16)
+        hipLaunchKernelGGL(rfu_apply_kernel<16>, rg, rb, 0, ctx->stream, N, U, ctx->d_lub_n, ctx->d_lub_j,
+                           ctx->d_lub_a, ctx->d_lub_b, ctx->d_lub_self, ctx->maxlub, xF, sign_out, F, order);
+    else
+        hipLaunchKernelGGL(rfu_apply_kernel<8>, rg, rb, 0, ctx->stream, N, U, ctx->d_lub_n, ctx->d_lub_j,
+                           ctx->d_lub_a, ctx->d_lub_b, ctx->d_lub_self, ctx->maxlub, xF, sign_out, F, order);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_RFU, ev);
     return 0;

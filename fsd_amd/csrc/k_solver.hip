@@ -215,7 +215,7 @@ int k_saddle_precond_tp(fsd_ctx *ctx, const float *b, float *x);
 int k_saddle_precond(fsd_ctx *ctx, const float *b, float *x)
 {
     if (ctx->precond_mode == 2 && lp_active(ctx)) return lp_apply(ctx, b, b + 11 * (int64_t)ctx->N, b, x, 1);
-    if (ctx->precond_mode == 1) return k_saddle_precond_jacobi(ctx, b, x);
+    if (ctx->ic_jac_k > 0) return k_saddle_precond_jacobi(ctx, b, x);
     // one sweep pair on d = b_F - b_U when the one-launch sweeps are built
     const int tp = k_saddle_precond_tp(ctx, b, x);
     if (tp <= 0) return tp;

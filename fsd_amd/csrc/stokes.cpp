@@ -24,7 +24,7 @@ void fsd_ewald_self(float xi, float self_out[2]);
 // ------------------------------------------------------------ profiling
 static const char *kProfNames[PROF_COUNT] = {"spread", "green", "contract", "fft", "realspace", "rfu",
                                              "trsv", "ichol", "gmres_vec", "nlist", "precond_build", "sweep",
-                                             "plan_host", "finish_host", "ljit", "learned_net"};
+                                             "plan_host", "finish_host", "ljit", "learned_net", "jacobi"};
 
 void prof_host(fsd_ctx *ctx, int slot, double ms)
 {
@@ -42,14 +42,16 @@ void prof_begin(fsd_ctx *ctx, int slot, hipEvent_t *ev)
     hipEventRecord(*ev, ctx->stream);
 }
 
-void prof_end(fsd_ctx *ctx, int slot, hipEvent_t ev)
+void prof_end_n(fsd_ctx *ctx, int slot, hipEvent_t ev, int n)
 {
     if (!(ctx->prof & (1u << slot)) || !ev) return;
     hipEvent_t e2;
     hipEventCreate(&e2);
     hipEventRecord(e2, ctx->stream);
-    ctx->slots[slot].pending.push_back({ev, e2});
+    ctx->slots[slot].pending.push_back({ev, e2, n});
 }
+
+void prof_end(fsd_ctx *ctx, int slot, hipEvent_t ev) { prof_end_n(ctx, slot, ev, 1); }
 
 static void prof_collect(fsd_ctx *ctx)
 {
@@ -57,11 +59,11 @@ static void prof_collect(fsd_ctx *ctx)
     for (auto &s : ctx->slots) {
         for (auto &pr : s.pending) {
             float ms = 0.f;
-            hipEventElapsedTime(&ms, pr.first, pr.second);
+            hipEventElapsedTime(&ms, pr.b, pr.e);
             s.ms += ms;
-            s.launches += 1;
-            hipEventDestroy(pr.first);
-            hipEventDestroy(pr.second);
+            s.launches += pr.n;
+            hipEventDestroy(pr.b);
+            hipEventDestroy(pr.e);
         }
         s.pending.clear();
     }
@@ -211,10 +213,10 @@ static int alloc_all(fsd_ctx *ctx)
     if (ctx->tile + ps.P - 1 > 32) { ctx->err = "spreading support P too large for the tile halo"; return FSD_E_PARAM; }
     ctx->contract_lds = k_contract_lds(ctx);
     // FSD_TRSV_LEVELS=1 runs the triangular sweeps one launch per level; IC(0) applied by Jacobi
-    // iterations needs no sweep plans (the learned mode keeps them for its IC(0) fallback)
+    // iterations needs no sweep plans (the learned mode keeps them for an exact IC(0) fallback)
     {
         const char *lv = getenv("FSD_TRSV_LEVELS");
-        if ((lv && lv[0] == '1') || ctx->precond_mode == 1) ctx->tp_R = 0;
+        if ((lv && lv[0] == '1') || ctx->ic_jac_k > 0) ctx->tp_R = 0;
         else RC(k_sweep_setup(ctx));
         // FSD_SWEEP_JOINT=1 runs both right-hand sides of a sweep in one workgroup
         const char *jt = getenv("FSD_SWEEP_JOINT");
@@ -382,7 +384,7 @@ static void free_all(fsd_ctx *ctx)
     if (ctx->info_fwd) rocfft_execution_info_destroy(ctx->info_fwd);
     if (ctx->info_inv) rocfft_execution_info_destroy(ctx->info_inv);
     for (auto &s : ctx->slots)
-        for (auto &pr : s.pending) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
+        for (auto &pr : s.pending) { hipEventDestroy(pr.b); hipEventDestroy(pr.e); }
 }
 
 // ------------------------------------------------------------ preconditioner
@@ -519,7 +521,7 @@ static int precond_ichol(fsd_ctx *ctx)
     }
     ctx->ichol_relaxer = lambda;
     lp_set_active(ctx, false);
-    if (ctx->precond_mode == 1) RC(k_jacobi_pack(ctx));
+    if (ctx->ic_jac_k > 0) RC(k_jacobi_pack(ctx));
     else if (ctx->pch.tp_ok) RC(k_sweep_pack(ctx));
     return 0;
 }
@@ -1066,8 +1068,10 @@ int fsd_create(const fsd_params *p, fsd_ctx **out)
         *out = ctx;
         return FSD_E_PARAM;
     }
+    if (q.fallback_jacobi < 0) q.fallback_jacobi = 0;
     ctx->precond_mode = q.precond;
     ctx->jac_k = q.jacobi_iters;
+    ctx->ic_jac_k = q.precond == 1 ? q.jacobi_iters : (q.precond == 2 ? q.fallback_jacobi : 0);
     if (q.N <= 0 || q.xi <= 0.f || q.error <= 0.f || q.error >= 1.f || q.Lx <= 0.f || q.Ly <= 0.f || q.Lz <= 0.f) {
         ctx->err = "invalid fsd_params";
         *out = ctx;
@@ -1334,6 +1338,46 @@ int fsd_mobility(fsd_ctx *ctx, const float *d_pos4, const float *d_gF11, float *
     return 0;
 }
 
+int fsd_autotune(fsd_ctx *ctx, const float *d_pos4, int32_t *h_choice8, float *h_us3)
+{
+    const float4 *pos = reinterpret_cast<const float4 *>(d_pos4);
+    RC(ensure_lub(ctx, pos));
+    ctx->spread_valid = false;
+    ctx->tune = fsd_tuning();
+    RC(autotune_run(ctx, pos));
+    return fsd_get_tuning(ctx, h_choice8, h_us3);
+}
+
+int fsd_get_tuning(fsd_ctx *ctx, int32_t *h_choice8, float *h_us3)
+{
+    const fsd_tuning &t = ctx->tune;
+    if (h_choice8) {
+        const int32_t c[8] = {t.spread, t.contract, t.xpass, t.real_g, t.real_blk, t.rfu_g, t.rfu_blk, t.tuned};
+        for (int i = 0; i < 8; ++i) h_choice8[i] = c[i];
+    }
+    if (h_us3)
+        for (int i = 0; i < 3; ++i) h_us3[i] = t.us[i];
+    return 0;
+}
+
+int fsd_set_tuning(fsd_ctx *ctx, const int32_t *h_choice7)
+{
+    const int32_t *c = h_choice7;
+    const bool ok = (c[0] == 0 || c[0] == 1) && (c[1] == 0 || c[1] == 1) && (c[2] == 0 || c[2] == 1) &&
+                    (c[3] == 4 || c[3] == 8 || c[3] == 16) && (c[4] == 128 || c[4] == 256) &&
+                    (c[5] == 8 || c[5] == 16) && (c[6] == 128 || c[6] == 256);
+    if (!ok) {
+        ctx->err = "tuning: spread, contract, xpass in {0, 1}; real lanes in {4, 8, 16}; R_FU lanes in {8, 16}; "
+                   "workgroups of 128 or 256";
+        return FSD_E_PARAM;
+    }
+    fsd_tuning &t = ctx->tune;
+    t.spread = c[0]; t.contract = c[1]; t.xpass = c[2];
+    t.real_g = c[3]; t.real_blk = c[4]; t.rfu_g = c[5]; t.rfu_blk = c[6];
+    t.tuned = 1;
+    return 0;
+}
+
 int fsd_build_precond(fsd_ctx *ctx, const float *d_pos4, fsd_solve_stats *stats)
 {
     RC(current_nlist(ctx, reinterpret_cast<const float4 *>(d_pos4)));
@@ -1356,7 +1400,7 @@ int fsd_precond_apply(fsd_ctx *ctx, const float *d_x6, float *d_y6)
 {
     if (!ctx->precond_valid) { ctx->err = "preconditioner not built"; return FSD_E_STATE; }
     if (ctx->precond_mode == 2 && lp_active(ctx)) RC(lp_apply(ctx, d_x6, nullptr, nullptr, d_y6, 0));
-    else if (ctx->precond_mode == 1) RC(k_precond_apply_jacobi(ctx, d_x6, d_y6));
+    else if (ctx->ic_jac_k > 0) RC(k_precond_apply_jacobi(ctx, d_x6, d_y6));
     else RC(k_precond_apply2(ctx, d_x6, d_x6, d_y6, d_y6, 1));
     CK(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -1589,6 +1633,12 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     RC(k_update_nlist(ctx, pos, &rebuilt));
     ctx->nlist_builds += rebuilt;
     RC(k_gridk(ctx));
+    if (ctx->p.autotune && !ctx->tune.tuned) {   // launch shapes timed at this step's positions
+        RC(k_build_lub_pairs(ctx, pos));
+        ctx->spread_valid = false;
+        RC(autotune_run(ctx, pos));
+        ctx->spread_valid = false;
+    }
     // preconditioner (Precondition_Wrap): its host phase runs on a second thread while the
     // kernels that do not depend on it are enqueued
     RC(precond_fetch(ctx, pos));

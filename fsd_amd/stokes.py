@@ -39,7 +39,7 @@ class Stokes:
                  dt=1e-3, max_strain=0.5, gmres_tol=None, gmres_restart=50, gmres_maxiter=1000,
                  parity=True, gmres_true_tol=0.0, guess="previous", gram_schmidt_passes=2,
                  precond="ichol", jacobi_iters=3, factor_sweeps=0, learned_weights="default",
-                 learned_calibration=50, refine=False):
+                 learned_calibration=50, refine=False, fallback_jacobi=0, autotune=False):
         self.lib = _lib.load()
         self.torch = _torch()
         L = box if not np.isscalar(box) else (box, box, box, 0.0, 0.0, 0.0)
@@ -77,6 +77,8 @@ class Stokes:
         if refine and not gmres_true_tol:
             raise ValueError("refine needs gmres_true_tol (the FP64 true residual it refines to)")
         p.refine = 1 if refine else 0
+        p.fallback_jacobi = int(fallback_jacobi)
+        p.autotune = 1 if autotune else 0
         self.params = p
         self.N = int(N)
         self.fileprefix = fileprefix
@@ -87,7 +89,7 @@ class Stokes:
         self._check(rc)
         self.setResistanceTable()
         if p.precond == 2:
-            # the learned factor: the committed trained weights unless the caller passes its own
+            # the learned factor: the committed weights unless the caller passes its own
             # (None: no weights, every step then runs IC(0))
             if learned_weights is not None:
                 from . import learned
@@ -258,6 +260,33 @@ class Stokes:
         self._sync()
         self._check(self.lib.fsd_build_precond(self.ctx, _ptr(pos), C.byref(st)))
         return st.as_dict()
+
+    TUNING_KEYS = ("spread", "contract", "xpass", "real_lanes", "real_block", "rfu_lanes", "rfu_block")
+
+    def autotune(self, pos):
+        """time the variants of the mobility and R_FU kernels at these positions and keep the
+        fastest; returns (choices, best microseconds per apply of the wave part, real part, R_FU)"""
+        c = np.zeros(8, dtype=np.int32)
+        us = np.zeros(3, dtype=np.float32)
+        self._sync()
+        self._check(self.lib.fsd_autotune(self.ctx, _ptr(pos), c.ctypes.data_as(C.c_void_p),
+                                          us.ctypes.data_as(C.c_void_p)))
+        return dict(zip(self.TUNING_KEYS, c.tolist())), dict(zip(("wave", "real", "rfu"), us.tolist()))
+
+    def tuning(self):
+        c = np.zeros(8, dtype=np.int32)
+        us = np.zeros(3, dtype=np.float32)
+        self._check(self.lib.fsd_get_tuning(self.ctx, c.ctypes.data_as(C.c_void_p), us.ctypes.data_as(C.c_void_p)))
+        d = dict(zip(self.TUNING_KEYS, c.tolist()))
+        d["tuned"] = int(c[7])
+        return d, dict(zip(("wave", "real", "rfu"), us.tolist()))
+
+    def set_tuning(self, **choices):
+        """force launch shapes: any of TUNING_KEYS (the others keep their current values)"""
+        cur = self.tuning()[0]
+        cur.update(choices)
+        c = np.array([cur[k] for k in self.TUNING_KEYS], dtype=np.int32)
+        self._check(self.lib.fsd_set_tuning(self.ctx, c.ctypes.data_as(C.c_void_p)))
 
     def get_precond(self, with_values=True):
         N = self.N

@@ -87,3 +87,33 @@ CONFIGS = {
     "C2": dict(N=4096, phi=0.30, shear_rate=0.0, gmres_tol=1e-3, T=1.0),
     "C3": dict(N=32768, phi=0.45, shear_rate=1.0, gmres_tol=1e-4, T=0.0),
 }
+
+
+def jittered_lattice(n_side, phi, seed=0, radius=1.0, min_gap=1e-3):
+    """n_side^3 spheres on a simple cubic lattice filling a periodic cube at volume
+    fraction phi, each coordinate displaced uniformly by up to a quarter of the lattice
+    gap less min_gap (two lattice neighbours then approach by at most half the gap, and
+    no two spheres overlap).  Returns (pos float32[N,3] in [-L/2, L/2), L)."""
+    N = n_side ** 3
+    L = box_length(N, phi, radius)
+    a = L / n_side
+    gap = a - 2.0 * radius
+    if gap <= 2.0 * min_gap:
+        raise ValueError("phi %g is too dense for a simple cubic lattice" % phi)
+    rng = np.random.default_rng(seed)
+    g = (np.arange(n_side) + 0.5) * a - 0.5 * L
+    x = np.stack(np.meshgrid(g, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    j = 0.25 * (gap - min_gap)
+    x = x + rng.uniform(-j, j, size=x.shape)
+    pos = x.astype(np.float32)
+    pos = np.where(pos >= np.float32(0.5 * L), pos - np.float32(L), pos)
+    pos = np.where(pos < np.float32(-0.5 * L), pos + np.float32(L), pos).astype(np.float32)
+    return pos, float(L)
+
+
+def random_orientations(N, seed=0):
+    """unit vectors uniform on the sphere, float32[N, 3]"""
+    rng = np.random.default_rng(seed + 104729)
+    v = rng.standard_normal((N, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return v.astype(np.float32)

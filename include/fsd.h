@@ -79,7 +79,7 @@ typedef struct fsd_params {
                                         Precondition.cu:1802-1890); 1 = IC(0), jacobi_iters Jacobi iterations
                                         per triangular solve; 2 = learned factor (fsd_learned_set_weights),
                                         jacobi_iters block Jacobi iterations per triangular solve, IC(0)
-                                        with exact solves as its fallback */
+                                        as its fallback (applied as fallback_jacobi says) */
     int32_t jacobi_iters;            /* Jacobi iterations per triangular solve in modes 1 and 2 (<= 0: 3) */
     int32_t factor_sweeps;           /* IC(0) by the level-scheduled recurrence (0) or by this many parallel
                                         fixed-point sweeps (Chow & Patel 2015) from the block-diagonal start */
@@ -87,6 +87,10 @@ typedef struct fsd_params {
                                         accumulated in FP64, r = b - A x with the lubrication product and the
                                         subtraction in FP64, FP32 GMRES for each correction) to the true
                                         residual gmres_true_tol; 0: plain FP32 GMRES (Solvers.cu:55-126) */
+    int32_t fallback_jacobi;         /* mode 2: the IC(0) of calibration and fallback steps applied by this
+                                        many Jacobi iterations per triangular solve (no sweep plans are
+                                        built); 0: exact solves */
+    int32_t autotune;                /* 1: the first fsd_step runs fsd_autotune at its positions */
 } fsd_params;
 
 /* Derived PSE parameters (Stokes.cc:131-261). */
@@ -193,6 +197,18 @@ int fsd_mobility(fsd_ctx *ctx, const float *d_pos4, const float *d_gF11, float *
 /* Precondition_Wrap (Precondition.cu:1326-1497): prune, build R_FU^nf, RCM,
  * IC(0) with relaxer doubling.  Uses the current neighbour list. */
 int fsd_build_precond(fsd_ctx *ctx, const float *d_pos4, fsd_solve_stats *stats);
+/* ---- autotuner (no counterpart in the reference, whose launch shapes are fixed) ----
+ * Times every variant of the kernels that compute the same operator at these positions
+ * (spreading, contraction and x-transform kernels of the wave-space part; lanes per
+ * particle and workgroup size of the real-space sum and of R_FU) and keeps the fastest.
+ * h_choice8 (may be null): spread (0 tiled, 1 per-tile), contract (0 tiled, 1 per
+ * particle), xpass (0 fused x transforms, 1 3-D rocFFT), real-space lanes, real-space
+ * workgroup, R_FU lanes, R_FU workgroup, tuned; h_us3 (may be null): the best time per
+ * apply in microseconds of the wave part, the real part and R_FU.  Allocates scratch. */
+int fsd_autotune(fsd_ctx *ctx, const float *d_pos4, int32_t *h_choice8, float *h_us3);
+int fsd_get_tuning(fsd_ctx *ctx, int32_t *h_choice8, float *h_us3);
+/* sets the seven choices of fsd_autotune's h_choice8 (tests, or a stored tuning) */
+int fsd_set_tuning(fsd_ctx *ctx, const int32_t *h_choice7);
 /* Export of the preconditioner in the reference's formats:
  *   h_pn[N] pruned counts (self included), h_plist packed ascending (total = sum h_pn),
  *   h_perm[N] RCM permutation (new -> old),
@@ -271,9 +287,12 @@ int fsd_profile_enable(fsd_ctx *ctx, int32_t enable);
  * event records perturb a long run less; fsd_profile_get then reports the timed launches */
 int fsd_profile_every(fsd_ctx *ctx, int32_t every);
 /* name: "spread", "green", "contract", "fft", "realspace", "rfu", "trsv", "ichol",
- * "gmres_vec", "nlist", "precond_build", "sweep", and the host wall time of the preconditioner's
- * host plan ("plan_host") and of its upload/build phase ("finish_host"); returns launches (phases)
- * and total milliseconds */
+ * "gmres_vec", "nlist", "precond_build", "sweep", the host wall time of the preconditioner's
+ * host plan ("plan_host") and of its upload/build phase ("finish_host"), the forward block
+ * Jacobi iterations of the learned factor ("ljit") or of IC(0) ("jacobi"; one event pair
+ * around the k iterations of an application, counted as k launches), and the learned
+ * factor's network and assembly ("learned_net"); returns launches (phases) and total
+ * milliseconds */
 int fsd_profile_get(fsd_ctx *ctx, const char *name, int64_t *launches, double *total_ms);
 int fsd_profile_reset(fsd_ctx *ctx);
 
@@ -282,8 +301,9 @@ int fsd_profile_reset(fsd_ctx *ctx);
  * saddle preconditioner (Precondition.cu:1116-1307, 1448-1489) with a factor that a
  * message-passing network predicts on IC(0)'s own pattern (Precondition.cu:530-836).
  * The factor is block LDL^T, (D + W) D^-1 (D + W)^T, W on the pattern's lower pair
- * blocks and D one 6x6 pivot per particle; a step falls back to IC(0) when a pivot
- * of D is non-positive, and a solve finishes with IC(0) when it reaches the
+ * blocks and D one 6x6 pivot per particle (positive definite by construction); a step
+ * falls back to IC(0) when the network's outputs are not finite, and a solve finishes
+ * with IC(0) when it reaches the
  * iteration count IC(0) needed at the last calibration step (every
  * fsd_learned_set_calibration steps, default 50, one step runs IC(0)).
  * Parameters are FP32 (fsd_learned_nparams of them); the network runs them in BF16. */

@@ -100,12 +100,16 @@ def test_learned_gradient_parity(case):
     assert el < 1e-3 and eg < 3e-2
 
 
-def test_learned_pivot_fallback():
+@pytest.mark.parametrize("fallback_jacobi", [0, 8])
+def test_learned_pivot_fallback(fallback_jacobi):
     """a non-finite network output (injected at one particle's pivot shift) makes the
-    build fall back to IC(0) for that step; the solve is then the IC(0) solve"""
+    build fall back to IC(0) for that step; the solve is then the IC(0) solve, with exact
+    triangular solves or (fallback_jacobi = 8) the same application as precond
+    'ichol_jacobi' with 8 iterations (equal to FP32 rounding)"""
     import torch
+    from fsd_amd.stokes import Stokes
     w = _weights()
-    st, orc, lo, dpos, N = _setup("D1", w, gmres_true_tol=1e-5, gmres_restart=100)
+    st, orc, lo, dpos, N = _setup("D1", w, gmres_true_tol=1e-5, gmres_restart=100, fallback_jacobi=fallback_jacobi)
     st.learned_inject_pivot(5)
     info = st.build_precond(dpos)
     assert info["precond_used"] == 0 and info["fallback"] == 1
@@ -113,6 +117,14 @@ def test_learned_pivot_fallback():
     b = torch.from_numpy(random_vec(17 * N, 9).astype(np.float32)).cuda()
     x, si = st.saddle_solve(dpos, b)
     assert si["precond_used"] == 0 and si["converged"]
+    if fallback_jacobi:
+        ref = Stokes(N, st.params.Lx, precond="ichol_jacobi", jacobi_iters=fallback_jacobi)
+        ref.build_nlist(dpos)
+        ref.build_precond(dpos)
+        z = torch.from_numpy(random_vec(6 * N, 10).astype(np.float32)).cuda()
+        e = rel_err(st.precond_apply(z).cpu().numpy(), ref.precond_apply(z).cpu().numpy())
+        print("IC(0) fallback by Jacobi vs ichol_jacobi", e)
+        assert e < 1e-6
     # the next build is learned again
     info = st.build_precond(dpos)
     assert info["precond_used"] == 1 and info["fallback"] == 0
