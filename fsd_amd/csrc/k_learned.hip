@@ -380,11 +380,13 @@ __global__ void lp_edge_W_kernel(int64_t E, const float *geo, const float *oute,
     W[t] = w;
 }
 
-// D_P = A_PP + lambda diag(e^a_t, e^a_r) S - s_P sum_{Q < P} gamma_e A_e Dt_Q^-1 A_e^T with
-// gamma_e = 2 sigmoid(c_e + c0), A_e the pair block and Dt_Q = A_QQ + lambda S.  The pivot is
+// D_P = A_PP + lambda diag(e^a_t, e^a_r) S - s_P sum_{Q < P} gamma_e A_e F_Q A_e^T with
+// gamma_e = 2 sigmoid(c_e + c0), A_e the pair block and F_Q the inverse pivot the fill uses:
+// Dt_Q^-1 = (A_QQ + lambda S)^-1 in the first pass, the first pass's D_Q^-1 in the second
+// (a second parallel sweep of the pivot recurrence, closer to IC(0)'s pivots).  The pivot is
 // positive definite by construction: s_P is the first of 1, 1/2, 1/4, 1/8 for which the
 // Cholesky factorisation of D_P succeeds (level 0 .. 3); when none does, D_P = Dt_P
-// (level 4), positive definite by the check of lp_node_feat_kernel.  lvl[P] records the
+// (level 4, with Dtinv its inverse), positive definite by the check of lp_node_feat_kernel.  lvl[P] records the
 // level for the backward pass; fail[1] counts particles above level 0, fail[2] those at
 // level 4.  Non-finite network outputs set fail bit 1 (the factor is then not used).
 // The fill term uses the pair block, not W: the pivots then do not depend on W, and
@@ -392,7 +394,7 @@ __global__ void lp_edge_W_kernel(int64_t E, const float *geo, const float *oute,
 constexpr int LP_LEVELS = 4;
 __global__ void lp_node_D_kernel(int N, const int *lrow, const int *lcol, const float *Adiag, float lambda,
                                  const float *outn, const float *oute, const float *Ablk, const float *Dtinv,
-                                 float *Dinv, float *Dout, int *lvl, int *fail)
+                                 const float *Finv, float *Dinv, float *Dout, int *lvl, int *fail)
 {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= N) return;
@@ -408,8 +410,8 @@ __global__ void lp_node_D_kernel(int N, const int *lrow, const int *lcol, const 
         const float *o = oute + (int64_t)e * lp::OE;
         const float gam = lp_gamma(o[6]);
         for (int k = 0; k < 7; ++k) finite = finite && isfinite(o[k]);
-        const float *We = Ablk + (int64_t)e * 36, *Ti = Dtinv + (int64_t)lcol[e] * 36;
-        float T[36];   // A Dt^-1
+        const float *We = Ablk + (int64_t)e * 36, *Ti = Finv + (int64_t)lcol[e] * 36;
+        float T[36];   // A F
         for (int a = 0; a < 6; ++a)
             for (int c = 0; c < 6; ++c) {
                 float s = 0.f;
@@ -685,22 +687,28 @@ __global__ void lj_gd_kernel(int N, GTerms T, float *gD)
     for (int c = 0; c < 6; ++c) gD[P * 36 + a * 6 + c] = acc[c];
 }
 
-// assembly backward, per particle: Dbar = -Dinv sym(gDinv) Dinv; pivot-shift gradients;
-// per lower pair the fill-weight gradient -s_P gamma (1 - gamma) <Dbar, A Dt^-1 A^T> with
-// the pivot's level s_P = 2^-level (no output gradient at level 4)
-// (dbar_direct: gDinv already holds the gradient with respect to D itself, symmetric)
+// assembly backward of one pivot pass, per particle: Dbar = -Dinv sym(gDinv) Dinv;
+// pivot-shift gradients; per lower pair the fill-weight gradient
+// -s_P gamma'(c) <Dbar, A F A^T> with the pivot's level s_P = 2^-level (no output gradient
+// at level 4).  dbar_direct: gDinv already holds the gradient with respect to D itself
+// (symmetric); accumulate: add to goutn and goute[.., 6] (the first pass, after the
+// second); Dbar_out (may be null): Dbar of every particle (zero at level 4)
 __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, const float *Dinv, const float *gDinv,
                                      int dbar_direct, const float *oute, const float *outn, float lambda,
-                                     const float *Ablk, const float *Dtinv, const int *lvl, float *goutn,
-                                     float *goute)
+                                     const float *Ablk, const float *Finv, const int *lvl, int accumulate,
+                                     float *Dbar_out, float *goutn, float *goute)
 {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= N) return;
     float* go = goutn + (int64_t)P * lp::ON;
-    for (int c = 0; c < lp::ON; ++c) go[c] = 0.f;
+    if (!accumulate)
+        for (int c = 0; c < lp::ON; ++c) go[c] = 0.f;
     const int level = lvl[P];
     if (level >= LP_LEVELS) {   // D_P = A_PP + lambda S does not depend on the outputs
-        for (int e = lrow[P]; e < lrow[P + 1]; ++e) goute[(int64_t)e * lp::OE + 6] = 0.f;
+        if (!accumulate)
+            for (int e = lrow[P]; e < lrow[P + 1]; ++e) goute[(int64_t)e * lp::OE + 6] = 0.f;
+        if (Dbar_out)
+            for (int q = 0; q < 36; ++q) Dbar_out[(int64_t)P * 36 + q] = 0.f;
         return;
     }
     const float sc = ldexpf(1.0f, -level);
@@ -724,11 +732,15 @@ __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, co
                 Db[a * 6 + c] = -s;
             }
     }
-    go[0] = lambda * expf(outn[(int64_t)P * lp::ON]) * (Db[0] + Db[7] + Db[14]);
-    go[1] = lambda * (4.0f / 3.0f) * expf(outn[(int64_t)P * lp::ON + 1]) * (Db[21] + Db[28] + Db[35]);
+    if (Dbar_out)
+        for (int q = 0; q < 36; ++q) Dbar_out[(int64_t)P * 36 + q] = Db[q];
+    const float g0 = lambda * expf(outn[(int64_t)P * lp::ON]) * (Db[0] + Db[7] + Db[14]);
+    const float g1 = lambda * (4.0f / 3.0f) * expf(outn[(int64_t)P * lp::ON + 1]) * (Db[21] + Db[28] + Db[35]);
+    go[0] = accumulate ? go[0] + g0 : g0;
+    go[1] = accumulate ? go[1] + g1 : g1;
     for (int e = lrow[P]; e < lrow[P + 1]; ++e) {
-        const float *Ae = Ablk + (int64_t)e * 36, *Ti = Dtinv + (int64_t)lcol[e] * 36;
-        float AT[36];   // A Dt^-1
+        const float *Ae = Ablk + (int64_t)e * 36, *Ti = Finv + (int64_t)lcol[e] * 36;
+        float AT[36];   // A F
         for (int a = 0; a < 6; ++a)
             for (int c = 0; c < 6; ++c) {
                 float s = 0.f;
@@ -742,8 +754,60 @@ __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, co
                 for (int q = 0; q < 6; ++q) s += AT[a * 6 + q] * Ae[c * 6 + q];
                 gg += Db[a * 6 + c] * s;
             }
-        goute[(int64_t)e * lp::OE + 6] = -sc * gg * lp_gamma_d(oute[(int64_t)e * lp::OE + 6]);
+        const float gc = -sc * gg * lp_gamma_d(oute[(int64_t)e * lp::OE + 6]);
+        float *gp = goute + (int64_t)e * lp::OE + 6;
+        *gp = accumulate ? *gp + gc : gc;
     }
+}
+
+// from the second pivot pass to the first: the gradient with respect to the first pass's
+// pivots D1_Q, from the second pass's Dbar2 of every row particle R with a lower block
+// (R, Q): the gradient with respect to D1_Q^-1 is X = sum -s_R gamma_e A_e^T Dbar2_R A_e
+// (levels of the second pass, none at level 4), and D1bar_Q = -D1_Q^-1 sym(X) D1_Q^-1.
+// The column entries of Q are its upper list (ucol: the row particle, upos: the block).
+__global__ void lp_fill_bwd_kernel(int N, const int *urow, const int *ucol, const int *upos, const float *Ablk,
+                                   const float *oute, const int *lvl2, const float *Dbar2, const float *D1inv,
+                                   float *D1bar)
+{
+    const int Q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (Q >= N) return;
+    float X[36];
+    for (int q = 0; q < 36; ++q) X[q] = 0.f;
+    for (int u = urow[Q]; u < urow[Q + 1]; ++u) {
+        const int R = ucol[u], e = upos[u];
+        if (lvl2[R] >= LP_LEVELS) continue;
+        const float w = -ldexpf(1.0f, -lvl2[R]) * lp_gamma(oute[(int64_t)e * lp::OE + 6]);
+        const float *A = Ablk + (int64_t)e * 36, *Db = Dbar2 + (int64_t)R * 36;
+        float T[36];   // Dbar2_R A
+        for (int a = 0; a < 6; ++a)
+            for (int c = 0; c < 6; ++c) {
+                float s = 0.f;
+                for (int q = 0; q < 6; ++q) s += Db[a * 6 + q] * A[q * 6 + c];
+                T[a * 6 + c] = s;
+            }
+        for (int a = 0; a < 6; ++a)
+            for (int c = 0; c < 6; ++c) {
+                float s = 0.f;
+                for (int q = 0; q < 6; ++q) s += A[q * 6 + a] * T[q * 6 + c];
+                X[a * 6 + c] += w * s;
+            }
+    }
+    const float *Di = D1inv + (int64_t)Q * 36;
+    float S[36], T[36];
+    for (int a = 0; a < 6; ++a)
+        for (int c = 0; c < 6; ++c) S[a * 6 + c] = 0.5f * (X[a * 6 + c] + X[c * 6 + a]);
+    for (int a = 0; a < 6; ++a)
+        for (int c = 0; c < 6; ++c) {
+            float s = 0.f;
+            for (int q = 0; q < 6; ++q) s += Di[a * 6 + q] * S[q * 6 + c];
+            T[a * 6 + c] = s;
+        }
+    for (int a = 0; a < 6; ++a)
+        for (int c = 0; c < 6; ++c) {
+            float s = 0.f;
+            for (int q = 0; q < 6; ++q) s += T[a * 6 + q] * Di[q * 6 + c];
+            D1bar[(int64_t)Q * 36 + a * 6 + c] = -s;
+        }
 }
 
 // assembly backward, per edge: multiplier gradients (1 - tanh^2 c_k) <gW, basis_k>; other outputs 0
@@ -1068,6 +1132,9 @@ int lp_init(fsd_ctx *ctx)
     LRC(lalloc(ctx, s.gDinv, N * 36));
     LRC(lalloc(ctx, s.d_fail, 4));
     LRC(lalloc(ctx, s.lvl, N));
+    LRC(lalloc(ctx, s.lvl1, N));
+    LRC(lalloc(ctx, s.D1inv, N * 36));
+    LRC(lalloc(ctx, s.Dbar2, N * 36));
     LRC(lalloc(ctx, s.d_loss, 4));
     FSD_HIP(hipHostMalloc((void **)&s.h_fail, sizeof(int) * 4, hipHostMallocDefault));
     FSD_HIP(hipHostMalloc((void **)&s.h_loss, sizeof(double) * 4, hipHostMallocDefault));
@@ -1083,7 +1150,7 @@ void lp_free(fsd_ctx *ctx)
     LearnedState &s = *ctx->lp;
     void *ptrs[] = {s.d_w, s.d_grad, s.d_m, s.d_v, s.d_wn, s.xn0, s.zn0, s.agg, s.xh, s.outn, s.Dinv, s.Dtinv, s.D,
                     s.xe0, s.ze0, s.oute, s.geo, s.W, s.erow, s.d_fail, s.gh, s.gh2, s.ge, s.gm, s.gzn, s.gze, s.gxe,
-                    s.gxn, s.goute, s.goutn, s.gW, s.gDinv, s.gbf, s.part, s.jv, s.d_loss, s.d_wbak, s.lvl};
+                    s.gxn, s.goute, s.goutn, s.gW, s.gDinv, s.gbf, s.part, s.jv, s.d_loss, s.d_wbak, s.lvl, s.lvl1, s.D1inv, s.Dbar2};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (int l = 0; l <= lp::NMP; ++l) {
@@ -1230,8 +1297,15 @@ int lp_forward(fsd_ctx *ctx, const float4 *pos, bool train, int inject, int *fai
     // factor
     if (E)
         hipLaunchKernelGGL(lp_edge_W_kernel, dim3(nblk(E * 36, 256)), dim3(256), 0, ctx->stream, E, s.geo, s.oute, s.W);
+    // two passes of the pivots: the first with the uncorrected neighbour pivots in the fill
+    // terms, the second with the first's (only the second's safeguard counts are kept)
     hipLaunchKernelGGL(lp_node_D_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow, ctx->d_lcol,
-                       ctx->d_Adiag, lambda, s.outn, s.oute, ctx->d_Ablk, s.Dtinv, s.Dinv, s.D, s.lvl, s.d_fail);
+                       ctx->d_Adiag, lambda, s.outn, s.oute, ctx->d_Ablk, s.Dtinv, s.Dtinv, s.D1inv,
+                       (float *)nullptr, s.lvl1, s.d_fail);
+    hipLaunchKernelGGL(lp_fill_int_kernel, dim3(1), dim3(64), 0, ctx->stream, 2, 0, s.d_fail + 1);
+    hipLaunchKernelGGL(lp_node_D_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow, ctx->d_lcol,
+                       ctx->d_Adiag, lambda, s.outn, s.oute, ctx->d_Ablk, s.Dtinv, s.D1inv, s.Dinv, s.D, s.lvl,
+                       s.d_fail);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_NET, pev);
     FSD_HIP(hipMemcpyAsync(s.h_fail, s.d_fail, sizeof(int) * 3, hipMemcpyDeviceToHost, ctx->stream));
@@ -1287,9 +1361,15 @@ static int lp_backward(fsd_ctx *ctx, bool dbar_direct)
     const int N = ctx->N;
     const int64_t E = ctx->pch.nlow;
     // assembly backward
+    // second pivot pass, then its dependence on the first pass's pivots, then the first pass
     hipLaunchKernelGGL(lp_node_D_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow,
                        ctx->d_lcol, s.Dinv, s.gDinv, dbar_direct ? 1 : 0, s.oute, s.outn, s.lambda,
-                       ctx->d_Ablk, s.Dtinv, s.lvl, s.goutn, s.goute);
+                       ctx->d_Ablk, s.D1inv, s.lvl, 0, s.Dbar2, s.goutn, s.goute);
+    hipLaunchKernelGGL(lp_fill_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_urow, ctx->d_ucol,
+                       ctx->d_upos, ctx->d_Ablk, s.oute, s.lvl, s.Dbar2, s.D1inv, s.gDinv);
+    hipLaunchKernelGGL(lp_node_D_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow,
+                       ctx->d_lcol, s.D1inv, s.gDinv, 1, s.oute, s.outn, s.lambda, ctx->d_Ablk, s.Dtinv, s.lvl1,
+                       1, (float *)nullptr, s.goutn, s.goute);
     if (E) hipLaunchKernelGGL(lp_edge_W_bwd_kernel, dim3(nblk(E, 128)), dim3(128), 0, ctx->stream, E, s.geo, s.oute,
                               s.gW, s.goute);
     // heads

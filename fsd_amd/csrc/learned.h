@@ -52,7 +52,8 @@ struct LearnedState {
     float *geo = nullptr, *W = nullptr;
     int *erow = nullptr;                         // row particle of every lower block
     int *d_fail = nullptr, *h_fail = nullptr;   // [0] failure bits, [1] pivots above level 0, [2] at level 4
-    int *lvl = nullptr;                          // per particle: the level of its pivot safeguard
+    int *lvl = nullptr, *lvl1 = nullptr;         // per particle: the level of its pivot safeguard (second, first pass)
+    float *D1inv = nullptr, *Dbar2 = nullptr;    // first pass's inverse pivots; second pass's pivot gradient
     float lambda = 1.0f;                         // pivot shift of the last factor (doubled until A_PP + lambda S > 0)
     int shrunk = 0, plain = 0;                   // pivots of the last factor above level 0 / at level 4
     // backward buffers

@@ -201,6 +201,7 @@ class LearnedOracle:
         self.Dt = Ad + lam * np.einsum('c,cd->cd', S6, np.eye(6))[None]
         self.Dtinv = np.linalg.inv(self.Dt)
         self.levels = np.zeros(N, dtype=np.int64)
+        self.levels1 = np.zeros(N, dtype=np.int64)
         self.Bk = basis(self.geo)
 
     # ---- network
@@ -249,22 +250,32 @@ class LearnedOracle:
         B = self.Adiag.copy()
         B[:, [0, 1, 2], [0, 1, 2]] += self.lam * np.exp(outn[:, 0:1])
         B[:, [3, 4, 5], [3, 4, 5]] += self.lam * (4.0 / 3.0) * np.exp(outn[:, 1:2])
-        F = np.zeros_like(B)
-        if self.E:   # the fill term uses the pair blocks A (the pivots do not depend on W)
-            T = np.einsum('eaq,eqr,ecr->eac', self.Ablk, self.Dtinv[self.cols], self.Ablk)
-            np.add.at(F, self.rows, gamma(oute[:, 6])[:, None, None] * T)
+        g = gamma(oute[:, 6]) if self.E else np.zeros(0)
         failed = not (np.all(np.isfinite(outn[:, :2])) and np.all(np.isfinite(oute[:, :7])))
-        D = np.empty_like(B)
-        for P in range(self.N):
-            self.levels[P] = LEVELS
-            for j in range(LEVELS):
-                Dj = B[P] - 0.5 ** j * F[P]
-                Dj = np.tril(Dj) + np.tril(Dj, -1).T
-                if _is_pd(Dj):
-                    self.levels[P], D[P] = j, Dj
-                    break
-            if self.levels[P] == LEVELS:
-                D[P] = self.Dt[P]
+
+        def one_pass(Finv, levels):
+            # the fill term uses the pair blocks A (the pivots do not depend on W)
+            F = np.zeros_like(B)
+            if self.E:
+                T = np.einsum('eaq,eqr,ecr->eac', self.Ablk, Finv[self.cols], self.Ablk)
+                np.add.at(F, self.rows, g[:, None, None] * T)
+            D = np.empty_like(B)
+            for P in range(self.N):
+                levels[P] = LEVELS
+                for j in range(LEVELS):
+                    Dj = B[P] - 0.5 ** j * F[P]
+                    Dj = np.tril(Dj) + np.tril(Dj, -1).T
+                    if _is_pd(Dj):
+                        levels[P], D[P] = j, Dj
+                        break
+                if levels[P] == LEVELS:
+                    D[P] = self.Dt[P]
+            return D
+
+        # two passes: the fill with the uncorrected pivots, then with the first pass's
+        self.D1 = one_pass(self.Dtinv, self.levels1)
+        self.D1inv = np.linalg.inv(self.D1)
+        D = one_pass(self.D1inv, self.levels)
         Dinv = np.linalg.inv(D)
         return W, D, Dinv, failed
 
@@ -408,15 +419,32 @@ class LearnedOracle:
         N = self.N
         goutn = np.zeros((N, ON))
         outn = c["outn"]
-        live = self.levels < LEVELS          # level 4 pivots do not depend on the outputs
-        sc = np.where(live, 0.5 ** np.minimum(self.levels, LEVELS - 1), 0.0)
-        goutn[:, 0] = live * self.lam * np.exp(outn[:, 0]) * np.einsum('pii->p', Db[:, 0:3, 0:3])
-        goutn[:, 1] = live * self.lam * (4.0 / 3.0) * np.exp(outn[:, 1]) * np.einsum('pii->p', Db[:, 3:6, 3:6])
         goute = np.zeros((self.E, OE))
         oute = c["oute"]
+
+        def pass_backward(Db, levels, Finv):
+            # one pivot pass: the shift and fill-weight gradients (none at level 4)
+            live = levels < LEVELS
+            sc = np.where(live, 0.5 ** np.minimum(levels, LEVELS - 1), 0.0)
+            goutn[:, 0] += live * self.lam * np.exp(outn[:, 0]) * np.einsum('pii->p', Db[:, 0:3, 0:3])
+            goutn[:, 1] += live * self.lam * (4.0 / 3.0) * np.exp(outn[:, 1]) * np.einsum('pii->p', Db[:, 3:6, 3:6])
+            if self.E:
+                T = np.einsum('eaq,eqr,ecr->eac', self.Ablk, Finv[self.cols], self.Ablk)
+                goute[:, 6] += -sc[self.rows] * np.einsum('eac,eac->e', Db[self.rows], T) * gamma_d(oute[:, 6])
+            return live, sc
+
+        live2, sc2 = pass_backward(Db, self.levels, self.D1inv)
+        # the second pass depends on the first's pivots through its fill terms A D1_Q^-1 A^T
+        X = np.zeros((N, 6, 6))
         if self.E:
-            T = np.einsum('eaq,eqr,ecr->eac', self.Ablk, self.Dtinv[self.cols], self.Ablk)
-            goute[:, 6] = -sc[self.rows] * np.einsum('eac,eac->e', Db[self.rows], T) * gamma_d(oute[:, 6])
+            Db2 = Db * live2[:, None, None]
+            w = -sc2[self.rows] * gamma(oute[:, 6])
+            np.add.at(X, self.cols, w[:, None, None] * np.einsum('eqa,eqr,erc->eac', self.Ablk, Db2[self.rows],
+                                                                   self.Ablk))
+        Xs = 0.5 * (X + np.transpose(X, (0, 2, 1)))
+        D1b = -np.einsum('paq,pqr,prc->pac', self.D1inv, Xs, self.D1inv)
+        pass_backward(D1b, self.levels1, self.Dtinv)
+        if self.E:
             goute[:, :6] = np.einsum('eac,ekac->ek', gW, self.Bk) * (1.0 - np.tanh(oute[:, :6]) ** 2)
         return self._mlp_backward(c, goute, goutn, emulate)
 

@@ -31,8 +31,9 @@ def test_layouts_agree():
 
 
 def test_zero_outputs_give_one_step_fill(dense):
-    """with zero network outputs W is the pair blocks A and D = A_PP + lambda S minus
-    0.98 of every lower pair's fill term A Dt^-1 A^T"""
+    """with zero network outputs W is the pair blocks A and the pivots are two passes of
+    D = A_PP + lambda S minus 0.982 of every lower pair's fill term A F A^T, F = Dt^-1 in
+    the first pass and the first pass's D^-1 in the second"""
     from oracle.learned import init_weights
     orc, lo, X, B = dense
     c = lo.forward(init_weights(1, scale_out=0.0), emulate=False)
@@ -43,10 +44,16 @@ def test_zero_outputs_give_one_step_fill(dense):
     # the pivots written out pair by pair from the pattern
     from oracle.learned import gamma
     g = gamma(0.0)
+    assert abs(g - 0.982) < 1e-4
+    D1 = lo.Dt.copy()
+    for e in range(lo.E):
+        P, Q = lo.rows[e], lo.cols[e]
+        D1[P] -= g * lo.Ablk[e] @ np.linalg.inv(lo.Dt[Q]) @ lo.Ablk[e].T
     Dref = lo.Dt.copy()
     for e in range(lo.E):
         P, Q = lo.rows[e], lo.cols[e]
-        Dref[P] -= g * lo.Ablk[e] @ np.linalg.inv(lo.Dt[Q]) @ lo.Ablk[e].T
+        Dref[P] -= g * lo.Ablk[e] @ np.linalg.inv(D1[Q]) @ lo.Ablk[e].T
+    assert np.all(lo.levels == 0) and np.all(lo.levels1 == 0)
     assert np.abs(D - Dref).max() <= 1e-9 * np.abs(Dref).max()
 
 
@@ -91,7 +98,9 @@ def test_factor_loss_gradient_finite_differences(dense):
     w = init_weights(3, scale_out=0.3).astype(np.float64)
     loss, g = lo.factor_loss(w, Dic, Wic, emulate=False, grad=True)
     v = np.random.default_rng(6).standard_normal(w.size)
-    eps = 1e-6
+    # a small step: pivots near singular (fill weights up to 2) have large third derivatives,
+    # and the central difference's error falls as eps^2 (2e-5 at 1e-6, 1e-8 at 1e-8)
+    eps = 1e-8
     fd = (lo.factor_loss(w + eps * v, Dic, Wic, emulate=False) - lo.factor_loss(w - eps * v, Dic, Wic, emulate=False)) / (2 * eps)
     assert loss > 0 and abs(fd - g @ v) < 1e-6 * abs(fd)
     c = lo.forward(w, emulate=False)
@@ -132,23 +141,24 @@ def test_pivot_safeguard(overfilled):
     w = _shrunk_weights(1, -8.0, 8.0)
     c = lo.forward(w, emulate=False)
     W, D, Dinv, failed = lo.assemble(c["oute"], c["outn"])
-    lv = lo.levels.copy()
-    print("lambda", lo.lam, "levels", np.bincount(lv, minlength=LEVELS + 1))
-    assert not failed and np.all(np.bincount(lv, minlength=LEVELS + 1) > 0)
+    print("lambda", lo.lam, "levels (first, second pass)", np.bincount(lo.levels1, minlength=LEVELS + 1),
+          np.bincount(lo.levels, minlength=LEVELS + 1))
+    assert not failed and np.all(np.bincount(lo.levels1, minlength=LEVELS + 1) > 0)
     S = np.diag([1.0, 1.0, 1.0, 4.0 / 3.0, 4.0 / 3.0, 4.0 / 3.0])
-    F = np.zeros((lo.N, 6, 6))
-    for e in range(lo.E):
-        F[lo.rows[e]] += gamma(c["oute"][e, 6]) * lo.Ablk[e] @ lo.Dtinv[lo.cols[e]] @ lo.Ablk[e].T
-    for P in range(lo.N):
-        assert _is_pd(D[P])
-        Bp = lo.Adiag[P] + lo.lam * np.diag(np.exp(np.repeat(c["outn"][P, :2], 3))) @ S
-        if lv[P] < LEVELS:
-            ref = Bp - 0.5 ** lv[P] * F[P]
-            assert np.abs(D[P] - ref).max() <= 1e-9 * np.abs(ref).max()
-            for j in range(lv[P]):      # the levels before it were not positive definite
-                assert not _is_pd(Bp - 0.5 ** j * F[P])
-        else:
-            assert np.allclose(D[P], lo.Dt[P])
+    for lv, Finv, Dout in ((lo.levels1, lo.Dtinv, lo.D1), (lo.levels, lo.D1inv, D)):
+        F = np.zeros((lo.N, 6, 6))
+        for e in range(lo.E):
+            F[lo.rows[e]] += gamma(c["oute"][e, 6]) * lo.Ablk[e] @ Finv[lo.cols[e]] @ lo.Ablk[e].T
+        for P in range(lo.N):
+            assert _is_pd(Dout[P])
+            Bp = lo.Adiag[P] + lo.lam * np.diag(np.exp(np.repeat(c["outn"][P, :2], 3))) @ S
+            if lv[P] < LEVELS:
+                ref = Bp - 0.5 ** lv[P] * F[P]
+                assert np.abs(Dout[P] - ref).max() <= 1e-9 * np.abs(ref).max()
+                for j in range(lv[P]):      # the levels before it were not positive definite
+                    assert not _is_pd(Bp - 0.5 ** j * F[P])
+            else:
+                assert np.allclose(Dout[P], lo.Dt[P])
 
 
 def test_safeguard_gradient_finite_differences(overfilled):
@@ -158,13 +168,13 @@ def test_safeguard_gradient_finite_differences(overfilled):
     Dic, Wic = lo.factor_targets(orc)
     w = _shrunk_weights(2, -8.0, 8.0)
     loss, g = lo.factor_loss(w, Dic, Wic, emulate=False, grad=True)
-    lv = lo.levels.copy()
-    assert lv.max() > 0
+    lv, lv1 = lo.levels.copy(), lo.levels1.copy()
+    assert lv.max() > 0 and lv1.max() > 0
     v = np.random.default_rng(8).standard_normal(w.size)
     eps = 1e-7
     fp = lo.factor_loss(w + eps * v, Dic, Wic, emulate=False)
-    assert np.array_equal(lo.levels, lv)
+    assert np.array_equal(lo.levels, lv) and np.array_equal(lo.levels1, lv1)
     fm = lo.factor_loss(w - eps * v, Dic, Wic, emulate=False)
-    assert np.array_equal(lo.levels, lv)
+    assert np.array_equal(lo.levels, lv) and np.array_equal(lo.levels1, lv1)
     fd = (fp - fm) / (2 * eps)
     assert abs(fd - g @ v) < 1e-5 * abs(fd)
