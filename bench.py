@@ -182,7 +182,9 @@ def main():
                     help="classical Gram-Schmidt passes per Arnoldi step")
     ap.add_argument("--guess", default="extrapolate", choices=["previous", "extrapolate", "quadratic"],
                     help="initial iterate of each step's solve")
-    ap.add_argument("--jacobi", type=int, default=3, help="block Jacobi iterations per triangular solve (learned)")
+    ap.add_argument("--jacobi", type=int, default=5,
+                    help="block Jacobi iterations per triangular solve (learned; at N=32768: 2 -> 19.9 steps/s, "
+                         "3 -> 27.1, 5 -> 28.4, 8 -> 26.0)")
     ap.add_argument("--calibration", type=int, default=20,
                     help="learned arm: one IC(0) step every this many steps (its iteration count caps the learned "
                          "solves); 20 puts one such step in the 20 timed steps")

@@ -42,7 +42,7 @@ def run_active(n_side=64, phi=0.4, steps=50, dt=1e-3, T=1.0, seed=2026, calibrat
     ori = random_orientations(N, seed=seed)
     log("# lattice start: N=%d phi=%.2f L=%.3f (%.1f s)" % (N, phi, L, time.time() - t0))
     st = Stokes(N, L, T=T, dt=dt, seed=seed, sqm_B1=1.5, gmres_tol=1e-5, gmres_true_tol=tol, gmres_restart=100,
-                gmres_maxiter=3000, refine=True, guess="previous", precond="learned", jacobi_iters=3,
+                gmres_maxiter=3000, refine=True, guess="previous", precond="learned", jacobi_iters=5,
                 learned_calibration=calibration, fallback_jacobi=8, autotune=True)
     pse = st.pse_info()
     M = pse["Nx"] * pse["Ny"] * pse["Nz"]

@@ -154,13 +154,16 @@ class Stokes:
         return {k: getattr(info, k) for k, _ in info._fields_}
 
     # ---- the step
-    def integrateStepOne(self, timestep, state, force6=None, noise_ang=None, shear_rate=None):
+    def integrateStepOne(self, timestep, state, force6=None, noise_ang=None, shear_rate=None, sync=True):
         """advance `state` (a ParticleState of device tensors) by one step.
-        Returns the solve statistics; velocities are left in state.velocity."""
+        Returns the solve statistics; velocities are left in state.velocity.  sync=False
+        skips the device-wide synchronisation before the call (the caller has made the
+        inputs ready; used when several contexts step concurrently from host threads)."""
         if shear_rate is None:
             shear_rate = self.shear_function.get_shear_rate(timestep) if self.shear_function else 0.0
         st = _lib.FsdSolveStats()
-        self._sync()
+        if sync:
+            self._sync()
         rc = self.lib.fsd_step(self.ctx, int(timestep), float(shear_rate), _ptr(state.pos), _ptr(state.ori),
                                _ptr(state.pos_gb), _ptr(state.image), _ptr(force6), _ptr(noise_ang),
                                _ptr(state.velocity), C.byref(st))
