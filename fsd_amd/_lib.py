@@ -127,6 +127,7 @@ SIGNATURES = [
     ("fsd_learned_inject_pivot", _I, [_P, _I]),
     ("fsd_learned_stats", _I, [_P, _P]),
     ("fsd_learned_export", _I, [_P, _P, _P, _P, _P, C.c_int64]),
+    ("fsd_learned_levels", _I, [_P, _P, C.c_int64]),
     ("fsd_learned_loss_grad", _I, [_P, _P, _I, _P, _P, _I, C.POINTER(C.c_double)]),
     ("fsd_learned_factor_loss", _I, [_P, _P, _I, C.POINTER(C.c_double)]),
     ("fsd_learned_adam", _I, [_P, _F, _I]),

@@ -454,4 +454,5 @@ bool lp_active(const fsd_ctx *ctx);
 bool lp_have_weights(const fsd_ctx *ctx);
 void lp_set_active(fsd_ctx *ctx, bool on);
 int lp_export(fsd_ctx *ctx, float *h_oute, float *h_outn, float *h_W, float *h_Dinv);
+int lp_levels(fsd_ctx *ctx, int32_t *h_levels, int64_t n);
 

@@ -71,6 +71,12 @@ __device__ __forceinline__ float lp_gamma_d(float c)
     return 2.0f * sg * (1.0f - sg);
 }
 
+__global__ void lp_fill_kernel(int64_t n, float v, float *p)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void lp_fill_int_kernel(int n, int v, int *p)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -382,8 +388,8 @@ __global__ void lp_edge_W_kernel(int64_t E, const float *geo, const float *oute,
 
 // D_P = A_PP + lambda diag(e^a_t, e^a_r) S - s_P sum_{Q < P} gamma_e A_e F_Q A_e^T with
 // gamma_e = 2 sigmoid(c_e + c0), A_e the pair block and F_Q the inverse pivot the fill uses:
-// Dt_Q^-1 = (A_QQ + lambda S)^-1 in the first pass, the first pass's D_Q^-1 in the second
-// (a second parallel sweep of the pivot recurrence, closer to IC(0)'s pivots).  The pivot is
+// Dt_Q^-1 = (A_QQ + lambda S)^-1 in the first pass, the previous pass's D_Q^-1 in each later
+// one (parallel sweeps of IC(0)'s sequential pivot recurrence; lp::NPASS of them).  The pivot is
 // positive definite by construction: s_P is the first of 1, 1/2, 1/4, 1/8 for which the
 // Cholesky factorisation of D_P succeeds (level 0 .. 3); when none does, D_P = Dt_P
 // (level 4, with Dtinv its inverse), positive definite by the check of lp_node_feat_kernel.  lvl[P] records the
@@ -691,8 +697,8 @@ __global__ void lj_gd_kernel(int N, GTerms T, float *gD)
 // pivot-shift gradients; per lower pair the fill-weight gradient
 // -s_P gamma'(c) <Dbar, A F A^T> with the pivot's level s_P = 2^-level (no output gradient
 // at level 4).  dbar_direct: gDinv already holds the gradient with respect to D itself
-// (symmetric); accumulate: add to goutn and goute[.., 6] (the first pass, after the
-// second); Dbar_out (may be null): Dbar of every particle (zero at level 4)
+// (symmetric); accumulate: add to goutn and goute[.., 6] (every pass but the last, which
+// runs first); Dbar_out (may be null): Dbar of every particle (zero at level 4)
 __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, const float *Dinv, const float *gDinv,
                                      int dbar_direct, const float *oute, const float *outn, float lambda,
                                      const float *Ablk, const float *Finv, const int *lvl, int accumulate,
@@ -760,10 +766,108 @@ __global__ void lp_node_D_bwd_kernel(int N, const int *lrow, const int *lcol, co
     }
 }
 
-// from the second pivot pass to the first: the gradient with respect to the first pass's
-// pivots D1_Q, from the second pass's Dbar2 of every row particle R with a lower block
+// one-step fill correction of the pair blocks (IC(0)'s W_PQ = A_PQ - sum_R W_PR D_R^-1 W_QR^T
+// with the pair blocks for W on the right): W_e -= sum_R A_PR Dinv_R A_QR^T over the common
+// lower neighbours R of the edge's particles (R < Q < P; both lower lists are sorted), with
+// the factor's final pivots.  One thread per lower block.
+__device__ __forceinline__ void lp_mat6_atb(const float *A, const float *M, const float *B, float *out, float w,
+                                            bool accumulate)
+{
+    // out (+)= w * A M B^T
+    float T[36];
+    for (int a = 0; a < 6; ++a)
+        for (int c = 0; c < 6; ++c) {
+            float s = 0.f;
+            for (int q = 0; q < 6; ++q) s += A[a * 6 + q] * M[q * 6 + c];
+            T[a * 6 + c] = s;
+        }
+    for (int a = 0; a < 6; ++a)
+        for (int c = 0; c < 6; ++c) {
+            float s = 0.f;
+            for (int q = 0; q < 6; ++q) s += T[a * 6 + q] * B[c * 6 + q];
+            out[a * 6 + c] = (accumulate ? out[a * 6 + c] : 0.f) + w * s;
+        }
+}
+
+__global__ void lp_wfix_kernel(int64_t E, const int *erow, const int *lcol, const int *lrow, const float *Ablk,
+                               const float *Dinv, float *W)
+{
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int P = erow[e], Q = lcol[e];
+    int i = lrow[P], j = lrow[Q];
+    const int ie = lrow[P + 1], je = lrow[Q + 1];
+    float acc[36];
+    bool any = false;
+    while (i < ie && j < je) {
+        const int rp = lcol[i], rq = lcol[j];
+        if (rp < rq) { ++i; continue; }
+        if (rq < rp) { ++j; continue; }
+        lp_mat6_atb(Ablk + (int64_t)i * 36, Dinv + (int64_t)rp * 36, Ablk + (int64_t)j * 36, acc, 1.0f, any);
+        any = true;
+        ++i; ++j;
+    }
+    if (!any) return;
+    float *w = W + e * 36;
+    for (int q = 0; q < 36; ++q) w[q] -= acc[q];
+}
+
+// backward of the fill correction: the gradient with respect to the final inverse pivots,
+// G_R += -A_PR^T gW_e A_QR for every common lower neighbour R of edge e = (P, Q)
+__global__ void lp_wfix_bwd_kernel(int64_t E, const int *erow, const int *lcol, const int *lrow, const float *Ablk,
+                                   const float *gW, float *G)
+{
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int P = erow[e], Q = lcol[e];
+    int i = lrow[P], j = lrow[Q];
+    const int ie = lrow[P + 1], je = lrow[Q + 1];
+    const float *g = gW + e * 36;
+    while (i < ie && j < je) {
+        const int rp = lcol[i], rq = lcol[j];
+        if (rp < rq) { ++i; continue; }
+        if (rq < rp) { ++j; continue; }
+        // A_PR^T gW A_QR: (A_PR^T) (gW) (A_QR^T)^T
+        const float *Ap = Ablk + (int64_t)i * 36, *Aq = Ablk + (int64_t)j * 36;
+        float At[36], Bt[36], out[36];
+        for (int a = 0; a < 6; ++a)
+            for (int c = 0; c < 6; ++c) {
+                At[a * 6 + c] = Ap[c * 6 + a];
+                Bt[a * 6 + c] = Aq[c * 6 + a];
+            }
+        lp_mat6_atb(At, g, Bt, out, -1.0f, false);
+        float *gr = G + (int64_t)rp * 36;
+        for (int q = 0; q < 36; ++q) atomicAdd(gr + q, out[q]);
+        ++i; ++j;
+    }
+}
+
+// adds the fill correction's gradient G (with respect to the final inverse pivots) to gDinv:
+// as it is when gDinv is itself a gradient with respect to the inverse pivots, as
+// -Dinv sym(G) Dinv when it holds the gradient with respect to the pivots (dbar_direct)
+__global__ void lp_wfix_dbar_kernel(int N, int dbar_direct, const float *Dinv, const float *G, float *gDinv)
+{
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= N) return;
+    const float *g = G + (int64_t)P * 36;
+    float *o = gDinv + (int64_t)P * 36;
+    if (!dbar_direct) {
+        for (int q = 0; q < 36; ++q) o[q] += g[q];
+        return;
+    }
+    float S[36];
+    for (int a = 0; a < 6; ++a)
+        for (int c = 0; c < 6; ++c) S[a * 6 + c] = 0.5f * (g[a * 6 + c] + g[c * 6 + a]);
+    const float *Di = Dinv + (int64_t)P * 36;
+    float out[36];
+    lp_mat6_atb(Di, S, Di, out, -1.0f, false);   // Dinv symmetric: Dinv S Dinv^T = Dinv S Dinv
+    for (int q = 0; q < 36; ++q) o[q] += out[q];
+}
+
+// from a pivot pass to the one before it: the gradient with respect to the earlier pass's
+// pivots D1_Q, from the later pass's Dbar2 of every row particle R with a lower block
 // (R, Q): the gradient with respect to D1_Q^-1 is X = sum -s_R gamma_e A_e^T Dbar2_R A_e
-// (levels of the second pass, none at level 4), and D1bar_Q = -D1_Q^-1 sym(X) D1_Q^-1.
+// (levels of the later pass, none at level 4), and D1bar_Q = -D1_Q^-1 sym(X) D1_Q^-1.
 // The column entries of Q are its upper list (ucol: the row particle, upos: the block).
 __global__ void lp_fill_bwd_kernel(int N, const int *urow, const int *ucol, const int *upos, const float *Ablk,
                                    const float *oute, const int *lvl2, const float *Dbar2, const float *D1inv,
@@ -1060,6 +1164,17 @@ int pack_weights(fsd_ctx *ctx)
     return 0;
 }
 
+// inverse pivots that pass p's fill terms use (the uncorrected ones for the first pass) and
+// that pass p writes (the factor's own for the last)
+const float *pass_finv(const LearnedState &s, int64_t N, int p)
+{
+    return p == 0 ? s.Dtinv : s.Dpinv + (int64_t)(p - 1) * N * 36;
+}
+float *pass_dinv(const LearnedState &s, int64_t N, int p)
+{
+    return p == lp::NPASS - 1 ? s.Dinv : s.Dpinv + (int64_t)p * N * 36;
+}
+
 int grow_edges(fsd_ctx *ctx, int64_t E)
 {
     LearnedState &s = *ctx->lp;
@@ -1131,9 +1246,8 @@ int lp_init(fsd_ctx *ctx)
     LRC(lalloc(ctx, s.goutn, N * lp::ON));
     LRC(lalloc(ctx, s.gDinv, N * 36));
     LRC(lalloc(ctx, s.d_fail, 4));
-    LRC(lalloc(ctx, s.lvl, N));
-    LRC(lalloc(ctx, s.lvl1, N));
-    LRC(lalloc(ctx, s.D1inv, N * 36));
+    LRC(lalloc(ctx, s.lvlp, N * lp::NPASS));
+    LRC(lalloc(ctx, s.Dpinv, N * 36 * (lp::NPASS - 1)));
     LRC(lalloc(ctx, s.Dbar2, N * 36));
     LRC(lalloc(ctx, s.d_loss, 4));
     FSD_HIP(hipHostMalloc((void **)&s.h_fail, sizeof(int) * 4, hipHostMallocDefault));
@@ -1150,7 +1264,7 @@ void lp_free(fsd_ctx *ctx)
     LearnedState &s = *ctx->lp;
     void *ptrs[] = {s.d_w, s.d_grad, s.d_m, s.d_v, s.d_wn, s.xn0, s.zn0, s.agg, s.xh, s.outn, s.Dinv, s.Dtinv, s.D,
                     s.xe0, s.ze0, s.oute, s.geo, s.W, s.erow, s.d_fail, s.gh, s.gh2, s.ge, s.gm, s.gzn, s.gze, s.gxe,
-                    s.gxn, s.goute, s.goutn, s.gW, s.gDinv, s.gbf, s.part, s.jv, s.d_loss, s.d_wbak, s.lvl, s.lvl1, s.D1inv, s.Dbar2};
+                    s.gxn, s.goute, s.goutn, s.gW, s.gDinv, s.gbf, s.part, s.jv, s.d_loss, s.d_wbak, s.lvlp, s.Dpinv, s.Dbar2};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (int l = 0; l <= lp::NMP; ++l) {
@@ -1297,15 +1411,19 @@ int lp_forward(fsd_ctx *ctx, const float4 *pos, bool train, int inject, int *fai
     // factor
     if (E)
         hipLaunchKernelGGL(lp_edge_W_kernel, dim3(nblk(E * 36, 256)), dim3(256), 0, ctx->stream, E, s.geo, s.oute, s.W);
-    // two passes of the pivots: the first with the uncorrected neighbour pivots in the fill
-    // terms, the second with the first's (only the second's safeguard counts are kept)
-    hipLaunchKernelGGL(lp_node_D_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow, ctx->d_lcol,
-                       ctx->d_Adiag, lambda, s.outn, s.oute, ctx->d_Ablk, s.Dtinv, s.Dtinv, s.D1inv,
-                       (float *)nullptr, s.lvl1, s.d_fail);
-    hipLaunchKernelGGL(lp_fill_int_kernel, dim3(1), dim3(64), 0, ctx->stream, 2, 0, s.d_fail + 1);
-    hipLaunchKernelGGL(lp_node_D_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow, ctx->d_lcol,
-                       ctx->d_Adiag, lambda, s.outn, s.oute, ctx->d_Ablk, s.Dtinv, s.D1inv, s.Dinv, s.D, s.lvl,
-                       s.d_fail);
+    // NPASS parallel passes of the pivot recurrence: the first with the uncorrected neighbour
+    // pivots in the fill terms, each later one with the previous pass's (only the last pass's
+    // safeguard counts are kept)
+    for (int p = 0; p < lp::NPASS; ++p) {
+        const bool last = p == lp::NPASS - 1;
+        if (last) hipLaunchKernelGGL(lp_fill_int_kernel, dim3(1), dim3(64), 0, ctx->stream, 2, 0, s.d_fail + 1);
+        hipLaunchKernelGGL(lp_node_D_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow,
+                           ctx->d_lcol, ctx->d_Adiag, lambda, s.outn, s.oute, ctx->d_Ablk, s.Dtinv, pass_finv(s, N, p),
+                           pass_dinv(s, N, p), last ? s.D : (float *)nullptr, s.lvlp + (int64_t)p * N, s.d_fail);
+    }
+    if (E)   // the pair blocks' one-step fill correction with the final pivots
+        hipLaunchKernelGGL(lp_wfix_kernel, dim3(nblk(E, 128)), dim3(128), 0, ctx->stream, E, s.erow, ctx->d_lcol,
+                           ctx->d_lrow, ctx->d_Ablk, s.Dinv, s.W);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_NET, pev);
     FSD_HIP(hipMemcpyAsync(s.h_fail, s.d_fail, sizeof(int) * 3, hipMemcpyDeviceToHost, ctx->stream));
@@ -1360,16 +1478,29 @@ static int lp_backward(fsd_ctx *ctx, bool dbar_direct)
     LearnedState &s = *ctx->lp;
     const int N = ctx->N;
     const int64_t E = ctx->pch.nlow;
-    // assembly backward
-    // second pivot pass, then its dependence on the first pass's pivots, then the first pass
-    hipLaunchKernelGGL(lp_node_D_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow,
-                       ctx->d_lcol, s.Dinv, s.gDinv, dbar_direct ? 1 : 0, s.oute, s.outn, s.lambda,
-                       ctx->d_Ablk, s.D1inv, s.lvl, 0, s.Dbar2, s.goutn, s.goute);
-    hipLaunchKernelGGL(lp_fill_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_urow, ctx->d_ucol,
-                       ctx->d_upos, ctx->d_Ablk, s.oute, s.lvl, s.Dbar2, s.D1inv, s.gDinv);
-    hipLaunchKernelGGL(lp_node_D_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow,
-                       ctx->d_lcol, s.D1inv, s.gDinv, 1, s.oute, s.outn, s.lambda, ctx->d_Ablk, s.Dtinv, s.lvl1,
-                       1, (float *)nullptr, s.goutn, s.goute);
+    // the pair blocks' fill correction depends on the final pivots: its gradient joins gDinv
+    if (E) {
+        hipLaunchKernelGGL(lp_fill_kernel, dim3(nblk((int64_t)N * 36, 256)), dim3(256), 0, ctx->stream,
+                           (int64_t)N * 36, 0.0f, s.Dbar2);
+        hipLaunchKernelGGL(lp_wfix_bwd_kernel, dim3(nblk(E, 128)), dim3(128), 0, ctx->stream, E, s.erow, ctx->d_lcol,
+                           ctx->d_lrow, ctx->d_Ablk, s.gW, s.Dbar2);
+        hipLaunchKernelGGL(lp_wfix_dbar_kernel, dim3(nblk(N, 128)), dim3(128), 0, ctx->stream, N,
+                           dbar_direct ? 1 : 0, s.Dinv, s.Dbar2, s.gDinv);
+    }
+    // assembly backward: the pivot passes from the last to the first; between two passes, the
+    // later pass's dependence on the earlier one's pivots (gradient into gDinv, which the
+    // earlier pass then reads as its pivot gradient)
+    for (int p = lp::NPASS - 1; p >= 0; --p) {
+        const bool last = p == lp::NPASS - 1;
+        const int *lvl = s.lvlp + (int64_t)p * N;
+        hipLaunchKernelGGL(lp_node_D_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_lrow,
+                           ctx->d_lcol, pass_dinv(s, N, p), s.gDinv, (last && !dbar_direct) ? 0 : 1, s.oute, s.outn,
+                           s.lambda, ctx->d_Ablk, pass_finv(s, N, p), lvl, last ? 0 : 1,
+                           p > 0 ? s.Dbar2 : (float *)nullptr, s.goutn, s.goute);
+        if (p > 0)
+            hipLaunchKernelGGL(lp_fill_bwd_kernel, dim3(nblk(N, 64)), dim3(64), 0, ctx->stream, N, ctx->d_urow,
+                               ctx->d_ucol, ctx->d_upos, ctx->d_Ablk, s.oute, lvl, s.Dbar2, pass_finv(s, N, p), s.gDinv);
+    }
     if (E) hipLaunchKernelGGL(lp_edge_W_bwd_kernel, dim3(nblk(E, 128)), dim3(128), 0, ctx->stream, E, s.geo, s.oute,
                               s.gW, s.goute);
     // heads
@@ -1565,6 +1696,19 @@ int lp_adam(fsd_ctx *ctx, float lr, int step)
     LRC(pack_weights(ctx));
     FSD_HIP(hipStreamSynchronize(ctx->stream));
     s.have_w = true;
+    return 0;
+}
+
+// safeguard levels of the last factor's pivots, [NPASS][N]
+int lp_levels(fsd_ctx *ctx, int32_t *h_levels, int64_t n)
+{
+    LearnedState &s = *ctx->lp;
+    if (n != (int64_t)lp::NPASS * ctx->N) {
+        ctx->err = "learned levels: expected " + std::to_string((int64_t)lp::NPASS * ctx->N) + " values";
+        return FSD_E_PARAM;
+    }
+    FSD_HIP(hipMemcpyAsync(h_levels, s.lvlp, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+    FSD_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 

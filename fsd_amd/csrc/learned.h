@@ -24,6 +24,7 @@ constexpr int OE = 32;       // edge head outputs (7 used)
 constexpr int ON = 32;       // particle head outputs (2 used)
 constexpr int NLAYER = 10;
 constexpr int NMP = 3;       // message-passing layers
+constexpr int NPASS = 4;     // parallel passes of the pivot recurrence
 enum { L_ENC_N = 0, L_ENC_E = 1, L_MSG = 2, /* L_MSG + 2 l, L_UPD + 2 l */ L_UPD = 3, L_HEAD_E = 8, L_HEAD_N = 9 };
 struct Layer {
     int ki, no, act;         // inputs, outputs, SiLU (1) or linear (0)
@@ -52,8 +53,9 @@ struct LearnedState {
     float *geo = nullptr, *W = nullptr;
     int *erow = nullptr;                         // row particle of every lower block
     int *d_fail = nullptr, *h_fail = nullptr;   // [0] failure bits, [1] pivots above level 0, [2] at level 4
-    int *lvl = nullptr, *lvl1 = nullptr;         // per particle: the level of its pivot safeguard (second, first pass)
-    float *D1inv = nullptr, *Dbar2 = nullptr;    // first pass's inverse pivots; second pass's pivot gradient
+    int *lvlp = nullptr;                         // [NPASS][N]: the level of each pivot's safeguard in each pass
+    float *Dpinv = nullptr;                      // [NPASS - 1][N][36]: inverse pivots of the passes before the last
+    float *Dbar2 = nullptr;                      // a pass's pivot gradient (backward scratch)
     float lambda = 1.0f;                         // pivot shift of the last factor (doubled until A_PP + lambda S > 0)
     int shrunk = 0, plain = 0;                   // pivots of the last factor above level 0 / at level 4
     // backward buffers

@@ -1854,6 +1854,12 @@ int fsd_learned_export(fsd_ctx *ctx, float *h_oute, float *h_outn, float *h_W, f
     return lp_export(ctx, h_oute, h_outn, h_W, h_Dinv);
 }
 
+int fsd_learned_levels(fsd_ctx *ctx, int32_t *h_levels, int64_t n)
+{
+    if (!ctx->lp) { ctx->err = "no learned factor"; return FSD_E_STATE; }
+    return lp_levels(ctx, h_levels, n);
+}
+
 int fsd_learned_factor_loss(fsd_ctx *ctx, const float *d_pos4, int32_t want_grad, double *loss)
 {
     if (ctx->precond_mode != 2) { ctx->err = "the learned factor needs precond mode 2"; return FSD_E_STATE; }

@@ -408,6 +408,13 @@ class Stokes:
         return dict(steps=int(out[0]), pivot_fallbacks=int(out[1]), iteration_fallbacks=int(out[2]),
                     calibrations=int(out[3]), reduced_fill_pivots=int(out[4]), plain_pivots=int(out[5]))
 
+    def learned_levels(self, npass=4):
+        """safeguard level of every pivot in each pass of the last learned factor, [pass, particle]
+        in the pattern's order"""
+        lv = np.zeros((npass, self.N), dtype=np.int32)
+        self._check(self.lib.fsd_learned_levels(self.ctx, lv.ctypes.data_as(C.c_void_p), lv.size))
+        return lv
+
     def learned_export(self, n_lower):
         N = self.N
         oute = np.zeros((n_lower, 32), dtype=np.float32)

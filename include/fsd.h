@@ -327,6 +327,10 @@ int fsd_learned_stats(fsd_ctx *ctx, int64_t *h_out6);
  * block) and inverse pivots D^-1 (36 per particle) of the active learned factor, in
  * the pattern's order (fsd_get_precond's permuted lower blocks); any may be null */
 int fsd_learned_export(fsd_ctx *ctx, float *h_oute, float *h_outn, float *h_W, float *h_Dinv, int64_t edge_cap);
+/* pivot safeguard level (0-3: fill scaled by 2^-level; 4: the uncorrected pivot) of every
+ * particle in each of the 4 passes of the last learned factor, [pass][particle] in the
+ * pattern's order; n = 4 N */
+int fsd_learned_levels(fsd_ctx *ctx, int32_t *h_levels, int64_t n);
 /* Training loss for nsamp samples: d_x17 holds x_s, d_b17 holds b_s = A x_s (A the saddle
  * operator, fsd_saddle_multiply), 17N each, consecutive.  With g_s = G (b_F - b_U)_s the
  * learned preconditioner's G (jacobi_iters iterations each way), the loss is

@@ -2,9 +2,11 @@
 
 FP64 numpy version of what fsd_amd/csrc/k_learned.hip computes: the edge and
 particle features, the 3-layer message-passing network (hidden 64), the block
-LDL^T factor it defines on IC(0)'s pattern, the preconditioner G applied by k
-block Jacobi iterations each way, the training loss and its gradient with
-respect to every network parameter.
+LDL^T factor it defines on IC(0)'s pattern (pivots from NPASS parallel passes of
+IC(0)'s pivot recurrence with the positive-definite safeguard, pair blocks with
+their one-step fill correction), the preconditioner G applied by k block Jacobi
+iterations each way, the training losses and their gradients with respect to
+every network parameter.
 
 The learned factor has no counterpart in the reference; what it replaces is
 the IC(0) factorisation of the saddle preconditioner (Precondition.cu:1116-1307,
@@ -17,7 +19,9 @@ gradients that enter the input-gradient products.  `emulate=False` is plain
 FP64; the finite-difference tests of the analytic gradient use it.
 
 Parity of this module's gradient is pinned by central finite differences of
-its own FP64 loss (tests/test_learned_cpu.py), and its G by the identity
+its own FP64 losses, also where pivots sit at every safeguard level
+(tests/test_learned_cpu.py); its factor by the zero-output case written out
+pair by pair (the pivot passes and the fill correction); its G by the identity
 G = ((D + W) D^-1 (D + W)^T)^-1 for k at least the depth of the pattern.
 """
 import numpy as np
@@ -61,6 +65,7 @@ def bf16(x):
 
 GAMMA0 = -0.036004   # logit(0.491): gamma(0) = 0.982
 LEVELS = 4   # pivot safeguard: fill scales 1, 1/2, 1/4, 1/8, then the uncorrected pivot
+NPASS = 4    # parallel passes of the pivot recurrence
 
 
 def _is_pd(M):
@@ -200,8 +205,20 @@ class LearnedOracle:
         self.lam = lam
         self.Dt = Ad + lam * np.einsum('c,cd->cd', S6, np.eye(6))[None]
         self.Dtinv = np.linalg.inv(self.Dt)
-        self.levels = np.zeros(N, dtype=np.int64)
-        self.levels1 = np.zeros(N, dtype=np.int64)
+        # triangles of the lower pattern: (edge (P, Q), edge (P, R), edge (Q, R)) for every common
+        # lower neighbour R of an edge's particles (the pair blocks' fill correction)
+        low = {}
+        for e in range(self.E):
+            low.setdefault(int(self.rows[e]), {})[int(self.cols[e])] = e
+        tri = []
+        for e in range(self.E):
+            lp_, lq = low.get(int(self.rows[e]), {}), low.get(int(self.cols[e]), {})
+            for R, eq in lq.items():
+                if R in lp_:
+                    tri.append((e, lp_[R], eq, R))
+        self.tri = np.array(tri, dtype=np.int64).reshape(-1, 4)
+        self.levels_p = [np.zeros(N, dtype=np.int64) for _ in range(NPASS)]
+        self.levels = self.levels_p[-1]
         self.Bk = basis(self.geo)
 
     # ---- network
@@ -272,12 +289,20 @@ class LearnedOracle:
                     D[P] = self.Dt[P]
             return D
 
-        # two passes: the fill with the uncorrected pivots, then with the first pass's
-        self.D1 = one_pass(self.Dtinv, self.levels1)
-        self.D1inv = np.linalg.inv(self.D1)
-        D = one_pass(self.D1inv, self.levels)
-        Dinv = np.linalg.inv(D)
-        return W, D, Dinv, failed
+        # NPASS passes: the fill with the uncorrected pivots, then each with the previous pass's
+        self.Dp, self.Dpinv = [], []
+        Finv = self.Dtinv
+        for p in range(NPASS):
+            Dn = one_pass(Finv, self.levels_p[p])
+            self.Dp.append(Dn)
+            Finv = np.linalg.inv(Dn)
+            self.Dpinv.append(Finv)
+        # the pair blocks' one-step fill correction with the final pivots (k_learned.hip
+        # lp_wfix_kernel): W_PQ -= sum_R A_PR D_R^-1 A_QR^T
+        if len(self.tri):
+            t = self.tri
+            np.add.at(W, t[:, 0], -np.einsum('taq,tqr,tcr->tac', self.Ablk[t[:, 1]], Finv[t[:, 3]], self.Ablk[t[:, 2]]))
+        return W, self.Dp[-1], self.Dpinv[-1], failed
 
     def lower(self, W, v):
         out = np.zeros((self.N, 6))
@@ -433,17 +458,30 @@ class LearnedOracle:
                 goute[:, 6] += -sc[self.rows] * np.einsum('eac,eac->e', Db[self.rows], T) * gamma_d(oute[:, 6])
             return live, sc
 
-        live2, sc2 = pass_backward(Db, self.levels, self.D1inv)
-        # the second pass depends on the first's pivots through its fill terms A D1_Q^-1 A^T
-        X = np.zeros((N, 6, 6))
-        if self.E:
-            Db2 = Db * live2[:, None, None]
-            w = -sc2[self.rows] * gamma(oute[:, 6])
-            np.add.at(X, self.cols, w[:, None, None] * np.einsum('eqa,eqr,erc->eac', self.Ablk, Db2[self.rows],
-                                                                   self.Ablk))
-        Xs = 0.5 * (X + np.transpose(X, (0, 2, 1)))
-        D1b = -np.einsum('paq,pqr,prc->pac', self.D1inv, Xs, self.D1inv)
-        pass_backward(D1b, self.levels1, self.Dtinv)
+        # the pair blocks' fill correction depends on the final pivots: -A_PR^T gW A_QR with
+        # respect to D_R^-1, i.e. -D^-1 sym(.) D^-1 with respect to D_R
+        if len(self.tri):
+            t = self.tri
+            G = np.zeros((N, 6, 6))
+            np.add.at(G, t[:, 3], -np.einsum('tqa,tqr,trc->tac', self.Ablk[t[:, 1]], gW[t[:, 0]], self.Ablk[t[:, 2]]))
+            Gs = 0.5 * (G + np.transpose(G, (0, 2, 1)))
+            Dfi = self.Dpinv[-1]
+            Db = Db - np.einsum('paq,pqr,prc->pac', Dfi, Gs, Dfi)
+        # from the last pass to the first; each later pass depends on the earlier one's pivots
+        # through its fill terms A D_Q^-1 A^T
+        for p in range(NPASS - 1, -1, -1):
+            Finv = self.Dpinv[p - 1] if p > 0 else self.Dtinv
+            live, sc = pass_backward(Db, self.levels_p[p], Finv)
+            if p == 0:
+                break
+            X = np.zeros((N, 6, 6))
+            if self.E:
+                Dbl = Db * live[:, None, None]
+                w = -sc[self.rows] * gamma(oute[:, 6])
+                np.add.at(X, self.cols, w[:, None, None] * np.einsum('eqa,eqr,erc->eac', self.Ablk, Dbl[self.rows],
+                                                                       self.Ablk))
+            Xs = 0.5 * (X + np.transpose(X, (0, 2, 1)))
+            Db = -np.einsum('paq,pqr,prc->pac', Finv, Xs, Finv)
         if self.E:
             goute[:, :6] = np.einsum('eac,ekac->ek', gW, self.Bk) * (1.0 - np.tanh(oute[:, :6]) ** 2)
         return self._mlp_backward(c, goute, goutn, emulate)

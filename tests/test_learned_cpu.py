@@ -31,29 +31,43 @@ def test_layouts_agree():
 
 
 def test_zero_outputs_give_one_step_fill(dense):
-    """with zero network outputs W is the pair blocks A and the pivots are two passes of
-    D = A_PP + lambda S minus 0.982 of every lower pair's fill term A F A^T, F = Dt^-1 in
-    the first pass and the first pass's D^-1 in the second"""
+    """with zero network outputs the pivots are NPASS passes of D = A_PP + lambda S minus
+    0.982 of every lower pair's fill term A F A^T, F = Dt^-1 in the first pass and the
+    previous pass's D^-1 in each later one, and W is the pair blocks A minus their one-step
+    fill A_PR D_R^-1 A_QR^T over the common lower neighbours R (written out pair by pair)"""
     from oracle.learned import init_weights
     orc, lo, X, B = dense
     c = lo.forward(init_weights(1, scale_out=0.0), emulate=False)
     assert np.abs(c["oute"]).max() == 0 and np.abs(c["outn"]).max() == 0
     W, D, Dinv, failed = lo.assemble(c["oute"], c["outn"])
     assert lo.E > 100 and not failed
-    assert np.abs(W - lo.Ablk).max() <= 1e-5 * np.abs(lo.Ablk).max()
+    Wref = lo.Ablk.copy()
+    low = {}
+    for e in range(lo.E):
+        low.setdefault(lo.rows[e], {})[lo.cols[e]] = e
+    ntri = 0
+    for e in range(lo.E):
+        P, Q = lo.rows[e], lo.cols[e]
+        for R, eq in low.get(Q, {}).items():
+            if R in low.get(P, {}):
+                Wref[e] -= lo.Ablk[low[P][R]] @ np.linalg.inv(D[R]) @ lo.Ablk[eq].T
+                ntri += 1
+    assert ntri > 0
+    # 1e-5: W's basis takes the table coefficients in FP32, the pair blocks A in FP64
+    assert np.abs(W - Wref).max() <= 1e-5 * np.abs(Wref).max()
     # the pivots written out pair by pair from the pattern
     from oracle.learned import gamma
+    from oracle.learned import NPASS
     g = gamma(0.0)
-    assert abs(g - 0.982) < 1e-4
-    D1 = lo.Dt.copy()
-    for e in range(lo.E):
-        P, Q = lo.rows[e], lo.cols[e]
-        D1[P] -= g * lo.Ablk[e] @ np.linalg.inv(lo.Dt[Q]) @ lo.Ablk[e].T
-    Dref = lo.Dt.copy()
-    for e in range(lo.E):
-        P, Q = lo.rows[e], lo.cols[e]
-        Dref[P] -= g * lo.Ablk[e] @ np.linalg.inv(D1[Q]) @ lo.Ablk[e].T
-    assert np.all(lo.levels == 0) and np.all(lo.levels1 == 0)
+    assert abs(g - 0.982) < 1e-4 and NPASS == 4
+    prev = lo.Dt.copy()
+    for p in range(NPASS):
+        Dref = lo.Dt.copy()
+        for e in range(lo.E):
+            P, Q = lo.rows[e], lo.cols[e]
+            Dref[P] -= g * lo.Ablk[e] @ np.linalg.inv(prev[Q]) @ lo.Ablk[e].T
+        prev = Dref
+    assert all(np.all(lv == 0) for lv in lo.levels_p)
     assert np.abs(D - Dref).max() <= 1e-9 * np.abs(Dref).max()
 
 
@@ -141,11 +155,12 @@ def test_pivot_safeguard(overfilled):
     w = _shrunk_weights(1, -8.0, 8.0)
     c = lo.forward(w, emulate=False)
     W, D, Dinv, failed = lo.assemble(c["oute"], c["outn"])
-    print("lambda", lo.lam, "levels (first, second pass)", np.bincount(lo.levels1, minlength=LEVELS + 1),
-          np.bincount(lo.levels, minlength=LEVELS + 1))
-    assert not failed and np.all(np.bincount(lo.levels1, minlength=LEVELS + 1) > 0)
+    print("lambda", lo.lam, "levels per pass", [np.bincount(lv, minlength=LEVELS + 1).tolist() for lv in lo.levels_p])
+    assert not failed and np.all(np.bincount(lo.levels_p[0], minlength=LEVELS + 1) > 0)
     S = np.diag([1.0, 1.0, 1.0, 4.0 / 3.0, 4.0 / 3.0, 4.0 / 3.0])
-    for lv, Finv, Dout in ((lo.levels1, lo.Dtinv, lo.D1), (lo.levels, lo.D1inv, D)):
+    passes = [(lo.levels_p[p], lo.Dpinv[p - 1] if p else lo.Dtinv, lo.Dp[p]) for p in range(len(lo.Dp))]
+    assert np.array_equal(passes[-1][2], D)
+    for lv, Finv, Dout in passes:
         F = np.zeros((lo.N, 6, 6))
         for e in range(lo.E):
             F[lo.rows[e]] += gamma(c["oute"][e, 6]) * lo.Ablk[e] @ Finv[lo.cols[e]] @ lo.Ablk[e].T
@@ -168,13 +183,13 @@ def test_safeguard_gradient_finite_differences(overfilled):
     Dic, Wic = lo.factor_targets(orc)
     w = _shrunk_weights(2, -8.0, 8.0)
     loss, g = lo.factor_loss(w, Dic, Wic, emulate=False, grad=True)
-    lv, lv1 = lo.levels.copy(), lo.levels1.copy()
-    assert lv.max() > 0 and lv1.max() > 0
+    lvs = [lv.copy() for lv in lo.levels_p]
+    assert all(lv.max() > 0 for lv in lvs)
     v = np.random.default_rng(8).standard_normal(w.size)
     eps = 1e-7
     fp = lo.factor_loss(w + eps * v, Dic, Wic, emulate=False)
-    assert np.array_equal(lo.levels, lv) and np.array_equal(lo.levels1, lv1)
+    assert all(np.array_equal(a, b) for a, b in zip(lo.levels_p, lvs))
     fm = lo.factor_loss(w - eps * v, Dic, Wic, emulate=False)
-    assert np.array_equal(lo.levels, lv) and np.array_equal(lo.levels1, lv1)
+    assert all(np.array_equal(a, b) for a, b in zip(lo.levels_p, lvs))
     fd = (fp - fm) / (2 * eps)
     assert abs(fd - g @ v) < 1e-5 * abs(fd)
