@@ -170,6 +170,15 @@ def load():
         return _lib
     if not os.path.exists(LIBPATH):
         raise RuntimeError("libfsd.so is not built; run `python -m fsd_amd.build` first")
+    # PyTorch-ROCm brings its own HIP runtime: it is loaded and brings the GPU up before the
+    # library (and with it the system HIP runtime) is loaded; in the other order the library
+    # finds no device
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except ImportError:
+        pass
     lib = C.CDLL(LIBPATH)
     for name, res, args in SIGNATURES:
         fn = getattr(lib, name)

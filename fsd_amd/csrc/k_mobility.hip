@@ -1001,6 +1001,7 @@ __global__ void __launch_bounds__(512) contract_tile2_kernel(SpreadGeom g, const
     // this lane's stencil column (ty, tz) and its offsets
     const int ty = lane / PP, tz = lane - PP * (lane / PP);
     const float aty = g.hy * (float)ty, atz = g.hz * (float)tz;
+    const float gq = __expf(-2.0f * g.expfac * g.hx * g.hx);
     for (int q = wv; q < total; q += nw) {
         int i, bpk;
         float4 c;
@@ -1021,14 +1022,19 @@ __global__ void __launch_bounds__(512) contract_tile2_kernel(SpreadGeom g, const
             const float dyz = dy * dy + dz * dz;
             const float dx0 = c.x + g.xy * uy;
             const int rb = (bx * S + by + ty) * S + bz + tz;
+            // the weights along x by the fast-Gaussian-gridding recurrence (Greengard and Lee,
+            // SIAM Rev. 2004): w(tx + 1) = w(tx) r(tx), r(tx + 1) = r(tx) q with
+            // r(0) = exp(-a hx (2 dx0 + hx)) and q = exp(-2 a hx^2): two exponentials per lane
+            float w = __expf(-g.expfac * (dx0 * dx0 + dyz));
+            float r = __expf(-g.expfac * g.hx * (2.0f * dx0 + g.hx));
 #pragma unroll
             for (int tx = 0; tx < PP; ++tx) {
-                const float dx = dx0 + g.hx * (float)tx;
-                const float w = __expf(-g.expfac * (dx * dx + dyz));
-                const int r = rb + tx * S * S;
+                const int ro = rb + tx * S * S;
 #pragma unroll
                 for (int c2 = 0; c2 < CT_NF; ++c2)
-                    if (c2 < nf) acc[c2] += w * s_g[c2 * S3 + r];
+                    if (c2 < nf) acc[c2] += w * s_g[c2 * S3 + ro];
+                w *= r;
+                r *= gq;
             }
         }
         // wave sums: rows of 16 by DPP, then the three rows that hold the 36 lanes.  Lane c < nf

@@ -1082,12 +1082,28 @@ int fsd_create(const fsd_params *p, fsd_ctx **out)
     int rc = pse_params(ctx);
     if (rc) { *out = ctx; return rc; }
     cell_dims(ctx);
+    // the HIP runtime is initialised on the current device before anything else (this may be
+    // the process's first HIP call), then rocFFT once per process
+    {
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipSetDevice(dev);
+        if (e == hipSuccess) e = hipFree(nullptr);
+        if (e != hipSuccess) {
+            ctx->err = std::string("HIP runtime initialisation failed: ") + hipGetErrorString(e);
+            *out = ctx;
+            return FSD_E_HIP;
+        }
+    }
     static bool fft_ready = false;
     if (!fft_ready) { rocfft_setup(); fft_ready = true; }
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-        ctx->err = "hipStreamCreate failed";
-        *out = ctx;
-        return FSD_E_HIP;
+    {
+        const hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            ctx->err = std::string("hipStreamCreate failed: ") + hipGetErrorString(e);
+            *out = ctx;
+            return FSD_E_HIP;
+        }
     }
     // device address of the allocation-free entry point's scratch array (once per device)
     if (noalloc_init_device() != 0) {

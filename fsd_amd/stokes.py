@@ -40,8 +40,8 @@ class Stokes:
                  parity=True, gmres_true_tol=0.0, guess="previous", gram_schmidt_passes=2,
                  precond="ichol", jacobi_iters=3, factor_sweeps=0, learned_weights="default",
                  learned_calibration=50, refine=False, fallback_jacobi=0, autotune=False):
-        self.lib = _lib.load()
         self.torch = _torch()
+        self.lib = _lib.load()
         L = box if not np.isscalar(box) else (box, box, box, 0.0, 0.0, 0.0)
         L = tuple(L) + (0.0,) * (6 - len(L))
         p = _lib.FsdParams()
