@@ -7,16 +7,20 @@ deterministic random forces U[-1,1]^3 (zero mean), T = 0, xi = 0.5, PSE error
 <= 1e-4.  A "step" is one full Stokes::integrateStepOne: neighbour list,
 preconditioner build, force, right-hand side, GMRES solve, explicit Euler move.
 
-Two arms run on the same packing from the same start:
-  ichol   - the reference's preconditioner: IC(0) with exact triangular solves;
-  ichol_jacobi (optional) - IC(0) applied by block Jacobi iterations;
-  learned - the learned factor (a message-passing network on the lubrication
-            graph) applied by block Jacobi iterations, with the IC(0) fallbacks
-            (non-positive pivot; iteration count above IC(0)'s at the last
-            calibration step, one IC(0) step every --calibration steps) and
-            FP64-residual iterative refinement of the solve.
-`value` is the learned arm's steps/s (all inputs resident in HBM); the IC(0)
-arm and the ratio are in `config.arms`.
+Three arms run on the same packing from the same start (--arms):
+  ichol        - the reference's preconditioner: IC(0) with exact triangular solves;
+  ichol_jacobi - IC(0) applied by 8 block Jacobi iterations per triangular solve;
+  learned      - the learned factor (a message-passing network on the lubrication
+                 graph correcting a block LDL^T factor on IC(0)'s pattern: four
+                 parallel pivot passes, the pair blocks' fill correction) applied by
+                 5 block Jacobi iterations, with the IC(0) fallbacks (non-finite
+                 network outputs; iteration count above IC(0)'s at the last
+                 calibration step, one IC(0) step every --calibration steps; its
+                 IC(0) steps applied by 8 Jacobi iterations) and FP64-residual
+                 iterative refinement of the solve.
+Each context autotunes its mobility and R_FU launch shapes at its first step.
+`value` is the learned arm's steps/s (all inputs resident in HBM); every arm and
+the learned/IC(0) ratio are in `config.arms`.
 
 Prints one JSON line.  `roofline` is computed for the kernel class that takes
 the most time in the learned arm, from its algorithmic bytes and its launch
