@@ -32,7 +32,8 @@ def contract_bytes(N, M):
     return 44 * M + 72 * N
 
 
-def run_active(n_side=64, phi=0.4, steps=50, dt=1e-3, T=1.0, seed=2026, calibration=50, tol=1e-6, log=print):
+def run_active(n_side=64, phi=0.4, steps=50, dt=1e-3, T=1.0, seed=2026, calibration=50, tol=1e-6, skip_after=2,
+               log=print):
     import torch
     from .stokes import Stokes, ParticleState
     from .workloads import jittered_lattice, random_orientations
@@ -43,7 +44,7 @@ def run_active(n_side=64, phi=0.4, steps=50, dt=1e-3, T=1.0, seed=2026, calibrat
     log("# lattice start: N=%d phi=%.2f L=%.3f (%.1f s)" % (N, phi, L, time.time() - t0))
     st = Stokes(N, L, T=T, dt=dt, seed=seed, sqm_B1=1.5, gmres_tol=1e-5, gmres_true_tol=tol, gmres_restart=100,
                 gmres_maxiter=3000, refine=True, guess="previous", precond="learned", jacobi_iters=5,
-                learned_calibration=calibration, fallback_jacobi=8, autotune=True)
+                learned_calibration=calibration, fallback_jacobi=8, autotune=True, learned_skip_after=skip_after)
     pse = st.pse_info()
     M = pse["Nx"] * pse["Ny"] * pse["Nz"]
     log("# PSE grid %dx%dx%d, P=%d" % (pse["Nx"], pse["Ny"], pse["Nz"], pse["P"]))
@@ -54,14 +55,16 @@ def run_active(n_side=64, phi=0.4, steps=50, dt=1e-3, T=1.0, seed=2026, calibrat
     t1 = time.perf_counter()
     for s in range(steps):
         before = state.positions().astype(np.float64)
+        ts = time.perf_counter()
         info = st.integrateStepOne(s, state, force6=None, shear_rate=0.0)
+        ms = 1e3 * (time.perf_counter() - ts)
         d = state.positions().astype(np.float64) - before
         d -= L * np.rint(d / L)
         o = state.ori.view(-1, 3).cpu().numpy().astype(np.float64)
         swim.append(float(np.mean(np.sum(d * o, axis=1))) / dt)   # mean speed along the directors
         infos.append(info)
-        log("# step %d: iterations %d, true residual %.2e, preconditioner %d, fallback %d, Lanczos %d/%d, "
-            "speed along director %.4f" % (s, info["iterations"], info["true_residual"], info["precond_used"],
+        log("# step %d: %.0f ms, iterations %d, true residual %.2e, preconditioner %d, fallback %d, Lanczos %d/%d, "
+            "speed along director %.4f" % (s, ms, info["iterations"], info["true_residual"], info["precond_used"],
                                            info["fallback"], info["lanczos_ff"], info["lanczos_nf"], swim[-1]))
     torch.cuda.synchronize()
     t2 = time.perf_counter()
@@ -80,6 +83,8 @@ def run_active(n_side=64, phi=0.4, steps=50, dt=1e-3, T=1.0, seed=2026, calibrat
         "gmres_iterations_mean": float(np.mean([i["iterations"] for i in infos])),
         "learned_steps": int(learned_steps),
         "fallback_fraction": float(sum(i["fallback"] != 0 for i in infos)) / steps,
+        "fallbacks_by_kind": {k: int(sum(i["fallback"] == c for i in infos))
+                              for c, k in ((1, "pivot"), (2, "iterations"), (3, "skipped"))},
         "calibration_steps": int(sum(i["precond_used"] == 0 and i["fallback"] == 0 for i in infos)),
         "mean_speed_along_director": float(np.mean(swim)),
         "hbm": rates, "launch_shapes": st.tuning()[0],
@@ -91,8 +96,12 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--n-side", type=int, default=64)
+    ap.add_argument("--skip-after", type=int, default=2,
+                    help="iteration fallbacks in a row after which the learned factor is skipped until the next "
+                         "calibration step (0: never)")
     a = ap.parse_args(argv)
-    out = run_active(n_side=a.n_side, steps=a.steps, log=lambda m: print(m, file=sys.stderr, flush=True))
+    out = run_active(n_side=a.n_side, steps=a.steps, skip_after=a.skip_after,
+                     log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(out), flush=True)
 
 

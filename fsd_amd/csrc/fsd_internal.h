@@ -307,7 +307,11 @@ struct fsd_ctx {
     int lp_iter_cap = 0;           // IC(0)'s iterations at the last such step (0: none yet)
     int lp_inject = -1;            // particle whose learned pivot the next factor forces non-positive (tests)
     bool lp_force_ichol = false;   // this step's preconditioner is IC(0) (calibration)
-    int lp_step_fallback = 0;      // 1: this step's learned factor had a non-positive pivot
+    int lp_skip_after = 2;         // this many iteration fallbacks in a row skip the learned factor until the
+                                   // next calibration step (0: never skip)
+    int lp_consec_fb = 0;          // iteration fallbacks in a row since the last learned solve that converged
+    bool lp_skip = false;          // this step skips the learned factor (IC(0), fallback 3)
+    int lp_step_fallback = 0;      // 1: this step's learned factor had a non-positive pivot, 3: skipped
     int64_t lp_steps = 0, lp_pivot_fallbacks = 0, lp_iter_fallbacks = 0, lp_calibrations = 0;
 
     // saddle / GMRES

@@ -510,10 +510,30 @@ __global__ void __launch_bounds__(256) lj_fwd_kernel(int N, const int *lrow, con
     vout[6 * (int64_t)P + a] = v;
 }
 
-// backward iteration: s = W^T x_in (stored when sout != null), x_out = v_k - Dinv s
+// the preconditioner's output of a row's value g (unpermuted): mode 0: out = g (6N);
+// mode 1: the saddle preconditioner's combination x_F = b_F - g, x_S = b_S, x_U = g
+// (Saddle.cu:171-264 with g = G (b_F - b_U))
+__device__ __forceinline__ void lj_post_write(int N, int P, int a, const int *perm, float g, const float *b, float *out,
+                                              int mode)
+{
+    const int64_t i = perm[P], n6 = 6 * (int64_t)N, n11 = 11 * (int64_t)N;
+    if (mode == 0) {
+        out[6 * i + a] = g;
+        return;
+    }
+    out[6 * i + a] = b[6 * i + a] - g;
+    out[n11 + 6 * i + a] = g;
+    if (a < 5) out[n6 + 5 * i + a] = b[n6 + 5 * i + a];
+}
+
+// backward iteration: s = W^T x_in (stored when sout != null), x_out = v_k - Dinv s; the last
+// iteration of an application also writes the preconditioner's output (post_mode 0 or 1,
+// lj_post_write; -1: none)
 __global__ void __launch_bounds__(256) lj_bwd_kernel(int N, const int *urow, const int *ucol, const int *upos,
                                                      const float *W, const float *Dinv, const float *vk,
-                                                     const float *xin, float *sout, float *xout)
+                                                     const float *xin, float *sout, float *xout,
+                                                     const int *perm = nullptr, const float *bpost = nullptr,
+                                                     float *outpost = nullptr, int post_mode = -1)
 {
     int P, a;
     const bool ok = row_lane(N, P, a);
@@ -534,7 +554,9 @@ __global__ void __launch_bounds__(256) lj_bwd_kernel(int N, const int *urow, con
 #pragma unroll
     for (int c = 0; c < 6; ++c) q += Dp[c] * s6[c];
     if (sout) sout[6 * (int64_t)P + a] = s;
-    xout[6 * (int64_t)P + a] = vk[6 * (int64_t)P + a] - q;
+    const float x = vk[6 * (int64_t)P + a] - q;
+    xout[6 * (int64_t)P + a] = x;
+    if (post_mode >= 0) lj_post_write(N, P, a, perm, x, bpost, outpost, post_mode);
 }
 
 // g = x unpermuted; mode 0: out = g (6N); mode 1: the saddle preconditioner's
@@ -543,15 +565,7 @@ __global__ void lj_post_kernel(int N, const int *perm, const float *x, const flo
 {
     int P, a;
     if (!row_lane(N, P, a)) return;
-    const float g = x[6 * (int64_t)P + a];
-    const int64_t i = perm[P], n6 = 6 * (int64_t)N, n11 = 11 * (int64_t)N;
-    if (mode == 0) {
-        out[6 * i + a] = g;
-        return;
-    }
-    out[6 * i + a] = b[6 * i + a] - g;
-    out[n11 + 6 * i + a] = g;
-    if (a < 5) out[n6 + 5 * i + a] = b[n6 + 5 * i + a];
+    lj_post_write(N, P, a, perm, x[6 * (int64_t)P + a], b, out, mode);
 }
 
 // ---------------------------------------------------- training loss, adjoint
@@ -1459,13 +1473,15 @@ int lp_apply(fsd_ctx *ctx, const float *b0, const float *b1, const float *bsaddl
     // backward iterates alternate between the two buffers vk does not occupy
     float *bo[2] = {xb[0], (vk == v[0]) ? v[1] : v[0]};
     const float *xc = vk;
-    for (int it = 0; it < k; ++it) {
+    for (int it = 0; it < k; ++it) {   // the last iteration writes the preconditioner's output as well
         float *out = bo[it & 1];
+        const bool last = it == k - 1;
         hipLaunchKernelGGL(lj_bwd_kernel, g, b, 0, ctx->stream, N, ctx->d_urow, ctx->d_ucol, ctx->d_upos, s.W, s.Dinv,
-                           vk, xc, (float *)nullptr, out);
+                           vk, xc, (float *)nullptr, out, last ? ctx->d_perm : (const int *)nullptr,
+                           last ? bsaddle : (const float *)nullptr, last ? x : (float *)nullptr, last ? mode : -1);
         xc = out;
     }
-    hipLaunchKernelGGL(lj_post_kernel, g, b, 0, ctx->stream, N, ctx->d_perm, xc, bsaddle, x, mode);
+    if (k == 0) hipLaunchKernelGGL(lj_post_kernel, g, b, 0, ctx->stream, N, ctx->d_perm, xc, bsaddle, x, mode);
     FSD_HIP(hipGetLastError());
     prof_end(ctx, PROF_TRSV, ev);
     return 0;

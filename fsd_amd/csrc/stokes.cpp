@@ -531,10 +531,15 @@ static int precond_finish(fsd_ctx *ctx, const float4 *pos)
     const double t0 = wall_ms();
     RC(precond_upload_blocks(ctx, pos));
     ctx->lp_step_fallback = 0;
-    // learned factor (mode 2), unless this step calibrates IC(0) or the factor has a non-positive pivot
+    // learned factor (mode 2), unless this step calibrates IC(0), skips the learned factor after
+    // repeated iteration fallbacks, or the factor has a non-positive pivot
     bool learned = false;
     if (ctx->precond_mode == 2) {
         learned = lp_have_weights(ctx) && !ctx->lp_force_ichol;
+        if (learned && ctx->lp_skip) {
+            ctx->lp_step_fallback = 3;
+            learned = false;
+        }
         if (learned) {
             int failed = 0;
             RC(lp_forward(ctx, pos, false, ctx->lp_inject, &failed));
@@ -1693,6 +1698,8 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     if (rci < 0) return rci;
     ctx->lp_force_ichol = ctx->precond_mode == 2 && lp_have_weights(ctx) && ctx->lp_calib_every > 0 &&
                           ctx->lp_steps % ctx->lp_calib_every == 0;
+    ctx->lp_skip = ctx->precond_mode == 2 && !ctx->lp_force_ichol && ctx->lp_skip_after > 0 &&
+                   ctx->lp_consec_fb >= ctx->lp_skip_after;
     RC(precond_finish(ctx, pos));
     if (brownian) {
         // rhs[0:11N] = U_B^ff, rhs[11N:17N] = F_B^nf, Velocity[0:6N] = RFD drift
@@ -1715,8 +1722,14 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
         if (ctx->lp_force_ichol) {   // calibration step: IC(0)'s iteration count caps the learned solves
             ctx->lp_iter_cap = st.iterations;
             ctx->lp_calibrations++;
+            ctx->lp_consec_fb = 0;   // and the next step tries the learned factor again
+        } else if (st.fallback == 2) {
+            ctx->lp_consec_fb++;
+        } else if (st.precond_used == 1) {
+            ctx->lp_consec_fb = 0;
         }
         ctx->lp_force_ichol = false;
+        ctx->lp_skip = false;
         ctx->lp_steps++;
     }
     RC(k_axpby(ctx, 6 * (int64_t)N, 1.0f, ctx->d_sol + 11 * (int64_t)N, 1.0f, d_vel11, d_vel11));
@@ -1817,6 +1830,14 @@ int fsd_learned_set_calibration(fsd_ctx *ctx, int32_t every)
     if (every < 0) { ctx->err = "calibration period must be >= 0"; return FSD_E_PARAM; }
     ctx->lp_calib_every = every;
     ctx->lp_steps = 0;
+    return 0;
+}
+
+int fsd_learned_set_skip(fsd_ctx *ctx, int32_t after)
+{
+    if (after < 0) { ctx->err = "skip count must be >= 0"; return FSD_E_PARAM; }
+    ctx->lp_skip_after = after;
+    ctx->lp_consec_fb = 0;
     return 0;
 }
 

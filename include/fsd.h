@@ -123,7 +123,8 @@ typedef struct fsd_solve_stats {
     int32_t precond_used;            /* the solve's factor: 0 IC(0), 1 the learned factor (precond mode 2) */
     int32_t fallback;                /* mode 2: 0 none, 1 the learned factor had a non-positive pivot (IC(0)
                                         used), 2 the learned solve reached IC(0)'s iteration count and was
-                                        finished with IC(0) */
+                                        finished with IC(0), 3 the step skipped the learned factor (IC(0)
+                                        used) after fsd_learned_set_skip iteration fallbacks in a row */
 } fsd_solve_stats;
 
 /* ---- lifecycle: Stokes::Stokes + setParams + setSparseMath + AllocateWorkSpaces
@@ -316,6 +317,11 @@ int fsd_learned_get_grad(fsd_ctx *ctx, float *h_g, int64_t n);
 int fsd_learned_get_adam(fsd_ctx *ctx, float *h_m, float *h_v, int64_t n);
 int fsd_learned_set_adam(fsd_ctx *ctx, const float *h_m, const float *h_v, int64_t n);
 int fsd_learned_set_calibration(fsd_ctx *ctx, int32_t every);
+/* after this many iteration fallbacks in a row (fallback 2) the steps up to the next
+ * calibration step use IC(0) without building the learned factor (fallback 3); the
+ * calibration step resets the count (default 2; 0: never skip).  No counterpart in the
+ * reference, which has no learned factor. */
+int fsd_learned_set_skip(fsd_ctx *ctx, int32_t after);
 /* the next learned factor gets a non-finite pivot shift at this particle, which makes
  * the build fall back to IC(0) (failure-path tests) */
 int fsd_learned_inject_pivot(fsd_ctx *ctx, int32_t particle);

@@ -130,6 +130,54 @@ def test_learned_pivot_fallback(fallback_jacobi):
     assert info["precond_used"] == 1 and info["fallback"] == 0
 
 
+def _fallback_sequence(skip_after, steps=9, calibration=6):
+    """steps of the dense packing under fresh random forces (the warm start then
+    helps neither factor), the learned factor applied by a single Jacobi iteration
+    each way, so that its solves reach IC(0)'s iteration count (exact triangular
+    solves at the calibration step) and fall back"""
+    import torch
+    from fsd_amd.stokes import Stokes, ParticleState
+    N, phi, seed = CASES["D1"]
+    pos, L = packing(N, phi, seed)
+    st = Stokes(N, L, precond="learned", jacobi_iters=1, learned_weights=_weights(head=0.0), gmres_tol=1e-5,
+                gmres_restart=100, gmres_maxiter=2000, learned_calibration=calibration,
+                learned_skip_after=skip_after)
+    state = ParticleState(pos)
+    seq = []
+    for s in range(steps):
+        f = torch.from_numpy(random_vec(6 * N, 100 + s).astype(np.float32) * 1e-2).cuda()
+        info = st.integrateStepOne(s, state, force6=f, shear_rate=0.0)
+        assert info["converged"]
+        seq.append((info["precond_used"], info["fallback"]))
+    print("skip_after", skip_after, "(precond_used, fallback) per step", seq, st.learned_stats())
+    return seq
+
+
+def test_learned_skip_after_iteration_fallbacks():
+    """after two iteration fallbacks in a row (fallback 2) the steps up to the next
+    calibration step use IC(0) without building the learned factor (fallback 3); the
+    calibration step (every 6th) resets the count, so the step after it tries the
+    learned factor again; with the skip switched off every non-calibration step
+    tries it"""
+    calibration = 6
+    seq = _fallback_sequence(2, calibration=calibration)
+    consec = 0
+    for s, (used, fb) in enumerate(seq):
+        if s % calibration == 0:
+            assert (used, fb) == (0, 0)
+            consec = 0
+            continue
+        if consec >= 2:
+            assert (used, fb) == (0, 3)
+            continue
+        assert fb in (0, 2)
+        consec = consec + 1 if fb == 2 else 0
+    assert sum(fb == 3 for _, fb in seq) > 0, "no step reached the skip: the case does not exercise it"
+    assert seq[calibration + 1][1] != 3
+    seq0 = _fallback_sequence(0, calibration=calibration)
+    assert all(fb != 3 for _, fb in seq0)
+
+
 def test_trained_factor_gain():
     """the committed trained network against the method without it (zero network
     outputs: the pair blocks with uncorrected pivots, block symmetric Gauss-Seidel)
