@@ -304,7 +304,8 @@ struct fsd_ctx {
     // learned factor (precond mode 2, learned.h) and its fallbacks to IC(0)
     struct LearnedState *lp = nullptr;
     int lp_calib_every = 50;       // every this many steps one step runs IC(0) to measure its iteration count
-    int lp_iter_cap = 0;           // IC(0)'s iterations at the last such step (0: none yet)
+    int lp_iter_cap[3] = {0, 0, 0};   // IC(0)'s iterations at the last such step (0: none yet), for the
+                                      // step's solve and the two random-finite-difference solves
     int lp_inject = -1;            // particle whose learned pivot the next factor forces non-positive (tests)
     bool lp_force_ichol = false;   // this step's preconditioner is IC(0) (calibration)
     int lp_skip_after = 2;         // this many iteration fallbacks in a row skip the learned factor until the

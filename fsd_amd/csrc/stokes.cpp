@@ -803,21 +803,25 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
 }
 
 // The saddle solve with this step's preconditioner.  With the learned factor
-// the solve may take at most IC(0)'s iteration count of the last calibration
-// step; a solve that reaches it without converging continues from its iterate
-// with IC(0), built then (the iteration fallback).
+// the solve may take at most IC(0)'s iteration count for the same solve
+// (which: 0 the step's solve, 1 and 2 the two random-finite-difference solves
+// of a Brownian step) at the last calibration step, which records it; a solve
+// that reaches it without converging continues from its iterate with IC(0),
+// built then (the iteration fallback), and IC(0) stays for the rest of the step.
 static int solve_saddle(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, int maxit, fsd_solve_stats *st,
-                        bool r0_in_t)
+                        bool r0_in_t, int which = 0)
 {
     const bool learned = ctx->precond_mode == 2 && lp_active(ctx);
     int cap = maxit;
-    if (learned && ctx->lp_iter_cap > 0) cap = std::min(maxit, ctx->lp_iter_cap);
+    if (learned && ctx->lp_iter_cap[which] > 0) cap = std::min(maxit, ctx->lp_iter_cap[which]);
     int rc = gmres(ctx, pos, b, x, ctx->p.gmres_tol, cap, ctx->p.gmres_restart, st, r0_in_t);
     if (rc < 0) return rc;
+    if (ctx->lp_force_ichol) ctx->lp_iter_cap[which] = st->iterations;
     st->precond_used = learned ? 1 : 0;
     st->fallback = ctx->lp_step_fallback;
     if (learned && rc == FSD_W_NOT_CONVERGED && cap < maxit) {
         ctx->lp_iter_fallbacks++;
+        ctx->lp_step_fallback = 2;
         RC(precond_ichol(ctx));
         fsd_solve_stats s2;
         memset(&s2, 0, sizeof s2);
@@ -1015,14 +1019,15 @@ static int bro_rfd(fsd_ctx *ctx, const float4 *pos, uint32_t timestep, float *dr
     RC(k_fill(ctx, n17, 0.f, ctx->d_rfd_rhs));
     RC(k_fill(ctx, n17, 0.f, ctx->d_rfd_sol));
     RC(k_axpby(ctx, 6 * N, -1.0f, psi, 0.0f, nullptr, ctx->d_rfd_rhs + 11 * N));
-    const int maxit = ctx->p.gmres_maxiter, R = ctx->p.gmres_restart;
+    const int maxit = ctx->p.gmres_maxiter;
     for (int side = 0; side < 2; ++side) {
         RC(k_shift_wrap(ctx, pos, psi, side == 0 ? 0.5f * eps : -0.5f * eps, ctx->d_posp));
         RC(k_build_lub_pairs(ctx, ctx->d_posp));
         ctx->spread_valid = false;
         fsd_solve_stats st;
         memset(&st, 0, sizeof st);
-        int rc = gmres(ctx, ctx->d_posp, ctx->d_rfd_rhs, ctx->d_rfd_sol, ctx->p.gmres_tol, maxit, R, &st);
+        st.true_residual = -1.f;
+        const int rc = solve_saddle(ctx, ctx->d_posp, ctx->d_rfd_rhs, ctx->d_rfd_sol, maxit, &st, false, 1 + side);
         if (rc < 0) return rc;
         if (side == 0) RC(k_copy(ctx, 6 * N, ctx->d_rfd_sol + 11 * N, drift));
     }
@@ -1719,8 +1724,7 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     ctx->n_solved++;
     if (rc < 0) return rc;
     if (ctx->precond_mode == 2) {
-        if (ctx->lp_force_ichol) {   // calibration step: IC(0)'s iteration count caps the learned solves
-            ctx->lp_iter_cap = st.iterations;
+        if (ctx->lp_force_ichol) {   // calibration step: solve_saddle recorded IC(0)'s iteration counts
             ctx->lp_calibrations++;
             ctx->lp_consec_fb = 0;   // and the next step tries the learned factor again
         } else if (st.fallback == 2) {

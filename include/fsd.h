@@ -122,9 +122,11 @@ typedef struct fsd_solve_stats {
                                         moved r_buff / 2 since the last build, or the box changed) */
     int32_t precond_used;            /* the solve's factor: 0 IC(0), 1 the learned factor (precond mode 2) */
     int32_t fallback;                /* mode 2: 0 none, 1 the learned factor had a non-positive pivot (IC(0)
-                                        used), 2 the learned solve reached IC(0)'s iteration count and was
-                                        finished with IC(0), 3 the step skipped the learned factor (IC(0)
-                                        used) after fsd_learned_set_skip iteration fallbacks in a row */
+                                        used), 2 a learned solve of the step (its own or, in a Brownian step,
+                                        a random-finite-difference solve) reached IC(0)'s iteration count for
+                                        that solve and was finished with IC(0), which the rest of the step
+                                        then used, 3 the step skipped the learned factor (IC(0) used) after
+                                        fsd_learned_set_skip iteration fallbacks in a row */
 } fsd_solve_stats;
 
 /* ---- lifecycle: Stokes::Stokes + setParams + setSparseMath + AllocateWorkSpaces
@@ -304,9 +306,9 @@ int fsd_profile_reset(fsd_ctx *ctx);
  * The factor is block LDL^T, (D + W) D^-1 (D + W)^T, W on the pattern's lower pair
  * blocks and D one 6x6 pivot per particle (positive definite by construction); a step
  * falls back to IC(0) when the network's outputs are not finite, and a solve finishes
- * with IC(0) when it reaches the
- * iteration count IC(0) needed at the last calibration step (every
- * fsd_learned_set_calibration steps, default 50, one step runs IC(0)).
+ * with IC(0) when it reaches the iteration count IC(0) needed for the same solve at the
+ * last calibration step (every fsd_learned_set_calibration steps, default 50, one step
+ * runs IC(0)).
  * Parameters are FP32 (fsd_learned_nparams of them); the network runs them in BF16. */
 int fsd_learned_nparams(fsd_ctx *ctx, int64_t *n);
 int fsd_learned_set_weights(fsd_ctx *ctx, const float *h_w, int64_t n);
