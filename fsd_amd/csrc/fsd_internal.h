@@ -324,6 +324,7 @@ struct fsd_ctx {
     float *d_rhs = nullptr, *d_sol = nullptr;
     // FP64-residual iterative refinement (allocated on first use)
     double *d_x64 = nullptr, *d_r64 = nullptr, *d_ybot64 = nullptr, *d_norm64 = nullptr;
+    double ref_r0[2] = {0.0, 0.0};   // |r|^2, |b|^2 of a first residual formed ahead of the solve
     float *d_refx32 = nullptr, *d_refy32 = nullptr, *d_refr32 = nullptr, *d_refd = nullptr;
     float *d_sol_prev = nullptr;  // x_{n-2} when the initial guess extrapolates (guess >= 1)
     float *d_sol_prev2 = nullptr; // x_{n-3} for the quadratic extrapolation (guess = 2)

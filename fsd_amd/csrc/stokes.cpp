@@ -584,13 +584,14 @@ static int gmres_core(fsd_ctx *ctx, const float4 *pos, const float *b, float *x,
 // residual estimate is the true residual ||b - A x||: it stops at
 // gmres_true_tol, and the true residual is then recomputed and the solve
 // continued (warm) if FP32 drift left it above the tolerance.
+// (refined: r0_in_t means refine_first_residual has formed the first FP64 residual)
 static int gmres_refined(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, int maxit, int R,
-                         fsd_solve_stats *st);
+                         fsd_solve_stats *st, bool r0_ready);
 
 static int gmres(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, float tol, int maxit, int R,
                  fsd_solve_stats *st, bool r0_in_t = false)
 {
-    if (ctx->p.refine && ctx->p.gmres_true_tol > 0.f) return gmres_refined(ctx, pos, b, x, maxit, R, st);
+    if (ctx->p.refine && ctx->p.gmres_true_tol > 0.f) return gmres_refined(ctx, pos, b, x, maxit, R, st, r0_in_t);
     const float ttol = ctx->p.gmres_true_tol;
     if (ttol <= 0.f) return gmres_core(ctx, pos, b, x, tol, maxit, R, false, false, st);
     int total = 0, restarts = 0, rc = 0;
@@ -618,28 +619,60 @@ static int gmres(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, floa
 // FP64 (k_residual64), stops at |r| <= gmres_true_tol |b|, and otherwise solves
 // A d = r with the FP32 right-preconditioned GMRES to a relative tolerance aimed at
 // the target (at most 0.5, at least 1e-4, the FP32 floor) and adds d in FP64.
+static int refine_alloc(fsd_ctx *ctx)
+{
+    if (ctx->d_x64) return 0;
+    const int64_t n = 17 * (int64_t)ctx->N;
+    CK(hipMalloc(&ctx->d_x64, sizeof(double) * n));
+    CK(hipMalloc(&ctx->d_r64, sizeof(double) * n));
+    CK(hipMalloc(&ctx->d_ybot64, sizeof(double) * 6 * (size_t)ctx->N));
+    CK(hipMalloc(&ctx->d_norm64, sizeof(double) * 2));
+    for (float **pp : {&ctx->d_refx32, &ctx->d_refy32, &ctx->d_refr32, &ctx->d_refd})
+        CK(hipMalloc(pp, sizeof(float) * n));
+    return 0;
+}
+
+// r = b - A x64 in FP64 (d_r64, and in FP32 d_refr32), nr = |r|^2, |b|^2
+static int refine_residual(fsd_ctx *ctx, const float4 *pos, const float *b, double nr[2])
+{
+    RC(k_residual64(ctx, pos, b, ctx->d_x64, ctx->d_refx32, ctx->d_refy32, ctx->d_ybot64, ctx->d_r64,
+                    ctx->d_refr32, ctx->d_norm64));
+    CK(hipMemcpyAsync(ctx->h_part, ctx->d_norm64, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    nr[0] = ctx->h_part[0];
+    nr[1] = ctx->h_part[1];
+    return 0;
+}
+
+// the refinement's first round before the preconditioner exists: x64 = x and its residual,
+// whose norms gmres_refined then takes from ctx->ref_r0 (fsd_step forms them while the host
+// plans the preconditioner)
+static int refine_first_residual(fsd_ctx *ctx, const float4 *pos, const float *b, const float *x)
+{
+    RC(refine_alloc(ctx));
+    RC(k_to64(ctx, 17 * (int64_t)ctx->N, x, ctx->d_x64));
+    return refine_residual(ctx, pos, b, ctx->ref_r0);
+}
+
 static int gmres_refined(fsd_ctx *ctx, const float4 *pos, const float *b, float *x, int maxit, int R,
-                         fsd_solve_stats *st)
+                         fsd_solve_stats *st, bool r0_ready)
 {
     const int64_t n = 17 * (int64_t)ctx->N;
-    if (!ctx->d_x64) {
-        CK(hipMalloc(&ctx->d_x64, sizeof(double) * n));
-        CK(hipMalloc(&ctx->d_r64, sizeof(double) * n));
-        CK(hipMalloc(&ctx->d_ybot64, sizeof(double) * 6 * (size_t)ctx->N));
-        CK(hipMalloc(&ctx->d_norm64, sizeof(double) * 2));
-        for (float **pp : {&ctx->d_refx32, &ctx->d_refy32, &ctx->d_refr32, &ctx->d_refd})
-            CK(hipMalloc(pp, sizeof(float) * n));
-    }
+    RC(refine_alloc(ctx));
     const double ttol = ctx->p.gmres_true_tol;
-    RC(k_to64(ctx, n, x, ctx->d_x64));
+    if (!r0_ready) RC(k_to64(ctx, n, x, ctx->d_x64));
     int total = 0, restarts = 0, rounds = 0;
     double rel = 1.0, r0 = -1.0;
     for (;;) {
-        RC(k_residual64(ctx, pos, b, ctx->d_x64, ctx->d_refx32, ctx->d_refy32, ctx->d_ybot64, ctx->d_r64,
-                        ctx->d_refr32, ctx->d_norm64));
-        CK(hipMemcpyAsync(ctx->h_part, ctx->d_norm64, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        CK(hipStreamSynchronize(ctx->stream));
-        rel = std::sqrt(ctx->h_part[0]) / std::max(std::sqrt(ctx->h_part[1]), 1e-300);
+        double nr[2];
+        if (r0_ready) {
+            nr[0] = ctx->ref_r0[0];
+            nr[1] = ctx->ref_r0[1];
+            r0_ready = false;
+        } else {
+            RC(refine_residual(ctx, pos, b, nr));
+        }
+        rel = std::sqrt(nr[0]) / std::max(std::sqrt(nr[1]), 1e-300);
         if (r0 < 0.0) r0 = rel;
         if (rel <= ttol || total >= maxit || rounds >= 12) break;
         RC(k_fill(ctx, n, 0.f, ctx->d_refd));
@@ -1670,7 +1703,7 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     RC(precond_fetch(ctx, pos));
     std::thread planner(precond_plan, ctx);
     const bool brownian = ctx->p.T > 0.f;
-    bool r0_in_t = false;   // d_t = A x0 - b formed below
+    bool r0_in_t = false;   // the first residual formed below (d_t = A x0 - b, or the refinement's FP64 one)
     auto independent = [&]() -> int {
         // applied force: injected, or the reference force law with the external torque
         if (d_force6) {
@@ -1693,6 +1726,9 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
             if (ctx->p.gmres_true_tol > 0.f && !ctx->p.refine) {
                 RC(k_saddle_multiply(ctx, pos, ctx->d_sol, ctx->d_t));
                 RC(k_axpby(ctx, n17, 1.0f, ctx->d_t, -1.0f, ctx->d_rhs, ctx->d_t));
+                r0_in_t = true;
+            } else if (ctx->p.gmres_true_tol > 0.f) {   // the refined solve's first FP64 residual
+                RC(refine_first_residual(ctx, pos, ctx->d_rhs, ctx->d_sol));
                 r0_in_t = true;
             }
         }
