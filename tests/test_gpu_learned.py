@@ -178,6 +178,33 @@ def test_learned_skip_after_iteration_fallbacks():
     assert all(fb != 3 for _, fb in seq0)
 
 
+def test_learned_brownian_solves_capped():
+    """Brownian steps (kT = 1) run three saddle solves: the step's own and the two
+    random-finite-difference solves.  With the weak learned application of the test
+    above, every one of them is capped at IC(0)'s iteration count for the same solve
+    (recorded at the calibration step) and finishes with IC(0) when it reaches it:
+    each non-calibration step reports the iteration fallback and converges, and the
+    count of iteration fallbacks grows by at least one per such step"""
+    import torch
+    from fsd_amd.stokes import Stokes, ParticleState
+    N, phi, seed = CASES["D1"]
+    pos, L = packing(N, phi, seed)
+    st = Stokes(N, L, T=1.0, seed=5, precond="learned", jacobi_iters=1, learned_weights=_weights(head=0.0),
+                gmres_tol=1e-5, gmres_restart=100, gmres_maxiter=2000, learned_calibration=4, learned_skip_after=0)
+    state = ParticleState(pos)
+    seq, fbs = [], []
+    for s in range(4):
+        info = st.integrateStepOne(s, state, force6=None, shear_rate=0.0)
+        assert info["converged"]
+        seq.append((info["precond_used"], info["fallback"], info["iterations"]))
+        fbs.append(st.learned_stats()["iteration_fallbacks"])
+    print("(precond_used, fallback, iterations) per step", seq, "iteration fallbacks", fbs)
+    assert seq[0][:2] == (0, 0) and fbs[0] == 0
+    for s in range(1, 4):
+        assert seq[s][:2] == (0, 2)
+        assert fbs[s] >= fbs[s - 1] + 1
+
+
 def test_trained_factor_gain():
     """the committed trained network against the method without it (zero network
     outputs: the pair blocks with uncorrected pivots, block symmetric Gauss-Seidel)
