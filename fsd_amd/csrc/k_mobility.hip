@@ -638,7 +638,10 @@ __global__ void __launch_bounds__(256) fft_x_green_kernel(float2 *spec, const fl
     const int tid = threadIdx.x;
     const int Nzh = g.Nz / 2 + 1;
     const int nchunk = (Nzh + FX_KC - 1) / FX_KC;
-    const int ky = blockIdx.x / nchunk, kz0 = (blockIdx.x % nchunk) * FX_KC;
+    // the chunks of a ky row share cache lines (a chunk is FX_KC of the row's Nzh modes): a contiguous
+    // range of rows per XCD keeps every line's readers and writers on one L2
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int ky = blk / nchunk, kz0 = (blk % nchunk) * FX_KC;
     const int kc = min(FX_KC, Nzh - kz0);
     if (tid < NX) {
         double sn, cs;
