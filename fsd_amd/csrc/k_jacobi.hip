@@ -24,7 +24,8 @@ constexpr int JROWS = 10;   // block rows per 64-lane wave (lanes 60..63 idle)
 __device__ __forceinline__ bool jac_row(int N, int &P, int &a)
 {
     const int lane = threadIdx.x & 63;
-    const int wave = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    // rows in one contiguous range per XCD (cache lines shared by neighbouring rows stay in one L2)
+    const int wave = (int)((xcd_remap(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x) >> 6);
     const int g = lane / 6;
     a = lane - 6 * g;
     P = wave * JROWS + g;

@@ -150,8 +150,10 @@ __device__ __forceinline__ float basis_entry(int k, int a, int c, const float r[
 // six lanes per row, ten rows per wave
 __device__ __forceinline__ bool row_lane(int N, int &P, int &a)
 {
+    // rows in one contiguous range per XCD: the blocks and iterate entries that neighbouring rows
+    // share a cache line with are then read through the same L2
     const int lane = threadIdx.x & 63;
-    const int wave = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    const int wave = (int)((xcd_remap(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x) >> 6);
     const int g = lane / 6;
     a = lane - 6 * g;
     P = wave * 10 + g;
