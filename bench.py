@@ -17,7 +17,9 @@ Three arms run on the same packing from the same start (--arms):
                  network outputs; iteration count above IC(0)'s at the last
                  calibration step, one IC(0) step every --calibration steps; its
                  IC(0) steps applied by 8 Jacobi iterations) and FP64-residual
-                 iterative refinement of the solve.
+                 iterative refinement of the solve; the particles keep the
+                 calibration step's RCM ordering until the next one
+                 (--rcm-every-step: the RCM of every step's graph).
 Each context autotunes its mobility and R_FU launch shapes at its first step.
 `value` is the learned arm's steps/s (all inputs resident in HBM); every arm and
 the learned/IC(0) ratio are in `config.arms`.
@@ -199,6 +201,9 @@ def main():
                     help="comma-separated arms to run (ichol, ichol_jacobi, learned); the last is `value`")
     ap.add_argument("--profile-all", action="store_true", help="time every kernel class (diagnostics)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--rcm-every-step", action="store_true",
+                    help="learned arm: order the particles by the RCM of every step's graph (default: the "
+                         "calibration step's ordering is kept until the next calibration step)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MB",
                     help="only run 5 device copies of MB megabytes (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -230,7 +235,8 @@ def main():
     arm_kw = {"ichol": dict(precond="ichol"),
               "ichol_jacobi": dict(precond="ichol_jacobi", jacobi_iters=args.ichol_jacobi),
               "learned": dict(precond="learned", jacobi_iters=args.jacobi, learned_calibration=args.calibration,
-                              refine=True, fallback_jacobi=args.ichol_jacobi)}
+                              refine=True, fallback_jacobi=args.ichol_jacobi,
+                              learned_keep_order=not args.rcm_every_step)}
     names = [a for a in args.arms.split(",") if a]
     arms, head = {}, names[-1]
     for name in names:

@@ -124,6 +124,7 @@ SIGNATURES = [
     ("fsd_learned_get_adam", _I, [_P, _P, _P, C.c_int64]),
     ("fsd_learned_set_adam", _I, [_P, _P, _P, C.c_int64]),
     ("fsd_learned_set_calibration", _I, [_P, _I]),
+    ("fsd_learned_set_keep_order", _I, [_P, _I]),
     ("fsd_learned_set_skip", _I, [_P, _I]),
     ("fsd_learned_inject_pivot", _I, [_P, _I]),
     ("fsd_learned_stats", _I, [_P, _P]),

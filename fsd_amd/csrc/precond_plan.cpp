@@ -51,15 +51,19 @@ void levels(int N, const std::vector<int> &row, const std::vector<int> &col, boo
 
 }  // namespace
 
-void plan_precond(int N, const int *pn, const int *plstride, int stride, int ring, int threads, PrecondHost &h)
+void plan_precond(int N, const int *pn, const int *plstride, int stride, int ring, int threads, PrecondHost &h,
+                  bool keep_order)
 {
     const double t0 = now_ms();
     threads = std::max(1, threads);
-    // RCM of the pruned graph (self loops included), as Precondition_Reorder passes it to genrcm
-    h.perm.resize(N);
-    host_rcm_rows(N, pn, plstride, stride, h.perm.data());
-    h.iperm.resize(N);
-    for (int P = 0; P < N; ++P) h.iperm[h.perm[P]] = P;
+    // RCM of the pruned graph (self loops included), as Precondition_Reorder passes it to genrcm;
+    // keep_order keeps the last plan's ordering instead (any ordering gives a valid factor)
+    if (!keep_order || (int)h.perm.size() != N || (int)h.iperm.size() != N) {
+        h.perm.resize(N);
+        host_rcm_rows(N, pn, plstride, stride, h.perm.data());
+        h.iperm.resize(N);
+        for (int P = 0; P < N; ++P) h.iperm[h.perm[P]] = P;
+    }
     const double t1 = now_ms();
 
     // block adjacency on the permuted graph.  The graph is symmetric, so row P's blocks

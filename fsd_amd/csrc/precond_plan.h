@@ -34,5 +34,7 @@ struct PrecondHost {
 
 // pn[i] <= PLAN_MAX_ROW: entries of particle i's pruned list (self included,
 // ascending), stored at plist[i * stride ...]; ring: LDS ring rows of the sweep kernel (0: no
-// single-workgroup plans); threads: worker threads for the parallel parts.
-void plan_precond(int N, const int *pn, const int *plist, int stride, int ring, int threads, PrecondHost &h);
+// single-workgroup plans); threads: worker threads for the parallel parts; keep_order: keep the
+// ordering of the last plan in h (of the same N) instead of computing the RCM of this graph.
+void plan_precond(int N, const int *pn, const int *plist, int stride, int ring, int threads, PrecondHost &h,
+                  bool keep_order = false);

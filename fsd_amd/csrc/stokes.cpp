@@ -460,10 +460,10 @@ static double wall_ms()
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-static void precond_plan(fsd_ctx *ctx)
+static void precond_plan(fsd_ctx *ctx, bool keep_order = false)
 {
     const double t0 = wall_ms();
-    plan_precond(ctx->N, ctx->h_pn.data(), ctx->h_plstride.data(), FSD_PSTRIDE, ctx->tp_R, 4, ctx->pch);
+    plan_precond(ctx->N, ctx->h_pn.data(), ctx->h_plstride.data(), FSD_PSTRIDE, ctx->tp_R, 4, ctx->pch, keep_order);
     prof_host(ctx, PROF_PLAN_HOST, wall_ms() - t0);
 }
 
@@ -1701,7 +1701,14 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     // preconditioner (Precondition_Wrap): its host phase runs on a second thread while the
     // kernels that do not depend on it are enqueued
     RC(precond_fetch(ctx, pos));
-    std::thread planner(precond_plan, ctx);
+    // precond mode 2: IC(0) calibrates on every lp_calib_every-th step; the steps between keep that
+    // step's ordering of the particles (fsd_learned_set_keep_order), so their host plan skips the RCM
+    const bool calib = ctx->precond_mode == 2 && lp_have_weights(ctx) && ctx->lp_calib_every > 0 &&
+                       ctx->lp_steps % ctx->lp_calib_every == 0;
+    const int reorder_every = ctx->lp_calib_every > 0 ? ctx->lp_calib_every : 50;
+    const bool keep_order = ctx->precond_mode == 2 && lp_have_weights(ctx) && ctx->lp_keep_order && !calib &&
+                            ctx->lp_steps % reorder_every != 0;
+    std::thread planner(precond_plan, ctx, keep_order);
     const bool brownian = ctx->p.T > 0.f;
     bool r0_in_t = false;   // the first residual formed below (d_t = A x0 - b, or the refinement's FP64 one)
     auto independent = [&]() -> int {
@@ -1737,8 +1744,7 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     const int rci = independent();
     planner.join();
     if (rci < 0) return rci;
-    ctx->lp_force_ichol = ctx->precond_mode == 2 && lp_have_weights(ctx) && ctx->lp_calib_every > 0 &&
-                          ctx->lp_steps % ctx->lp_calib_every == 0;
+    ctx->lp_force_ichol = calib;
     ctx->lp_skip = ctx->precond_mode == 2 && !ctx->lp_force_ichol && ctx->lp_skip_after > 0 &&
                    ctx->lp_consec_fb >= ctx->lp_skip_after;
     RC(precond_finish(ctx, pos));
@@ -1870,6 +1876,12 @@ int fsd_learned_set_calibration(fsd_ctx *ctx, int32_t every)
     if (every < 0) { ctx->err = "calibration period must be >= 0"; return FSD_E_PARAM; }
     ctx->lp_calib_every = every;
     ctx->lp_steps = 0;
+    return 0;
+}
+
+int fsd_learned_set_keep_order(fsd_ctx *ctx, int32_t on)
+{
+    ctx->lp_keep_order = on != 0;
     return 0;
 }
 

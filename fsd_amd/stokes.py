@@ -39,7 +39,8 @@ class Stokes:
                  dt=1e-3, max_strain=0.5, gmres_tol=None, gmres_restart=50, gmres_maxiter=1000,
                  parity=True, gmres_true_tol=0.0, guess="previous", gram_schmidt_passes=2,
                  precond="ichol", jacobi_iters=3, factor_sweeps=0, learned_weights="default",
-                 learned_calibration=50, refine=False, fallback_jacobi=0, autotune=False, learned_skip_after=2):
+                 learned_calibration=50, refine=False, fallback_jacobi=0, autotune=False, learned_skip_after=2,
+                 learned_keep_order=True):
         self.torch = _torch()
         self.lib = _lib.load()
         L = box if not np.isscalar(box) else (box, box, box, 0.0, 0.0, 0.0)
@@ -98,6 +99,7 @@ class Stokes:
                 self.learned_set_weights(w)
             self._check(self.lib.fsd_learned_set_calibration(self.ctx, int(learned_calibration)))
             self._check(self.lib.fsd_learned_set_skip(self.ctx, int(learned_skip_after)))
+            self._check(self.lib.fsd_learned_set_keep_order(self.ctx, 1 if learned_keep_order else 0))
 
     # ---- error handling
     def _check(self, rc):

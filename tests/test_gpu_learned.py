@@ -205,6 +205,38 @@ def test_learned_brownian_solves_capped():
         assert fbs[s] >= fbs[s - 1] + 1
 
 
+def test_learned_keep_order_steps():
+    """steps between calibration steps keep the calibration step's ordering of the
+    particles (no RCM of their own graph): the factor is then built on another
+    ordering of the same pattern, the solves converge to the same tolerance, and the
+    velocities agree with those of steps that order every graph by its RCM to the
+    solve tolerance"""
+    import torch
+    from fsd_amd.stokes import Stokes, ParticleState
+    N, phi, seed = CASES["D1"]
+    pos, L = packing(N, phi, seed)
+    f = torch.from_numpy(random_vec(6 * N, 77).astype(np.float32) * 1e-2).cuda()
+    out = {}
+    for keep in (True, False):
+        st = Stokes(N, L, precond="learned", jacobi_iters=5, learned_weights=_weights(head=0.0), gmres_tol=1e-6,
+                    gmres_true_tol=2e-7, refine=True, gmres_restart=100, learned_calibration=4,
+                    learned_keep_order=keep, dt=1e-2)
+        state = ParticleState(pos)
+        vel, its = [], []
+        for s in range(4):
+            info = st.integrateStepOne(s, state, force6=f, shear_rate=1.0)
+            assert info["converged"] and info["true_residual"] <= 2e-7
+            vel.append(state.velocity.cpu().numpy().astype(np.float64))
+            its.append(info["iterations"])
+        out[keep] = (vel, its, state.positions().astype(np.float64))
+        print("keep_order", keep, "iterations", its)
+    for s in range(4):
+        e = rel_err2(out[True][0][s], out[False][0][s])
+        print("step", s, "velocity rel diff", e)
+        assert e < 1e-5
+    assert np.max(np.abs(out[True][2] - out[False][2])) < 1e-5
+
+
 def test_trained_factor_gain():
     """the committed trained network against the method without it (zero network
     outputs: the pair blocks with uncorrected pivots, block symmetric Gauss-Seidel)
