@@ -17,8 +17,8 @@ Three arms run on the same packing from the same start (--arms):
                  network outputs; iteration count above IC(0)'s at the last
                  calibration step, one IC(0) step every --calibration steps; its
                  IC(0) steps applied by 8 Jacobi iterations) and FP64-residual
-                 iterative refinement of the solve; the particles keep the
-                 calibration step's RCM ordering until the next one
+                 iterative refinement of the solve; a step that does not calibrate
+                 keeps the last RCM ordering while its neighbour list stands
                  (--rcm-every-step: the RCM of every step's graph).
 Each context autotunes its mobility and R_FU launch shapes at its first step.
 `value` is the learned arm's steps/s (all inputs resident in HBM); every arm and
@@ -203,7 +203,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--rcm-every-step", action="store_true",
                     help="learned arm: order the particles by the RCM of every step's graph (default: the "
-                         "calibration step's ordering is kept until the next calibration step)")
+                         "last ordering is kept while the neighbour list it was computed under stands)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MB",
                     help="only run 5 device copies of MB megabytes (calibrates the HBM byte counters) and exit")
     args = ap.parse_args()

@@ -308,7 +308,8 @@ struct fsd_ctx {
                                       // step's solve and the two random-finite-difference solves
     int lp_inject = -1;            // particle whose learned pivot the next factor forces non-positive (tests)
     bool lp_force_ichol = false;   // this step's preconditioner is IC(0) (calibration)
-    bool lp_keep_order = true;     // steps between calibration steps keep the calibration step's ordering
+    bool lp_keep_order = true;     // steps keep the last ordering while its neighbour list stands
+    int64_t order_nlist = -1;      // nlist_builds when the preconditioner's ordering was last computed
     int lp_skip_after = 2;         // this many iteration fallbacks in a row skip the learned factor until the
                                    // next calibration step (0: never skip)
     int lp_consec_fb = 0;          // iteration fallbacks in a row since the last learned solve that converged

@@ -464,6 +464,7 @@ static void precond_plan(fsd_ctx *ctx, bool keep_order = false)
 {
     const double t0 = wall_ms();
     plan_precond(ctx->N, ctx->h_pn.data(), ctx->h_plstride.data(), FSD_PSTRIDE, ctx->tp_R, 4, ctx->pch, keep_order);
+    if (!keep_order) ctx->order_nlist = ctx->nlist_builds;   // the list this ordering was computed under
     prof_host(ctx, PROF_PLAN_HOST, wall_ms() - t0);
 }
 
@@ -1701,13 +1702,14 @@ int fsd_step(fsd_ctx *ctx, uint32_t timestep, float shear_rate, float *d_pos4, f
     // preconditioner (Precondition_Wrap): its host phase runs on a second thread while the
     // kernels that do not depend on it are enqueued
     RC(precond_fetch(ctx, pos));
-    // precond mode 2: IC(0) calibrates on every lp_calib_every-th step; the steps between keep that
-    // step's ordering of the particles (fsd_learned_set_keep_order), so their host plan skips the RCM
+    // precond mode 2: IC(0) calibrates on every lp_calib_every-th step.  The other steps keep the
+    // last computed ordering of the particles (fsd_learned_set_keep_order), so that their host plan
+    // skips the RCM, while the neighbour list it was computed under stands: no particle has moved
+    // r_buff / 2 since, so the pruned graph differs from the one it orders by a few pairs at most
     const bool calib = ctx->precond_mode == 2 && lp_have_weights(ctx) && ctx->lp_calib_every > 0 &&
                        ctx->lp_steps % ctx->lp_calib_every == 0;
-    const int reorder_every = ctx->lp_calib_every > 0 ? ctx->lp_calib_every : 50;
     const bool keep_order = ctx->precond_mode == 2 && lp_have_weights(ctx) && ctx->lp_keep_order && !calib &&
-                            ctx->lp_steps % reorder_every != 0;
+                            ctx->order_nlist == ctx->nlist_builds;
     std::thread planner(precond_plan, ctx, keep_order);
     const bool brownian = ctx->p.T > 0.f;
     bool r0_in_t = false;   // the first residual formed below (d_t = A x0 - b, or the refinement's FP64 one)

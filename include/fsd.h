@@ -319,11 +319,12 @@ int fsd_learned_get_grad(fsd_ctx *ctx, float *h_g, int64_t n);
 int fsd_learned_get_adam(fsd_ctx *ctx, float *h_m, float *h_v, int64_t n);
 int fsd_learned_set_adam(fsd_ctx *ctx, const float *h_m, const float *h_v, int64_t n);
 int fsd_learned_set_calibration(fsd_ctx *ctx, int32_t every);
-/* on (default): in fsd_step, the steps between calibration steps keep the calibration
- * step's ordering of the particles (the reference's RCM of that step's pruned graph,
- * Precondition.cu:1009) instead of computing the RCM of their own graph, which any
- * ordering factors validly; the ordering is refreshed at least every 50 steps.  Off: every
- * step orders as the reference does.  No counterpart in the reference. */
+/* on (default): in fsd_step, a step that does not calibrate keeps the last computed
+ * ordering of the particles (the reference's RCM of an earlier step's pruned graph,
+ * Precondition.cu:1009) instead of computing the RCM of its own graph, which any ordering
+ * factors validly, as long as the neighbour list that ordering was computed under has not
+ * been rebuilt (no particle moved r_buff / 2 since).  Off: every step orders as the
+ * reference does.  No counterpart in the reference. */
 int fsd_learned_set_keep_order(fsd_ctx *ctx, int32_t on);
 /* after this many iteration fallbacks in a row (fallback 2) the steps up to the next
  * calibration step use IC(0) without building the learned factor (fallback 3); the

@@ -206,11 +206,11 @@ def test_learned_brownian_solves_capped():
 
 
 def test_learned_keep_order_steps():
-    """steps between calibration steps keep the calibration step's ordering of the
-    particles (no RCM of their own graph): the factor is then built on another
-    ordering of the same pattern, the solves converge to the same tolerance, and the
-    velocities agree with those of steps that order every graph by its RCM to the
-    solve tolerance"""
+    """steps that do not calibrate keep the last ordering of the particles while its
+    neighbour list stands (no RCM of their own graph): the factor is then built on
+    another ordering of the same pattern, the solves converge to the same tolerance,
+    and the velocities agree with those of steps that order every graph by its RCM to
+    the solve tolerance"""
     import torch
     from fsd_amd.stokes import Stokes, ParticleState
     N, phi, seed = CASES["D1"]
