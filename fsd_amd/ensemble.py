@@ -7,7 +7,7 @@ preconditioner's host plan, the Lanczos eigensolves) overlap with the kernels of
 the others.  The steps are not captured into one hipGraph and no kernel is
 persistent: each step keeps those host synchronisations.
 
-    python -m fsd_amd.ensemble [--replicas 128] [--steps 5] [--threads 32]
+    python -m fsd_amd.ensemble [--replicas 128] [--steps 5] [--threads 16]
 
 prints one JSON line: aggregate particle-steps/s over the timed steps and the
 p50/p99 of the per-replica step times.  Replica sizes are uniform in
@@ -42,7 +42,7 @@ def replica_params(T, dt, seed, i, precond, autotune):
                 guess="extrapolate", gram_schmidt_passes=1, precond=precond, jacobi_iters=8, autotune=autotune)
 
 
-def run_ensemble(R=128, steps=5, warmup=2, threads=32, T=1.0, dt=1e-3, seed=0, precond="ichol_jacobi",
+def run_ensemble(R=128, steps=5, warmup=2, threads=16, T=1.0, dt=1e-3, seed=0, precond="ichol_jacobi",
                  autotune=True, log=print, replicas=None, keep=False):
     """steps the replicas (make_replicas(R, seed) unless given) warmup + steps times with a
     pool of host threads; returns the summary (and, with keep, the replicas with their final
@@ -106,7 +106,7 @@ def main(argv=None):
     ap.add_argument("--replicas", type=int, default=128)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--threads", type=int, default=32)
+    ap.add_argument("--threads", type=int, default=16)
     a = ap.parse_args(argv)
     out = run_ensemble(R=a.replicas, steps=a.steps, warmup=a.warmup, threads=a.threads,
                        log=lambda m: print(m, file=sys.stderr, flush=True))
