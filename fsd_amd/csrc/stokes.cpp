@@ -645,7 +645,7 @@ static int refine_residual(fsd_ctx *ctx, const float4 *pos, const float *b, doub
     return 0;
 }
 
-// the refinement's first round before the preconditioner exists: x64 = x and its residual,
+// the refinement's first residual, formed before the preconditioner exists: x64 = x and r,
 // whose norms gmres_refined then takes from ctx->ref_r0 (fsd_step forms them while the host
 // plans the preconditioner)
 static int refine_first_residual(fsd_ctx *ctx, const float4 *pos, const float *b, const float *x)
