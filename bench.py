@@ -301,6 +301,8 @@ def main():
                                             GUESS_TEXT[args.guess], head),
             "N": N, "phi": args.phi, "grid": [pse["Nx"], pse["Ny"], pse["Nz"]], "P": pse["P"],
             "preconditioner": head, "jacobi_iters": args.jacobi, "calibration_every": args.calibration,
+            "learned_ordering": "RCM of every step" if args.rcm_every_step else
+                                "last RCM kept while its neighbour list stands",
             "gmres_restart": args.restart, "initial_guess": args.guess, "gram_schmidt_passes": args.gram_schmidt,
             "arms": arms, "speedup_learned_vs_ichol": ratio,
             "parallelism": "single GPU",
