@@ -77,6 +77,21 @@ def test_sweep_schedule_emulation(tmp_path, args):
     assert "hazards=0 oob=0" in r.stdout
 
 
+@pytest.mark.parametrize("N,seed", [(2000, 1), (5000, 7)])
+def test_plan_keep_order(tmp_path, N, seed):
+    """the host plan with the last ordering kept (precond mode 2 while the neighbour
+    list stands): on a perturbed graph it keeps the ordering and builds the new graph's
+    block pattern and level schedules under it; without keep_order it orders by the
+    new graph's RCM"""
+    exe = str(tmp_path / "plan_keep_order")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", exe, os.path.join(CPU, "plan_keep_order.cpp"),
+                    os.path.join(CSRC, "precond_plan.cpp"), os.path.join(CSRC, "rcm.cpp"),
+                    os.path.join(CSRC, "sweep_plan.cpp"), "-lpthread"], check=True)
+    r = subprocess.run([exe, str(N), str(seed)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "bad=0" in r.stdout
+
+
 def test_tridiagonal_square_root(tmp_path):
     """FP64 T^(1/2) e_1 of the Lanczos sampler (lanczos_host.cpp) satisfies
     t^T t = T_11 and t^T T t = (T^2)_11 up to m = 99"""
